@@ -1,0 +1,235 @@
+"""MeshBVH: the drop-in mesh query object, backed by the gfx950 HIP library.
+
+Same constructor, methods, keyword arguments and result types as Atom3D
+atom3d/core/mesh_bvh.py:42-1397 for the hot path:
+
+* ``MeshBVH(vertices, faces, device=None, face_normals=None, vertex_normals=None)``
+  (:59-117) — casts, bounds, face normals, Baerentzen-Aanaes pseudo-normal
+  tables (:129-232) and a device BVH;
+* ``udf(points, return_grad, return_closest, return_uvw, return_face_ids)`` (:601-700);
+* ``sdf(points, mode="pseudo_normal", ...)`` (:702-827);
+* ``intersect_aabb(aabb_min, aabb_max, mode)`` (:297-479).
+
+There is no silent fallback: the reference catches every native failure and
+falls back to slower code with different maths (:122-127, :352-379, :656-683);
+here a failure raises ``RuntimeError`` and the device must be a GPU.
+"""
+from __future__ import annotations
+
+from typing import Optional, Tuple, Union
+
+import torch
+
+from ..kernels import _lib
+from ..kernels.bvh import BVHAccelerator
+from .data_structures import AABBIntersectResult, ClosestPointResult
+from .device_utils import resolve_device
+
+
+class _PointDistance(torch.autograd.Function):
+    """|p - c| per row with c constant; hand-written HIP forward and adjoint."""
+
+    @staticmethod
+    def forward(ctx, points: torch.Tensor, closest: torch.Tensor) -> torch.Tensor:
+        p = points.detach().contiguous().float()
+        c = closest.detach().contiguous().float()
+        d = torch.empty(p.shape[0], dtype=torch.float32, device=p.device)
+        with torch.cuda.device(p.device):
+            _lib.call("a3d_point_dist", _lib.ptr(p), _lib.ptr(c), p.shape[0], _lib.ptr(d), _lib.stream(p.device))
+        ctx.save_for_backward(p, c, d)
+        ctx.in_dtype = points.dtype
+        return d
+
+    @staticmethod
+    def backward(ctx, grad: torch.Tensor):
+        p, c, d = ctx.saved_tensors
+        g = grad.contiguous().float()
+        gp = torch.empty_like(p)
+        with torch.cuda.device(p.device):
+            _lib.call("a3d_point_dist_backward", _lib.ptr(p), _lib.ptr(c), _lib.ptr(d), _lib.ptr(g), p.shape[0],
+                      _lib.ptr(gp), _lib.stream(p.device))
+        return gp.to(ctx.in_dtype), None
+
+
+class MeshBVH:
+    """Triangle mesh with a device BVH for closest-point, SDF and box queries."""
+
+    def __init__(
+        self,
+        vertices: torch.Tensor,
+        faces: torch.Tensor,
+        device: Optional[Union[str, torch.device]] = None,
+        face_normals: Optional[torch.Tensor] = None,
+        vertex_normals: Optional[torch.Tensor] = None,
+    ):
+        dev = resolve_device(vertices, faces, device=device, default="cuda")
+        if dev.type != "cuda":
+            raise RuntimeError(f"atom3d_amd.MeshBVH runs on the GPU only; resolved device is {dev}")
+        self.vertices = vertices.to(dev).float()
+        self.faces = faces.to(dev).int()
+        self.device = dev
+        self.num_vertices = int(self.vertices.shape[0])
+        self.num_faces = int(self.faces.shape[0])
+        if self.num_vertices > 0:
+            self._bounds = torch.stack([self.vertices.min(dim=0)[0], self.vertices.max(dim=0)[0]])
+        else:
+            self._bounds = torch.zeros(2, 3, device=dev)
+        self._face_verts_flat = None
+        self._is_cuda = True
+        if face_normals is not None:
+            # The reference moves user normals with the unresolved `device`
+            # argument (mesh_bvh.py:101); the resolved device is used here.
+            self._face_normals = face_normals.to(dev).float().contiguous()
+        else:
+            self._face_normals = self._precompute_face_normals()
+        if vertex_normals is not None:
+            self._vertex_pn = vertex_normals.to(dev).float().contiguous()
+            self._edge_pn = None
+            self._he_edge_idx = None
+            self._face_angles = None
+        else:
+            self._precompute_pseudo_normal_data()
+        self._vertex_normals = self._vertex_pn
+        self._build_bvh()
+
+    # ------------------------------------------------------------------ setup
+    def _build_bvh(self) -> None:
+        self._bvh = BVHAccelerator(self.vertices, self.faces)
+
+    def _tri(self):
+        f = self.faces.long()
+        return self.vertices[f[:, 0]], self.vertices[f[:, 1]], self.vertices[f[:, 2]]
+
+    def _precompute_face_normals(self) -> torch.Tensor:
+        """Unit face normals cross(v1-v0, v2-v0) / (|.| + 1e-8) (mesh_bvh.py:129-139)."""
+        v0, v1, v2 = self._tri()
+        n = torch.cross(v1 - v0, v2 - v0, dim=1)
+        return (n / (n.norm(dim=1, keepdim=True) + 1e-8)).contiguous()
+
+    def _precompute_pseudo_normal_data(self) -> None:
+        """Angle-weighted vertex pseudo-normals and edge pseudo-normals
+        (mesh_bvh.py:141-232): vertex n = normalise(sum_faces angle * n_f), edge
+        n = normalise(sum of the adjacent unit face normals), each normalised
+        with +1e-12; half-edge 3f+k is edge (v_k, v_(k+1) mod 3) of face f."""
+        F = self.faces.long()
+        dev = self.vertices.device
+        v0, v1, v2 = self._tri()
+
+        def ang(a, b):
+            c = (a * b).sum(-1) / (a.norm(dim=-1) * b.norm(dim=-1) + 1e-12)
+            return torch.acos(c.clamp(-1.0, 1.0))
+
+        self._face_angles = torch.stack([ang(v1 - v0, v2 - v0), ang(v0 - v1, v2 - v1), ang(v0 - v2, v1 - v2)], dim=-1)
+        fn = torch.cross(v1 - v0, v2 - v0, dim=-1)
+        fn = fn / (fn.norm(dim=-1, keepdim=True) + 1e-12)
+        vn = torch.zeros_like(self.vertices)
+        for k in range(3):
+            vn.scatter_add_(0, F[:, k:k + 1].expand(-1, 3), self._face_angles[:, k:k + 1] * fn)
+        self._vertex_pn = (vn / (vn.norm(dim=-1, keepdim=True) + 1e-12)).contiguous()
+        he = torch.stack([F, F.roll(-1, dims=1)], dim=-1).reshape(-1, 2)  # [3F, 2]
+        key = torch.minimum(he[:, 0], he[:, 1]) * self.num_vertices + torch.maximum(he[:, 0], he[:, 1])
+        uniq, inv = torch.unique(key, return_inverse=True)
+        face_of_he = torch.arange(self.num_faces, device=dev).repeat_interleave(3)
+        epn = torch.zeros(uniq.shape[0], 3, device=dev)
+        epn.scatter_add_(0, inv.unsqueeze(-1).expand(-1, 3), fn[face_of_he])
+        self._edge_pn = (epn / (epn.norm(dim=-1, keepdim=True) + 1e-12)).contiguous()
+        self._he_edge_idx = inv.contiguous()
+
+    def _get_face_verts_flat(self) -> torch.Tensor:
+        if self._face_verts_flat is None:
+            self._face_verts_flat = self.vertices[self.faces.long()].reshape(-1, 9).contiguous()
+        return self._face_verts_flat
+
+    # ------------------------------------------------------------------ properties
+    def get_bounds(self) -> torch.Tensor:
+        return self._bounds
+
+    def get_face_normals(self) -> torch.Tensor:
+        return self._face_normals
+
+    def get_vertex_normals(self) -> Optional[torch.Tensor]:
+        return self._vertex_normals
+
+    def get_face_aabb(self, face_indices: Optional[torch.Tensor] = None) -> Tuple[torch.Tensor, torch.Tensor]:
+        f = self.faces if face_indices is None else self.faces[face_indices]
+        tv = self.vertices[f.long()]
+        return tv.min(dim=1)[0], tv.max(dim=1)[0]
+
+    # ------------------------------------------------------------------ boxes
+    def intersect_aabb(self, aabb_min: torch.Tensor, aabb_max: torch.Tensor, mode: int = 1) -> AABBIntersectResult:
+        """Triangle-box intersection.  mode 0: hit mask; 1: + (aabb_id, face_id)
+        pairs sorted by (aabb, face); 2/3: + clipped-polygon data."""
+        if mode not in (0, 1, 2, 3):
+            raise ValueError(f"mode must be 0..3, got {mode}")
+        if mode == 0:
+            return AABBIntersectResult(hit=self._bvh.aabb_hit(aabb_min, aabb_max))
+        if mode == 1:
+            hit, a, f = self._bvh.aabb_intersect(aabb_min, aabb_max)
+            return AABBIntersectResult(hit=hit, aabb_ids=a, face_ids=f)
+        return self._intersect_aabb_clip(aabb_min, aabb_max, mode)
+
+    def _intersect_aabb_clip(self, aabb_min, aabb_max, mode):
+        raise NotImplementedError("intersect_aabb modes 2/3 (polygon clip) are not built yet")
+
+    # ------------------------------------------------------------------ distances
+    def udf(self, points: torch.Tensor, return_grad: bool = False, return_closest: bool = False,
+            return_uvw: bool = False, return_face_ids: bool = False) -> Union[torch.Tensor, ClosestPointResult]:
+        """Unsigned distance; a bare tensor unless a return_* flag is set."""
+        res = self._udf_with_grad(points) if return_grad else self._udf_query(points)
+        return self._shape_result(res, return_closest, return_uvw, return_face_ids)
+
+    @staticmethod
+    def _shape_result(res: ClosestPointResult, rc: bool, ru: bool, rf: bool):
+        if not (rc or ru or rf):
+            return res.distances
+        if not rc:
+            res.closest_points = None
+        if not ru:
+            res.uvw = None
+        if not rf:
+            res.face_ids = None
+        return res
+
+    def _udf_query(self, points: torch.Tensor) -> ClosestPointResult:
+        d, f, c, w = self._bvh.udf(points)
+        return ClosestPointResult(distances=d, face_ids=f, closest_points=c, uvw=w)
+
+    def _udf_with_grad(self, points: torch.Tensor) -> ClosestPointResult:
+        with torch.no_grad():
+            r = self._udf_query(points)
+        d = _PointDistance.apply(points, r.closest_points)
+        return ClosestPointResult(distances=d, face_ids=r.face_ids, closest_points=r.closest_points, uvw=r.uvw)
+
+    def sdf(self, points: torch.Tensor, mode: str = "pseudo_normal", return_grad: bool = False,
+            return_closest: bool = False, return_uvw: bool = False, return_face_ids: bool = False,
+            **kwargs) -> Union[torch.Tensor, ClosestPointResult]:
+        """Signed distance with the pseudo-normal sign (negative inside)."""
+        if mode == "parity":
+            raise NotImplementedError("sdf(mode='parity') (ray-parity sign) is not built yet")
+        if mode != "pseudo_normal":
+            raise ValueError(f"unknown sdf mode {mode!r}")
+        res = self._udf_with_grad(points) if return_grad else self._udf_query(points)
+        signs = self._compute_sign(points, res.closest_points, res.face_ids, uvw=res.uvw)
+        res.distances = res.distances * signs
+        return self._shape_result(res, return_closest, return_uvw, return_face_ids)
+
+    def _compute_sign(self, points: torch.Tensor, closest_points: torch.Tensor, face_ids: torch.Tensor,
+                      uvw: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """+-1 per point from the pseudo-normal at the closest point (HIP kernel)."""
+        p = points.detach().contiguous().float()
+        K = p.shape[0]
+        signs = torch.ones(K, dtype=torch.float32, device=self.device)
+        if K == 0:
+            return signs
+        use_pn = uvw is not None and self._edge_pn is not None and self._he_edge_idx is not None
+        c = closest_points.detach().contiguous().float()
+        f = face_ids.contiguous().int()
+        w = uvw.detach().contiguous().float() if use_pn else None
+        with torch.cuda.device(self.device):
+            _lib.call(
+                "a3d_sdf_sign", _lib.ptr(p), K, _lib.ptr(c), _lib.ptr(f), _lib.ptr(w), _lib.ptr(self._face_normals),
+                _lib.ptr(self._vertex_pn) if use_pn else None, _lib.ptr(self._edge_pn) if use_pn else None,
+                _lib.ptr(self._he_edge_idx) if use_pn else None, _lib.ptr(self.faces) if use_pn else None,
+                _lib.ptr(signs), _lib.stream(self.device),
+            )
+        return signs

@@ -1,0 +1,251 @@
+// Shared device code for the atom3d_amd HIP library (gfx950).
+//
+// Geometry predicates follow the reference's FP32 operation order exactly
+// (Atom3D atom3d/kernels/bvh_kernels.cu:77-353) and the library is compiled
+// with -ffp-contract=off and IEEE division/sqrt, so every predicate rounds the
+// same way on the GPU as the CPU oracle does.  That is what lets closest-face
+// ids, SAT hit masks and octree voxel sets be compared bit for bit.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/atom3d_amd.h"
+
+namespace a3d {
+
+// ------------------------------------------------------------------ errors
+void set_error(const char* fmt, ...);
+int fail(int code, const char* fmt, ...);
+
+#define A3D_HIP_CHECK(expr)                                                     \
+  do {                                                                          \
+    hipError_t _e = (expr);                                                     \
+    if (_e != hipSuccess)                                                       \
+      return ::a3d::fail(A3D_ERR_HIP, "%s failed: %s (%s:%d)", #expr,           \
+                         hipGetErrorString(_e), __FILE__, __LINE__);            \
+  } while (0)
+
+#define A3D_LAUNCH_CHECK() A3D_HIP_CHECK(hipGetLastError())
+
+// ------------------------------------------------------------------ vectors
+struct v3 { float x, y, z; };
+__host__ __device__ __forceinline__ v3 mk(float x, float y, float z) { v3 r; r.x = x; r.y = y; r.z = z; return r; }
+__host__ __device__ __forceinline__ float dot3(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__host__ __device__ __forceinline__ v3 cross3(v3 a, v3 b) {
+  return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__host__ __device__ __forceinline__ v3 sub3(v3 a, v3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__host__ __device__ __forceinline__ v3 add3(v3 a, v3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__host__ __device__ __forceinline__ v3 mul3(v3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ float clampf(float x, float lo, float hi) { return fmaxf(lo, fminf(hi, x)); }
+__device__ __forceinline__ float sgn(float x) { return x >= 0.0f ? 1.0f : -1.0f; }
+
+// ------------------------------------------------------------------ layout
+// BVH blob (one device allocation, owned by the caller):
+//   [0, 256)            BvhHeader
+//   nodes               Node64[n_node_cap]   internal nodes, root = node 0
+//   tris                Tri48[n_tris]        triangles in leaf order
+// Node64 keeps BOTH children's boxes so one 64-byte fetch decides both
+// children (4 x 16-byte loads, no dependent second fetch).
+struct BvhHeader {
+  int32_t magic;
+  int32_t n_tris;
+  int32_t n_nodes;      // internal nodes actually used (written by the build)
+  int32_t n_node_cap;   // capacity reserved in the blob
+  int64_t nodes_off;    // byte offsets from the blob start
+  int64_t tris_off;
+  float scene_lo[3];    // exact min/max over all referenced vertices
+  float scene_hi[3];
+  float coord_max;      // max |coordinate| over referenced vertices
+  float tri_pad_max;    // max per-triangle distance error bound (see bvh_build)
+  int32_t leaf_max;
+  int32_t depth;        // max internal-node depth (root = 0), for stack sizing
+  int32_t pad[46];
+};
+static_assert(sizeof(BvhHeader) == 256, "header is 256 bytes");
+constexpr int32_t kBvhMagic = 0x41334442;  // 'A3DB'
+
+// Child reference: >= 0 internal node index; < 0 leaf: ~ref = start << 3 | (count-1)
+__host__ __device__ __forceinline__ int32_t leaf_ref(int32_t start, int32_t count) {
+  return ~((start << 3) | (count - 1));
+}
+__device__ __forceinline__ int32_t leaf_start(int32_t ref) { return (~ref) >> 3; }
+__device__ __forceinline__ int32_t leaf_count(int32_t ref) { return ((~ref) & 7) + 1; }
+
+struct Node64 {
+  float4 a;  // lmin.x lmin.y lmin.z lmax.x
+  float4 b;  // lmax.y lmax.z rmin.x rmin.y
+  float4 c;  // rmin.z rmax.x rmax.y rmax.z
+  int4 d;    // left ref, right ref, 0, 0
+};
+static_assert(sizeof(Node64) == 64, "node is 64 bytes");
+
+// Triangle in leaf order: a, b, c and the original face id (bit-cast).
+struct Tri48 {
+  float4 p;  // a.x a.y a.z b.x
+  float4 q;  // b.y b.z c.x c.y
+  float4 r;  // c.z id 0 0
+};
+static_assert(sizeof(Tri48) == 48, "triangle is 48 bytes");
+
+__device__ __forceinline__ void load_tri(const Tri48* __restrict__ t, int32_t i, v3& a, v3& b, v3& c,
+                                         int32_t& id) {
+  float4 p = t[i].p, q = t[i].q, r = t[i].r;
+  a = mk(p.x, p.y, p.z);
+  b = mk(p.w, q.x, q.y);
+  c = mk(q.z, q.w, r.x);
+  id = __float_as_int(r.y);
+}
+
+__host__ __device__ __forceinline__ const BvhHeader* header(const void* blob) {
+  return reinterpret_cast<const BvhHeader*>(blob);
+}
+
+// ------------------------------------------------------------------ predicates
+// Reference Triangle::distance_sq (bvh_kernels.cu:101-135).
+__device__ __forceinline__ float tri_dist2(v3 a, v3 b, v3 c, v3 pos) {
+  v3 v21 = sub3(b, a), p1 = sub3(pos, a);
+  v3 v32 = sub3(c, b), p2 = sub3(pos, b);
+  v3 v13 = sub3(a, c), p3 = sub3(pos, c);
+  v3 nor = cross3(v21, v13);
+  float nor_sq = dot3(nor, nor);
+  bool degen = nor_sq < 1e-12f;
+  float st = 0.0f;
+  if (!degen) {
+    st = sgn(dot3(cross3(v21, nor), p1)) + sgn(dot3(cross3(v32, nor), p2)) + sgn(dot3(cross3(v13, nor), p3));
+  }
+  if (degen || st < 2.0f) {
+    float d, e1, e2, e3;
+    v3 q;
+    d = clampf(dot3(v21, p1) / fmaxf(dot3(v21, v21), 1e-12f), 0.0f, 1.0f);
+    q = sub3(mul3(v21, d), p1); e1 = dot3(q, q);
+    d = clampf(dot3(v32, p2) / fmaxf(dot3(v32, v32), 1e-12f), 0.0f, 1.0f);
+    q = sub3(mul3(v32, d), p2); e2 = dot3(q, q);
+    d = clampf(dot3(v13, p3) / fmaxf(dot3(v13, v13), 1e-12f), 0.0f, 1.0f);
+    q = sub3(mul3(v13, d), p3); e3 = dot3(q, q);
+    return fminf(e1, fminf(e2, e3));
+  }
+  float d = dot3(nor, p1);
+  return d * d / fmaxf(nor_sq, 1e-12f);
+}
+
+// Reference Triangle::closest_point (bvh_kernels.cu:138-182).
+__device__ __forceinline__ v3 tri_closest(v3 a, v3 b, v3 c, v3 pos) {
+  v3 v21 = sub3(b, a), p1 = sub3(pos, a);
+  v3 v32 = sub3(c, b), p2 = sub3(pos, b);
+  v3 v13 = sub3(a, c), p3 = sub3(pos, c);
+  v3 nor = cross3(v21, v13);
+  float nor_sq = dot3(nor, nor);
+  bool degen = nor_sq < 1e-12f;
+  float st = 0.0f;
+  if (!degen) {
+    st = sgn(dot3(cross3(v21, nor), p1)) + sgn(dot3(cross3(v32, nor), p2)) + sgn(dot3(cross3(v13, nor), p3));
+  }
+  if (degen || st < 2.0f) {
+    float d;
+    d = clampf(dot3(v21, p1) / fmaxf(dot3(v21, v21), 1e-12f), 0.0f, 1.0f);
+    v3 c1 = add3(a, mul3(v21, d));
+    d = clampf(dot3(v32, p2) / fmaxf(dot3(v32, v32), 1e-12f), 0.0f, 1.0f);
+    v3 c2 = add3(b, mul3(v32, d));
+    d = clampf(dot3(v13, p3) / fmaxf(dot3(v13, v13), 1e-12f), 0.0f, 1.0f);
+    v3 c3 = add3(c, mul3(v13, d));
+    v3 q1 = sub3(c1, pos), q2 = sub3(c2, pos), q3 = sub3(c3, pos);
+    float d1 = dot3(q1, q1), d2 = dot3(q2, q2), d3 = dot3(q3, q3);
+    if (d1 < d2 && d1 < d3) return c1;
+    if (d2 < d3) return c2;
+    return c3;
+  }
+  float d = dot3(nor, p1);
+  v3 proj = mul3(nor, d / fmaxf(nor_sq, 1e-12f));
+  return sub3(pos, proj);
+}
+
+// Reference Triangle::barycentric (bvh_kernels.cu:205-222).
+__device__ __forceinline__ v3 tri_bary(v3 a, v3 b, v3 c, v3 p) {
+  v3 v0 = sub3(b, a), v1 = sub3(c, a), v2 = sub3(p, a);
+  float d00 = dot3(v0, v0), d01 = dot3(v0, v1), d11 = dot3(v1, v1);
+  float d20 = dot3(v2, v0), d21 = dot3(v2, v1);
+  float denom = d00 * d11 - d01 * d01;
+  float v = (d11 * d20 - d01 * d21) / fmaxf(denom, 1e-10f);
+  float w = (d00 * d21 - d01 * d20) / fmaxf(denom, 1e-10f);
+  float u = 1.0f - v - w;
+  return mk(u, v, w);
+}
+
+// Reference point_aabb_dist_sq (bvh_kernels.cu:248-253).
+__device__ __forceinline__ float box_d2(v3 p, float lx, float ly, float lz, float hx, float hy, float hz) {
+  float dx = fmaxf(fmaxf(lx - p.x, 0.0f), p.x - hx);
+  float dy = fmaxf(fmaxf(ly - p.y, 0.0f), p.y - hy);
+  float dz = fmaxf(fmaxf(lz - p.z, 0.0f), p.z - hz);
+  return dx * dx + dy * dy + dz * dz;
+}
+
+// Reference aabb_overlap (bvh_kernels.cu:241-245), inclusive.
+__device__ __forceinline__ bool box_overlap(float lx, float ly, float lz, float hx, float hy, float hz, v3 qlo,
+                                            v3 qhi) {
+  return (lx <= qhi.x && hx >= qlo.x) && (ly <= qhi.y && hy >= qlo.y) && (lz <= qhi.z && hz >= qlo.z);
+}
+
+// Reference triangle_aabb_sat (bvh_kernels.cu:285-353), eps 1e-6 on all 13 axes.
+__device__ __forceinline__ bool sat_axis(v3 v0, v3 v1, v3 v2, v3 h, v3 axis) {
+  const float eps = 1e-6f;
+  float p0 = dot3(v0, axis), p1 = dot3(v1, axis), p2 = dot3(v2, axis);
+  float mn = fminf(fminf(p0, p1), p2), mx = fmaxf(fmaxf(p0, p1), p2);
+  float rad = h.x * fabsf(axis.x) + h.y * fabsf(axis.y) + h.z * fabsf(axis.z) + eps;
+  return !(mn > rad || mx < -rad);
+}
+
+__device__ __forceinline__ bool tri_aabb_sat(v3 ta, v3 tb, v3 tc, v3 lo, v3 hi) {
+  const float eps = 1e-6f;
+  v3 ctr = mk((lo.x + hi.x) * 0.5f, (lo.y + hi.y) * 0.5f, (lo.z + hi.z) * 0.5f);
+  v3 h = mk((hi.x - lo.x) * 0.5f, (hi.y - lo.y) * 0.5f, (hi.z - lo.z) * 0.5f);
+  v3 v0 = sub3(ta, ctr), v1 = sub3(tb, ctr), v2 = sub3(tc, ctr);
+  float mn, mx;
+  mn = fminf(fminf(v0.x, v1.x), v2.x); mx = fmaxf(fmaxf(v0.x, v1.x), v2.x);
+  if (mn > h.x + eps || mx < -h.x - eps) return false;
+  mn = fminf(fminf(v0.y, v1.y), v2.y); mx = fmaxf(fmaxf(v0.y, v1.y), v2.y);
+  if (mn > h.y + eps || mx < -h.y - eps) return false;
+  mn = fminf(fminf(v0.z, v1.z), v2.z); mx = fmaxf(fmaxf(v0.z, v1.z), v2.z);
+  if (mn > h.z + eps || mx < -h.z - eps) return false;
+  v3 e0 = sub3(v1, v0), e1 = sub3(v2, v1), e2 = sub3(v0, v2);
+  if (!sat_axis(v0, v1, v2, h, mk(0.0f, e0.z, -e0.y))) return false;
+  if (!sat_axis(v0, v1, v2, h, mk(-e0.z, 0.0f, e0.x))) return false;
+  if (!sat_axis(v0, v1, v2, h, mk(e0.y, -e0.x, 0.0f))) return false;
+  if (!sat_axis(v0, v1, v2, h, mk(0.0f, e1.z, -e1.y))) return false;
+  if (!sat_axis(v0, v1, v2, h, mk(-e1.z, 0.0f, e1.x))) return false;
+  if (!sat_axis(v0, v1, v2, h, mk(e1.y, -e1.x, 0.0f))) return false;
+  if (!sat_axis(v0, v1, v2, h, mk(0.0f, e2.z, -e2.y))) return false;
+  if (!sat_axis(v0, v1, v2, h, mk(-e2.z, 0.0f, e2.x))) return false;
+  if (!sat_axis(v0, v1, v2, h, mk(e2.y, -e2.x, 0.0f))) return false;
+  v3 n = cross3(e0, sub3(v2, v0));
+  float d = -dot3(n, v0);
+  float r = h.x * fabsf(n.x) + h.y * fabsf(n.y) + h.z * fabsf(n.z) + eps;
+  if (fabsf(d) > r) return false;
+  return true;
+}
+
+// Tree-independent hit predicate: the triangle's own AABB overlaps the box
+// (inclusive) and the reference SAT passes.  With one triangle per leaf this
+// is exactly what the reference's BVH walk decides.
+__device__ __forceinline__ bool tri_box_hit(v3 a, v3 b, v3 c, v3 lo, v3 hi) {
+  float tlx = fminf(fminf(a.x, b.x), c.x), tly = fminf(fminf(a.y, b.y), c.y), tlz = fminf(fminf(a.z, b.z), c.z);
+  float thx = fmaxf(fmaxf(a.x, b.x), c.x), thy = fmaxf(fmaxf(a.y, b.y), c.y), thz = fmaxf(fmaxf(a.z, b.z), c.z);
+  if (!box_overlap(tlx, tly, tlz, thx, thy, thz, lo, hi)) return false;
+  return tri_aabb_sat(a, b, c, lo, hi);
+}
+
+// ------------------------------------------------------------------ launch helpers
+// Blocks are handed to the 8 XCDs round-robin; remap so each XCD gets a
+// contiguous run of logical blocks (neighbouring queries share its L2).
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
+  const uint32_t nx = 8;
+  uint32_t per = nb / nx, rem = nb % nx;
+  uint32_t x = b % nx, k = b / nx;
+  // XCD x owns logical blocks [x*per + min(x,rem), ...)
+  uint32_t base = x * per + (x < rem ? x : rem);
+  return base + k;
+}
+
+inline int64_t div_up(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace a3d

@@ -1,0 +1,94 @@
+"""Deterministic synthetic workloads (meshes, query points, SDF grids).
+
+All generators draw from ``numpy.random.default_rng`` with fixed seeds in
+float64 and round once to float32, so the same arrays come out on every
+machine.  They build the benchmark configurations:
+
+* ``noisy_icosphere``: icosphere subdivided 5 times (10,242 vertices, 20,480
+  faces), radius 0.5, each vertex moved radially by N(0, 0.01), seed 0;
+* ``sphere_union``: 200 such icospheres (4,096,000 faces), centres uniform in
+  [-0.8, 0.8]^3, radii log-uniform in [0.02, 0.2], uniformly random rotations,
+  seed 1;
+* ``uniform_points``: query points uniform in a box.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+_T = (1.0 + 5.0 ** 0.5) / 2.0
+_ICO_V = np.array([
+    [-1, _T, 0], [1, _T, 0], [-1, -_T, 0], [1, -_T, 0],
+    [0, -1, _T], [0, 1, _T], [0, -1, -_T], [0, 1, -_T],
+    [_T, 0, -1], [_T, 0, 1], [-_T, 0, -1], [-_T, 0, 1],
+], dtype=np.float64)
+_ICO_F = np.array([
+    [0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11],
+    [1, 5, 9], [5, 11, 4], [11, 10, 2], [10, 7, 6], [7, 1, 8],
+    [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9],
+    [4, 9, 5], [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1],
+], dtype=np.int64)
+
+
+def icosphere(subdivisions: int = 5, radius: float = 1.0):
+    """(vertices float64 [V,3], faces int64 [F,3]) of a geodesic sphere: the
+    golden-ratio icosahedron, each triangle split in 4 through edge midpoints
+    ``subdivisions`` times, every vertex projected to the sphere.  Faces are
+    counter-clockwise seen from outside."""
+    v = _ICO_V / np.linalg.norm(_ICO_V, axis=1, keepdims=True)
+    f = _ICO_F.copy()
+    for _ in range(subdivisions):
+        e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]], axis=0)
+        lo, hi = np.minimum(e[:, 0], e[:, 1]), np.maximum(e[:, 0], e[:, 1])
+        key = lo * len(v) + hi
+        uniq, inv = np.unique(key, return_inverse=True)
+        mids = (v[uniq // len(v)] + v[uniq % len(v)]) * 0.5
+        mids /= np.linalg.norm(mids, axis=1, keepdims=True)
+        m = inv.reshape(3, -1).T + len(v)  # midpoint of edges 01, 12, 20 per face
+        v = np.concatenate([v, mids], axis=0)
+        a, b, c = f[:, 0], f[:, 1], f[:, 2]
+        m01, m12, m20 = m[:, 0], m[:, 1], m[:, 2]
+        f = np.concatenate([
+            np.stack([a, m01, m20], 1), np.stack([b, m12, m01], 1),
+            np.stack([c, m20, m12], 1), np.stack([m01, m12, m20], 1),
+        ], axis=0)
+    return v * radius, f
+
+
+def noisy_icosphere(subdivisions: int = 5, radius: float = 0.5, sigma: float = 0.01, seed: int = 0):
+    """Icosphere with each vertex displaced radially by N(0, sigma)."""
+    v, f = icosphere(subdivisions, 1.0)
+    rng = np.random.default_rng(seed)
+    r = radius + rng.normal(0.0, sigma, size=len(v))
+    return torch.from_numpy((v * r[:, None]).astype(np.float32)), torch.from_numpy(f.astype(np.int32))
+
+
+def _random_rotations(rng, n):
+    # Uniform rotations from normalised Gaussian quaternions.
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    return np.stack([
+        np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)], 1),
+        np.stack([2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)], 1),
+        np.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], 1),
+    ], 1)
+
+
+def sphere_union(n_spheres: int = 200, subdivisions: int = 5, seed: int = 1, r_min: float = 0.02,
+                 r_max: float = 0.2, extent: float = 0.8):
+    """Concatenation of ``n_spheres`` rotated, scaled, translated icospheres."""
+    v0, f0 = icosphere(subdivisions, 1.0)
+    rng = np.random.default_rng(seed)
+    centres = rng.uniform(-extent, extent, size=(n_spheres, 3))
+    radii = np.exp(rng.uniform(np.log(r_min), np.log(r_max), size=n_spheres))
+    rots = _random_rotations(rng, n_spheres)
+    vs = np.einsum("sij,vj->svi", rots, v0) * radii[:, None, None] + centres[:, None, :]
+    fs = f0[None, :, :] + (np.arange(n_spheres) * len(v0))[:, None, None]
+    return (torch.from_numpy(vs.reshape(-1, 3).astype(np.float32)),
+            torch.from_numpy(fs.reshape(-1, 3).astype(np.int32)))
+
+
+def uniform_points(n: int, lo: float = -1.0, hi: float = 1.0, seed: int = 0) -> torch.Tensor:
+    rng = np.random.default_rng([seed, 1])
+    return torch.from_numpy(rng.uniform(lo, hi, size=(n, 3)).astype(np.float32))

@@ -1,0 +1,108 @@
+"""ctypes binding of the gfx950 HIP library ``libatom3d_amd.so``.
+
+Replaces the reference's JIT-built pybind11 modules (Atom3D
+atom3d/kernels/bvh.py:21-51, kernels/__init__.py:15-83,
+kernels/flood_fill.py:20-62).  The library is built ahead of time in-tree
+(``python -m atom3d_amd.build``); there is no JIT step and no fallback: if the
+library is missing or a call fails, a RuntimeError says why.
+
+Calls go through ``ctypes.CDLL`` (the GIL is released during each call) and
+pass raw device pointers plus the caller's current HIP stream, so every kernel
+is ordered with the surrounding PyTorch work.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import torch
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG, "_lib", "libatom3d_amd.so")
+
+_lib = None
+_lock = threading.Lock()
+
+_vp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_i32 = ctypes.c_int
+_f32 = ctypes.c_float
+_pi64 = ctypes.POINTER(ctypes.c_int64)
+
+# name -> argtypes.  Every entry point returns int (0 = OK).
+SIGNATURES: dict[str, list] = {
+    "a3d_bvh_sizes": [_i64, _i32, _pi64, _pi64],
+    "a3d_bvh_build": [_vp, _i64, _vp, _i64, _i32, _vp, _i64, _vp, _i64, _vp],
+    "a3d_bvh_info": [_vp, _pi64, _vp],
+    "a3d_bvh_udf": [_vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp],
+    "a3d_point_dist": [_vp, _vp, _i64, _vp, _vp],
+    "a3d_point_dist_backward": [_vp, _vp, _vp, _vp, _i64, _vp, _vp],
+    "a3d_sdf_sign": [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "a3d_bvh_aabb_hit": [_vp, _vp, _vp, _i64, _vp, _vp],
+    "a3d_bvh_aabb_count": [_vp, _vp, _vp, _i64, _vp, _vp],
+    "a3d_bvh_aabb_pairs": [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp],
+    "a3d_pairs_sort_ws_bytes": [_i64, _pi64],
+    "a3d_pairs_sort": [_vp, _vp, _i64, _vp, _i64, _vp],
+    "a3d_octree_ws_bytes": [_i64, _pi64],
+    "a3d_octree_level": [_vp, ctypes.POINTER(ctypes.c_float), _i32, _vp, _i64, _vp, _vp, _vp, _i64, _vp],
+    "a3d_octree_unpack": [_vp, _i64, _vp, _vp],
+}
+
+
+def _load():
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"atom3d_amd: HIP library not found at {LIB_PATH}; build it with `python -m atom3d_amd.build`"
+            )
+        # torch first: the library then binds to the HIP runtime torch already
+        # loaded (same soname), so pointers and streams are shared.
+        torch.cuda.is_available  # noqa: B018  (import side effect only)
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, argtypes in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.argtypes = argtypes
+            fn.restype = ctypes.c_int
+        lib.a3d_last_error.restype = ctypes.c_char_p
+        lib.a3d_last_error.argtypes = []
+        lib.a3d_version.restype = ctypes.c_char_p
+        lib.a3d_version.argtypes = []
+        _lib = lib
+        return lib
+
+
+def lib():
+    return _lib if _lib is not None else _load()
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = lib().a3d_last_error().decode(errors="replace")
+        raise RuntimeError(f"atom3d_amd {what} failed (code {rc}): {msg}")
+
+
+def call(name: str, *args) -> None:
+    check(getattr(lib(), name)(*args), name)
+
+
+def ptr(t: torch.Tensor | None) -> int | None:
+    if t is None:
+        return None
+    return t.data_ptr()
+
+
+def stream(device: torch.device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def require_cuda(t: torch.Tensor, name: str) -> None:
+    if not t.is_cuda:
+        raise RuntimeError(f"atom3d_amd: {name} must be a GPU tensor (got device {t.device})")
+
+
+def version() -> str:
+    return lib().a3d_version().decode()

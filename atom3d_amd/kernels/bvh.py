@@ -1,0 +1,172 @@
+"""BVHAccelerator: device BVH build and queries over the HIP library.
+
+Mirrors Atom3D atom3d/kernels/bvh.py:54-207 (same constructor arguments, same
+method names and return tuples).  Differences, all deliberate:
+
+* the tree is built on the GPU (Morton LBVH) instead of on the host, and is
+  held as one opaque byte blob (``self.blob``) instead of ``nodes``/``triangles``
+  tensors;
+* the closest face is tree independent (minimum FP32 squared distance, ties to
+  the lowest face index) and ``uvw`` is zero where no face was found (the
+  reference leaves it uninitialised);
+* ``aabb_intersect`` never truncates: it counts pairs first, then writes them
+  sorted by (box, face) (the reference appends with an atomic counter, capped
+  at min(100*K, 5e7) pairs, in nondeterministic order).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _lib
+
+
+class BVHAccelerator:
+    def __init__(self, vertices: torch.Tensor, faces: torch.Tensor, n_primitives_per_leaf: int = 4):
+        _lib.require_cuda(vertices, "vertices")
+        self.device = vertices.device
+        self.vertices = vertices.contiguous().float()
+        self.faces = faces.to(self.device).contiguous().int()
+        self.num_faces = int(self.faces.shape[0])
+        self.leaf_max = int(n_primitives_per_leaf)
+        if self.num_faces > 0:
+            fmin, fmax = int(self.faces.min()), int(self.faces.max())
+            if fmin < 0 or fmax >= self.vertices.shape[0]:
+                raise ValueError(
+                    f"faces index out of range [0, {self.vertices.shape[0]}): min {fmin}, max {fmax}"
+                )
+        blob_b, ws_b = ctypes.c_int64(), ctypes.c_int64()
+        _lib.call("a3d_bvh_sizes", self.num_faces, self.leaf_max, ctypes.byref(blob_b), ctypes.byref(ws_b))
+        with torch.cuda.device(self.device):
+            self.blob = torch.empty(blob_b.value, dtype=torch.uint8, device=self.device)
+            ws = torch.empty(ws_b.value, dtype=torch.uint8, device=self.device)
+            _lib.call(
+                "a3d_bvh_build",
+                _lib.ptr(self.vertices), self.vertices.shape[0], _lib.ptr(self.faces), self.num_faces,
+                self.leaf_max, _lib.ptr(self.blob), blob_b.value, _lib.ptr(ws), ws_b.value,
+                _lib.stream(self.device),
+            )
+            del ws
+
+    def info(self) -> dict:
+        out = (ctypes.c_int64 * 4)()
+        with torch.cuda.device(self.device):
+            _lib.call("a3d_bvh_info", _lib.ptr(self.blob), out, _lib.stream(self.device))
+        return {"n_tris": out[0], "n_nodes": out[1], "depth": out[2], "leaf_max": out[3]}
+
+    def check_tree(self) -> dict:
+        """Copy the blob to the host and verify its structure: every internal
+        reference points at a written node, leaves partition the triangles,
+        each child box contains its triangles.  Raises AssertionError."""
+        import numpy as np
+
+        info = self.info()
+        raw = self.blob.cpu().numpy()
+        hdr = raw[:256].view(np.int32)
+        n_tris, n_nodes, cap = int(hdr[1]), int(hdr[2]), int(hdr[3])
+        assert hdr[0] == 0x41334442 and n_tris == self.num_faces, "bad header"
+        nodes_off, tris_off = raw[16:24].view(np.int64)[0], raw[24:32].view(np.int64)[0]
+        nodes = raw[nodes_off:nodes_off + 64 * cap].view(np.float32).reshape(cap, 16)[:n_nodes]
+        refs = nodes[:, 12:14].view(np.int32)
+        tris = raw[tris_off:tris_off + 48 * n_tris].view(np.float32).reshape(n_tris, 12)
+        ids = tris[:, 10].view(np.int32)
+        assert 1 <= n_nodes <= cap
+        if n_tris:
+            assert np.array_equal(np.sort(ids), np.arange(n_tris)), "triangle ids are not a permutation"
+        covered = np.zeros(max(n_tris, 1), np.int32)
+        seen = np.zeros(n_nodes, np.int32)
+        seen[0] = 1
+        tv = tris[:, :9].reshape(-1, 3, 3) if n_tris else None
+        for side in range(2):
+            r = refs[:, side]
+            box = nodes[:, side * 6:side * 6 + 6]
+            internal = r >= 0
+            assert np.all(r[internal] < n_nodes) and np.all(r[internal] > 0), "internal ref out of range"
+            np.add.at(seen, r[internal], 1)
+            if n_tris == 0:
+                continue
+            leaf = (~internal) & (box[:, 0] <= box[:, 3])  # empty children carry inverted boxes
+            enc = ~r[leaf]
+            start, cnt = enc >> 3, (enc & 7) + 1
+            assert np.all(start + cnt <= n_tris), "leaf range out of range"
+            tri_idx = np.repeat(start, cnt) + (np.arange(cnt.sum()) - np.repeat(np.cumsum(cnt) - cnt, cnt))
+            np.add.at(covered, tri_idx, 1)
+            owner = np.repeat(np.nonzero(leaf)[0], cnt)
+            t = tv[tri_idx]
+            assert np.all(t.min(1) >= box[owner, :3]) and np.all(t.max(1) <= box[owner, 3:]), "leaf box"
+        assert np.all(seen == 1), "every node is referenced exactly once (root by the walk)"
+        if n_tris:
+            assert np.all(covered[:n_tris] == 1), "leaves do not partition the triangles"
+        return info
+
+    def _points(self, points: torch.Tensor) -> torch.Tensor:
+        _lib.require_cuda(points, "points")
+        if points.device != self.device:
+            raise RuntimeError(f"atom3d_amd: points on {points.device}, BVH on {self.device}")
+        if points.dim() != 2 or points.shape[1] != 3:
+            raise ValueError(f"points must be [K, 3], got {tuple(points.shape)}")
+        return points.contiguous().float()
+
+    def udf(self, points: torch.Tensor):
+        """Closest point on the mesh: (distances [K], face_ids [K] int32,
+        closest_points [K,3], uvw [K,3])."""
+        p = self._points(points)
+        K = p.shape[0]
+        dist = torch.empty(K, dtype=torch.float32, device=self.device)
+        face = torch.empty(K, dtype=torch.int32, device=self.device)
+        closest = torch.empty(K, 3, dtype=torch.float32, device=self.device)
+        uvw = torch.empty(K, 3, dtype=torch.float32, device=self.device)
+        with torch.cuda.device(self.device):
+            _lib.call(
+                "a3d_bvh_udf", _lib.ptr(self.blob), _lib.ptr(p), K, _lib.ptr(dist), _lib.ptr(face),
+                _lib.ptr(closest), _lib.ptr(uvw), _lib.stream(self.device),
+            )
+        return dist, face, closest, uvw
+
+    def aabb_hit(self, query_min: torch.Tensor, query_max: torch.Tensor) -> torch.Tensor:
+        """Hit mask only (stops at the first intersecting face per box)."""
+        lo, hi = self._points(query_min), self._points(query_max)
+        K = lo.shape[0]
+        hit = torch.empty(K, dtype=torch.uint8, device=self.device)
+        with torch.cuda.device(self.device):
+            _lib.call("a3d_bvh_aabb_hit", _lib.ptr(self.blob), _lib.ptr(lo), _lib.ptr(hi), K, _lib.ptr(hit),
+                      _lib.stream(self.device))
+        return hit.view(torch.bool)
+
+    def aabb_intersect(self, query_min: torch.Tensor, query_max: torch.Tensor):
+        """(hit_mask [K] bool, aabb_ids [P] int32, face_ids [P] int32), pairs
+        sorted by (aabb_id, face_id)."""
+        lo, hi = self._points(query_min), self._points(query_max)
+        K = lo.shape[0]
+        dev = self.device
+        with torch.cuda.device(dev):
+            counts = torch.empty(K, dtype=torch.int32, device=dev)
+            s = _lib.stream(dev)
+            _lib.call("a3d_bvh_aabb_count", _lib.ptr(self.blob), _lib.ptr(lo), _lib.ptr(hi), K, _lib.ptr(counts), s)
+            hit = counts > 0
+            if K == 0:
+                e = torch.empty(0, dtype=torch.int32, device=dev)
+                return hit, e, e.clone()
+            incl = torch.cumsum(counts, 0, dtype=torch.int64)
+            total = int(incl[-1])
+            offsets = incl - counts
+            pa = torch.empty(total, dtype=torch.int32, device=dev)
+            pf = torch.empty(total, dtype=torch.int32, device=dev)
+            if total > 0:
+                _lib.call("a3d_bvh_aabb_pairs", _lib.ptr(self.blob), _lib.ptr(lo), _lib.ptr(hi), K,
+                          _lib.ptr(offsets), _lib.ptr(pa), _lib.ptr(pf), s)
+                wsb = ctypes.c_int64()
+                _lib.call("a3d_pairs_sort_ws_bytes", total, ctypes.byref(wsb))
+                ws = torch.empty(wsb.value, dtype=torch.uint8, device=dev)
+                _lib.call("a3d_pairs_sort", _lib.ptr(pa), _lib.ptr(pf), total, _lib.ptr(ws), wsb.value, s)
+        return hit, pa, pf
+
+
+def bvh_available() -> bool:
+    """True when the HIP library loads and a GPU is visible."""
+    try:
+        _lib.lib()
+    except Exception:
+        return False
+    return torch.cuda.is_available()

@@ -1,0 +1,143 @@
+/*
+ * atom3d_amd — C ABI of the MI355X (gfx950) geometry library.
+ *
+ * Drop-in boundary for the hot path of Atom3D (Luo-Yihao/Atom3d).  The
+ * reference binds three JIT-built pybind11 modules (bvh_cuda, cumtv_cuda,
+ * floodfill_cuda); each entry point below replaces one of their functions,
+ * cited as file:line into the reference.  The Python package atom3d_amd binds
+ * these symbols with ctypes.
+ *
+ * Conventions (all entry points):
+ *   - return 0 on success, a negative A3D_ERR_* code on failure; the message
+ *     is kept per thread and read with a3d_last_error();
+ *   - every pointer argument is a DEVICE pointer unless stated otherwise;
+ *     sizes are int64_t element counts;
+ *   - work is enqueued asynchronously on the given HIP stream (the caller's
+ *     current stream); nothing is allocated by the library: outputs, the BVH
+ *     blob and workspaces are buffers the caller owns and sizes with the
+ *     *_bytes queries;
+ *   - no function synchronises the device except where documented.
+ */
+#ifndef ATOM3D_AMD_H
+#define ATOM3D_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* a3d_stream_t; /* == hipStream_t */
+
+enum {
+  A3D_OK = 0,
+  A3D_ERR_ARG = -1,      /* invalid argument (null pointer, bad size, bad mode) */
+  A3D_ERR_HIP = -2,      /* a HIP runtime call failed */
+  A3D_ERR_CAPACITY = -3, /* caller buffer too small */
+  A3D_ERR_FORMAT = -4    /* blob is not a BVH built by this library */
+};
+
+/* Last error message of the calling thread ("" if none). */
+const char* a3d_last_error(void);
+/* Library version string, e.g. "atom3d_amd 0.1.0 gfx950". */
+const char* a3d_version(void);
+
+/* ------------------------------------------------------------------ BVH build
+ * Replaces build_bvh (bvh_kernels.cu:860-925, bound at :1245) and
+ * BVHAccelerator.__init__ (kernels/bvh.py:66-93).  The reference builds a
+ * binned-SAH tree on the host; this builds a Morton LBVH on the device
+ * (63-bit Morton keys of face centroids, Karras 2012) with leaves of up to leaf_max <= 8
+ * triangles.  The query results of every entry point below do not depend on
+ * the tree.
+ *
+ * a3d_bvh_sizes: bytes of the BVH blob and of the build workspace for nF faces.
+ * a3d_bvh_build: V float32[nV*3], F int32[nF*3] (device) -> blob.
+ */
+int a3d_bvh_sizes(int64_t nF, int leaf_max, int64_t* blob_bytes, int64_t* ws_bytes);
+int a3d_bvh_build(const float* V, int64_t nV, const int32_t* F, int64_t nF, int leaf_max,
+                  void* blob, int64_t blob_bytes, void* ws, int64_t ws_bytes, a3d_stream_t stream);
+/* Copies the blob header fields to HOST memory (synchronises the stream):
+ * out[0]=n_tris, out[1]=n_nodes, out[2]=depth, out[3]=leaf_max. */
+int a3d_bvh_info(const void* blob, int64_t* out4, a3d_stream_t stream);
+
+/* ------------------------------------------------------------------ UDF
+ * Replaces bvh_udf (bvh_kernels.cu:1028-1066 / bvh_udf_kernel :363-463, bound
+ * at :1246) and BVHAccelerator.udf (kernels/bvh.py:95-117).
+ * P float32[K*3] -> dist float32[K], face int32[K] (original face index, -1
+ * if none), closest float32[K*3], uvw float32[K*3] (nullable; zeros when
+ * face == -1).  Closest face = minimum of the reference's FP32 squared
+ * distance over ALL faces, ties to the lowest face index.
+ */
+int a3d_bvh_udf(const void* blob, const float* P, int64_t K, float* dist, int32_t* face,
+                float* closest, float* uvw, a3d_stream_t stream);
+
+/* ------------------------------------------------------------------ UDF adjoint
+ * Replaces the autograd re-evaluation in MeshBVH._udf_with_grad
+ * (core/mesh_bvh.py:685-700): dist = |P - C| per row with C constant;
+ * backward grad_P = grad * (P - C) / dist, 0 where dist == 0.
+ */
+int a3d_point_dist(const float* P, const float* C, int64_t K, float* dist, a3d_stream_t stream);
+int a3d_point_dist_backward(const float* P, const float* C, const float* dist, const float* grad,
+                            int64_t K, float* grad_P, a3d_stream_t stream);
+
+/* ------------------------------------------------------------------ SDF sign
+ * Replaces MeshBVH._compute_sign (core/mesh_bvh.py:759-827): Baerentzen-
+ * Aanaes pseudo-normal chosen by uvw (< 1e-4 per coordinate), sign of
+ * dot(p - closest, n), 0 -> +1, face -1 -> +1.  Multiplies dist in place.
+ * face_n float32[nF*3]; vertex_pn float32[nV*3], edge_pn float32[nE*3] and
+ * he_edge int64[3*nF] are nullable (then the face normal is used).
+ * F int32[nF*3] is needed only with vertex_pn.
+ */
+int a3d_sdf_sign(const float* P, int64_t K, const float* closest, const int32_t* face,
+                 const float* uvw, const float* face_n, const float* vertex_pn,
+                 const float* edge_pn, const int64_t* he_edge, const int32_t* F, float* dist,
+                 a3d_stream_t stream);
+
+/* ------------------------------------------------------------------ AABB / SAT
+ * Replaces bvh_aabb_intersect (bvh_kernels.cu:1192-1242 /
+ * bvh_aabb_intersect_kernel :555-610, bound at :1249).  A (box, face) pair
+ * hits when the face's AABB overlaps the box (inclusive) and the reference
+ * SAT (eps 1e-6) passes.
+ * a3d_bvh_aabb_hit:   hit uint8[K] only, first-hit early exit (mode 0).
+ * a3d_bvh_aabb_count: counts int32[K] = pairs per box.
+ * a3d_bvh_aabb_pairs: writes the pairs of box q at offsets[q] (int64[K],
+ *                     exclusive prefix sum of counts), in traversal order
+ *                     within each box; pair_a int32, pair_f int32.
+ * a3d_pairs_sort:     sorts n pairs by (box, face) in place (the reference's
+ *                     atomic append order is nondeterministic; this gives the
+ *                     canonical order); workspace from a3d_pairs_sort_ws_bytes.
+ */
+int a3d_bvh_aabb_hit(const void* blob, const float* qmin, const float* qmax, int64_t K,
+                     uint8_t* hit, a3d_stream_t stream);
+int a3d_bvh_aabb_count(const void* blob, const float* qmin, const float* qmax, int64_t K,
+                       int32_t* counts, a3d_stream_t stream);
+int a3d_bvh_aabb_pairs(const void* blob, const float* qmin, const float* qmax, int64_t K,
+                       const int64_t* offsets, int32_t* pair_a, int32_t* pair_f,
+                       a3d_stream_t stream);
+int a3d_pairs_sort_ws_bytes(int64_t n, int64_t* bytes);
+int a3d_pairs_sort(int32_t* pair_a, int32_t* pair_f, int64_t n, void* ws, int64_t ws_bytes,
+                   a3d_stream_t stream);
+
+/* ------------------------------------------------------------------ octree
+ * Replaces the level loop of OctreeIndexer.octree_traverse
+ * (grid/octree_indexer.py:470-509) and its per-level boxes
+ * (cube_aabb_level :302-337, subdivide :191-216).  Cubes are packed uint64
+ * keys i<<42 | j<<21 | k (level <= 21).
+ * a3d_octree_ws_bytes: workspace for up to max_cubes candidate cubes.
+ * a3d_octree_level: candidates are all res^3 cubes of `level` in meshgrid('ij')
+ *   order when parents == NULL, else the 8 children (CUBE_CORNERS order) of
+ *   each parent key (level-1).  Writes the hit candidates, order preserved,
+ *   to kept (capacity: the candidate count) and their number to n_kept
+ *   (device int64).  bounds = host float[6] (b0 xyz, b1 xyz).
+ * a3d_octree_unpack: keys -> int64[n*3] ijk.
+ */
+int a3d_octree_ws_bytes(int64_t max_cubes, int64_t* bytes);
+int a3d_octree_level(const void* blob, const float* bounds_host, int level, const uint64_t* parents,
+                     int64_t n_parents, uint64_t* kept, int64_t* n_kept, void* ws, int64_t ws_bytes,
+                     a3d_stream_t stream);
+int a3d_octree_unpack(const uint64_t* keys, int64_t n, int64_t* ijk, a3d_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ATOM3D_AMD_H */

@@ -1,0 +1,180 @@
+"""CPU oracle (test infrastructure only).
+
+ctypes binding of ``liba3d_oracle.so`` — the C++ restatement of the
+reference's algorithms in this folder.  Imported only by tests/, by
+__graft_entry__.smoke() and by bench.py's cpu_baseline leg, as the checker;
+the product (atom3d_amd) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_DIR, "liba3d_oracle.so")
+_lib = None
+
+_f = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_d = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_i = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_l = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_u8 = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+_i64 = ctypes.c_int64
+_int = ctypes.c_int
+_vp = ctypes.c_void_p
+
+_SIGS = {
+    "orc_num_threads": (_int, []),
+    "orc_tri_query": (None, [_f, _f, _f, _f, _f]),
+    "orc_tri_aabb_sat": (_int, [_f, _f, _f]),
+    "orc_sah_build": (_i64, [_f, _i64, _i, _i64, _int, _f, _f]),
+    "orc_bvh_udf": (None, [_f, _f, _f, _i64, _f, _i, _f, _f, _vp, _vp, _int]),
+    "orc_brute_udf": (None, [_f, _i, _i64, _f, _i64, _f, _i, _f, _f, _int]),
+    "orc_fp64_udf": (None, [_f, _i, _i64, _f, _i64, _d, _d, _int]),
+    "orc_tri_closest_fp64": (None, [_d, _d, _d]),
+    "orc_bvh_aabb": (_i64, [_f, _f, _f, _f, _i64, _u8, _i, _i, _i64, _int]),
+    "orc_brute_aabb_hit": (None, [_f, _i, _i64, _f, _f, _i64, _u8, _vp, _int]),
+    "orc_octree_traverse": (_i64, [_f, _f, _f, _int, _int, _l, _i64, _vp, _vp, _int]),
+}
+
+
+def build() -> str:
+    subprocess.run(["make", "-C", _DIR, "-j8"], check=True, capture_output=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _f32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+
+
+def _i32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int32))
+
+
+def num_threads() -> int:
+    return lib().orc_num_threads()
+
+
+def tri_query(tri9, p):
+    """(d2, closest[3], uvw[3]) of the reference Triangle maths for one pair."""
+    d2 = np.zeros(1, np.float32)
+    c = np.zeros(3, np.float32)
+    w = np.zeros(3, np.float32)
+    lib().orc_tri_query(_f32(tri9).reshape(9), _f32(p).reshape(3), d2, c, w)
+    return float(d2[0]), c, w
+
+
+def tri_aabb_sat(tri9, bmin, bmax) -> bool:
+    return bool(lib().orc_tri_aabb_sat(_f32(tri9).reshape(9), _f32(bmin).reshape(3), _f32(bmax).reshape(3)))
+
+
+class SahBvh:
+    """The reference's host SAH tree (nodes [Nn, 9] f32, tris [F, 10] f32)."""
+
+    def __init__(self, V, F, leaf: int = 4):
+        V, F = _f32(V), _i32(F)
+        nF = F.shape[0]
+        nodes = np.zeros((2 * max(nF, 1) + 1, 9), np.float32)
+        tris = np.zeros((max(nF, 1), 10), np.float32)
+        nn = lib().orc_sah_build(V.reshape(-1), V.shape[0], F.reshape(-1), nF, leaf, nodes.reshape(-1), tris.reshape(-1))
+        self.nodes = np.ascontiguousarray(nodes[:nn])
+        self.tris = tris
+        self.num_faces = nF
+
+    def udf(self, P, threads: int = 0, counts: bool = False):
+        P = _f32(P).reshape(-1, 3)
+        K = P.shape[0]
+        d = np.zeros(K, np.float32)
+        f = np.zeros(K, np.int32)
+        c = np.zeros((K, 3), np.float32)
+        w = np.zeros((K, 3), np.float32)
+        nv = np.zeros(K, np.int32) if counts else None
+        nt = np.zeros(K, np.int32) if counts else None
+        lib().orc_bvh_udf(self.nodes.reshape(-1), self.tris.reshape(-1), P.reshape(-1), K, d, f, c.reshape(-1),
+                          w.reshape(-1), nv.ctypes.data if counts else None, nt.ctypes.data if counts else None,
+                          threads)
+        return (d, f, c, w, nv, nt) if counts else (d, f, c, w)
+
+    def aabb(self, qmin, qmax, canonical: bool = True, cap: int | None = None):
+        qmin, qmax = _f32(qmin).reshape(-1, 3), _f32(qmax).reshape(-1, 3)
+        K = qmin.shape[0]
+        hit = np.zeros(K, np.uint8)
+        cap = cap if cap is not None else max(64 * K, 1024)
+        pa = np.zeros(cap, np.int32)
+        pf = np.zeros(cap, np.int32)
+        n = lib().orc_bvh_aabb(self.nodes.reshape(-1), self.tris.reshape(-1), qmin.reshape(-1), qmax.reshape(-1), K,
+                               hit, pa, pf, cap, int(canonical))
+        if n > cap:
+            return self.aabb(qmin, qmax, canonical, cap=n)
+        return hit.astype(bool), pa[:n], pf[:n]
+
+    def octree(self, bounds, min_level: int, max_level: int, threads: int = 0):
+        b = _f32(bounds).reshape(6)
+        cap = 1 << 20
+        tested = np.zeros(max_level + 2, np.int64)
+        hits = np.zeros(max_level + 2, np.int64)
+        while True:
+            out = np.zeros((cap, 3), np.int64)
+            n = lib().orc_octree_traverse(self.nodes.reshape(-1), self.tris.reshape(-1), b, min_level, max_level,
+                                          out.reshape(-1), cap, tested.ctypes.data, hits.ctypes.data, threads)
+            if n <= cap:
+                return out[:n], tested, hits
+            cap = int(n)
+
+
+def brute_udf(V, F, P, threads: int = 0):
+    """Tree-independent closest face (ties to the lowest face index)."""
+    V, F, P = _f32(V), _i32(F), _f32(P).reshape(-1, 3)
+    K = P.shape[0]
+    d = np.zeros(K, np.float32)
+    f = np.zeros(K, np.int32)
+    c = np.zeros((K, 3), np.float32)
+    w = np.zeros((K, 3), np.float32)
+    lib().orc_brute_udf(V.reshape(-1), F.reshape(-1), F.shape[0], P.reshape(-1), K, d, f, c.reshape(-1),
+                        w.reshape(-1), threads)
+    return d, f, c, w
+
+
+def fp64_udf(V, F, P, threads: int = 0):
+    """Exact FP64 distance and closest point (Ericson's Voronoi-region method)."""
+    V, F, P = _f32(V), _i32(F), _f32(P).reshape(-1, 3)
+    K = P.shape[0]
+    d = np.zeros(K, np.float64)
+    c = np.zeros((K, 3), np.float64)
+    lib().orc_fp64_udf(V.reshape(-1), F.reshape(-1), F.shape[0], P.reshape(-1), K, d, c.reshape(-1), threads)
+    return d, c
+
+
+def tri_closest_fp64(tri9, p):
+    out = np.zeros(3, np.float64)
+    lib().orc_tri_closest_fp64(np.ascontiguousarray(tri9, np.float64).reshape(9),
+                               np.ascontiguousarray(p, np.float64).reshape(3), out)
+    return out
+
+
+def brute_aabb_hit(V, F, qmin, qmax, threads: int = 0):
+    V, F = _f32(V), _i32(F)
+    qmin, qmax = _f32(qmin).reshape(-1, 3), _f32(qmax).reshape(-1, 3)
+    K = qmin.shape[0]
+    hit = np.zeros(K, np.uint8)
+    cnt = np.zeros(K, np.int32)
+    lib().orc_brute_aabb_hit(V.reshape(-1), F.reshape(-1), F.shape[0], qmin.reshape(-1), qmax.reshape(-1), K, hit,
+                             cnt.ctypes.data, threads)
+    return hit.astype(bool), cnt
