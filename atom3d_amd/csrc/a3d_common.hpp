@@ -234,6 +234,29 @@ __device__ __forceinline__ bool tri_box_hit(v3 a, v3 b, v3 c, v3 lo, v3 hi) {
   return tri_aabb_sat(a, b, c, lo, hi);
 }
 
+// ------------------------------------------------------------------ traversal
+constexpr int kStackTotal = 112;  // >= 97: every tree level adds >= 1 bit of Morton prefix
+
+__device__ __forceinline__ const Node64* nodes_of(const void* blob) {
+  return reinterpret_cast<const Node64*>((const char*)blob + header(blob)->nodes_off);
+}
+__device__ __forceinline__ const Tri48* tris_of(const void* blob) {
+  return reinterpret_cast<const Tri48*>((const char*)blob + header(blob)->tris_off);
+}
+
+// Squared-distance threshold above which a box cannot hold a face the
+// reference formula would rank at or below `best` (see the UDF kernels).
+__device__ __forceinline__ float prune_thr(float best, float slack) {
+  float t = (sqrtf(best) + slack) * 1.00000095367431640625f;  // x (1 + 2^-20)
+  return t * t;
+}
+
+// Absolute error slack of a query: 16 ulp of the largest magnitude entering a
+// coordinate difference.
+__device__ __forceinline__ float query_slack(v3 p, float coord_max) {
+  return 16.0f * 5.9604645e-08f * (fabsf(p.x) + fabsf(p.y) + fabsf(p.z) + 3.0f * coord_max);
+}
+
 // ------------------------------------------------------------------ launch helpers
 // Blocks are handed to the 8 XCDs round-robin; remap so each XCD gets a
 // contiguous run of logical blocks (neighbouring queries share its L2).

@@ -26,120 +26,8 @@ namespace a3d {
 namespace {
 
 constexpr int kBlk = 256;
-constexpr int kLds = 16;           // LDS stack entries per lane
-constexpr int kStackTotal = 112;   // >= 97 (max depth) + margin
+constexpr int kLds = 16;  // LDS stack entries per lane
 constexpr int kOvf = kStackTotal - kLds;
-
-__device__ __forceinline__ const Node64* nodes_of(const void* blob) {
-  return reinterpret_cast<const Node64*>((const char*)blob + header(blob)->nodes_off);
-}
-__device__ __forceinline__ const Tri48* tris_of(const void* blob) {
-  return reinterpret_cast<const Tri48*>((const char*)blob + header(blob)->tris_off);
-}
-
-__device__ __forceinline__ float prune_thr(float best, float slack) {
-  float t = (sqrtf(best) + slack) * 1.00000095367431640625f;  // x (1 + 2^-20)
-  return t * t;
-}
-
-__global__ __launch_bounds__(kBlk) void k_udf(const void* __restrict__ blob, const float* __restrict__ P, int64_t K,
-                                               float* __restrict__ dist, int32_t* __restrict__ face,
-                                               float* __restrict__ closest, float* __restrict__ uvw) {
-  __shared__ int32_t s_ref[kLds * kBlk];
-  __shared__ float s_d2[kLds * kBlk];
-  const int tid = threadIdx.x;
-  const int64_t q = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlk + tid;
-  if (q >= K) return;
-  const BvhHeader* H = header(blob);
-  const Node64* __restrict__ nodes = nodes_of(blob);
-  const Tri48* __restrict__ tris = tris_of(blob);
-  const v3 p = mk(P[q * 3], P[q * 3 + 1], P[q * 3 + 2]);
-  // 16 ulp of the largest magnitude entering a difference.
-  const float slack = 16.0f * 5.9604645e-08f * (fabsf(p.x) + fabsf(p.y) + fabsf(p.z) + 3.0f * H->coord_max);
-
-  float best = 1e30f;
-  int32_t best_id = 0x7fffffff, best_tri = -1;
-  float thr = prune_thr(best, slack);
-  int32_t ovf_ref[kOvf];
-  float ovf_d2[kOvf];
-  int sp = 0;
-  int32_t cur = 0;
-  while (true) {
-    if (cur >= 0) {
-      const Node64* nd = nodes + cur;
-      float4 a = nd->a, b = nd->b, c = nd->c;
-      int4 d = nd->d;
-      float dl = box_d2(p, a.x, a.y, a.z, a.w, b.x, b.y);
-      float dr = box_d2(p, b.z, b.w, c.x, c.y, c.z, c.w);
-      bool tl = dl <= thr, tr = dr <= thr;
-      if (tl && tr) {
-        int32_t near_ref = d.x, far_ref = d.y;
-        float far_d = dr;
-        if (dr < dl) { near_ref = d.y; far_ref = d.x; far_d = dl; }
-        if (sp < kLds) {
-          s_ref[sp * kBlk + tid] = far_ref;
-          s_d2[sp * kBlk + tid] = far_d;
-        } else if (sp < kStackTotal) {
-          ovf_ref[sp - kLds] = far_ref;
-          ovf_d2[sp - kLds] = far_d;
-        }
-        sp = sp < kStackTotal ? sp + 1 : sp;
-        cur = near_ref;
-        continue;
-      }
-      if (tl) { cur = d.x; continue; }
-      if (tr) { cur = d.y; continue; }
-    } else {
-      const int32_t s0 = leaf_start(cur), n = leaf_count(cur);
-      for (int32_t i = s0; i < s0 + n; ++i) {
-        v3 ta, tb, tc;
-        int32_t id;
-        load_tri(tris, i, ta, tb, tc, id);
-        float d2 = tri_dist2(ta, tb, tc, p);
-        if (d2 < best || (d2 == best && id < best_id)) {
-          best = d2;
-          best_id = id;
-          best_tri = i;
-          thr = prune_thr(best, slack);
-        }
-      }
-    }
-    bool found = false;
-    while (sp > 0) {
-      --sp;
-      int32_t r;
-      float dd;
-      if (sp < kLds) {
-        r = s_ref[sp * kBlk + tid];
-        dd = s_d2[sp * kBlk + tid];
-      } else {
-        r = ovf_ref[sp - kLds];
-        dd = ovf_d2[sp - kLds];
-      }
-      if (dd <= thr) { cur = r; found = true; break; }
-    }
-    if (!found) break;
-  }
-
-  dist[q] = sqrtf(best);
-  face[q] = best_tri >= 0 ? best_id : -1;
-  v3 cp = p, w = mk(0.0f, 0.0f, 0.0f);
-  if (best_tri >= 0) {
-    v3 ta, tb, tc;
-    int32_t id;
-    load_tri(tris, best_tri, ta, tb, tc, id);
-    cp = tri_closest(ta, tb, tc, p);
-    w = tri_bary(ta, tb, tc, cp);
-  }
-  closest[q * 3] = cp.x;
-  closest[q * 3 + 1] = cp.y;
-  closest[q * 3 + 2] = cp.z;
-  if (uvw) {
-    uvw[q * 3] = w.x;
-    uvw[q * 3 + 1] = w.y;
-    uvw[q * 3 + 2] = w.z;
-  }
-}
 
 // ---------------------------------------------------------------- AABB walks
 // MODE 0: hit byte, stop at the first hit.  MODE 1: pair count.  MODE 2: write
@@ -369,17 +257,6 @@ inline int64_t al256(int64_t x) { return (x + 255) & ~int64_t(255); }
 }  // namespace a3d
 
 using namespace a3d;
-
-extern "C" int a3d_bvh_udf(const void* blob, const float* P, int64_t K, float* dist, int32_t* face, float* closest,
-                           float* uvw, a3d_stream_t stream) {
-  if (int rc = check_blob(blob, "a3d_bvh_udf")) return rc;
-  if (K < 0 || K > (int64_t(1) << 40)) return fail(A3D_ERR_ARG, "a3d_bvh_udf: K=%lld out of range", (long long)K);
-  if (K == 0) return A3D_OK;
-  if (!P || !dist || !face || !closest) return fail(A3D_ERR_ARG, "a3d_bvh_udf: null input/output");
-  hipLaunchKernelGGL(k_udf, dim3(grid_for(K)), dim3(kBlk), 0, (hipStream_t)stream, blob, P, K, dist, face, closest, uvw);
-  A3D_LAUNCH_CHECK();
-  return A3D_OK;
-}
 
 extern "C" int a3d_sdf_sign(const float* P, int64_t K, const float* closest, const int32_t* face, const float* uvw,
                             const float* face_n, const float* vertex_pn, const float* edge_pn, const int64_t* he_edge,

@@ -1,0 +1,381 @@
+// Closest-point (UDF) queries for gfx950.
+//
+// Replaces bvh_udf_kernel + bvh_udf_cuda (Atom3D
+// atom3d/kernels/bvh_kernels.cu:363-463, 1028-1066), which walk the tree with
+// one thread per query, a 128-entry private stack and 36-byte AoS nodes.
+//
+// Default path (wave-coherent):
+//   1. queries are ordered along a 30-bit Morton curve of their position
+//      (radix sort of (key, index)), so the 64 queries of a wavefront are
+//      spatial neighbours;
+//   2. each wavefront walks the tree as one packet: a node or triangle is
+//      fetched once per wave with scalar loads (64 bytes of node, 48 bytes of
+//      triangle) and every lane tests it against its own query; a child is
+//      entered when any lane's pruning test accepts it, nearer-by-vote first;
+//      the stack (reference + child box, 32 bytes an entry) is shared by the
+//      wave in LDS and re-tested against every lane when popped;
+//   3. each lane writes its results to the query's original position.
+// Fallback path (flag A3D_UDF_LANE): one independent walk per lane with a
+// per-lane LDS stack.
+//
+// Closest-face definition (tree and order independent): the face minimising
+// the reference's FP32 squared distance (Triangle::distance_sq, :101-135),
+// ties to the lowest original face id.  A box is skipped only when its
+// distance exceeds the lane's best distance by more than the query's FP32
+// error slack (and thin-triangle error is folded into the leaf boxes at build
+// time), so no face the formula ranks first can be skipped; the minimum over
+// the tested faces is therefore the minimum over all faces, whatever order or
+// extra nodes a wave visits.
+#include <hipcub/hipcub.hpp>
+
+#include "a3d_common.hpp"
+
+namespace a3d {
+namespace {
+
+constexpr int kBlk = 256;
+constexpr int kWaves = kBlk / 64;
+constexpr int kLaneLds = 16;  // per-lane LDS stack entries in the lane path
+constexpr int kLaneOvf = kStackTotal - kLaneLds;
+
+__device__ __forceinline__ int32_t uni(int32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+
+__device__ __forceinline__ void write_result(int64_t q, v3 p, float best, int32_t best_id, int32_t best_tri,
+                                             const Tri48* __restrict__ tris, float* __restrict__ dist,
+                                             int32_t* __restrict__ face, float* __restrict__ closest,
+                                             float* __restrict__ uvw) {
+  dist[q] = sqrtf(best);
+  face[q] = best_tri >= 0 ? best_id : -1;
+  v3 cp = p, w = mk(0.0f, 0.0f, 0.0f);
+  if (best_tri >= 0) {
+    v3 ta, tb, tc;
+    int32_t id;
+    load_tri(tris, best_tri, ta, tb, tc, id);
+    cp = tri_closest(ta, tb, tc, p);
+    w = tri_bary(ta, tb, tc, cp);
+  }
+  closest[q * 3] = cp.x;
+  closest[q * 3 + 1] = cp.y;
+  closest[q * 3 + 2] = cp.z;
+  if (uvw) {
+    uvw[q * 3] = w.x;
+    uvw[q * 3 + 1] = w.y;
+    uvw[q * 3 + 2] = w.z;
+  }
+}
+
+// ------------------------------------------------------------ wave-coherent walk
+__global__ __launch_bounds__(kBlk) void k_udf_wave(const void* __restrict__ blob, const float* __restrict__ P,
+                                                    const uint32_t* __restrict__ perm, int64_t K,
+                                                    float* __restrict__ dist, int32_t* __restrict__ face,
+                                                    float* __restrict__ closest, float* __restrict__ uvw) {
+  __shared__ float4 s_stk[kWaves][kStackTotal][2];  // (lo.xyz, ref bits), (hi.xyz, -)
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlk + threadIdx.x;
+  const bool valid = i < K;
+  if (!__any(valid)) return;
+  const int64_t q = valid ? (perm ? (int64_t)perm[i] : i) : 0;
+  const BvhHeader* H = header(blob);
+  const Node64* __restrict__ nodes = nodes_of(blob);
+  const Tri48* __restrict__ tris = tris_of(blob);
+  const v3 p = valid ? mk(P[q * 3], P[q * 3 + 1], P[q * 3 + 2]) : mk(0.0f, 0.0f, 0.0f);
+  const float slack = query_slack(p, H->coord_max);
+
+  float best = 1e30f;
+  int32_t best_id = 0x7fffffff, best_tri = -1;
+  float thr = valid ? prune_thr(best, slack) : -1.0f;  // invalid lanes accept nothing
+  int sp = 0;
+  int32_t cur = 0;
+  while (true) {
+    if (cur >= 0) {
+      const Node64 nd = nodes[cur];
+      float dl = box_d2(p, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y);
+      float dr = box_d2(p, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w);
+      bool wl = dl <= thr, wr = dr <= thr;
+      uint64_t bl = __ballot(wl), br = __ballot(wr);
+      if (bl && br) {
+        int nl = __popcll(__ballot((wl || wr) && dl <= dr));
+        int nr = __popcll(bl | br) - nl;
+        bool left_first = nl >= nr;
+        int32_t near_ref = left_first ? nd.d.x : nd.d.y;
+        int32_t far_ref = left_first ? nd.d.y : nd.d.x;
+        if (sp < kStackTotal) {
+          if (lane == 0) {
+            if (left_first) {
+              s_stk[wv][sp][0] = make_float4(nd.b.z, nd.b.w, nd.c.x, __int_as_float(far_ref));
+              s_stk[wv][sp][1] = make_float4(nd.c.y, nd.c.z, nd.c.w, 0.0f);
+            } else {
+              s_stk[wv][sp][0] = make_float4(nd.a.x, nd.a.y, nd.a.z, __int_as_float(far_ref));
+              s_stk[wv][sp][1] = make_float4(nd.a.w, nd.b.x, nd.b.y, 0.0f);
+            }
+          }
+          ++sp;
+        }
+        cur = uni(near_ref);
+        continue;
+      }
+      if (bl) { cur = uni(nd.d.x); continue; }
+      if (br) { cur = uni(nd.d.y); continue; }
+    } else {
+      const int32_t s0 = leaf_start(cur), n = leaf_count(cur);
+      for (int32_t t = s0; t < s0 + n; ++t) {
+        v3 ta, tb, tc;
+        int32_t id;
+        load_tri(tris, t, ta, tb, tc, id);
+        float d2 = tri_dist2(ta, tb, tc, p);
+        if (d2 < best || (d2 == best && id < best_id)) {
+          best = d2;
+          best_id = id;
+          best_tri = t;
+          thr = valid ? prune_thr(best, slack) : -1.0f;
+        }
+      }
+    }
+    bool found = false;
+    while (sp > 0) {
+      --sp;
+      float4 e0 = s_stk[wv][sp][0], e1 = s_stk[wv][sp][1];
+      float d = box_d2(p, e0.x, e0.y, e0.z, e1.x, e1.y, e1.z);
+      if (__ballot(d <= thr)) {
+        cur = uni(__float_as_int(e0.w));
+        found = true;
+        break;
+      }
+    }
+    if (!found) break;
+  }
+  if (valid) write_result(q, p, best, best_id, best_tri, tris, dist, face, closest, uvw);
+}
+
+// ------------------------------------------------------------ per-lane walk
+__global__ __launch_bounds__(kBlk) void k_udf_lane(const void* __restrict__ blob, const float* __restrict__ P,
+                                                    const uint32_t* __restrict__ perm, int64_t K,
+                                                    float* __restrict__ dist, int32_t* __restrict__ face,
+                                                    float* __restrict__ closest, float* __restrict__ uvw) {
+  __shared__ int32_t s_ref[kLaneLds * kBlk];
+  __shared__ float s_d2[kLaneLds * kBlk];
+  const int tid = threadIdx.x;
+  const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlk + tid;
+  if (i >= K) return;
+  const int64_t q = perm ? (int64_t)perm[i] : i;
+  const BvhHeader* H = header(blob);
+  const Node64* __restrict__ nodes = nodes_of(blob);
+  const Tri48* __restrict__ tris = tris_of(blob);
+  const v3 p = mk(P[q * 3], P[q * 3 + 1], P[q * 3 + 2]);
+  const float slack = query_slack(p, H->coord_max);
+
+  float best = 1e30f;
+  int32_t best_id = 0x7fffffff, best_tri = -1;
+  float thr = prune_thr(best, slack);
+  int32_t ovf_ref[kLaneOvf];
+  float ovf_d2[kLaneOvf];
+  int sp = 0;
+  int32_t cur = 0;
+  while (true) {
+    if (cur >= 0) {
+      const Node64* nd = nodes + cur;
+      float4 a = nd->a, b = nd->b, c = nd->c;
+      int4 d = nd->d;
+      float dl = box_d2(p, a.x, a.y, a.z, a.w, b.x, b.y);
+      float dr = box_d2(p, b.z, b.w, c.x, c.y, c.z, c.w);
+      bool tl = dl <= thr, tr = dr <= thr;
+      if (tl && tr) {
+        int32_t near_ref = d.x, far_ref = d.y;
+        float far_d = dr;
+        if (dr < dl) { near_ref = d.y; far_ref = d.x; far_d = dl; }
+        if (sp < kLaneLds) {
+          s_ref[sp * kBlk + tid] = far_ref;
+          s_d2[sp * kBlk + tid] = far_d;
+        } else if (sp < kStackTotal) {
+          ovf_ref[sp - kLaneLds] = far_ref;
+          ovf_d2[sp - kLaneLds] = far_d;
+        }
+        sp = sp < kStackTotal ? sp + 1 : sp;
+        cur = near_ref;
+        continue;
+      }
+      if (tl) { cur = d.x; continue; }
+      if (tr) { cur = d.y; continue; }
+    } else {
+      const int32_t s0 = leaf_start(cur), n = leaf_count(cur);
+      for (int32_t t = s0; t < s0 + n; ++t) {
+        v3 ta, tb, tc;
+        int32_t id;
+        load_tri(tris, t, ta, tb, tc, id);
+        float d2 = tri_dist2(ta, tb, tc, p);
+        if (d2 < best || (d2 == best && id < best_id)) {
+          best = d2;
+          best_id = id;
+          best_tri = t;
+          thr = prune_thr(best, slack);
+        }
+      }
+    }
+    bool found = false;
+    while (sp > 0) {
+      --sp;
+      int32_t r;
+      float dd;
+      if (sp < kLaneLds) {
+        r = s_ref[sp * kBlk + tid];
+        dd = s_d2[sp * kBlk + tid];
+      } else {
+        r = ovf_ref[sp - kLaneLds];
+        dd = ovf_d2[sp - kLaneLds];
+      }
+      if (dd <= thr) { cur = r; found = true; break; }
+    }
+    if (!found) break;
+  }
+  write_result(q, p, best, best_id, best_tri, tris, dist, face, closest, uvw);
+}
+
+// ------------------------------------------------------------ query ordering
+__device__ __forceinline__ uint32_t f2ord(float f) {
+  uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ord2f(uint32_t u) {
+  return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
+__global__ void k_qinit(uint32_t* red) {
+  int t = threadIdx.x;
+  if (t < 6) red[t] = t < 3 ? 0xffffffffu : 0u;
+}
+
+__global__ __launch_bounds__(kBlk) void k_qbounds(const float* __restrict__ P, int64_t K, uint32_t* red) {
+  float lo[3] = {3.0e38f, 3.0e38f, 3.0e38f}, hi[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
+  for (int64_t i = blockIdx.x * (int64_t)kBlk + threadIdx.x; i < K; i += (int64_t)gridDim.x * kBlk) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      float v = P[i * 3 + k];
+      lo[k] = fminf(lo[k], v);
+      hi[k] = fmaxf(hi[k], v);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    for (int o = 32; o > 0; o >>= 1) {
+      lo[k] = fminf(lo[k], __shfl_xor(lo[k], o));
+      hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], o));
+    }
+  }
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      atomicMin(&red[k], f2ord(lo[k]));
+      atomicMax(&red[3 + k], f2ord(hi[k]));
+    }
+  }
+}
+
+__device__ __forceinline__ uint32_t spread10(uint32_t x) {
+  x &= 0x3ffu;
+  x = (x | (x << 16)) & 0x030000ffu;
+  x = (x | (x << 8)) & 0x0300f00fu;
+  x = (x | (x << 4)) & 0x030c30c3u;
+  x = (x | (x << 2)) & 0x09249249u;
+  return x;
+}
+
+__global__ __launch_bounds__(kBlk) void k_qmorton(const float* __restrict__ P, int64_t K,
+                                                   const uint32_t* __restrict__ red, uint32_t* __restrict__ keys,
+                                                   uint32_t* __restrict__ vals) {
+  int64_t i = blockIdx.x * (int64_t)kBlk + threadIdx.x;
+  if (i >= K) return;
+  uint32_t c[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    float lo = ord2f(red[k]), ext = ord2f(red[3 + k]) - lo;
+    float t = ext > 0.0f ? (P[i * 3 + k] - lo) * (1023.0f / ext) : 0.0f;
+    t = fminf(fmaxf(t, 0.0f), 1023.0f);
+    c[k] = (uint32_t)t;
+  }
+  keys[i] = (spread10(c[0]) << 2) | (spread10(c[1]) << 1) | spread10(c[2]);
+  vals[i] = (uint32_t)i;
+}
+
+struct QWs {
+  uint32_t* red;
+  uint32_t* k0;
+  uint32_t* k1;
+  uint32_t* v0;
+  void* cub;
+  size_t cub_bytes;
+  int64_t total;
+};
+
+inline int64_t al256(int64_t x) { return (x + 255) & ~int64_t(255); }
+
+QWs carve_q(void* base, int64_t K) {
+  QWs w;
+  int64_t n = K > 1 ? K : 2;
+  char* b = (char*)base;
+  int64_t off = 0;
+  auto take = [&](int64_t bytes) { char* p = b ? b + off : nullptr; off += al256(bytes); return p; };
+  w.red = (uint32_t*)take(64);
+  w.k0 = (uint32_t*)take(4 * n);
+  w.k1 = (uint32_t*)take(4 * n);
+  w.v0 = (uint32_t*)take(4 * n);
+  size_t tb = 0;
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, tb, (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                           (uint32_t*)nullptr, (int)n, 0, 30);
+  w.cub_bytes = tb;
+  w.cub = take((int64_t)tb);
+  w.total = off;
+  return w;
+}
+
+inline unsigned grid_for(int64_t n) { return (unsigned)div_up(n > 0 ? n : 1, kBlk); }
+
+}  // namespace
+}  // namespace a3d
+
+using namespace a3d;
+
+extern "C" int a3d_query_order_ws_bytes(int64_t K, int64_t* bytes) {
+  if (K < 0 || K > 0x7fffffff || !bytes) return fail(A3D_ERR_ARG, "a3d_query_order_ws_bytes: K=%lld not in [0, 2^31)", (long long)K);
+  *bytes = carve_q(nullptr, K).total;
+  return A3D_OK;
+}
+
+extern "C" int a3d_query_order(const float* P, int64_t K, uint32_t* perm, void* ws, int64_t ws_bytes,
+                               a3d_stream_t stream) {
+  int64_t need = 0;
+  if (int rc = a3d_query_order_ws_bytes(K, &need)) return rc;
+  if (K == 0) return A3D_OK;
+  if (!P || !perm || !ws) return fail(A3D_ERR_ARG, "a3d_query_order: null argument");
+  if (ws_bytes < need) return fail(A3D_ERR_CAPACITY, "a3d_query_order: workspace %lld < %lld bytes", (long long)ws_bytes, (long long)need);
+  if ((uintptr_t)ws & 255) return fail(A3D_ERR_ARG, "a3d_query_order: workspace must be 256-byte aligned");
+  hipStream_t s = (hipStream_t)stream;
+  QWs w = carve_q(ws, K);
+  hipLaunchKernelGGL(k_qinit, dim3(1), dim3(64), 0, s, w.red);
+  A3D_LAUNCH_CHECK();
+  unsigned g = grid_for(K);
+  hipLaunchKernelGGL(k_qbounds, dim3(g < 2048 ? g : 2048), dim3(kBlk), 0, s, P, K, w.red);
+  A3D_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_qmorton, dim3(g), dim3(kBlk), 0, s, P, K, w.red, w.k0, w.v0);
+  A3D_LAUNCH_CHECK();
+  size_t tb = w.cub_bytes;
+  A3D_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(w.cub, tb, w.k0, w.k1, w.v0, perm, (int)K, 0, 30, s));
+  return A3D_OK;
+}
+
+extern "C" int a3d_bvh_udf(const void* blob, const float* P, int64_t K, const uint32_t* perm, float* dist,
+                           int32_t* face, float* closest, float* uvw, int flags, a3d_stream_t stream) {
+  if (!blob) return fail(A3D_ERR_ARG, "a3d_bvh_udf: null BVH blob");
+  if (K < 0 || K > (int64_t(1) << 40)) return fail(A3D_ERR_ARG, "a3d_bvh_udf: K=%lld out of range", (long long)K);
+  if (perm && K > 0xffffffffll) return fail(A3D_ERR_ARG, "a3d_bvh_udf: perm needs K < 2^32");
+  if (flags & ~A3D_UDF_LANE) return fail(A3D_ERR_ARG, "a3d_bvh_udf: unknown flags 0x%x", flags);
+  if (K == 0) return A3D_OK;
+  if (!P || !dist || !face || !closest) return fail(A3D_ERR_ARG, "a3d_bvh_udf: null input/output");
+  hipStream_t s = (hipStream_t)stream;
+  if (flags & A3D_UDF_LANE)
+    hipLaunchKernelGGL(k_udf_lane, dim3(grid_for(K)), dim3(kBlk), 0, s, blob, P, perm, K, dist, face, closest, uvw);
+  else
+    hipLaunchKernelGGL(k_udf_wave, dim3(grid_for(K)), dim3(kBlk), 0, s, blob, P, perm, K, dist, face, closest, uvw);
+  A3D_LAUNCH_CHECK();
+  return A3D_OK;
+}

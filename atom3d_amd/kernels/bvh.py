@@ -70,7 +70,7 @@ class BVHAccelerator:
         nodes = raw[nodes_off:nodes_off + 64 * cap].view(np.float32).reshape(cap, 16)[:n_nodes]
         refs = nodes[:, 12:14].view(np.int32)
         tris = raw[tris_off:tris_off + 48 * n_tris].view(np.float32).reshape(n_tris, 12)
-        ids = tris[:, 10].view(np.int32)
+        ids = tris[:, 9].view(np.int32)
         assert 1 <= n_nodes <= cap
         if n_tris:
             assert np.array_equal(np.sort(ids), np.arange(n_tris)), "triangle ids are not a permutation"
@@ -108,19 +108,41 @@ class BVHAccelerator:
             raise ValueError(f"points must be [K, 3], got {tuple(points.shape)}")
         return points.contiguous().float()
 
-    def udf(self, points: torch.Tensor):
+    UDF_LANE = 1
+    ORDER_MIN_QUERIES = 4096
+
+    def query_order(self, points: torch.Tensor) -> torch.Tensor:
+        """Morton order of the query points (uint32 permutation stored in an
+        int32 tensor)."""
+        p = self._points(points)
+        K = p.shape[0]
+        wsb = ctypes.c_int64()
+        _lib.call("a3d_query_order_ws_bytes", K, ctypes.byref(wsb))
+        with torch.cuda.device(self.device):
+            perm = torch.empty(K, dtype=torch.int32, device=self.device)
+            ws = torch.empty(max(wsb.value, 1), dtype=torch.uint8, device=self.device)
+            _lib.call("a3d_query_order", _lib.ptr(p), K, _lib.ptr(perm), _lib.ptr(ws), wsb.value,
+                      _lib.stream(self.device))
+        return perm
+
+    def udf(self, points: torch.Tensor, flags: int = 0, order: bool | None = None):
         """Closest point on the mesh: (distances [K], face_ids [K] int32,
-        closest_points [K,3], uvw [K,3])."""
+        closest_points [K,3], uvw [K,3]).  ``order`` (default: K >= 4096)
+        walks the queries in Morton order; ``flags`` (UDF_LANE) picks the
+        traversal variant.  Results are identical either way."""
         p = self._points(points)
         K = p.shape[0]
         dist = torch.empty(K, dtype=torch.float32, device=self.device)
         face = torch.empty(K, dtype=torch.int32, device=self.device)
         closest = torch.empty(K, 3, dtype=torch.float32, device=self.device)
         uvw = torch.empty(K, 3, dtype=torch.float32, device=self.device)
+        if order is None:
+            order = self.ORDER_MIN_QUERIES <= K < 2 ** 31
+        perm = self.query_order(p) if order and K > 0 else None
         with torch.cuda.device(self.device):
             _lib.call(
-                "a3d_bvh_udf", _lib.ptr(self.blob), _lib.ptr(p), K, _lib.ptr(dist), _lib.ptr(face),
-                _lib.ptr(closest), _lib.ptr(uvw), _lib.stream(self.device),
+                "a3d_bvh_udf", _lib.ptr(self.blob), _lib.ptr(p), K, _lib.ptr(perm), _lib.ptr(dist), _lib.ptr(face),
+                _lib.ptr(closest), _lib.ptr(uvw), int(flags), _lib.stream(self.device),
             )
         return dist, face, closest, uvw
 

@@ -19,6 +19,7 @@ C++) on this host's cores on a bounded sample.
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -90,28 +91,37 @@ def main():
     face = torch.empty(K, dtype=torch.int32, device=dev)
     closest = torch.empty(K, 3, device=dev)
     uvw = torch.empty(K, 3, device=dev)
+    perm = torch.empty(K, dtype=torch.int32, device=dev)
+    wsb = ctypes.c_int64()
+    _lib.call("a3d_query_order_ws_bytes", K, ctypes.byref(wsb))
+    ws = torch.empty(wsb.value, dtype=torch.uint8, device=dev)
     stream = torch.cuda.current_stream(dev)
     sp = stream.cuda_stream
     lib = _lib.lib()
-    args = (_lib.ptr(bvh.blob), _lib.ptr(P), K, _lib.ptr(dist), _lib.ptr(face), _lib.ptr(closest), _lib.ptr(uvw), sp)
+    order_args = (_lib.ptr(P), K, _lib.ptr(perm), _lib.ptr(ws), wsb.value, sp)
+    udf_args = (_lib.ptr(bvh.blob), _lib.ptr(P), K, _lib.ptr(perm), _lib.ptr(dist), _lib.ptr(face),
+                _lib.ptr(closest), _lib.ptr(uvw), 0, sp)
 
-    def step():
-        _lib.check(lib.a3d_bvh_udf(*args), "a3d_bvh_udf")
+    # one step = Morton ordering of the queries + the wave-coherent walk
+    def step(ev=None):
+        _lib.check(lib.a3d_query_order(*order_args), "a3d_query_order")
+        if ev is not None:
+            ev[0].record(stream)
+        _lib.check(lib.a3d_bvh_udf(*udf_args), "a3d_bvh_udf")
+        if ev is not None:
+            ev[1].record(stream)
 
     for _ in range(a.warmup):
         step()
     torch.cuda.synchronize()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(a.steps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    ev0.record(stream)
-    for _ in range(a.steps):
-        step()
-    ev1.record(stream)
+    for k in range(a.steps):
+        step(evs[k])
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    kern_ms = ev0.elapsed_time(ev1) / a.steps  # one kernel launch per step
+    kern_ms = sum(e0.elapsed_time(e1) for e0, e1 in evs) / a.steps  # the walk kernel alone
     ms_per_step = wall * 1e3 / a.steps
     qps = K / (ms_per_step * 1e-3)
 
@@ -172,7 +182,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": "k_udf",
+            "kernel": "k_udf_wave",
             "kernel_ms": kern_ms,
             "bytes_per_query": bq,
             "n_node": n_node,

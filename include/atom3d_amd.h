@@ -66,10 +66,23 @@ int a3d_bvh_info(const void* blob, int64_t* out4, a3d_stream_t stream);
  * P float32[K*3] -> dist float32[K], face int32[K] (original face index, -1
  * if none), closest float32[K*3], uvw float32[K*3] (nullable; zeros when
  * face == -1).  Closest face = minimum of the reference's FP32 squared
- * distance over ALL faces, ties to the lowest face index.
+ * distance over ALL faces, ties to the lowest face index; results do not
+ * depend on perm, the flags or the tree.
+ * perm (nullable, uint32[K]): process the queries in this order (a
+ * permutation of 0..K-1, e.g. from a3d_query_order); results are still
+ * written at each query's own index.  flags: A3D_UDF_LANE = one independent
+ * walk per lane instead of one shared walk per wavefront.
+ *
+ * a3d_query_order: perm = the queries sorted along a 30-bit Morton curve of
+ * their bounding box (stable), so each wavefront covers a compact region.
+ * Workspace of a3d_query_order_ws_bytes(K) bytes, 256-byte aligned; K < 2^31.
  */
-int a3d_bvh_udf(const void* blob, const float* P, int64_t K, float* dist, int32_t* face,
-                float* closest, float* uvw, a3d_stream_t stream);
+#define A3D_UDF_LANE 1
+int a3d_query_order_ws_bytes(int64_t K, int64_t* bytes);
+int a3d_query_order(const float* P, int64_t K, uint32_t* perm, void* ws, int64_t ws_bytes,
+                    a3d_stream_t stream);
+int a3d_bvh_udf(const void* blob, const float* P, int64_t K, const uint32_t* perm, float* dist,
+                int32_t* face, float* closest, float* uvw, int flags, a3d_stream_t stream);
 
 /* ------------------------------------------------------------------ UDF adjoint
  * Replaces the autograd re-evaluation in MeshBVH._udf_with_grad

@@ -16,17 +16,25 @@ pytestmark = pytest.mark.gpu
 DEV = "cuda:0"
 
 
-def _gpu_udf(V, F, P):
+def _gpu_udf(V, F, P, **kw):
     b = BVHAccelerator(torch.as_tensor(V).to(DEV), torch.as_tensor(F).to(DEV))
-    d, f, c, w = b.udf(torch.as_tensor(P).to(DEV))
+    d, f, c, w = b.udf(torch.as_tensor(P).to(DEV), **kw)
     torch.cuda.synchronize()
     return d.cpu().numpy(), f.cpu().numpy(), c.cpu().numpy(), w.cpu().numpy()
 
 
+VARIANTS = [dict(order=True), dict(order=False), dict(order=True, flags=1), dict(order=False, flags=1)]
+
+
 def _assert_bitexact(V, F, P, label):
-    print(f"[udf parity] {label}: V={len(V)} F={len(F)} K={len(P)}", flush=True)
-    d, f, c, w = _gpu_udf(V, F, P)
     rd, rf, rc, rw = oracle.brute_udf(V, F, P)
+    for kw in VARIANTS:
+        print(f"[udf parity] {label} {kw}: V={len(V)} F={len(F)} K={len(P)}", flush=True)
+        _check_one(V, F, P, kw, label, rd, rf, rc, rw)
+
+
+def _check_one(V, F, P, kw, label, rd, rf, rc, rw):
+    d, f, c, w = _gpu_udf(V, F, P, **kw)
     bad = np.nonzero((d != rd) | (f != rf))[0]
     assert bad.size == 0, f"{label}: {bad.size} mismatches, first {bad[:5]} gpu {d[bad[:5]]} {f[bad[:5]]} ref {rd[bad[:5]]} {rf[bad[:5]]}"
     np.testing.assert_array_equal(c, rc)
