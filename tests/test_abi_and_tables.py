@@ -1,0 +1,90 @@
+"""Host-side checks (no GPU): the C-ABI library loads and exports every
+declared symbol, argument validation errors surface as messages, the cube
+tables equal the reference's, and the synthetic workloads are deterministic."""
+import ctypes
+import hashlib
+import os
+import re
+
+import numpy as np
+import pytest
+import torch
+
+from atom3d_amd import constants as C
+from atom3d_amd import datasets
+from atom3d_amd.kernels import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "atom3d_amd.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(a3d_\w+)\s*\(", src, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    names = _declared()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(lib, n), f"{n} declared in include/atom3d_amd.h but not exported"
+    # and every binding the Python side sets up is declared
+    assert set(_lib.SIGNATURES) <= set(names)
+
+
+def test_version_and_argument_errors():
+    assert "gfx950" in _lib.version()
+    b, w = ctypes.c_int64(), ctypes.c_int64()
+    with pytest.raises(RuntimeError, match="leaf_max"):
+        _lib.call("a3d_bvh_sizes", 100, 0, ctypes.byref(b), ctypes.byref(w))
+    with pytest.raises(RuntimeError, match="null BVH blob"):
+        _lib.call("a3d_bvh_udf", None, None, 5, None, None, None, None, None, 0, None)
+    with pytest.raises(RuntimeError, match="unknown flags"):
+        _lib.call("a3d_bvh_udf", 1, None, 5, None, None, None, None, None, 8, None)
+    with pytest.raises(RuntimeError, match="level"):
+        _lib.call("a3d_octree_level", 1, (ctypes.c_float * 6)(), 22, None, 0, 1, 1, 1, 1 << 20, None)
+    _lib.call("a3d_bvh_sizes", 4096000, 4, ctypes.byref(b), ctypes.byref(w))
+    assert b.value == 256 + 64 * 4095999 + 48 * 4096000
+
+
+def test_cube_tables_match_reference_values():
+    # literal values of atom3d/constants.py:11-64 (fixture data)
+    assert C.CUBE_CORNERS.tolist() == [[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0],
+                                       [0, 0, 1], [1, 0, 1], [0, 1, 1], [1, 1, 1]]
+    assert C.CUBE_EDGES.tolist() == [[0, 1], [2, 3], [4, 5], [6, 7], [0, 2], [1, 3], [4, 6], [5, 7],
+                                     [0, 4], [1, 5], [2, 6], [3, 7]]
+    assert C.NEIGHBOR_OFFSETS_26.shape == (26, 3)
+    assert C.NEIGHBOR_OFFSETS_26[0].tolist() == [-1, -1, -1] and C.NEIGHBOR_OFFSETS_26[13].tolist() == [1, 0, 0]
+    assert C.EDGE_NEIGHBOR_OFFSETS[0].tolist() == [[0, 1, 0], [0, 0, 1], [0, 1, 1]]
+    assert C.EDGE_NEIGHBOR_OFFSETS[3].tolist() == [[0, -1, 0], [0, 0, -1], [0, -1, -1]]
+    assert C.EDGE_NEIGHBOR_OFFSETS[6].tolist() == [[1, 0, 0], [0, 0, -1], [1, 0, -1]]
+    assert C.EDGE_NEIGHBOR_OFFSETS[11].tolist() == [[-1, 0, 0], [0, -1, 0], [-1, -1, 0]]
+    assert C.CORNER_NEIGHBOR_OFFSETS[7].tolist()[-1] == [1, 1, 1]
+    assert C.CORNER_NEIGHBOR_OFFSETS[0].tolist()[-1] == [-1, -1, -1]
+
+
+def test_icosphere_counts_orientation_and_determinism():
+    v, f = datasets.icosphere(5)
+    assert v.shape == (10242, 3) and f.shape == (20480, 3)
+    a, b, c = v[f[:, 0]], v[f[:, 1]], v[f[:, 2]]
+    assert np.all(np.einsum("ij,ij->i", np.cross(b - a, c - a), a + b + c) > 0)  # outward winding
+    V, F = datasets.noisy_icosphere()
+    V2, F2 = datasets.noisy_icosphere()
+    assert torch.equal(V, V2) and torch.equal(F, F2)
+    assert V.dtype == torch.float32 and F.dtype == torch.int32
+    Vu, Fu = datasets.sphere_union(n_spheres=3)
+    assert Fu.shape == (3 * 20480, 3) and int(Fu.max()) == 3 * 10242 - 1
+
+
+def test_octree_cpu_geometry_helpers():
+    from atom3d_amd import OctreeIndexer
+    oc = OctreeIndexer(max_level=4, device="cpu")
+    cubes = oc.all_cubes_at_level(2)
+    assert cubes.shape == (64, 3) and cubes[1].tolist() == [0, 0, 1]
+    kids = oc.subdivide(cubes[:1] + 1, 2)
+    assert kids.tolist()[:3] == [[2, 2, 2], [3, 2, 2], [2, 3, 2]]
+    lo, hi = oc.cube_aabb_level(torch.tensor([[1, 2, 3]]), 2)
+    assert lo.tolist() == [[-0.5, 0.0, 0.5]] and hi.tolist() == [[0.0, 0.5, 1.0]]
+    idx = oc.ijk_to_cube_level(torch.tensor([[1, 2, 3]]), 2)
+    assert idx.item() == (1 * 4 + 2) * 4 + 3
+    assert oc.cube_to_ijk_level(idx, 2).tolist() == [[1, 2, 3]]
