@@ -168,8 +168,38 @@ class MeshBVH:
             return AABBIntersectResult(hit=hit, aabb_ids=a, face_ids=f)
         return self._intersect_aabb_clip(aabb_min, aabb_max, mode)
 
-    def _intersect_aabb_clip(self, aabb_min, aabb_max, mode):
-        raise NotImplementedError("intersect_aabb modes 2/3 (polygon clip) are not built yet")
+    def _intersect_aabb_clip(self, aabb_min: torch.Tensor, aabb_max: torch.Tensor, mode: int,
+                             eps: float = 1e-8) -> AABBIntersectResult:
+        """Modes 2/3 (mesh_bvh.py:381-479): SAT pairs as candidates, then the
+        polygon clip; pairs whose clipped polygon is empty are dropped.  Ids
+        are int64 here, as in the reference's clip path."""
+        lo = aabb_min.contiguous().float()
+        hi = aabb_max.contiguous().float()
+        N = lo.shape[0]
+        dev = self.device
+        _, cand_a, cand_t = self._bvh.aabb_intersect(lo, hi)
+        if cand_a.numel() == 0:
+            return AABBIntersectResult(hit=torch.zeros(N, dtype=torch.bool, device=dev))
+        ca, ct = cand_a.long(), cand_t.long()
+        M = ca.shape[0]
+        clip_mode = 1 if mode == 2 else 2
+        hit = torch.empty(M, dtype=torch.uint8, device=dev)
+        counts = torch.empty(M, dtype=torch.int32, device=dev)
+        cen = torch.empty(M, 3, dtype=torch.float32, device=dev)
+        areas = torch.empty(M, dtype=torch.float32, device=dev)
+        poly = torch.empty(M, 8, 3, dtype=torch.float32, device=dev) if clip_mode == 2 else None
+        with torch.cuda.device(dev):
+            _lib.call("a3d_sat_clip", _lib.ptr(lo), _lib.ptr(hi), _lib.ptr(self._get_face_verts_flat()),
+                      _lib.ptr(ca), _lib.ptr(ct), M, clip_mode, float(eps), _lib.ptr(hit), _lib.ptr(counts),
+                      _lib.ptr(poly), _lib.ptr(cen), _lib.ptr(areas), _lib.stream(dev))
+        valid = hit.view(torch.bool)
+        box_hit = torch.zeros(N, dtype=torch.bool, device=dev)
+        box_hit[ca[valid]] = True
+        res = AABBIntersectResult(hit=box_hit, aabb_ids=ca[valid], face_ids=ct[valid],
+                                  centroids=cen[valid], areas=areas[valid], poly_counts=counts[valid])
+        if mode == 3:
+            res.poly_verts = poly[valid]
+        return res
 
     # ------------------------------------------------------------------ distances
     def udf(self, points: torch.Tensor, return_grad: bool = False, return_closest: bool = False,

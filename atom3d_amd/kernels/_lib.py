@@ -46,6 +46,7 @@ SIGNATURES: dict[str, list] = {
     "a3d_bvh_aabb_pairs": [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp],
     "a3d_pairs_sort_ws_bytes": [_i64, _pi64],
     "a3d_pairs_sort": [_vp, _vp, _i64, _vp, _i64, _vp],
+    "a3d_sat_clip": [_vp, _vp, _vp, _vp, _vp, _i64, _i32, _f32, _vp, _vp, _vp, _vp, _vp, _vp],
     "a3d_octree_ws_bytes": [_i64, _pi64],
     "a3d_octree_level": [_vp, ctypes.POINTER(ctypes.c_float), _i32, _vp, _i64, _vp, _vp, _vp, _i64, _vp],
     "a3d_octree_unpack": [_vp, _i64, _vp, _vp],

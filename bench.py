@@ -42,20 +42,21 @@ def parse():
     ap.add_argument("--queries", type=int, default=N_QUERIES)
     ap.add_argument("--cpu-sample", type=int, default=1 << 21, help="queries in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="run one plain device copy of this many MiB before timing (counter calibration)")
     return ap.parse_args()
 
 
 def traffic_from_profiles():
-    """HBM bytes per launch of the traversal kernel from the committed
-    rocprofv3 counter summary, if present."""
-    p = os.path.join(ROOT, "profiles", "udf_sphere_union_traffic.json")
+    """HBM bytes per launch of the traversal kernel, from the committed
+    rocprofv3 counter summary (line ``hbm_bytes_per_launch: N``), if present."""
+    p = os.path.join(ROOT, "profiles", "k_udf_wave_sphere_union_counters.md")
     if not os.path.exists(p):
         return None
-    try:
-        with open(p) as f:
-            return float(json.load(f)["hbm_bytes_per_launch"])
-    except Exception:
-        return None
+    import re
+
+    m = re.search(r"hbm_bytes_per_launch:\s*([0-9.eE+]+)", open(p).read())
+    return float(m.group(1)) if m else None
 
 
 def main():
@@ -85,6 +86,13 @@ def main():
     bvh = BVHAccelerator(Vd, Fd)  # second build: steady-state (allocator warm)
     torch.cuda.synchronize()
     build_s = min(build_s, time.perf_counter() - t0)
+
+    if a.copy_probe_mb:
+        src = torch.ones(a.copy_probe_mb << 18, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     K = P.shape[0]
     dist = torch.empty(K, device=dev)

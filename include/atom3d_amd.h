@@ -131,6 +131,20 @@ int a3d_pairs_sort_ws_bytes(int64_t n, int64_t* bytes);
 int a3d_pairs_sort(int32_t* pair_a, int32_t* pair_f, int64_t n, void* ws, int64_t ws_bytes,
                    a3d_stream_t stream);
 
+/* ------------------------------------------------------------------ polygon clip
+ * Replaces sat_clip_polygon (cumtv_kernels.cu:1035-1100 /
+ * sat_clip_polygon_kernel :193-460, bound at :1111), used by
+ * MeshBVH.intersect_aabb modes 2 and 3 (core/mesh_bvh.py:381-479).
+ * For each candidate k (box cand_a[k] of bmin/bmax float32[*,3], triangle
+ * cand_t[k] of tris9 float32[M,9]): hit uint8[N], counts int32[N] (clipped
+ * polygon vertices, <= 8), centroids float32[N,3], areas float32[N], and for
+ * mode 2 poly float32[N,8,3] (zero padded).  mode: 0 hit only, 1 + centroid
+ * and area, 2 + polygon.  Every output element is written.
+ */
+int a3d_sat_clip(const float* bmin, const float* bmax, const float* tris9, const int64_t* cand_a,
+                 const int64_t* cand_t, int64_t N, int mode, float eps, uint8_t* hit, int32_t* counts,
+                 float* poly, float* centroids, float* areas, a3d_stream_t stream);
+
 /* ------------------------------------------------------------------ octree
  * Replaces the level loop of OctreeIndexer.octree_traverse
  * (grid/octree_indexer.py:470-509) and its per-level boxes

@@ -37,6 +37,7 @@ _SIGS = {
     "orc_tri_closest_fp64": (None, [_d, _d, _d]),
     "orc_bvh_aabb": (_i64, [_f, _f, _f, _f, _i64, _u8, _i, _i, _i64, _int]),
     "orc_brute_aabb_hit": (None, [_f, _i, _i64, _f, _f, _i64, _u8, _vp, _int]),
+    "orc_sat_clip": (None, [_f, _f, _f, _l, _l, _i64, _int, ctypes.c_float, _u8, _i, _f, _f, _f]),
     "orc_octree_traverse": (_i64, [_f, _f, _f, _int, _int, _l, _i64, _vp, _vp, _int]),
 }
 
@@ -178,3 +179,20 @@ def brute_aabb_hit(V, F, qmin, qmax, threads: int = 0):
     lib().orc_brute_aabb_hit(V.reshape(-1), F.reshape(-1), F.shape[0], qmin.reshape(-1), qmax.reshape(-1), K, hit,
                              cnt.ctypes.data, threads)
     return hit.astype(bool), cnt
+
+
+def sat_clip(bmin, bmax, tris9, cand_a, cand_t, mode: int = 2, eps: float = 1e-8):
+    """Reference polygon clip per (box, triangle) candidate:
+    (hit [N] bool, counts [N], poly [N,8,3], centroids [N,3], areas [N])."""
+    bmin, bmax, tris9 = _f32(bmin).reshape(-1, 3), _f32(bmax).reshape(-1, 3), _f32(tris9).reshape(-1, 9)
+    ca = np.ascontiguousarray(cand_a, np.int64)
+    ct = np.ascontiguousarray(cand_t, np.int64)
+    N = len(ca)
+    hit = np.zeros(N, np.uint8)
+    cnt = np.zeros(N, np.int32)
+    poly = np.zeros((N, 8, 3), np.float32)
+    cen = np.zeros((N, 3), np.float32)
+    area = np.zeros(N, np.float32)
+    lib().orc_sat_clip(bmin.reshape(-1), bmax.reshape(-1), tris9.reshape(-1), ca, ct, N, mode, eps, hit, cnt,
+                       poly.reshape(-1), cen.reshape(-1), area)
+    return hit.astype(bool), cnt, poly, cen, area
