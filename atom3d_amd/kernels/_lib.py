@@ -50,6 +50,13 @@ SIGNATURES: dict[str, list] = {
     "a3d_octree_ws_bytes": [_i64, _pi64],
     "a3d_octree_level": [_vp, ctypes.POINTER(ctypes.c_float), _i32, _vp, _i64, _vp, _vp, _vp, _i64, _vp],
     "a3d_octree_unpack": [_vp, _i64, _vp, _vp],
+    "a3d_flood_sizes": [_i32, _i32, _i32, _pi64, _pi64],
+    "a3d_flood_pack_dense": [_vp, _i32, _i32, _i32, _vp, _vp],
+    "a3d_flood_fill": [_vp, _i64, _pi64, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _i64, _pi64,
+                       _pi64, _vp],
+    "a3d_flood_extract": [_vp, _i32, _i32, _i32, _i32, _pi64, _vp, _vp, _i64, _vp],
+    "a3d_flood_dense_mask": [_vp, _i32, _i32, _i32, _vp, _vp],
+    "a3d_flood_dam_mask": [_vp, _i32, _i32, _i32, _vp, _i64, _pi64, _vp, _vp],
 }
 
 

@@ -164,6 +164,41 @@ int a3d_octree_level(const void* blob, const float* bounds_host, int level, cons
                      a3d_stream_t stream);
 int a3d_octree_unpack(const uint64_t* keys, int64_t n, int64_t* ijk, a3d_stream_t stream);
 
+/* ------------------------------------------------------------------ flood fill
+ * Replaces flood_fill (flood_fill_kernels.cu:117-175, kernel :21-87, bound
+ * at :178) and the dense post-processing of flood_fill_3d_sparse
+ * (kernels/flood_fill.py:118-258) and get_dam_neighborhood_signs
+ * (apps/sparse_flood_fill.py:34-100).  The crop of D x H x W voxels (axis
+ * order i, j, k) is held as A3D_FLOOD_NPLANES bit planes of plane_words
+ * uint32 each (caller-owned `state`):
+ *   OCC 0, REACHED 1, WATER 2 (mask 2), COLLISION 3 (mask 1), DAM 4 (mask -1
+ *   incl. the extended dam), DRY 5 (mask -2), EXTENDED 6, NEIGHBORHOOD 7
+ *   (k-ring 26-dilation of DAM), EXT_TIDE 8 (= NEIGHBORHOOD & (WATER |
+ *   COLLISION)), DRY_ADJ 9 (= NEIGHBORHOOD & DRY), scratch 10.
+ * a3d_flood_fill: occupancy from dam_ijk int64[N,3] (global coordinates,
+ *   crop origin lo_host int64[3] on the host), or, with dam_ijk == NULL, from
+ *   a prior a3d_flood_pack_dense; start (si, sj, sk) local; conn 6/18/26.
+ *   Synchronises the stream.  totals_host[NPLANES] = set bits per plane;
+ *   passes_host = global passes run, or -1 when the start voxel is occupied
+ *   (every plane then empty, as the reference returns empty results).
+ * a3d_flood_extract: coordinates (lo + local ijk) of one plane's set bits,
+ *   lexicographic order (= torch.nonzero), int64[totals[plane], 3].
+ * a3d_flood_dense_mask: the reference's int32 mask (-2/-1/1/2, dam without
+ *   the extended dam) as a dense D x H x W array.
+ * a3d_flood_dam_mask: uint8[N] = DAM bit at each dam_ijk row.
+ */
+#define A3D_FLOOD_NPLANES 11
+int a3d_flood_sizes(int D, int H, int W, int64_t* plane_words, int64_t* ws_bytes);
+int a3d_flood_pack_dense(const uint8_t* occ, int D, int H, int W, uint32_t* state, a3d_stream_t stream);
+int a3d_flood_fill(const int64_t* dam_ijk, int64_t N, const int64_t* lo_host, int D, int H, int W,
+                   int si, int sj, int sk, int conn, int k_ring, uint32_t* state, void* ws,
+                   int64_t ws_bytes, int64_t* totals_host, int64_t* passes_host, a3d_stream_t stream);
+int a3d_flood_extract(const uint32_t* state, int D, int H, int W, int plane, const int64_t* lo_host,
+                      int64_t* out_ijk, void* ws, int64_t ws_bytes, a3d_stream_t stream);
+int a3d_flood_dense_mask(const uint32_t* state, int D, int H, int W, int32_t* mask, a3d_stream_t stream);
+int a3d_flood_dam_mask(const uint32_t* state, int D, int H, int W, const int64_t* dam_ijk, int64_t N,
+                       const int64_t* lo_host, uint8_t* out, a3d_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif
