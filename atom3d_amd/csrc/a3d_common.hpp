@@ -80,11 +80,12 @@ struct Node64 {
 };
 static_assert(sizeof(Node64) == 64, "node is 64 bytes");
 
-// Triangle in leaf order: a, b, c and the original face id (bit-cast).
+// Triangle in leaf order: a, b, c, the original face id (bit-cast) and the
+// triangle's distance-error pad (see bvh_build.hip, tri_pad).
 struct Tri48 {
   float4 p;  // a.x a.y a.z b.x
   float4 q;  // b.y b.z c.x c.y
-  float4 r;  // c.z id 0 0
+  float4 r;  // c.z id pad 0
 };
 static_assert(sizeof(Tri48) == 48, "triangle is 48 bytes");
 
@@ -96,6 +97,8 @@ __device__ __forceinline__ void load_tri(const Tri48* __restrict__ t, int32_t i,
   c = mk(q.z, q.w, r.x);
   id = __float_as_int(r.y);
 }
+
+__device__ __forceinline__ float tri_pad_of(const Tri48* __restrict__ t, int32_t i) { return t[i].r.z; }
 
 __host__ __device__ __forceinline__ const BvhHeader* header(const void* blob) {
   return reinterpret_cast<const BvhHeader*>(blob);
@@ -178,6 +181,16 @@ __device__ __forceinline__ float box_d2(v3 p, float lx, float ly, float lz, floa
   float dy = fmaxf(fmaxf(ly - p.y, 0.0f), p.y - hy);
   float dz = fmaxf(fmaxf(lz - p.z, 0.0f), p.z - hz);
   return dx * dx + dy * dy + dz * dz;
+}
+
+// Lower bound used to skip exact triangle tests: squared distance from p to
+// the triangle's box widened by its error pad.  Whenever this exceeds a
+// lane's pruning threshold the reference formula cannot rank the triangle at
+// or below the lane's best (same argument as for node boxes).
+__device__ __forceinline__ float tri_box_d2(v3 a, v3 b, v3 c, float pad, v3 p) {
+  return box_d2(p, fminf(fminf(a.x, b.x), c.x) - pad, fminf(fminf(a.y, b.y), c.y) - pad,
+                fminf(fminf(a.z, b.z), c.z) - pad, fmaxf(fmaxf(a.x, b.x), c.x) + pad,
+                fmaxf(fmaxf(a.y, b.y), c.y) + pad, fmaxf(fmaxf(a.z, b.z), c.z) + pad);
 }
 
 // Reference aabb_overlap (bvh_kernels.cu:241-245), inclusive.

@@ -355,7 +355,7 @@ __global__ __launch_bounds__(kBlk) void k_emit_tris(const float* __restrict__ V,
   Tri48 t;
   t.p = make_float4(a.x, a.y, a.z, b.x);
   t.q = make_float4(b.y, b.z, c.x, c.y);
-  t.r = make_float4(c.z, __int_as_float((int32_t)f), 0.0f, 0.0f);
+  t.r = make_float4(c.z, __int_as_float((int32_t)f), tri_pad(a, b, c), 0.0f);
   out[j] = t;
 }
 
@@ -373,7 +373,7 @@ __global__ void k_small_root(const float* __restrict__ V, const int32_t* __restr
     Tri48 t;
     t.p = make_float4(a.x, a.y, a.z, bb.x);
     t.q = make_float4(bb.y, bb.z, c.x, c.y);
-    t.r = make_float4(c.z, __int_as_float((int32_t)j), 0.0f, 0.0f);
+    t.r = make_float4(c.z, __int_as_float((int32_t)j), tri_pad(a, bb, c), 0.0f);
     tris[j] = t;
   }
   const float inf = __int_as_float(0x7f800000);

@@ -65,10 +65,12 @@ __device__ __forceinline__ void write_result(int64_t q, v3 p, float best, int32_
 }
 
 // ------------------------------------------------------------ wave-coherent walk
+template <bool STATS>
 __global__ __launch_bounds__(kBlk) void k_udf_wave(const void* __restrict__ blob, const float* __restrict__ P,
                                                     const uint32_t* __restrict__ perm, int64_t K,
                                                     float* __restrict__ dist, int32_t* __restrict__ face,
-                                                    float* __restrict__ closest, float* __restrict__ uvw) {
+                                                    float* __restrict__ closest, float* __restrict__ uvw,
+                                                    unsigned long long* __restrict__ counters) {
   __shared__ float4 s_stk[kWaves][kStackTotal][2];  // (lo.xyz, ref bits), (hi.xyz, -)
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
@@ -87,8 +89,10 @@ __global__ __launch_bounds__(kBlk) void k_udf_wave(const void* __restrict__ blob
   float thr = valid ? prune_thr(best, slack) : -1.0f;  // invalid lanes accept nothing
   int sp = 0;
   int32_t cur = 0;
+  uint32_t n_node = 0, n_leaf_tri = 0, n_test = 0;
   while (true) {
     if (cur >= 0) {
+      if (STATS) ++n_node;
       const Node64 nd = nodes[cur];
       float dl = box_d2(p, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y);
       float dr = box_d2(p, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w);
@@ -123,6 +127,9 @@ __global__ __launch_bounds__(kBlk) void k_udf_wave(const void* __restrict__ blob
         v3 ta, tb, tc;
         int32_t id;
         load_tri(tris, t, ta, tb, tc, id);
+        if (STATS) ++n_leaf_tri;
+        if (!__ballot(tri_box_d2(ta, tb, tc, tri_pad_of(tris, t), p) <= thr)) continue;
+        if (STATS) ++n_test;
         float d2 = tri_dist2(ta, tb, tc, p);
         if (d2 < best || (d2 == best && id < best_id)) {
           best = d2;
@@ -144,6 +151,12 @@ __global__ __launch_bounds__(kBlk) void k_udf_wave(const void* __restrict__ blob
       }
     }
     if (!found) break;
+  }
+  if (STATS && lane == 0) {
+    atomicAdd(&counters[0], 1ull);
+    atomicAdd(&counters[1], (unsigned long long)n_node);
+    atomicAdd(&counters[2], (unsigned long long)n_leaf_tri);
+    atomicAdd(&counters[3], (unsigned long long)n_test);
   }
   if (valid) write_result(q, p, best, best_id, best_tri, tris, dist, face, closest, uvw);
 }
@@ -203,6 +216,7 @@ __global__ __launch_bounds__(kBlk) void k_udf_lane(const void* __restrict__ blob
         v3 ta, tb, tc;
         int32_t id;
         load_tri(tris, t, ta, tb, tc, id);
+        if (tri_box_d2(ta, tb, tc, tri_pad_of(tris, t), p) > thr) continue;
         float d2 = tri_dist2(ta, tb, tc, p);
         if (d2 < best || (d2 == best && id < best_id)) {
           best = d2;
@@ -364,18 +378,25 @@ extern "C" int a3d_query_order(const float* P, int64_t K, uint32_t* perm, void* 
 }
 
 extern "C" int a3d_bvh_udf(const void* blob, const float* P, int64_t K, const uint32_t* perm, float* dist,
-                           int32_t* face, float* closest, float* uvw, int flags, a3d_stream_t stream) {
+                           int32_t* face, float* closest, float* uvw, int flags, int64_t* counters,
+                           a3d_stream_t stream) {
   if (!blob) return fail(A3D_ERR_ARG, "a3d_bvh_udf: null BVH blob");
   if (K < 0 || K > (int64_t(1) << 40)) return fail(A3D_ERR_ARG, "a3d_bvh_udf: K=%lld out of range", (long long)K);
   if (perm && K > 0xffffffffll) return fail(A3D_ERR_ARG, "a3d_bvh_udf: perm needs K < 2^32");
   if (flags & ~A3D_UDF_LANE) return fail(A3D_ERR_ARG, "a3d_bvh_udf: unknown flags 0x%x", flags);
+  if (counters && (flags & A3D_UDF_LANE)) return fail(A3D_ERR_ARG, "a3d_bvh_udf: counters need the wave walk");
   if (K == 0) return A3D_OK;
   if (!P || !dist || !face || !closest) return fail(A3D_ERR_ARG, "a3d_bvh_udf: null input/output");
   hipStream_t s = (hipStream_t)stream;
+  unsigned long long* c = (unsigned long long*)counters;
   if (flags & A3D_UDF_LANE)
     hipLaunchKernelGGL(k_udf_lane, dim3(grid_for(K)), dim3(kBlk), 0, s, blob, P, perm, K, dist, face, closest, uvw);
+  else if (counters)
+    hipLaunchKernelGGL(k_udf_wave<true>, dim3(grid_for(K)), dim3(kBlk), 0, s, blob, P, perm, K, dist, face, closest,
+                       uvw, c);
   else
-    hipLaunchKernelGGL(k_udf_wave, dim3(grid_for(K)), dim3(kBlk), 0, s, blob, P, perm, K, dist, face, closest, uvw);
+    hipLaunchKernelGGL(k_udf_wave<false>, dim3(grid_for(K)), dim3(kBlk), 0, s, blob, P, perm, K, dist, face, closest,
+                       uvw, c);
   A3D_LAUNCH_CHECK();
   return A3D_OK;
 }

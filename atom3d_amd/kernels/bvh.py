@@ -125,11 +125,13 @@ class BVHAccelerator:
                       _lib.stream(self.device))
         return perm
 
-    def udf(self, points: torch.Tensor, flags: int = 0, order: bool | None = None):
+    def udf(self, points: torch.Tensor, flags: int = 0, order: bool | None = None,
+            counters: torch.Tensor | None = None):
         """Closest point on the mesh: (distances [K], face_ids [K] int32,
         closest_points [K,3], uvw [K,3]).  ``order`` (default: K >= 4096)
         walks the queries in Morton order; ``flags`` (UDF_LANE) picks the
-        traversal variant.  Results are identical either way."""
+        traversal variant; ``counters`` (int64[4] on the device) accumulates
+        the wave walk's work counts.  Results are identical either way."""
         p = self._points(points)
         K = p.shape[0]
         dist = torch.empty(K, dtype=torch.float32, device=self.device)
@@ -142,7 +144,7 @@ class BVHAccelerator:
         with torch.cuda.device(self.device):
             _lib.call(
                 "a3d_bvh_udf", _lib.ptr(self.blob), _lib.ptr(p), K, _lib.ptr(perm), _lib.ptr(dist), _lib.ptr(face),
-                _lib.ptr(closest), _lib.ptr(uvw), int(flags), _lib.stream(self.device),
+                _lib.ptr(closest), _lib.ptr(uvw), int(flags), _lib.ptr(counters), _lib.stream(self.device),
             )
         return dist, face, closest, uvw
 

@@ -90,7 +90,7 @@ def main():
     if a.copy_probe_mb:
         src = torch.ones(a.copy_probe_mb << 18, device=dev)
         dst = torch.empty_like(src)
-        dst.copy_(src)
+        torch.mul(src, 2.0, out=dst)  # one elementwise kernel: reads and writes copy_probe_mb MiB
         torch.cuda.synchronize()
         del src, dst
 
@@ -108,7 +108,7 @@ def main():
     lib = _lib.lib()
     order_args = (_lib.ptr(P), K, _lib.ptr(perm), _lib.ptr(ws), wsb.value, sp)
     udf_args = (_lib.ptr(bvh.blob), _lib.ptr(P), K, _lib.ptr(perm), _lib.ptr(dist), _lib.ptr(face),
-                _lib.ptr(closest), _lib.ptr(uvw), 0, sp)
+                _lib.ptr(closest), _lib.ptr(uvw), 0, None, sp)
 
     # one step = Morton ordering of the queries + the wave-coherent walk
     def step(ev=None):

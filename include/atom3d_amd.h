@@ -71,7 +71,9 @@ int a3d_bvh_info(const void* blob, int64_t* out4, a3d_stream_t stream);
  * perm (nullable, uint32[K]): process the queries in this order (a
  * permutation of 0..K-1, e.g. from a3d_query_order); results are still
  * written at each query's own index.  flags: A3D_UDF_LANE = one independent
- * walk per lane instead of one shared walk per wavefront.
+ * walk per lane instead of one shared walk per wavefront.  counters (nullable,
+ * device int64[4], accumulated, wave walk only): wavefronts, node visits,
+ * leaf triangles seen and exact triangle tests run, summed over wavefronts.
  *
  * a3d_query_order: perm = the queries sorted along a 30-bit Morton curve of
  * their bounding box (stable), so each wavefront covers a compact region.
@@ -82,7 +84,8 @@ int a3d_query_order_ws_bytes(int64_t K, int64_t* bytes);
 int a3d_query_order(const float* P, int64_t K, uint32_t* perm, void* ws, int64_t ws_bytes,
                     a3d_stream_t stream);
 int a3d_bvh_udf(const void* blob, const float* P, int64_t K, const uint32_t* perm, float* dist,
-                int32_t* face, float* closest, float* uvw, int flags, a3d_stream_t stream);
+                int32_t* face, float* closest, float* uvw, int flags, int64_t* counters,
+                a3d_stream_t stream);
 
 /* ------------------------------------------------------------------ UDF adjoint
  * Replaces the autograd re-evaluation in MeshBVH._udf_with_grad
