@@ -89,7 +89,7 @@ __global__ __launch_bounds__(kBlk) void k_udf_wave(const void* __restrict__ blob
   float thr = valid ? prune_thr(best, slack) : -1.0f;  // invalid lanes accept nothing
   int sp = 0;
   int32_t cur = 0;
-  uint32_t n_node = 0, n_leaf_tri = 0, n_test = 0;
+  uint32_t n_node = 0, n_leaf_tri = 0, n_test = 0, n_lane = 0;
   while (true) {
     if (cur >= 0) {
       if (STATS) ++n_node;
@@ -128,7 +128,9 @@ __global__ __launch_bounds__(kBlk) void k_udf_wave(const void* __restrict__ blob
         int32_t id;
         load_tri(tris, t, ta, tb, tc, id);
         if (STATS) ++n_leaf_tri;
-        if (!__ballot(tri_box_d2(ta, tb, tc, tri_pad_of(tris, t), p) <= thr)) continue;
+        bool need = tri_box_d2(ta, tb, tc, tri_pad_of(tris, t), p) <= thr;
+        if (STATS) n_lane += need ? 1u : 0u;
+        if (!__ballot(need)) continue;
         if (STATS) ++n_test;
         float d2 = tri_dist2(ta, tb, tc, p);
         if (d2 < best || (d2 == best && id < best_id)) {
@@ -152,11 +154,16 @@ __global__ __launch_bounds__(kBlk) void k_udf_wave(const void* __restrict__ blob
     }
     if (!found) break;
   }
-  if (STATS && lane == 0) {
-    atomicAdd(&counters[0], 1ull);
-    atomicAdd(&counters[1], (unsigned long long)n_node);
-    atomicAdd(&counters[2], (unsigned long long)n_leaf_tri);
-    atomicAdd(&counters[3], (unsigned long long)n_test);
+  if (STATS) {
+    if (!valid) n_lane = 0;
+    for (int o = 32; o > 0; o >>= 1) n_lane += __shfl_xor(n_lane, o);
+    if (lane == 0) {
+      atomicAdd(&counters[0], 1ull);
+      atomicAdd(&counters[1], (unsigned long long)n_node);
+      atomicAdd(&counters[2], (unsigned long long)n_leaf_tri);
+      atomicAdd(&counters[3], (unsigned long long)n_test);
+      atomicAdd(&counters[4], (unsigned long long)n_lane);
+    }
   }
   if (valid) write_result(q, p, best, best_id, best_tri, tris, dist, face, closest, uvw);
 }

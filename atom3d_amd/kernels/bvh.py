@@ -130,7 +130,7 @@ class BVHAccelerator:
         """Closest point on the mesh: (distances [K], face_ids [K] int32,
         closest_points [K,3], uvw [K,3]).  ``order`` (default: K >= 4096)
         walks the queries in Morton order; ``flags`` (UDF_LANE) picks the
-        traversal variant; ``counters`` (int64[4] on the device) accumulates
+        traversal variant; ``counters`` (int64[5] on the device) accumulates
         the wave walk's work counts.  Results are identical either way."""
         p = self._points(points)
         K = p.shape[0]

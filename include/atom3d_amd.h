@@ -72,8 +72,9 @@ int a3d_bvh_info(const void* blob, int64_t* out4, a3d_stream_t stream);
  * permutation of 0..K-1, e.g. from a3d_query_order); results are still
  * written at each query's own index.  flags: A3D_UDF_LANE = one independent
  * walk per lane instead of one shared walk per wavefront.  counters (nullable,
- * device int64[4], accumulated, wave walk only): wavefronts, node visits,
- * leaf triangles seen and exact triangle tests run, summed over wavefronts.
+ * device int64[5], accumulated, wave walk only): wavefronts, node visits,
+ * leaf triangles seen, exact triangle tests run (all summed over wavefronts)
+ * and triangles whose bound admitted them per lane (summed over lanes).
  *
  * a3d_query_order: perm = the queries sorted along a 30-bit Morton curve of
  * their bounding box (stable), so each wavefront covers a compact region.
