@@ -92,3 +92,50 @@ def sphere_union(n_spheres: int = 200, subdivisions: int = 5, seed: int = 1, r_m
 def uniform_points(n: int, lo: float = -1.0, hi: float = 1.0, seed: int = 0) -> torch.Tensor:
     rng = np.random.default_rng([seed, 1])
     return torch.from_numpy(rng.uniform(lo, hi, size=(n, 3)).astype(np.float32))
+
+
+# ------------------------------------------------------------------ SDF shapes
+def sdf_shapes(n_spheres: int = 32, n_tori: int = 16, seed: int = 2) -> dict:
+    """Random spheres (r 0.05-0.3) and tori (R 0.1-0.3, r 0.02-0.08, axis x,
+    y or z) with centres in [-0.7, 0.7]^3: the sparse-DMC workload."""
+    rng = np.random.default_rng(seed)
+    return {
+        "sphere_c": rng.uniform(-0.7, 0.7, (n_spheres, 3)), "sphere_r": rng.uniform(0.05, 0.3, n_spheres),
+        "torus_c": rng.uniform(-0.7, 0.7, (n_tori, 3)), "torus_R": rng.uniform(0.1, 0.3, n_tori),
+        "torus_r": rng.uniform(0.02, 0.08, n_tori), "torus_axis": rng.integers(0, 3, n_tori),
+    }
+
+
+# two spheres and a torus: the small DMC test shape
+SMALL_SHAPES = {
+    "sphere_c": np.array([[-0.35, 0.0, 0.0], [0.4, 0.1, 0.05]]), "sphere_r": np.array([0.4, 0.3]),
+    "torus_c": np.array([[0.0, -0.3, 0.35]]), "torus_R": np.array([0.3]), "torus_r": np.array([0.1]),
+    "torus_axis": np.array([2]),
+}
+
+
+def shapes_sdf(P, shapes: dict | None = None):
+    """Signed distance of the union (min) of the shapes at points P [K,3]
+    (numpy or torch; torch stays on P's device and dtype)."""
+    sh = SMALL_SHAPES if shapes is None else shapes
+    tn = isinstance(P, torch.Tensor)
+    if tn:
+        T = {k: torch.as_tensor(v, device=P.device, dtype=P.dtype if k != "torus_axis" else torch.long)
+             for k, v in sh.items()}
+        norm = lambda x: torch.linalg.vector_norm(x, dim=-1)  # noqa: E731
+        out = torch.full(P.shape[:1], float("inf"), device=P.device, dtype=P.dtype)
+        mn = torch.minimum
+    else:
+        T = sh
+        norm = lambda x: np.linalg.norm(x, axis=-1)  # noqa: E731
+        out = np.full(P.shape[:1], np.inf)
+        mn = np.minimum
+    for c, r in zip(T["sphere_c"], T["sphere_r"]):
+        out = mn(out, norm(P - c) - r)
+    for c, R, r, ax in zip(T["torus_c"], T["torus_R"], T["torus_r"], T["torus_axis"]):
+        d = P - c
+        a = int(ax)
+        u, v = [k for k in range(3) if k != a]
+        q = ((d[:, u] ** 2 + d[:, v] ** 2) ** 0.5) - R
+        out = mn(out, (q ** 2 + d[:, a] ** 2) ** 0.5 - r)
+    return out

@@ -28,6 +28,7 @@ _vp = ctypes.c_void_p
 _i64 = ctypes.c_int64
 _i32 = ctypes.c_int
 _f32 = ctypes.c_float
+_f64 = ctypes.c_double
 _pi64 = ctypes.POINTER(ctypes.c_int64)
 
 # name -> argtypes.  Every entry point returns int (0 = OK).
@@ -57,6 +58,14 @@ SIGNATURES: dict[str, list] = {
     "a3d_flood_extract": [_vp, _i32, _i32, _i32, _i32, _pi64, _vp, _vp, _i64, _vp],
     "a3d_flood_dense_mask": [_vp, _i32, _i32, _i32, _vp, _vp],
     "a3d_flood_dam_mask": [_vp, _i32, _i32, _i32, _vp, _i64, _pi64, _vp, _vp],
+    "a3d_dmc_ws_bytes": [_i64, _pi64],
+    "a3d_dmc_topology": [_vp, _vp, _i32, _f64, _vp, _i64, _i64, _i32, _vp, _vp, _i64, _pi64, _vp],
+    "a3d_dmc_faces": [_vp, _i64, _pi64, _vp, _vp],
+    "a3d_dmc_forward": [_vp, _i64, _pi64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i32, _f64, _f64, _i32, _vp,
+                        _vp, _vp, _vp, _vp, _vp],
+    "a3d_dmc_backward": [_vp, _i64, _i64, _pi64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _i32, _f64, _f64,
+                         _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "a3d_dmc_topology_get": [_vp, _i64, _pi64, _i32, _vp, _vp],
 }
 
 

@@ -203,6 +203,52 @@ int a3d_flood_dense_mask(const uint32_t* state, int D, int H, int W, int32_t* ma
 int a3d_flood_dam_mask(const uint32_t* state, int D, int H, int W, const int64_t* dam_ijk, int64_t N,
                        const int64_t* lo_host, uint8_t* out, a3d_stream_t stream);
 
+/* ------------------------------------------------------------------ DMC
+ * Replaces SparseDiffDMC.forward in weighted_avg (training) mode and its
+ * autograd backward (mesh_extractor/sparse_diffdmc.py:76-232 forward,
+ * :234-261 weights and L_dev, :516-599 surface cubes / case ids / surface
+ * edges, :601-780 dual vertices, :782-826 triangulation, :859-871
+ * interpolation).  Inputs: vox int32[N,3] cube coordinates, ci int32[N,8]
+ * corner ids (CUBE_CORNERS order) into sdf[M]; values float32 (dtype 0) or
+ * float64 (dtype 1); optional deform [M,3], alpha [N,8], beta [N,12] (raw,
+ * normalised inside as tanh(x) * weight_scale + 1), features [M,C].
+ * a3d_dmc_ws_bytes: workspace for N cubes; holds the topology between calls.
+ * a3d_dmc_topology: case ids (with the check_table flip), dual-vertex
+ *   numbering, surface edges and quads, all integer and bit-exact with the
+ *   reference.  tables_host = int8 dmc_table[256*4*7] | num_vd[256] |
+ *   check_table[256*5].  Synchronises the stream; sizes_host[5] = surface
+ *   cubes, dual vertices Vd, edge-group entries E, surface edges, quads Q.
+ * a3d_dmc_faces: int64[4Q,3] training-mode faces (4 triangles per quad
+ *   around a centre vertex Vd + q).
+ * a3d_dmc_forward: verts [Vd+Q,3], L_dev [E], p_alpha [E,3], n_alpha [E,3],
+ *   optional vfeat [Vd+Q,C].
+ * a3d_dmc_backward: gradients of those outputs (each may be NULL = zero) to
+ *   g_sdf [M] and, where non-NULL, g_deform [M,3], g_alpha [N,8], g_beta
+ *   [N,12], g_feat [M,C] (all overwritten).  The interpolation weights and
+ *   dual vertices are recomputed, not stored.  gvf_scratch: [Vd,C] values,
+ *   needed with g_vfeat.
+ * a3d_dmc_topology_get: int32 copies of the topology (kind 0 surface cube
+ *   ids, 1 case ids, 2 entry -> dual vertex, 3 surface edges [.,2], 4 quads
+ *   [.,4]).
+ */
+int a3d_dmc_ws_bytes(int64_t N, int64_t* bytes);
+int a3d_dmc_topology(const int32_t* vox, const void* sdf, int dtype, double isovalue, const int32_t* ci, int64_t N,
+                     int64_t M, int res, const int8_t* tables_host, void* ws, int64_t ws_bytes, int64_t* sizes_host,
+                     a3d_stream_t stream);
+int a3d_dmc_faces(const void* ws, int64_t N, const int64_t* sizes, int64_t* faces, a3d_stream_t stream);
+int a3d_dmc_forward(const void* ws, int64_t N, const int64_t* sizes, const int32_t* vox, const int32_t* ci,
+                    const void* sdf, const void* deform, const void* alpha, const void* beta, const void* feat, int C,
+                    int dtype, double isovalue, double weight_scale, int res, void* verts, void* L_dev,
+                    void* p_alpha, void* n_alpha, void* vfeat, a3d_stream_t stream);
+int a3d_dmc_backward(const void* ws, int64_t N, int64_t M, const int64_t* sizes, const int32_t* vox,
+                     const int32_t* ci, const void* sdf, const void* deform, const void* alpha, const void* beta,
+                     const void* feat, int C, const void* vfeat, int dtype, double isovalue, double weight_scale,
+                     int res, const void* g_verts, const void* g_L_dev, const void* g_p_alpha, const void* g_n_alpha,
+                     const void* g_vfeat, void* gvf_scratch, void* g_sdf, void* g_deform, void* g_alpha,
+                     void* g_beta, void* g_feat, a3d_stream_t stream);
+int a3d_dmc_topology_get(const void* ws, int64_t N, const int64_t* sizes, int kind, int32_t* out,
+                         a3d_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

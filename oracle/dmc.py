@@ -1,0 +1,177 @@
+"""Oracle (test infrastructure only): SparseDiffDMC's training-mode forward
+restated with CPU torch ops, differentiable by autograd.
+
+Follows Atom3D atom3d/mesh_extractor/sparse_diffdmc.py:
+  corner positions and world transform     :151-170
+  surface cubes                            :516-521
+  weight normalisation                     :234-253
+  case ids and the neighbour check         :524-579
+  surface edges (lexicographic unique)     :582-599
+  dual vertices, weighted average, L_dev   :601-780 (weighted_avg branch), :255-260
+  triangulation, training 4-triangle split :782-826
+  linear interpolation                     :859-871
+Summation orders follow a sequential (CPU) evaluation.  Tables come from
+atom3d_amd.mesh_extractor.tables, which tests/test_dmc_tables.py pins to the
+reference's values.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from atom3d_amd.mesh_extractor.tables import EDGE_CORNERS, check_table, dmc_table, num_vd_table
+
+_EDGE_FLAT = torch.tensor([c for e in EDGE_CORNERS for c in e], dtype=torch.long)
+_CORNERS = torch.tensor([[(i >> 0) & 1, (i >> 1) & 1, (i >> 2) & 1] for i in range(8)], dtype=torch.long)
+
+
+def _interp(w, x):
+    """Zero crossing of a linear function along an edge: weights [.,2,1],
+    positions [.,2,C] -> (x0 w1 - x1 w0) / (w1 - w0)."""
+    ww = torch.cat([w[:, 1:2], -w[:, 0:1]], dim=1)
+    return (x * ww).sum(1) / ww.sum(1)
+
+
+def dmc_forward(voxel_coords, sdf, cube_idx, resolution, deform=None, beta=None, alpha=None, gamma=None,
+                isovalue=0.0, weight_scale=0.99, dtype=torch.float64, v_features=None, training=True):
+    """dict(vertices, faces, L_dev, p_alpha, n_alpha, v_features); inputs may
+    require grad.  training=False gives the eval-mode 2-triangle split
+    (sparse_diffdmc.py:802-810) of the weighted-average vertices."""
+    dev = torch.device("cpu")
+    vc = voxel_coords.to(dev).long()
+    ci = cube_idx.to(dev).long()
+    s = sdf.to(dev, dtype)
+    M, N = s.shape[0], vc.shape[0]
+    pos = torch.zeros(M, 3, dtype=dtype)
+    pos[ci.reshape(-1)] = (vc[:, None, :] + _CORNERS[None]).reshape(-1, 3).to(dtype)
+    world = pos * (2.0 / resolution) - 1.0
+    if deform is not None:
+        world = world + deform.to(dev, dtype)
+    f = s - isovalue
+    occ = (f < 0)[ci]
+    osum = occ.sum(1)
+    surf = (osum > 0) & (osum < 8)
+    if int(surf.sum()) == 0:
+        e3 = torch.zeros(0, 3, dtype=dtype)
+        return dict(vertices=e3, faces=torch.zeros(0, 3, dtype=torch.long), L_dev=torch.zeros(0, dtype=dtype),
+                    p_alpha=e3, n_alpha=e3, v_features=None)
+    b = torch.tanh(beta.to(dev, dtype)) * weight_scale + 1 if beta is not None else torch.ones(N, 12, dtype=dtype)
+    a = torch.tanh(alpha.to(dev, dtype)) * weight_scale + 1 if alpha is not None else torch.ones(N, 8, dtype=dtype)
+    b, a = b[surf], a[surf]
+    # case ids + neighbour check
+    case = (occ[surf].long() * (2 ** torch.arange(8))).sum(1)
+    chk = torch.as_tensor(check_table)[case]
+    prob = chk[:, 0] == 1
+    if bool(prob.any()):
+        pc = vc[surf][prob]
+        nb = pc + chk[prob][:, 1:4]
+        inr = ((nb >= 0) & (nb < resolution)).all(1)
+        R = int(resolution)
+        keys = set(((pc[:, 0] * R + pc[:, 1]) * R + pc[:, 2]).tolist())
+        nbk = ((nb[:, 0] * R + nb[:, 1]) * R + nb[:, 2]).tolist()
+        flip = torch.tensor([bool(r) and k in keys for r, k in zip(inr.tolist(), nbk)], dtype=torch.bool)
+        idx = torch.nonzero(prob).squeeze(1)[flip]
+        case[idx] = chk[idx, 4]
+    # surface edges: lexicographically unique corner pairs with a sign change
+    cs = ci[surf]
+    Ns = cs.shape[0]
+    all_e = cs[:, _EDGE_FLAT].reshape(-1, 2)
+    ue_keys, inv, counts = torch.unique(all_e, dim=0, return_inverse=True, return_counts=True)
+    neg = f < 0
+    hot = neg[ue_keys[:, 0]] != neg[ue_keys[:, 1]]
+    slot_hot = hot[inv]
+    slot_cnt = counts[inv]
+    remap = torch.full((ue_keys.shape[0],), -1, dtype=torch.long)
+    remap[hot] = torch.arange(int(hot.sum()))
+    idx_map = remap[inv].reshape(Ns, 12)
+    sedge = ue_keys[hot]
+    ex = world[sedge.reshape(-1)].reshape(-1, 2, 3)
+    es = f[sedge.reshape(-1)].reshape(-1, 2, 1)
+    zc = _interp(es, ex)
+    # edge-direction normals
+    ev = ex[:, 1] - ex[:, 0]
+    el = ev.norm(dim=-1, keepdim=True) + 1e-8
+    nrm = (ev / el) * torch.sign((es[:, 1] - es[:, 0]) / el)
+    nrm = nrm / nrm.norm(dim=-1, keepdim=True).clamp(min=1e-12)
+    # dual vertices grouped by ascending num_vd, cube order, table order
+    nvd = torch.as_tensor(num_vd_table)[case]
+    tab = torch.as_tensor(dmc_table)
+    ent_cube, ent_edge, ent_vd, vd_cnt = [], [], [], []
+    nv = 0
+    vd_of_slot = torch.zeros(Ns, 12, dtype=torch.long)
+    for num in sorted(set(nvd.tolist())):
+        for cube in torch.nonzero(nvd == num).squeeze(1).tolist():
+            for k in range(num):
+                row = [int(x) for x in tab[case[cube], k] if x >= 0]
+                for e in row:
+                    ent_cube.append(cube)
+                    ent_edge.append(e)
+                    ent_vd.append(nv)
+                    vd_of_slot[cube, e] = nv
+                vd_cnt.append(len(row))
+                nv += 1
+    ent_cube = torch.tensor(ent_cube, dtype=torch.long)
+    ent_edge = torch.tensor(ent_edge, dtype=torch.long)
+    ent_vd = torch.tensor(ent_vd, dtype=torch.long)
+    vd_cnt = torch.tensor(vd_cnt, dtype=torch.long)
+    g = idx_map[ent_cube, ent_edge]
+    zc_g, s_g, x_g, n_g = zc[g], es[g], ex[g], nrm[g]
+    ends = _EDGE_FLAT.reshape(12, 2)[ent_edge]
+    a_g = torch.stack([a[ent_cube, ends[:, 0]], a[ent_cube, ends[:, 1]]], 1).unsqueeze(-1)
+    p_alpha = _interp(s_g * a_g, x_g)
+    b_g = b[ent_cube, ent_edge].unsqueeze(-1)
+    bsum = torch.zeros(nv, 1, dtype=dtype).index_add(0, ent_vd, b_g)
+    vd = torch.zeros(nv, 3, dtype=dtype).index_add(0, ent_vd, p_alpha * b_g) / (bsum + 1e-8)
+    vf = None
+    if v_features is not None:
+        fe = v_features.to(dev, dtype)[sedge.reshape(-1)].reshape(sedge.shape[0], 2, -1)
+        uf = _interp(s_g * a_g, fe[g])
+        vf = torch.zeros(nv, fe.shape[-1], dtype=dtype).index_add(0, ent_vd, uf * b_g) / (bsum + 1e-8)
+    dist = (zc_g - vd[ent_vd]).norm(dim=-1)
+    mean = torch.zeros(nv, dtype=dtype).index_add(0, ent_vd, dist) / vd_cnt.to(dtype)
+    L_dev = (dist - mean[ent_vd]).abs()
+    # quads: surface edges shared by 4 surface cubes, in surface-edge order
+    sel = (slot_cnt.reshape(Ns, 12) == 4) & slot_hot.reshape(Ns, 12)
+    grp = idx_map[sel]
+    vds = vd_of_slot[sel]
+    order = torch.sort(grp, stable=True)[1]
+    quad_e = grp[order].reshape(-1, 4)[:, 0]
+    quad = vds[order].reshape(-1, 4)
+    first_pos = f[sedge[quad_e, 0]] > 0
+    quad = torch.where(first_pos[:, None], quad[:, [0, 1, 3, 2]], quad[:, [2, 3, 1, 0]])
+    if not training:
+        q = vd.detach()[quad.reshape(-1)].reshape(-1, 4, 3)
+        m = (q[:, 0] - q[:, 2]).norm(dim=1) < (q[:, 1] - q[:, 3]).norm(dim=1)
+        faces = torch.where(m[:, None], quad[:, [0, 1, 2, 0, 2, 3]], quad[:, [0, 1, 3, 3, 1, 2]]).reshape(-1, 3)
+        return dict(vertices=vd, faces=faces, L_dev=L_dev, p_alpha=p_alpha, n_alpha=n_g, v_features=vf)
+    centers = vd[quad.reshape(-1)].reshape(-1, 4, 3).mean(1)
+    cidx = torch.arange(quad.shape[0]) + nv
+    verts = torch.cat([vd, centers])
+    if vf is not None:
+        vf = torch.cat([vf, vf[quad.reshape(-1)].reshape(quad.shape[0], 4, -1).mean(1)])
+    pairs = quad[:, [0, 1, 1, 2, 2, 3, 3, 0]].reshape(-1, 4, 2)
+    faces = torch.cat([pairs, cidx.reshape(-1, 1, 1).repeat(1, 4, 1)], -1).reshape(-1, 3)
+    return dict(vertices=verts, faces=faces, L_dev=L_dev, p_alpha=p_alpha, n_alpha=n_g, v_features=vf)
+
+
+def grid_band(sdf_fn, resolution: int, band: float, dtype=np.float64):
+    """Sparse narrow-band input on the resolution^3 grid over [-1,1]^3: the
+    cubes whose 8 corner SDF values include one with |sdf| < band, their
+    unique corners (sdf values) and the cube -> corner index table (corners
+    in CUBE_CORNERS order).  Small grids only (dense evaluation)."""
+    r = resolution
+    g = np.arange(r + 1)
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    P = np.stack([X, Y, Z], -1).reshape(-1, 3).astype(dtype) * (2.0 / r) - 1.0
+    S = sdf_fn(P).reshape(r + 1, r + 1, r + 1)
+    c = np.stack(np.meshgrid(np.arange(r), np.arange(r), np.arange(r), indexing="ij"), -1).reshape(-1, 3)
+    corner_ids = []
+    for i in range(8):
+        o = c + np.array([(i >> 0) & 1, (i >> 1) & 1, (i >> 2) & 1])
+        corner_ids.append((o[:, 0] * (r + 1) + o[:, 1]) * (r + 1) + o[:, 2])
+    cid = np.stack(corner_ids, 1)
+    Sf = S.reshape(-1)
+    keep = (np.abs(Sf[cid]) < band).any(1)
+    cubes = c[keep]
+    used, inv = np.unique(cid[keep].reshape(-1), return_inverse=True)
+    return cubes.astype(np.int64), Sf[used], inv.reshape(-1, 8).astype(np.int64)
