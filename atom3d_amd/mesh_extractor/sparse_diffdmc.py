@@ -75,6 +75,14 @@ class DMCTopology:
         self.vox = voxel_coords.to(dev, torch.int32).contiguous()
         self.ci = cube_idx.to(dev, torch.int32).contiguous()
         self.res = int(resolution)
+        if self.N:
+            lo, hi = torch.aminmax(self.ci)
+            vlo, vhi = torch.aminmax(self.vox)
+            lo, hi, vlo, vhi = (int(x) for x in torch.stack([lo, hi, vlo, vhi]).tolist())
+            if lo < 0 or hi >= self.M:
+                raise ValueError(f"cube_idx values must lie in [0, {self.M}), got [{lo}, {hi}]")
+            if vlo < 0 or vhi >= self.res:
+                raise ValueError(f"voxel_coords must lie in [0, {self.res}), got [{vlo}, {vhi}]")
         self.iso = float(isovalue)
         self.wscale = float(weight_scale)
         self.dtype = sdf.dtype
