@@ -139,3 +139,38 @@ def shapes_sdf(P, shapes: dict | None = None):
         q = ((d[:, u] ** 2 + d[:, v] ** 2) ** 0.5) - R
         out = mn(out, (q ** 2 + d[:, a] ** 2) ** 0.5 - r)
     return out
+
+
+def dmc_band(res: int = 1024, shapes: dict | None = None, band: float = 0.8660254037844386, device="cuda",
+             slab: int = 16):
+    """Sparse narrow band for DMC on the res^3 grid over [-1,1]^3 (world =
+    grid * 2 / res - 1): the cubes whose centre has |sdf| < band * h
+    (default: half the cube diagonal, so every cube the surface crosses is
+    kept), in lexicographic order.  Returns voxel_coords int32 [N,3], sdf
+    float32 [M] at the unique corners (ascending linear corner id, evaluated
+    in float64) and cube_idx int32 [N,8] in CUBE_CORNERS order."""
+    sh = sdf_shapes() if shapes is None else shapes
+    h = 2.0 / res
+    g = torch.arange(res, device=device, dtype=torch.float64)
+    cubes = []
+    for x0 in range(0, res, slab):
+        xs = torch.arange(x0, min(res, x0 + slab), device=device, dtype=torch.float64)
+        X, Y, Z = torch.meshgrid(xs, g, g, indexing="ij")
+        P = (torch.stack([X, Y, Z], -1).reshape(-1, 3) + 0.5) * h - 1.0
+        idx = torch.nonzero(shapes_sdf(P, sh).abs() < band * h).squeeze(1)
+        cubes.append(torch.stack([idx // (res * res) + x0, (idx // res) % res, idx % res], 1))
+        del X, Y, Z, P
+    cubes = torch.cat(cubes)
+    R1 = res + 1
+    off = torch.tensor([[(i >> 0) & 1, (i >> 1) & 1, (i >> 2) & 1] for i in range(8)], device=device)
+    c = cubes[:, None, :] + off[None]
+    lin = (c[..., 0] * R1 + c[..., 1]) * R1 + c[..., 2]
+    used, inv = torch.unique(lin.reshape(-1), return_inverse=True)
+    del c, lin
+    sdf = torch.empty(used.shape[0], device=device, dtype=torch.float32)
+    step = 1 << 24
+    for s0 in range(0, used.shape[0], step):
+        u = used[s0:s0 + step]
+        Pc = torch.stack([u // (R1 * R1), (u // R1) % R1, u % R1], 1).double() * h - 1.0
+        sdf[s0:s0 + step] = shapes_sdf(Pc, sh).float()
+    return cubes.int(), sdf, inv.view(-1, 8).int()

@@ -42,6 +42,9 @@ def parse():
     ap.add_argument("--queries", type=int, default=N_QUERIES)
     ap.add_argument("--cpu-sample", type=int, default=1 << 21, help="queries in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--workload", choices=["udf", "dmc"], default="udf",
+                    help="udf: the closest-point headline; dmc: SparseDiffDMC forward+backward on a 1024^3 band")
+    ap.add_argument("--dmc-res", type=int, default=1024)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="run one plain device copy of this many MiB before timing (counter calibration)")
     return ap.parse_args()
@@ -63,8 +66,11 @@ def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
-              "the multi-GPU query sharding is described in DESIGN.md but not built.", file=sys.stderr)
+              "multi-GPU query sharding is not built.", file=sys.stderr)
         sys.exit(2)
+
+    if a.workload == "dmc":
+        return bench_dmc(a)
 
     import torch
 
@@ -198,6 +204,116 @@ def main():
         },
         "cpu_baseline": cpu,
         "bvh_build_s": build_s,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_dmc(a):
+    """SparseDiffDMC (training mode, float32) on the narrow band of 32 spheres
+    and 16 tori (seed 2) at res^3: one step = forward (topology + values) and
+    backward of sum(W * outputs) to sdf, deform, alpha and beta."""
+    import torch
+
+    from atom3d_amd import datasets
+    from atom3d_amd.mesh_extractor import SparseDiffDMC
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    res = a.dmc_res
+    tg = time.perf_counter()
+    vox, sdf, ci = datasets.dmc_band(res, device=dev)
+    torch.cuda.synchronize()
+    gen_s = time.perf_counter() - tg
+    N, M = vox.shape[0], sdf.shape[0]
+    g = torch.Generator(device=dev).manual_seed(2)
+    deform = (torch.rand(M, 3, device=dev, generator=g) - 0.5) * (0.2 / res)
+    alpha = torch.randn(N, 8, device=dev, generator=g)
+    beta = torch.randn(N, 12, device=dev, generator=g)
+    leaves = [t.requires_grad_() for t in (sdf, deform, alpha, beta)]
+    m = SparseDiffDMC(device=dev)
+    W = {}
+
+    def fwd():
+        return m(vox, leaves[0], ci, res, deform=leaves[1], alpha=leaves[2], beta=leaves[3])
+
+    def step(ev=None):
+        for t in leaves:
+            t.grad = None
+        if ev is not None:
+            ev[0].record()
+        o = fwd()
+        if ev is not None:
+            ev[1].record()
+        loss = sum((W[k] * getattr(o, k)).sum() for k in W)
+        loss.backward()
+        if ev is not None:
+            ev[2].record()
+        return o
+
+    o = fwd()
+    for k in ("vertices", "L_dev", "p_alpha", "n_alpha"):
+        W[k] = torch.randn(getattr(o, k).shape, device=dev, generator=g)
+    V, Ftri, E = o.vertices.shape[0], o.faces.shape[0], o.L_dev.shape[0]
+    del o
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    evs = [tuple(torch.cuda.Event(enable_timing=True) for _ in range(3)) for _ in range(a.steps)]
+    t0 = time.perf_counter()
+    for k in range(a.steps):
+        step(evs[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    fwd_ms = sum(e[0].elapsed_time(e[1]) for e in evs) / a.steps
+    bwd_ms = sum(e[1].elapsed_time(e[2]) for e in evs) / a.steps
+    ms = wall * 1e3 / a.steps
+    n_surf = int(((sdf.detach()[ci.long()] < 0).sum(1) % 8 != 0).sum())
+    fwd_bytes = 48 * N + 4 * M + 12 * V + 24 * Ftri
+    bwd_bytes = 12 * V + 4 * M + 48 * n_surf
+    achieved = (fwd_bytes + bwd_bytes) / (ms * 1e-3) / 1e9
+
+    cpu = None
+    if not a.no_cpu:
+        from oracle.dmc import dmc_forward
+
+        rs = 128
+        cv, cs, cc = datasets.dmc_band(rs, device="cpu")
+        cpu_leaves = [cs.clone().requires_grad_(), torch.zeros(cs.shape[0], 3, requires_grad=True),
+                      torch.randn(cv.shape[0], 8).requires_grad_(), torch.randn(cv.shape[0], 12).requires_grad_()]
+        tc = time.perf_counter()
+        oc = dmc_forward(cv.long(), cpu_leaves[0], cc.long(), rs, deform=cpu_leaves[1], alpha=cpu_leaves[2],
+                         beta=cpu_leaves[3], dtype=torch.float32)
+        sum(oc[k].sum() for k in ("vertices", "L_dev", "p_alpha", "n_alpha")).backward()
+        cpu_s = time.perf_counter() - tc
+        cpu = {"value": cv.shape[0] / cpu_s, "unit": "cubes/s", "cores": torch.get_num_threads(), "kind": "port",
+               "sample": f"the same shapes' band at {rs}^3 ({cv.shape[0]} cubes), forward + backward of the "
+                         f"reference's weighted-average algorithm restated in torch CPU ops (oracle/dmc.py), "
+                         f"float32"}
+    out = {
+        "metric": "SparseDiffDMC cubes/s (forward + backward)",
+        "value": N / (ms * 1e-3),
+        "unit": "cubes/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"SparseDiffDMC training step on the |sdf(centre)| < h*sqrt(3)/2 band of 32 spheres "
+                               f"+ 16 tori (seed 2) at {res}^3, deform/alpha/beta/sdf all requiring grad",
+                   "cubes": N, "corners": M, "surface_cubes": n_surf, "vertices": V, "faces": Ftri,
+                   "entries": E, "parallelism": "single GPU"},
+        "forward_ms": fwd_ms,
+        "backward_ms": bwd_ms,
+        "forward_cubes_per_s": N / (fwd_ms * 1e-3),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "whole step",
+                     "bytes_fwd": fwd_bytes, "bytes_bwd": bwd_bytes},
+        "cpu_baseline": cpu,
+        "band_build_s": gen_s,
     }
     print(json.dumps(out), flush=True)
 

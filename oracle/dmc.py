@@ -67,9 +67,10 @@ def dmc_forward(voxel_coords, sdf, cube_idx, resolution, deform=None, beta=None,
         nb = pc + chk[prob][:, 1:4]
         inr = ((nb >= 0) & (nb < resolution)).all(1)
         R = int(resolution)
-        keys = set(((pc[:, 0] * R + pc[:, 1]) * R + pc[:, 2]).tolist())
-        nbk = ((nb[:, 0] * R + nb[:, 1]) * R + nb[:, 2]).tolist()
-        flip = torch.tensor([bool(r) and k in keys for r, k in zip(inr.tolist(), nbk)], dtype=torch.bool)
+        keys = torch.sort((pc[:, 0] * R + pc[:, 1]) * R + pc[:, 2]).values
+        nbk = (nb[:, 0] * R + nb[:, 1]) * R + nb[:, 2]
+        pos = torch.searchsorted(keys, nbk).clamp(max=len(keys) - 1)
+        flip = inr & (keys[pos] == nbk)
         idx = torch.nonzero(prob).squeeze(1)[flip]
         case[idx] = chk[idx, 4]
     # surface edges: lexicographically unique corner pairs with a sign change
@@ -98,22 +99,18 @@ def dmc_forward(voxel_coords, sdf, cube_idx, resolution, deform=None, beta=None,
     tab = torch.as_tensor(dmc_table)
     ent_cube, ent_edge, ent_vd, vd_cnt = [], [], [], []
     nv = 0
+    for num in torch.unique(nvd).tolist():
+        cur = torch.nonzero(nvd == num).squeeze(1)
+        rows = tab[case[cur], :num]
+        mask = rows >= 0
+        ent_cube.append(cur.view(-1, 1, 1).expand(-1, num, 7)[mask])
+        ent_edge.append(rows[mask])
+        ent_vd.append((nv + torch.arange(len(cur) * num)).view(-1, num, 1).expand(-1, -1, 7)[mask])
+        vd_cnt.append(mask.sum(-1).reshape(-1))
+        nv += len(cur) * num
+    ent_cube, ent_edge, ent_vd, vd_cnt = (torch.cat(x) for x in (ent_cube, ent_edge, ent_vd, vd_cnt))
     vd_of_slot = torch.zeros(Ns, 12, dtype=torch.long)
-    for num in sorted(set(nvd.tolist())):
-        for cube in torch.nonzero(nvd == num).squeeze(1).tolist():
-            for k in range(num):
-                row = [int(x) for x in tab[case[cube], k] if x >= 0]
-                for e in row:
-                    ent_cube.append(cube)
-                    ent_edge.append(e)
-                    ent_vd.append(nv)
-                    vd_of_slot[cube, e] = nv
-                vd_cnt.append(len(row))
-                nv += 1
-    ent_cube = torch.tensor(ent_cube, dtype=torch.long)
-    ent_edge = torch.tensor(ent_edge, dtype=torch.long)
-    ent_vd = torch.tensor(ent_vd, dtype=torch.long)
-    vd_cnt = torch.tensor(vd_cnt, dtype=torch.long)
+    vd_of_slot[ent_cube, ent_edge] = ent_vd
     g = idx_map[ent_cube, ent_edge]
     zc_g, s_g, x_g, n_g = zc[g], es[g], ex[g], nrm[g]
     ends = _EDGE_FLAT.reshape(12, 2)[ent_edge]
