@@ -65,9 +65,14 @@ __device__ __forceinline__ void write_result(int64_t q, v3 p, float best, int32_
 }
 
 // ------------------------------------------------------------ wave-coherent walk
+// seed (optional, indexed like the outputs): an upper bound on each query's
+// distance; the walk starts with it as the best distance, so it never enters
+// a box farther than the bound.  A query with no face within its bound gets
+// face -1 and dist = its seed.
 template <bool STATS>
 __global__ __launch_bounds__(kBlk) void k_udf_wave(const void* __restrict__ blob, const float* __restrict__ P,
                                                     const uint32_t* __restrict__ perm, int64_t K,
+                                                    const float* __restrict__ seed,
                                                     float* __restrict__ dist, int32_t* __restrict__ face,
                                                     float* __restrict__ closest, float* __restrict__ uvw,
                                                     unsigned long long* __restrict__ counters) {
@@ -85,6 +90,10 @@ __global__ __launch_bounds__(kBlk) void k_udf_wave(const void* __restrict__ blob
   const float slack = query_slack(p, H->coord_max);
 
   float best = 1e30f;
+  if (seed && valid) {
+    const float sd = seed[q];
+    if (sd >= 0.0f && sd < 1e15f) best = fmaxf(sd * sd * (1.0f + 4.8e-7f), 1e-30f);
+  }
   int32_t best_id = 0x7fffffff, best_tri = -1;
   float thr = valid ? prune_thr(best, slack) : -1.0f;  // invalid lanes accept nothing
   int sp = 0;
@@ -165,13 +174,16 @@ __global__ __launch_bounds__(kBlk) void k_udf_wave(const void* __restrict__ blob
       atomicAdd(&counters[4], (unsigned long long)n_lane);
     }
   }
-  if (valid) write_result(q, p, best, best_id, best_tri, tris, dist, face, closest, uvw);
+  if (valid) {
+    if (seed && best_tri < 0) best = seed[q] * seed[q];
+    write_result(q, p, best, best_id, best_tri, tris, dist, face, closest, uvw);
+  }
 }
 
 // ------------------------------------------------------------ per-lane walk
 __global__ __launch_bounds__(kBlk) void k_udf_lane(const void* __restrict__ blob, const float* __restrict__ P,
                                                     const uint32_t* __restrict__ perm, int64_t K,
-                                                    float* __restrict__ dist, int32_t* __restrict__ face,
+                                                    const float* __restrict__ seed, float* __restrict__ dist, int32_t* __restrict__ face,
                                                     float* __restrict__ closest, float* __restrict__ uvw) {
   __shared__ int32_t s_ref[kLaneLds * kBlk];
   __shared__ float s_d2[kLaneLds * kBlk];
@@ -186,6 +198,10 @@ __global__ __launch_bounds__(kBlk) void k_udf_lane(const void* __restrict__ blob
   const float slack = query_slack(p, H->coord_max);
 
   float best = 1e30f;
+  if (seed) {
+    const float sd = seed[q];
+    if (sd >= 0.0f && sd < 1e15f) best = fmaxf(sd * sd * (1.0f + 4.8e-7f), 1e-30f);
+  }
   int32_t best_id = 0x7fffffff, best_tri = -1;
   float thr = prune_thr(best, slack);
   int32_t ovf_ref[kLaneOvf];
@@ -249,6 +265,7 @@ __global__ __launch_bounds__(kBlk) void k_udf_lane(const void* __restrict__ blob
     }
     if (!found) break;
   }
+  if (seed && best_tri < 0) best = seed[q] * seed[q];
   write_result(q, p, best, best_id, best_tri, tris, dist, face, closest, uvw);
 }
 
@@ -384,9 +401,9 @@ extern "C" int a3d_query_order(const float* P, int64_t K, uint32_t* perm, void* 
   return A3D_OK;
 }
 
-extern "C" int a3d_bvh_udf(const void* blob, const float* P, int64_t K, const uint32_t* perm, float* dist,
-                           int32_t* face, float* closest, float* uvw, int flags, int64_t* counters,
-                           a3d_stream_t stream) {
+namespace {
+int udf_launch(const void* blob, const float* P, int64_t K, const uint32_t* perm, const float* seed, float* dist,
+               int32_t* face, float* closest, float* uvw, int flags, int64_t* counters, a3d_stream_t stream) {
   if (!blob) return fail(A3D_ERR_ARG, "a3d_bvh_udf: null BVH blob");
   if (K < 0 || K > (int64_t(1) << 40)) return fail(A3D_ERR_ARG, "a3d_bvh_udf: K=%lld out of range", (long long)K);
   if (perm && K > 0xffffffffll) return fail(A3D_ERR_ARG, "a3d_bvh_udf: perm needs K < 2^32");
@@ -397,13 +414,30 @@ extern "C" int a3d_bvh_udf(const void* blob, const float* P, int64_t K, const ui
   hipStream_t s = (hipStream_t)stream;
   unsigned long long* c = (unsigned long long*)counters;
   if (flags & A3D_UDF_LANE)
-    hipLaunchKernelGGL(k_udf_lane, dim3(grid_for(K)), dim3(kBlk), 0, s, blob, P, perm, K, dist, face, closest, uvw);
+    hipLaunchKernelGGL(k_udf_lane, dim3(grid_for(K)), dim3(kBlk), 0, s, blob, P, perm, K, seed, dist, face, closest,
+                       uvw);
   else if (counters)
-    hipLaunchKernelGGL(k_udf_wave<true>, dim3(grid_for(K)), dim3(kBlk), 0, s, blob, P, perm, K, dist, face, closest,
+    hipLaunchKernelGGL(k_udf_wave<true>, dim3(grid_for(K)), dim3(kBlk), 0, s, blob, P, perm, K, seed, dist, face,
+                       closest,
                        uvw, c);
   else
-    hipLaunchKernelGGL(k_udf_wave<false>, dim3(grid_for(K)), dim3(kBlk), 0, s, blob, P, perm, K, dist, face, closest,
+    hipLaunchKernelGGL(k_udf_wave<false>, dim3(grid_for(K)), dim3(kBlk), 0, s, blob, P, perm, K, seed, dist, face,
+                       closest,
                        uvw, c);
   A3D_LAUNCH_CHECK();
   return A3D_OK;
+}
+}  // namespace
+
+extern "C" int a3d_bvh_udf(const void* blob, const float* P, int64_t K, const uint32_t* perm, float* dist,
+                           int32_t* face, float* closest, float* uvw, int flags, int64_t* counters,
+                           a3d_stream_t stream) {
+  return udf_launch(blob, P, K, perm, nullptr, dist, face, closest, uvw, flags, counters, stream);
+}
+
+extern "C" int a3d_bvh_udf_seeded(const void* blob, const float* P, int64_t K, const uint32_t* perm,
+                                  const float* seed, float* dist, int32_t* face, float* closest, float* uvw,
+                                  int flags, a3d_stream_t stream) {
+  if (!seed && K > 0) return fail(A3D_ERR_ARG, "a3d_bvh_udf_seeded: null seed");
+  return udf_launch(blob, P, K, perm, seed, dist, face, closest, uvw, flags, nullptr, stream);
 }

@@ -39,6 +39,7 @@ SIGNATURES: dict[str, list] = {
     "a3d_query_order_ws_bytes": [_i64, _pi64],
     "a3d_query_order": [_vp, _i64, _vp, _vp, _i64, _vp],
     "a3d_bvh_udf": [_vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp],
+    "a3d_bvh_udf_seeded": [_vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp],
     "a3d_point_dist": [_vp, _vp, _i64, _vp, _vp],
     "a3d_point_dist_backward": [_vp, _vp, _vp, _vp, _i64, _vp, _vp],
     "a3d_sdf_sign": [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],

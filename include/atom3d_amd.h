@@ -87,6 +87,15 @@ int a3d_query_order(const float* P, int64_t K, uint32_t* perm, void* ws, int64_t
 int a3d_bvh_udf(const void* blob, const float* P, int64_t K, const uint32_t* perm, float* dist,
                 int32_t* face, float* closest, float* uvw, int flags, int64_t* counters,
                 a3d_stream_t stream);
+/* a3d_bvh_udf_seeded: a3d_bvh_udf (wave walk) where seed[q] (float, indexed
+ * like the outputs) is an upper bound on query q's distance: the walk starts
+ * with it as its best distance and never enters a box farther away.  Results
+ * equal a3d_bvh_udf's wherever the true distance is <= seed[q]; elsewhere
+ * face[q] = -1 and dist[q] = seed[q] (the caller re-queries those without a
+ * seed).  flags: 0 or A3D_UDF_LANE.  No counterpart in the reference: the
+ * surrogate router's exact mode. */
+int a3d_bvh_udf_seeded(const void* blob, const float* P, int64_t K, const uint32_t* perm, const float* seed,
+                       float* dist, int32_t* face, float* closest, float* uvw, int flags, a3d_stream_t stream);
 
 /* ------------------------------------------------------------------ UDF adjoint
  * Replaces the autograd re-evaluation in MeshBVH._udf_with_grad

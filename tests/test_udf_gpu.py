@@ -166,3 +166,34 @@ def test_drop_in_result_shapes():
     assert r.closest_points.shape == (100, 3) and r.uvw is None and r.face_ids is None
     r = m.udf(P, return_face_ids=True, return_uvw=True)
     assert r.face_ids.dtype == torch.int32 and r.uvw.shape == (100, 3) and r.closest_points is None
+
+
+@pytest.mark.parametrize("flags", [0, 1])
+def test_seeded_walk_equals_plain_and_flags_short_seeds(flags):
+    """a3d_bvh_udf_seeded: a seed >= the true distance changes nothing (bit
+    for bit); a seed below it returns face -1 and dist = seed."""
+    from atom3d_amd.kernels import _lib
+
+    V, F = datasets.noisy_icosphere()
+    P = datasets.uniform_points(1 << 16, seed=4)
+    rd, rf, rc, rw = oracle.brute_udf(V.numpy(), F.numpy(), P.numpy())
+    b = BVHAccelerator(V.to(DEV), F.to(DEV))
+    Pd = P.to(DEV)
+    K = P.shape[0]
+    true = torch.tensor(rd, device=DEV)
+    rng = np.random.default_rng(9)
+    short = torch.tensor(rng.random(K) < 0.3, device=DEV)
+    seed = torch.where(short, true * 0.5, true * torch.tensor(rng.uniform(1.0, 1.5, K), device=DEV).float() + 1e-6)
+    perm = b.query_order(Pd)
+    d = torch.empty(K, device=DEV)
+    f = torch.empty(K, dtype=torch.int32, device=DEV)
+    c = torch.empty(K, 3, device=DEV)
+    w = torch.empty(K, 3, device=DEV)
+    print(f"[udf seeded] flags={flags} K={K}", flush=True)
+    _lib.call("a3d_bvh_udf_seeded", _lib.ptr(b.blob), _lib.ptr(Pd), K, _lib.ptr(perm), _lib.ptr(seed), _lib.ptr(d),
+              _lib.ptr(f), _lib.ptr(c), _lib.ptr(w), flags, _lib.stream(torch.device(DEV)))
+    s = short.cpu().numpy() & (rd > 0)
+    f, d, c = f.cpu().numpy(), d.cpu().numpy(), c.cpu().numpy()
+    assert np.all(f[s] == -1) and np.array_equal(d[s], seed.cpu().numpy()[s])
+    ok = ~short.cpu().numpy()
+    assert np.array_equal(f[ok], rf[ok]) and np.array_equal(d[ok], rd[ok]) and np.array_equal(c[ok], rc[ok])
