@@ -278,23 +278,59 @@ __global__ void k_ff_count(Geo g, const uint32_t* __restrict__ plane, uint32_t* 
   cnt[t] = __popc(plane[waddr(g, (int)(r / g.H), (int)(r % g.H), mw)]);
 }
 
-__global__ void k_ff_write(Geo g, const uint32_t* __restrict__ plane, const uint32_t* __restrict__ off, int64_t l0,
-                           int64_t l1, int64_t l2, int64_t* __restrict__ out) {
-  int64_t t = blockIdx.x * (int64_t)kBlk + threadIdx.x;
-  int64_t n = (int64_t)g.D * g.H * g.Wn;
-  if (t >= n) return;
-  int mw = (int)(t % g.Wn);
-  int64_t r = t / g.Wn;
-  int i = (int)(r / g.H), j = (int)(r % g.H);
-  uint32_t w = plane[waddr(g, i, j, mw)];
-  int64_t o = off[t];
-  while (w) {
-    int bit = __ffs(w) - 1;
-    w &= w - 1;
-    out[o * 3] = l0 + i;
-    out[o * 3 + 1] = l1 + j;
+// One block per kBlk consecutive words; the block's set bits are written as
+// consecutive output rows by consecutive threads (rank -> word by binary
+// search over the block's prefix counts in LDS, then the bit by a select in
+// the word), so the 24-byte rows are written coalesced.
+__global__ __launch_bounds__(kBlk) void k_ff_write(Geo g, const uint32_t* __restrict__ plane,
+                                                   const uint32_t* __restrict__ off, int64_t l0, int64_t l1,
+                                                   int64_t l2, int64_t* __restrict__ out) {
+  __shared__ uint32_t s_word[kBlk];
+  __shared__ uint32_t s_pre[kBlk + 1];
+  const int64_t n = (int64_t)g.D * g.H * g.Wn;
+  const int64_t t0 = blockIdx.x * (int64_t)kBlk;
+  const int64_t t = t0 + threadIdx.x;
+  uint32_t w = 0;
+  if (t < n) {
+    const int mw = (int)(t % g.Wn);
+    const int64_t r = t / g.Wn;
+    w = plane[waddr(g, (int)(r / g.H), (int)(r % g.H), mw)];
+  }
+  const uint32_t base = off[t0];
+  s_word[threadIdx.x] = w;
+  s_pre[threadIdx.x] = t < n ? off[t] - base : 0;
+  __syncthreads();
+  const int64_t last = (n - t0 < kBlk ? n - t0 : kBlk) - 1;
+  if (threadIdx.x == 0) s_pre[kBlk] = s_pre[last] + __popc(s_word[last]);
+  __syncthreads();
+  const uint32_t total = s_pre[kBlk];
+  for (uint32_t e = threadIdx.x; e < total; e += kBlk) {
+    int lo = 0, hi = (int)last;  // last word with s_pre <= e
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (s_pre[mid] <= e) lo = mid; else hi = mid - 1;
+    }
+    uint32_t x = s_word[lo];
+    uint32_t k = e - s_pre[lo];  // the k-th set bit of x (0-based)
+    int bit = 0;
+    for (int width = 16; width > 0; width >>= 1) {
+      const uint32_t m = x & ((1u << width) - 1u);
+      const uint32_t c = __popc(m);
+      if (k >= c) {
+        k -= c;
+        x >>= width;
+        bit += width;
+      } else {
+        x = m;
+      }
+    }
+    const int64_t tw = t0 + lo;
+    const int mw = (int)(tw % g.Wn);
+    const int64_t r = tw / g.Wn;
+    const int64_t o = (int64_t)base + e;
+    out[o * 3] = l0 + r / g.H;
+    out[o * 3 + 1] = l1 + r % g.H;
     out[o * 3 + 2] = l2 + mw * 32 + bit;
-    ++o;
   }
 }
 
@@ -373,10 +409,67 @@ int check_dims(int D, int H, int W, const char* who) {
 
 inline unsigned gridn(int64_t n) { return (unsigned)div_up(n > 0 ? n : 1, kBlk); }
 
+// Per-axis min and max of int64 [N,3] coordinates: a wave reduction, then
+// one 64-bit atomic per wave and axis.
+__global__ void k_ijk_bounds_init(long long* out) {
+  if (threadIdx.x < 3) out[threadIdx.x] = 0x7fffffffffffffffll;
+  else if (threadIdx.x < 6) out[threadIdx.x] = -0x7fffffffffffffffll - 1;
+}
+
+__global__ __launch_bounds__(256) void k_ijk_bounds(const int64_t* __restrict__ ijk, int64_t N,
+                                                    long long* __restrict__ out) {
+  long long lo[3] = {0x7fffffffffffffffll, 0x7fffffffffffffffll, 0x7fffffffffffffffll};
+  long long hi[3] = {-0x7fffffffffffffffll - 1, -0x7fffffffffffffffll - 1, -0x7fffffffffffffffll - 1};
+  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < N; i += (int64_t)gridDim.x * 256) {
+    for (int k = 0; k < 3; ++k) {
+      long long v = ijk[i * 3 + k];
+      lo[k] = v < lo[k] ? v : lo[k];
+      hi[k] = v > hi[k] ? v : hi[k];
+    }
+  }
+  for (int k = 0; k < 3; ++k) {
+    for (int o = 32; o > 0; o >>= 1) {
+      long long a = __shfl_xor(lo[k], o), b = __shfl_xor(hi[k], o);
+      lo[k] = a < lo[k] ? a : lo[k];
+      hi[k] = b > hi[k] ? b : hi[k];
+    }
+  }
+  __shared__ long long s_lo[4][3], s_hi[4][3];
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+    for (int k = 0; k < 3; ++k) {
+      s_lo[wv][k] = lo[k];
+      s_hi[wv][k] = hi[k];
+    }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int k = threadIdx.x;
+    long long a = s_lo[0][k], b = s_hi[0][k];
+    for (int w = 1; w < 4; ++w) {
+      a = s_lo[w][k] < a ? s_lo[w][k] : a;
+      b = s_hi[w][k] > b ? s_hi[w][k] : b;
+    }
+    atomicMin(&out[k], a);
+    atomicMax(&out[3 + k], b);
+  }
+}
+
 }  // namespace
 }  // namespace a3d
 
 using namespace a3d;
+
+extern "C" int a3d_ijk_bounds(const int64_t* ijk, int64_t N, int64_t* out6, a3d_stream_t stream) {
+  if (!out6 || (N > 0 && !ijk) || N < 0) return fail(A3D_ERR_ARG, "a3d_ijk_bounds: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_ijk_bounds_init, dim3(1), dim3(64), 0, s, (long long*)out6);
+  A3D_LAUNCH_CHECK();
+  if (N == 0) return A3D_OK;
+  int64_t g = (N + 255) / 256;
+  hipLaunchKernelGGL(k_ijk_bounds, dim3((unsigned)(g < 1024 ? g : 1024)), dim3(256), 0, s, ijk, N, (long long*)out6);
+  A3D_LAUNCH_CHECK();
+  return A3D_OK;
+}
 
 extern "C" int a3d_flood_sizes(int D, int H, int W, int64_t* plane_words, int64_t* ws_bytes) {
   if (int rc = check_dims(D, H, W, "a3d_flood_sizes")) return rc;

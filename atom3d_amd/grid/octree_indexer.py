@@ -108,6 +108,7 @@ class OctreeIndexer(CubeGrid):
             n_kept = torch.empty(1, dtype=torch.int64, device=dev)
             parents = None
             n_par = 0
+            self.last_level_counts = []  # (level, cubes tested, cubes hit)
             for level in range(min_level, max(self.max_level, min_level) + 1):
                 if level > min_level and n_par == 0:
                     break
@@ -119,6 +120,7 @@ class OctreeIndexer(CubeGrid):
                 _lib.call("a3d_octree_level", _lib.ptr(blob), bounds, level, _lib.ptr(parents), n_par,
                           _lib.ptr(kept), _lib.ptr(n_kept), _lib.ptr(ws), wsb.value, s)
                 n_par = int(n_kept.item())
+                self.last_level_counts.append((level, n_cand, n_par))
                 parents = kept
                 if level >= self.max_level:
                     break

@@ -59,6 +59,7 @@ SIGNATURES: dict[str, list] = {
     "a3d_flood_extract": [_vp, _i32, _i32, _i32, _i32, _pi64, _vp, _vp, _i64, _vp],
     "a3d_flood_dense_mask": [_vp, _i32, _i32, _i32, _vp, _vp],
     "a3d_flood_dam_mask": [_vp, _i32, _i32, _i32, _vp, _i64, _pi64, _vp, _vp],
+    "a3d_ijk_bounds": [_vp, _i64, _vp, _vp],
     "a3d_dmc_ws_bytes": [_i64, _pi64],
     "a3d_dmc_topology": [_vp, _vp, _i32, _f64, _vp, _i64, _i64, _i32, _vp, _vp, _i64, _pi64, _vp],
     "a3d_dmc_faces": [_vp, _i64, _pi64, _vp, _vp],

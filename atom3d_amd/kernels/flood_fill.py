@@ -109,9 +109,13 @@ def _empty_result(device):
 
 
 def _crop(dam_coords: torch.Tensor, resolution: int, source, padding: int):
-    lo = (dam_coords.min(dim=0)[0] - padding).clamp(min=0)
-    hi = (dam_coords.max(dim=0)[0] + padding).clamp(max=resolution - 1)
-    lo_l, hi_l = lo.tolist(), hi.tolist()
+    b = torch.empty(6, dtype=torch.int64, device=dam_coords.device)
+    with torch.cuda.device(dam_coords.device):
+        _lib.call("a3d_ijk_bounds", _lib.ptr(dam_coords), int(dam_coords.shape[0]), _lib.ptr(b),
+                  _lib.stream(dam_coords.device))
+    b = b.tolist()
+    lo_l = [max(v - padding, 0) for v in b[:3]]
+    hi_l = [min(v + padding, resolution - 1) for v in b[3:]]
     size = [h - l + 1 for l, h in zip(lo_l, hi_l)]
     ls = [int(s) - l for s, l in zip(source, lo_l)]
     inside = all(0 <= a < n for a, n in zip(ls, size))

@@ -201,6 +201,10 @@ int a3d_octree_unpack(const uint64_t* keys, int64_t n, int64_t* ijk, a3d_stream_
  * a3d_flood_dam_mask: uint8[N] = DAM bit at each dam_ijk row.
  */
 #define A3D_FLOOD_NPLANES 11
+/* a3d_ijk_bounds: out6 (device int64) = per-axis min (xyz) then max (xyz) of
+ * ijk int64[N,3] (min = INT64_MAX, max = INT64_MIN when N = 0); the crop
+ * bounds of flood_fill_3d_sparse (kernels/flood_fill.py:150-156). */
+int a3d_ijk_bounds(const int64_t* ijk, int64_t N, int64_t* out6, a3d_stream_t stream);
 int a3d_flood_sizes(int D, int H, int W, int64_t* plane_words, int64_t* ws_bytes);
 int a3d_flood_pack_dense(const uint8_t* occ, int D, int H, int W, uint32_t* state, a3d_stream_t stream);
 int a3d_flood_fill(const int64_t* dam_ijk, int64_t N, const int64_t* lo_host, int D, int H, int W,

@@ -38,7 +38,7 @@ _SIGS = {
     "orc_bvh_aabb": (_i64, [_f, _f, _f, _f, _i64, _u8, _i, _i, _i64, _int]),
     "orc_brute_aabb_hit": (None, [_f, _i, _i64, _f, _f, _i64, _u8, _vp, _int]),
     "orc_sat_clip": (None, [_f, _f, _f, _l, _l, _i64, _int, ctypes.c_float, _u8, _i, _f, _f, _f]),
-    "orc_octree_traverse": (_i64, [_f, _f, _f, _int, _int, _l, _i64, _vp, _vp, _int]),
+    "orc_octree_traverse": (_i64, [_f, _f, _f, _int, _int, _l, _i64, _vp, _vp, _int, _int, _vp]),
 }
 
 
@@ -126,17 +126,23 @@ class SahBvh:
             return self.aabb(qmin, qmax, canonical, cap=n)
         return hit.astype(bool), pa[:n], pf[:n]
 
-    def octree(self, bounds, min_level: int, max_level: int, threads: int = 0):
+    def octree(self, bounds, min_level: int, max_level: int, threads: int = 0, full: bool = False,
+               work: bool = False):
+        """Octree level loop on this tree: (ijk int64 [n,3], tested per level,
+        hits per level[, (node visits, triangle tests)]).  full walks every
+        cube's whole overlap set as the reference does (no first-hit exit)."""
         b = _f32(bounds).reshape(6)
         cap = 1 << 20
-        tested = np.zeros(max_level + 2, np.int64)
-        hits = np.zeros(max_level + 2, np.int64)
         while True:
+            tested = np.zeros(max_level + 2, np.int64)
+            hits = np.zeros(max_level + 2, np.int64)
+            w = np.zeros(2, np.int64)
             out = np.zeros((cap, 3), np.int64)
             n = lib().orc_octree_traverse(self.nodes.reshape(-1), self.tris.reshape(-1), b, min_level, max_level,
-                                          out.reshape(-1), cap, tested.ctypes.data, hits.ctypes.data, threads)
+                                          out.reshape(-1), cap, tested.ctypes.data, hits.ctypes.data, threads,
+                                          int(full), w.ctypes.data)
             if n <= cap:
-                return out[:n], tested, hits
+                return (out[:n], tested, hits, (int(w[0]), int(w[1]))) if work else (out[:n], tested, hits)
             cap = int(n)
 
 

@@ -594,9 +594,15 @@ static inline void cube_box(const int64_t* ijk, const float* b0, const float* ce
 // (int64) at max_level in the reference order; returns the count.
 // tested_out (nullable) receives the number of cube tests done per level,
 // hits_out the number of hits per level (max_level+1 entries each).
+// full = 1 walks each cube's whole overlap set as the reference does (its
+// octree calls intersect_aabb(mode=0), which enumerates every pair,
+// bvh_kernels.cu:555-610) instead of stopping at the first hit; work_out
+// (nullable, 2 entries) then receives the total node visits and triangle
+// SAT tests of all cube tests.
 int64_t orc_octree_traverse(const float* nodes_f, const float* tris, const float* bounds,
                             int min_level, int max_level, int64_t* out, int64_t cap,
-                            int64_t* tested_out, int64_t* hits_out, int nthreads) {
+                            int64_t* tested_out, int64_t* hits_out, int nthreads, int full,
+                            int64_t* work_out) {
   const Node* nodes = reinterpret_cast<const Node*>(nodes_f);
   if (nthreads > 0) omp_set_num_threads(nthreads);
   const float* b0 = bounds;
@@ -623,21 +629,28 @@ int64_t orc_octree_traverse(const float* nodes_f, const float* tris, const float
     for (int k = 0; k < 3; ++k) cell[k] = (b1[k] - b0[k]) / r;
     int64_t n = (int64_t)cur.size() / 3;
     std::vector<uint8_t> hit((size_t)n);
-#pragma omp parallel for schedule(dynamic, 256)
+    int64_t w_node = 0, w_tri = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : w_node, w_tri)
     for (int64_t q = 0; q < n; ++q) {
       f3 lo, hi;
       cube_box(&cur[(size_t)q * 3], b0, cell, &lo, &hi);
       bool any = false;
       int stack[128]; int sp = 0; stack[sp++] = 0;
-      while (sp > 0 && !any) {
+      while (sp > 0 && (full || !any)) {
         const Node& nd = nodes[stack[--sp]];
+        ++w_node;
         if (!aabb_overlap(nd.bmin, nd.bmax, lo, hi)) continue;
         if (nd.left < 0) {
           int s = -nd.left - 1, e = -nd.right - 1;
-          for (int i = s; i < e && !any; i++) {
+          for (int i = s; i < e && (full || !any); i++) {
             const float* t = tris + (int64_t)i * 10;
             f3 a = ld3(t), b = ld3(t + 3), c = ld3(t + 6);
-            if (tri_box_overlap(a, b, c, lo, hi) && tri_aabb_sat(a, b, c, lo, hi)) any = true;
+            ++w_tri;
+            if (full) {
+              if (tri_aabb_sat(a, b, c, lo, hi) && tri_box_overlap(a, b, c, lo, hi)) any = true;
+            } else if (tri_box_overlap(a, b, c, lo, hi) && tri_aabb_sat(a, b, c, lo, hi)) {
+              any = true;
+            }
           }
         } else if (sp < 127) {
           stack[sp++] = nd.right;
@@ -652,6 +665,10 @@ int64_t orc_octree_traverse(const float* nodes_f, const float* tris, const float
       if (hit[(size_t)q]) for (int k = 0; k < 3; ++k) kept.push_back(cur[(size_t)q * 3 + k]);
     if (tested_out) tested_out[level] = n;
     if (hits_out) hits_out[level] = (int64_t)kept.size() / 3;
+    if (work_out) {
+      work_out[0] += w_node;
+      work_out[1] += w_tri;
+    }
     cur.swap(kept);
   }
   int64_t n = (int64_t)cur.size() / 3;
