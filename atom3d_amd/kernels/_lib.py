@@ -29,6 +29,8 @@ _i64 = ctypes.c_int64
 _i32 = ctypes.c_int
 _f32 = ctypes.c_float
 _f64 = ctypes.c_double
+_u64 = ctypes.c_uint64
+_pf32 = ctypes.POINTER(ctypes.c_float)
 _pi64 = ctypes.POINTER(ctypes.c_int64)
 
 # name -> argtypes.  Every entry point returns int (0 = OK).
@@ -68,6 +70,15 @@ SIGNATURES: dict[str, list] = {
     "a3d_dmc_backward": [_vp, _i64, _i64, _pi64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _i32, _f64, _f64,
                          _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "a3d_dmc_topology_get": [_vp, _i64, _pi64, _i32, _vp, _vp],
+    "a3d_sg_sizes": [_pi64, _pi64],
+    "a3d_sg_init": [_vp, _vp, _pf32, _pf32, _u64, _vp],
+    "a3d_sg_sample": [_vp, _vp, _i64, _i64, _f32, _f32, _pf32, _pf32, _u64, _vp, _vp],
+    "a3d_sg_grad": [_vp, _vp, _vp, _vp, _i64, _vp, _vp],
+    "a3d_sg_table_grad": [_vp, _vp, _vp],
+    "a3d_sg_apply": [_vp, _vp, _f32, _f32, _f32, _f32, _i64, _vp],
+    "a3d_sg_infer": [_vp, _vp, _i64, _vp, _vp, _vp],
+    "a3d_sg_route_ws_bytes": [_i64, _pi64],
+    "a3d_sg_udf": [_vp, _vp, _vp, _i64, _vp, _f32, _vp, _vp, _vp, _vp, _vp, _i64, _pi64, _vp],
 }
 
 

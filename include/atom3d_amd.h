@@ -262,6 +262,51 @@ int a3d_dmc_backward(const void* ws, int64_t N, int64_t M, const int64_t* sizes,
 int a3d_dmc_topology_get(const void* ws, int64_t N, const int64_t* sizes, int kind, int32_t* out,
                          a3d_stream_t stream);
 
+/* ------------------------------------------------------------------ surrogate
+ * Learned UDF surrogate: no counterpart in the reference.  A multiresolution
+ * hash grid (16 levels, 2^19 entries of 2 features per level, resolutions
+ * 16..2048 over the box [lo, hi], levels with (N+1)^3 <= 2^19 indexed
+ * densely) feeding an MLP 32 -> 64 -> 64 -> 1 (ReLU; bf16 MFMA operands, fp32
+ * accumulation).  `params` (a3d_sg_sizes) holds what inference reads: meta,
+ * the fp16 table, bf16 W1 [64][32] and W2 [64][64], fp32 w3 [64] and b3.
+ * `state` holds the fp32 master copies, Adam moments, gradients and
+ * per-wave partial gradients used by training.
+ * a3d_sg_init: domain and level layout, table ~ U(-1e-4, 1e-4), weights ~
+ *   U(+-sqrt(6 / (fan_in + fan_out))), b3 = 0; clears the state.
+ * a3d_sg_sample: n training points, a fraction near_frac within `radius` of
+ *   the mesh surface, the rest uniform in [lo, hi].
+ * a3d_sg_grad: gradients of mean((net(P) - target)^2) over a batch of B:
+ *   the table gradient accumulates in the state; the MLP gradient vector
+ *   (W1, W2, w3, b3 flattened, then the batch loss; 6272 floats) is written
+ *   to the state and, if grad_out != NULL, copied there.
+ * a3d_sg_table_grad: copy of the accumulated table gradient (16*2^19*2 floats).
+ * a3d_sg_apply: one Adam step (bias-corrected for step t >= 1) on both, sparse
+ *   on the table (entries with zero gradient untouched); clears the table
+ *   gradient and refreshes the fp16 / bf16 copies in params.
+ * a3d_sg_infer: out[q] = network(P[q]) (perm: optional query order).
+ * a3d_sg_udf: the router.  Every query is answered by the network; those
+ *   predicted below `band` are re-answered exactly by the BVH wave walk
+ *   (a3d_bvh_udf results); the others get face -1, closest = the query
+ *   point, uvw 0.  Writes the exact-query count to *n_near_host
+ *   (synchronises the stream).  Workspace: a3d_sg_route_ws_bytes.
+ */
+int a3d_sg_sizes(int64_t* params_bytes, int64_t* state_bytes);
+int a3d_sg_init(void* params, void* state, const float* lo_host, const float* hi_host, uint64_t seed,
+                a3d_stream_t stream);
+int a3d_sg_sample(const float* V, const int32_t* F, int64_t nF, int64_t n, float near_frac, float radius,
+                  const float* lo_host, const float* hi_host, uint64_t seed, float* P, a3d_stream_t stream);
+int a3d_sg_grad(const void* params, void* state, const float* P, const float* target, int64_t B, float* grad_out,
+                a3d_stream_t stream);
+int a3d_sg_table_grad(const void* state, float* out, a3d_stream_t stream);
+int a3d_sg_apply(void* params, void* state, float lr, float beta1, float beta2, float eps, int64_t t,
+                 a3d_stream_t stream);
+int a3d_sg_infer(const void* params, const float* P, int64_t K, const uint32_t* perm, float* out,
+                 a3d_stream_t stream);
+int a3d_sg_route_ws_bytes(int64_t K, int64_t* bytes);
+int a3d_sg_udf(const void* params, const void* blob, const float* P, int64_t K, const uint32_t* perm, float band,
+               float* dist, int32_t* face, float* closest, float* uvw, void* ws, int64_t ws_bytes,
+               int64_t* n_near_host, a3d_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif
