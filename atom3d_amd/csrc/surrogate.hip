@@ -526,13 +526,23 @@ __global__ __launch_bounds__(kBlkS) void k_sg_train(const void* __restrict__ par
   }
 }
 
-// g[p] = sum over the waves' partials (deterministic order).
-__global__ void k_sg_reduce(const float* __restrict__ part, int64_t nparts, float* __restrict__ mg) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p > kOffLoss) return;
+// g[p] = sum over the waves' partials (fixed order: 16 interleaved partial
+// sums per parameter, then their sum).
+__global__ __launch_bounds__(1024) void k_sg_reduce(const float* __restrict__ part, int64_t nparts,
+                                                    float* __restrict__ mg) {
+  __shared__ float red[16][64];
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int p = blockIdx.x * 64 + tx;
   float s = 0.0f;
-  for (int64_t w = 0; w < nparts; ++w) s += part[w * kStride + p];
-  mg[p] = s;
+  if (p <= kOffLoss)
+    for (int64_t w = ty; w < nparts; w += 16) s += part[w * kStride + p];
+  red[ty][tx] = s;
+  __syncthreads();
+  if (ty == 0 && p <= kOffLoss) {
+    float t = 0.0f;
+    for (int k = 0; k < 16; ++k) t += red[k][tx];
+    mg[p] = t;
+  }
 }
 
 struct AdamArgs {
@@ -722,7 +732,7 @@ extern "C" int a3d_sg_grad(const void* params, void* state, const float* P, cons
   hipLaunchKernelGGL(k_sg_train, dim3((unsigned)blocks), dim3(kBlkS), 0, s, params, st.tab, P, target, B,
                      1.0f / (float)B, st.tg, st.part);
   A3D_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_sg_reduce, dim3(gsz(kOffLoss + 1)), dim3(256), 0, s, st.part, blocks * kWavesS, st.mg);
+  hipLaunchKernelGGL(k_sg_reduce, dim3(gsz(kOffLoss + 1, 64)), dim3(1024), 0, s, st.part, blocks * kWavesS, st.mg);
   A3D_LAUNCH_CHECK();
   if (grad_out) A3D_HIP_CHECK(hipMemcpyAsync(grad_out, st.mg, kStride * 4, hipMemcpyDeviceToDevice, s));
   return A3D_OK;
