@@ -65,40 +65,17 @@ __device__ __forceinline__ void write_result(int64_t q, v3 p, float best, int32_
 }
 
 // ------------------------------------------------------------ wave-coherent walk
-// seed (optional, indexed like the outputs): an upper bound on each query's
-// distance; the walk starts with it as the best distance, so it never enters
-// a box farther than the bound.  A query with no face within its bound gets
-// face -1 and dist = its seed.
+// The wave-coherent packet walk of one wavefront (see the file comment):
+// each lane's best squared distance / face enter as the seed state and leave
+// as the result.  stk: this wave's LDS stack.  n_*: work counters (STATS).
 template <bool STATS>
-__global__ __launch_bounds__(kBlk) void k_udf_wave(const void* __restrict__ blob, const float* __restrict__ P,
-                                                    const uint32_t* __restrict__ perm, int64_t K,
-                                                    const float* __restrict__ seed,
-                                                    float* __restrict__ dist, int32_t* __restrict__ face,
-                                                    float* __restrict__ closest, float* __restrict__ uvw,
-                                                    unsigned long long* __restrict__ counters) {
-  __shared__ float4 s_stk[kWaves][kStackTotal][2];  // (lo.xyz, ref bits), (hi.xyz, -)
-  const int lane = threadIdx.x & 63;
-  const int wv = threadIdx.x >> 6;
-  const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlk + threadIdx.x;
-  const bool valid = i < K;
-  if (!__any(valid)) return;
-  const int64_t q = valid ? (perm ? (int64_t)perm[i] : i) : 0;
-  const BvhHeader* H = header(blob);
-  const Node64* __restrict__ nodes = nodes_of(blob);
-  const Tri48* __restrict__ tris = tris_of(blob);
-  const v3 p = valid ? mk(P[q * 3], P[q * 3 + 1], P[q * 3 + 2]) : mk(0.0f, 0.0f, 0.0f);
-  const float slack = query_slack(p, H->coord_max);
-
-  float best = 1e30f;
-  if (seed && valid) {
-    const float sd = seed[q];
-    if (sd >= 0.0f && sd < 1e15f) best = fmaxf(sd * sd * (1.0f + 4.8e-7f), 1e-30f);
-  }
-  int32_t best_id = 0x7fffffff, best_tri = -1;
+__device__ __forceinline__ void wave_walk(const Node64* __restrict__ nodes, const Tri48* __restrict__ tris, v3 p,
+                                          bool valid, float slack, float4 (*stk)[2], int lane, float& best,
+                                          int32_t& best_id, int32_t& best_tri, uint32_t& n_node,
+                                          uint32_t& n_leaf_tri, uint32_t& n_test, uint32_t& n_lane) {
   float thr = valid ? prune_thr(best, slack) : -1.0f;  // invalid lanes accept nothing
   int sp = 0;
   int32_t cur = 0;
-  uint32_t n_node = 0, n_leaf_tri = 0, n_test = 0, n_lane = 0;
   while (true) {
     if (cur >= 0) {
       if (STATS) ++n_node;
@@ -116,11 +93,11 @@ __global__ __launch_bounds__(kBlk) void k_udf_wave(const void* __restrict__ blob
         if (sp < kStackTotal) {
           if (lane == 0) {
             if (left_first) {
-              s_stk[wv][sp][0] = make_float4(nd.b.z, nd.b.w, nd.c.x, __int_as_float(far_ref));
-              s_stk[wv][sp][1] = make_float4(nd.c.y, nd.c.z, nd.c.w, 0.0f);
+              stk[sp][0] = make_float4(nd.b.z, nd.b.w, nd.c.x, __int_as_float(far_ref));
+              stk[sp][1] = make_float4(nd.c.y, nd.c.z, nd.c.w, 0.0f);
             } else {
-              s_stk[wv][sp][0] = make_float4(nd.a.x, nd.a.y, nd.a.z, __int_as_float(far_ref));
-              s_stk[wv][sp][1] = make_float4(nd.a.w, nd.b.x, nd.b.y, 0.0f);
+              stk[sp][0] = make_float4(nd.a.x, nd.a.y, nd.a.z, __int_as_float(far_ref));
+              stk[sp][1] = make_float4(nd.a.w, nd.b.x, nd.b.y, 0.0f);
             }
           }
           ++sp;
@@ -153,7 +130,7 @@ __global__ __launch_bounds__(kBlk) void k_udf_wave(const void* __restrict__ blob
     bool found = false;
     while (sp > 0) {
       --sp;
-      float4 e0 = s_stk[wv][sp][0], e1 = s_stk[wv][sp][1];
+      float4 e0 = stk[sp][0], e1 = stk[sp][1];
       float d = box_d2(p, e0.x, e0.y, e0.z, e1.x, e1.y, e1.z);
       if (__ballot(d <= thr)) {
         cur = uni(__float_as_int(e0.w));
@@ -163,6 +140,41 @@ __global__ __launch_bounds__(kBlk) void k_udf_wave(const void* __restrict__ blob
     }
     if (!found) break;
   }
+}
+
+// seed (optional, indexed like the outputs): an upper bound on each query's
+// distance; the walk starts with it as the best distance, so it never enters
+// a box farther than the bound.  A query with no face within its bound gets
+// face -1 and dist = its seed.
+template <bool STATS>
+__global__ __launch_bounds__(kBlk) void k_udf_wave(const void* __restrict__ blob, const float* __restrict__ P,
+                                                    const uint32_t* __restrict__ perm, int64_t K,
+                                                    const float* __restrict__ seed,
+                                                    float* __restrict__ dist, int32_t* __restrict__ face,
+                                                    float* __restrict__ closest, float* __restrict__ uvw,
+                                                    unsigned long long* __restrict__ counters) {
+  __shared__ float4 s_stk[kWaves][kStackTotal][2];  // (lo.xyz, ref bits), (hi.xyz, -)
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlk + threadIdx.x;
+  const bool valid = i < K;
+  if (!__any(valid)) return;
+  const int64_t q = valid ? (perm ? (int64_t)perm[i] : i) : 0;
+  const BvhHeader* H = header(blob);
+  const Node64* __restrict__ nodes = nodes_of(blob);
+  const Tri48* __restrict__ tris = tris_of(blob);
+  const v3 p = valid ? mk(P[q * 3], P[q * 3 + 1], P[q * 3 + 2]) : mk(0.0f, 0.0f, 0.0f);
+  const float slack = query_slack(p, H->coord_max);
+
+  float best = 1e30f;
+  if (seed && valid) {
+    const float sd = seed[q];
+    if (sd >= 0.0f && sd < 1e15f) best = fmaxf(sd * sd * (1.0f + 4.8e-7f), 1e-30f);
+  }
+  int32_t best_id = 0x7fffffff, best_tri = -1;
+  uint32_t n_node = 0, n_leaf_tri = 0, n_test = 0, n_lane = 0;
+  wave_walk<STATS>(nodes, tris, p, valid, slack, s_stk[wv], lane, best, best_id, best_tri, n_node, n_leaf_tri,
+                   n_test, n_lane);
   if (STATS) {
     if (!valid) n_lane = 0;
     for (int o = 32; o > 0; o >>= 1) n_lane += __shfl_xor(n_lane, o);
@@ -178,6 +190,24 @@ __global__ __launch_bounds__(kBlk) void k_udf_wave(const void* __restrict__ blob
     if (seed && best_tri < 0) best = seed[q] * seed[q];
     write_result(q, p, best, best_id, best_tri, tris, dist, face, closest, uvw);
   }
+}
+
+
+// One packet of the batched walk: lane's query q (if valid) on `blob`.
+__device__ __forceinline__ void batch_walk_tile(const void* __restrict__ blob, const float* __restrict__ P, int64_t q,
+                                                bool valid, float4 (*stk)[2], int lane, float* __restrict__ dist,
+                                                int32_t* __restrict__ face, float* __restrict__ closest,
+                                                float* __restrict__ uvw) {
+  const BvhHeader* H = header(blob);
+  const Node64* __restrict__ nodes = nodes_of(blob);
+  const Tri48* __restrict__ tris = tris_of(blob);
+  const v3 p = valid ? mk(P[q * 3], P[q * 3 + 1], P[q * 3 + 2]) : mk(0.0f, 0.0f, 0.0f);
+  const float slack = query_slack(p, H->coord_max);
+  float best = 1e30f;
+  int32_t best_id = 0x7fffffff, best_tri = -1;
+  uint32_t n0 = 0, n1 = 0, n2 = 0, n3 = 0;
+  wave_walk<false>(nodes, tris, p, valid, slack, stk, lane, best, best_id, best_tri, n0, n1, n2, n3);
+  if (valid) write_result(q, p, best, best_id, best_tri, tris, dist, face, closest, uvw);
 }
 
 // ------------------------------------------------------------ per-lane walk
@@ -441,3 +471,146 @@ extern "C" int a3d_bvh_udf_seeded(const void* blob, const float* P, int64_t K, c
   if (!seed && K > 0) return fail(A3D_ERR_ARG, "a3d_bvh_udf_seeded: null seed");
   return udf_launch(blob, P, K, perm, seed, dist, face, closest, uvw, flags, nullptr, stream);
 }
+
+// ============================================================ batched walk
+// Batched closest-point queries over many meshes of uneven size in one
+// persistent launch, plus HIP-graph capture helpers.  No counterpart in the
+// reference (its MeshBVH answers one mesh per call).
+//
+// Plan (once per batch): every query gets the key mesh << 30 | its 30-bit
+// Morton code in its mesh's box; one radix sort makes the query order
+// mesh-major and spatially coherent within each mesh.  The caller splits
+// each mesh's sorted range into tiles of at most 64 queries (one wavefront
+// packet) and may order the tiles (e.g. most expensive meshes first).
+// Run: a persistent grid of waves takes tiles from one atomic counter and
+// walks each tile as a packet on its mesh's tree (the same walk and the same
+// closest-face definition as a3d_bvh_udf, so results are bit-identical),
+// writing results at the queries' positions.  Every launch is sync-free and
+// can be captured in a graph; the counter is reset by a kernel, not a memset.
+namespace a3d {
+
+namespace {
+
+constexpr int kBlkB = 256;
+
+__device__ __forceinline__ uint32_t spread10b(uint32_t v) {
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+__global__ void k_batch_keys(const void* const* __restrict__ blobs, int n_meshes, const float* __restrict__ P,
+                             int64_t K, const int64_t* __restrict__ q_off, uint64_t* __restrict__ keys,
+                             uint32_t* __restrict__ vals) {
+  const int64_t i = blockIdx.x * (int64_t)kBlkB + threadIdx.x;
+  if (i >= K) return;
+  int lo = 0, hi = n_meshes - 1;  // last mesh with q_off[m] <= i
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (q_off[mid] <= i) lo = mid; else hi = mid - 1;
+  }
+  const BvhHeader* H = header(blobs[lo]);
+  uint32_t c[3];
+  for (int k = 0; k < 3; ++k) {
+    const float l = H->scene_lo[k], ext = H->scene_hi[k] - l;
+    float t = ext > 0.0f ? (P[i * 3 + k] - l) * (1023.0f / ext) : 0.0f;
+    t = fminf(fmaxf(t, 0.0f), 1023.0f);
+    c[k] = (uint32_t)t;
+  }
+  keys[i] = ((uint64_t)lo << 30) | (spread10b(c[0]) << 2) | (spread10b(c[1]) << 1) | spread10b(c[2]);
+  vals[i] = (uint32_t)i;
+}
+
+__global__ void k_batch_reset(unsigned int* counter) {
+  if (threadIdx.x == 0) *counter = 0u;
+}
+
+// tiles: int64 pairs (start << 8 | count, mesh), start = index into perm.
+__global__ void k_udf_batch(const void* const* __restrict__ blobs, const float* __restrict__ P,
+                            const uint32_t* __restrict__ perm, const int64_t* __restrict__ tiles, int64_t n_tiles,
+                            unsigned int* __restrict__ counter, float* __restrict__ dist, int32_t* __restrict__ face,
+                            float* __restrict__ closest, float* __restrict__ uvw) {
+  extern __shared__ float4 s_dyn[];
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  float4(*stk)[2] = reinterpret_cast<float4(*)[2]>(s_dyn + (size_t)wv * kStackTotal * 2);
+  while (true) {
+    unsigned int t = 0;
+    if (lane == 0) t = atomicAdd(counter, 1u);
+    t = __builtin_amdgcn_readfirstlane(t);
+    if ((int64_t)t >= n_tiles) break;
+    const int64_t w0 = tiles[2 * (int64_t)t], mesh = tiles[2 * (int64_t)t + 1];
+    const int64_t start = w0 >> 8;
+    const int count = (int)(w0 & 0xff);
+    const bool valid = lane < count;
+    const int64_t q = valid ? (int64_t)perm[start + lane] : 0;
+    batch_walk_tile(blobs[mesh], P, q, valid, stk, lane, dist, face, closest, uvw);
+  }
+}
+
+size_t sort_bytes(int64_t K) {
+  size_t t = 0;
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, t, (uint64_t*)nullptr, (uint64_t*)nullptr, (uint32_t*)nullptr,
+                                           (uint32_t*)nullptr, (int)(K > 1 ? K : 2), 0, 40);
+  return t;
+}
+
+}  // namespace
+}  // namespace a3d
+
+using namespace a3d;
+
+extern "C" int a3d_batch_order_ws_bytes(int64_t K, int64_t* bytes) {
+  if (K < 0 || K > 0x7fffffffll || !bytes) return fail(A3D_ERR_ARG, "a3d_batch_order_ws_bytes: bad argument");
+  const int64_t n = K > 0 ? K : 1;
+  *bytes = 2 * al256(8 * n) + al256(4 * n) + al256((int64_t)sort_bytes(K));
+  return A3D_OK;
+}
+
+extern "C" int a3d_batch_order(const void* const* blobs, int n_meshes, const float* P, int64_t K,
+                               const int64_t* q_off, uint32_t* perm, void* ws, int64_t ws_bytes,
+                               a3d_stream_t stream) {
+  int64_t need = 0;
+  if (int rc = a3d_batch_order_ws_bytes(K, &need)) return rc;
+  if (n_meshes <= 0 || n_meshes > (1 << 24)) return fail(A3D_ERR_ARG, "a3d_batch_order: n_meshes out of range");
+  if (K == 0) return A3D_OK;
+  if (!blobs || !P || !q_off || !perm || !ws) return fail(A3D_ERR_ARG, "a3d_batch_order: null argument");
+  if (ws_bytes < need) return fail(A3D_ERR_CAPACITY, "a3d_batch_order: workspace too small");
+  hipStream_t s = (hipStream_t)stream;
+  char* w = (char*)ws;
+  uint64_t* k0 = (uint64_t*)w;
+  uint64_t* k1 = (uint64_t*)(w + al256(8 * K));
+  uint32_t* v0 = (uint32_t*)(w + 2 * al256(8 * K));
+  void* tmp = w + 2 * al256(8 * K) + al256(4 * K);
+  size_t tb = sort_bytes(K);
+  hipLaunchKernelGGL(k_batch_keys, dim3((unsigned)div_up(K, kBlkB)), dim3(kBlkB), 0, s, blobs, n_meshes, P, K, q_off,
+                     k0, v0);
+  A3D_LAUNCH_CHECK();
+  int bits = 30;
+  while ((1ll << (bits - 30)) < n_meshes) ++bits;
+  A3D_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, k0, k1, v0, perm, (int)K, 0, bits, s));
+  return A3D_OK;
+}
+
+extern "C" int a3d_batch_udf(const void* const* blobs, const float* P, const uint32_t* perm, const int64_t* tiles,
+                             int64_t n_tiles, unsigned int* counter, int waves_per_block, int blocks, float* dist,
+                             int32_t* face, float* closest, float* uvw, a3d_stream_t stream) {
+  if (n_tiles < 0 || n_tiles > 0xffffffffll) return fail(A3D_ERR_ARG, "a3d_batch_udf: n_tiles out of range");
+  if (waves_per_block < 1 || waves_per_block > 16 || blocks < 1 || blocks > (1 << 20))
+    return fail(A3D_ERR_ARG, "a3d_batch_udf: launch shape %d waves x %d blocks out of range", waves_per_block, blocks);
+  if (n_tiles == 0) return A3D_OK;
+  if (!blobs || !P || !perm || !tiles || !counter || !dist || !face || !closest)
+    return fail(A3D_ERR_ARG, "a3d_batch_udf: null argument");
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, counter);
+  A3D_LAUNCH_CHECK();
+  const size_t lds = (size_t)waves_per_block * kStackTotal * 2 * sizeof(float4);
+  hipLaunchKernelGGL(k_udf_batch, dim3(blocks), dim3(64 * waves_per_block), lds, s, blobs, P, perm, tiles, n_tiles,
+                     counter, dist, face, closest, uvw);
+  A3D_LAUNCH_CHECK();
+  return A3D_OK;
+}
+

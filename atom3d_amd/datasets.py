@@ -174,3 +174,39 @@ def dmc_band(res: int = 1024, shapes: dict | None = None, band: float = 0.866025
         Pc = torch.stack([u // (R1 * R1), (u // R1) % R1, u % R1], 1).double() * h - 1.0
         sdf[s0:s0 + step] = shapes_sdf(Pc, sh).float()
     return cubes.int(), sdf, inv.view(-1, 8).int()
+
+
+# ------------------------------------------------------------------ uneven batch
+def uneven_batch(n_meshes: int = 256, seed: int = 3, tri_min: int = 1000, tri_max: int = 200000,
+                 q_min: int = 1 << 14, q_max: int = 1 << 20):
+    """Meshes of log-uniform triangle counts (a noisy icosphere when one
+    sphere of a subdivision level reaches the count, else a union of such
+    spheres) paired with log-uniform query counts.  Returns a list of
+    (V float32 [n,3], F int32 [m,3], n_queries)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for m in range(n_meshes):
+        target = float(np.exp(rng.uniform(np.log(tri_min), np.log(tri_max))))
+        nq = int(np.exp(rng.uniform(np.log(q_min), np.log(q_max))))
+        s = int(min(6, max(2, np.floor(np.log(target / 20.0) / np.log(4.0)))))
+        k = max(1, int(round(target / (20 * 4 ** s))))
+        if k == 1:
+            V, F = noisy_icosphere(s, radius=0.5, sigma=0.01, seed=seed * 1000 + m)
+        else:
+            V, F = sphere_union(k, s, seed=seed * 1000 + m, r_min=0.05, r_max=0.3, extent=0.6)
+        out.append((V, F, nq))
+    return out
+
+
+def batch_queries(meshes, seed: int = 3, pad: float = 0.1, device="cuda"):
+    """Uniform queries in each mesh's bounding box padded by `pad` of its
+    extent, concatenated: (P float32 [K,3] on device, counts list)."""
+    g = torch.Generator(device=device).manual_seed(seed)
+    parts, counts = [], []
+    for V, _, nq in meshes:
+        lo, hi = V.min(0)[0], V.max(0)[0]
+        ext = hi - lo
+        lo, hi = (lo - pad * ext).to(device), (hi + pad * ext).to(device)
+        parts.append(torch.rand(nq, 3, generator=g, device=device) * (hi - lo) + lo)
+        counts.append(nq)
+    return torch.cat(parts).contiguous(), counts

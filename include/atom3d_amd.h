@@ -307,6 +307,32 @@ int a3d_sg_udf(const void* params, const void* blob, const float* P, int64_t K, 
                float* dist, int32_t* face, float* closest, float* uvw, void* ws, int64_t ws_bytes,
                int64_t* n_near_host, a3d_stream_t stream);
 
+/* ------------------------------------------------------------------ batches
+ * Closest-point queries over many meshes of uneven size in one persistent
+ * launch.  No counterpart in the reference (one mesh per MeshBVH call).
+ * blobs: device array of n_meshes BVH blob pointers; P: all queries
+ * concatenated, mesh m owning [q_off[m], q_off[m+1]) (q_off device int64).
+ * a3d_batch_order: perm (uint32 [K]) = queries sorted by mesh, then by Morton
+ *   code in the mesh's box.  Workspace: a3d_batch_order_ws_bytes.
+ * a3d_batch_udf: tiles = int64 pairs (start << 8 | count, mesh), count <= 64,
+ *   covering perm[start, start + count); a persistent grid of `blocks` blocks
+ *   of `waves_per_block` wavefronts takes tiles from *counter (device uint32,
+ *   reset by the call) in table order.  Results equal a3d_bvh_udf's on each
+ *   mesh, written at the queries' positions.  Sync-free: capturable.
+ * a3d_graph_begin / end / launch / destroy: capture the work queued on a
+ *   stream into an instantiated HIP graph (opaque handle) and replay it.
+ */
+int a3d_batch_order_ws_bytes(int64_t K, int64_t* bytes);
+int a3d_batch_order(const void* const* blobs, int n_meshes, const float* P, int64_t K, const int64_t* q_off,
+                    uint32_t* perm, void* ws, int64_t ws_bytes, a3d_stream_t stream);
+int a3d_batch_udf(const void* const* blobs, const float* P, const uint32_t* perm, const int64_t* tiles,
+                  int64_t n_tiles, unsigned int* counter, int waves_per_block, int blocks, float* dist, int32_t* face,
+                  float* closest, float* uvw, a3d_stream_t stream);
+int a3d_graph_begin(a3d_stream_t stream);
+int a3d_graph_end(a3d_stream_t stream, int64_t* handle);
+int a3d_graph_launch(int64_t handle, a3d_stream_t stream);
+int a3d_graph_destroy(int64_t handle);
+
 #ifdef __cplusplus
 }
 #endif
