@@ -96,6 +96,8 @@ struct DmcWs {
   int32_t* quads;     // [3N][4]
   // gradient scratch (backward): per dual vertex and per quad
   void* gvd;          // [4N][3] of T
+  void* aux;          // [4N][5] of T: forward's dual vertex xyz, beta sum + 1e-8, mean crossing distance
+  void* sgs;          // [4N] of T: sum over a vertex's entries of gL * sign(dist - mean)
   void* cub;
   size_t cub_bytes;
   int64_t total;
@@ -160,6 +162,8 @@ DmcWs carve(void* base, int64_t N) {
   w.qrank = (int32_t*)take(4 * (12 * n + 1));
   w.quads = (int32_t*)take(16 * 3 * n);
   w.gvd = take(8 * 3 * 4 * n);
+  w.aux = take(8 * 5 * 4 * n);
+  w.sgs = take(8 * 4 * n);
   w.cub_bytes = cub_need(N);
   w.cub = take((int64_t)w.cub_bytes);
   w.total = o;
@@ -484,7 +488,7 @@ __global__ void k_entry_fwd(Geom g, const int32_t* __restrict__ totals, Fields<T
 // interpolated with the same weights.
 template <typename T>
 __global__ void k_vd_fwd(Geom g, const int32_t* __restrict__ totals, Fields<T> f, T* __restrict__ verts,
-                         T* __restrict__ L_dev, T* __restrict__ vfeat) {
+                         T* __restrict__ L_dev, T* __restrict__ vfeat, T* __restrict__ aux) {
   int64_t v = blockIdx.x * (int64_t)kBlk + threadIdx.x;
   if (v >= totals[0]) return;
   const int32_t o0 = g.vd_off[v], o1 = g.vd_off[v + 1];
@@ -525,6 +529,9 @@ __global__ void k_vd_fwd(Geom g, const int32_t* __restrict__ totals, Fields<T> f
   }
   const T mean = sum / (T)(o1 - o0);
   for (int32_t e = o0; e < o1; ++e) L_dev[e] = fabs(L_dev[e] - mean);
+  for (int d = 0; d < 3; ++d) aux[v * 5 + d] = vd[d];
+  aux[v * 5 + 3] = D;
+  aux[v * 5 + 4] = mean;
 }
 
 // Training-mode quad centres: mean of the 4 dual vertices, appended after them
@@ -588,150 +595,147 @@ __device__ __forceinline__ void interp1_bwd(T w0, T w1, T x0, T x1, T g, T& gw0,
 template <typename T>
 __device__ __forceinline__ T sgn(T x) { return x > T(0) ? T(1) : (x < T(0) ? T(-1) : T(0)); }
 
+// Pass A, one thread per dual vertex: the L_dev terms that feed back into
+// the vertex, g_vd -= sum_e gdist_e (zc_e - vd) / dist_e with
+// gdist_e = gL_e sign(r_e) - (sum_k gL_k sign(r_k)) / n, r = dist - mean.
 template <typename T>
-__global__ void k_vd_bwd(Geom g, const int32_t* __restrict__ totals, Fields<T> f, const T* __restrict__ vfeat,
-                         const T* __restrict__ gvd, const T* __restrict__ gvf, const T* __restrict__ gL,
-                         const T* __restrict__ gP, T* __restrict__ g_sdf, T* __restrict__ g_deform,
-                         T* __restrict__ g_alpha, T* __restrict__ g_beta, T* __restrict__ g_feat) {
+__global__ void k_vd_bwd_a(Geom g, const int32_t* __restrict__ totals, Fields<T> f, const T* __restrict__ aux,
+                           const T* __restrict__ gL, T* __restrict__ gvd, T* __restrict__ sgs) {
   int64_t v = blockIdx.x * (int64_t)kBlk + threadIdx.x;
   if (v >= totals[0]) return;
   const int32_t o0 = g.vd_off[v], o1 = g.vd_off[v + 1];
-  const int C = f.C;
-  // recompute the dual vertex
-  T num[3] = {T(0), T(0), T(0)}, bs = T(0);
+  const T vd[3] = {aux[v * 5], aux[v * 5 + 1], aux[v * 5 + 2]};
+  const T mean = aux[v * 5 + 4];
+  const T n = (T)(o1 - o0);
+  T sg_sum = T(0);
   for (int32_t e = o0; e < o1; ++e) {
     EdgeVals<T> ev;
-    edge_vals(g, g.ent_slot[e], f, true, ev);
-    T ue[3];
-    interp(ev.s0 * ev.a0, ev.s1 * ev.a1, ev.x0, ev.x1, ue);
-    for (int d = 0; d < 3; ++d) num[d] = num[d] + ue[d] * ev.beta;
-    bs = bs + ev.beta;
+    edge_vals(g, g.ent_slot[e], f, false, ev);
+    T zc[3];
+    interp(ev.s0, ev.s1, ev.x0, ev.x1, zc);
+    T dd[3] = {zc[0] - vd[0], zc[1] - vd[1], zc[2] - vd[2]};
+    sg_sum += gL[e] * sgn(norm3(dd) - mean);
   }
-  const T D = bs + T(1e-8);
-  const T vd[3] = {num[0] / D, num[1] / D, num[2] / D};
   T gv[3] = {gvd[v * 3], gvd[v * 3 + 1], gvd[v * 3 + 2]};
-  // L_dev_e = |dist_e - mean|, dist_e = |zc_e - vd|, mean = sum(dist) / n
-  if (gL) {
-    const T n = (T)(o1 - o0);
-    T sum = T(0);
-    for (int32_t e = o0; e < o1; ++e) {
-      EdgeVals<T> ev;
-      edge_vals(g, g.ent_slot[e], f, false, ev);
-      T zc[3];
-      interp(ev.s0, ev.s1, ev.x0, ev.x1, zc);
-      T dd[3] = {zc[0] - vd[0], zc[1] - vd[1], zc[2] - vd[2]};
-      sum = sum + norm3(dd);
-    }
-    const T mean = sum / n;
-    T sg_sum = T(0);
-    for (int32_t e = o0; e < o1; ++e) {
-      EdgeVals<T> ev;
-      edge_vals(g, g.ent_slot[e], f, false, ev);
-      T zc[3];
-      interp(ev.s0, ev.s1, ev.x0, ev.x1, zc);
-      T dd[3] = {zc[0] - vd[0], zc[1] - vd[1], zc[2] - vd[2]};
-      sg_sum += gL[e] * sgn(norm3(dd) - mean);
-    }
-    for (int32_t e = o0; e < o1; ++e) {
-      EdgeVals<T> ev;
-      edge_vals(g, g.ent_slot[e], f, false, ev);
-      T zc[3];
-      interp(ev.s0, ev.s1, ev.x0, ev.x1, zc);
-      T dd[3] = {zc[0] - vd[0], zc[1] - vd[1], zc[2] - vd[2]};
-      const T dist = norm3(dd);
-      const T gdist = gL[e] * sgn(dist - mean) - sg_sum / n;
-      if (!(dist > T(0)) || gdist == T(0)) continue;
-      T gw0 = T(0), gw1 = T(0);
-      for (int d = 0; d < 3; ++d) {
-        const T gz = gdist * dd[d] / dist;
-        gv[d] -= gz;
-        T gx0, gx1;
-        interp1_bwd(ev.s0, ev.s1, ev.x0[d], ev.x1[d], gz, gw0, gw1, gx0, gx1);
-        if (g_deform) {
-          atomicAdd(&g_deform[ev.c0 * 3 + d], gx0);
-          atomicAdd(&g_deform[ev.c1 * 3 + d], gx1);
-        }
-      }
-      atomicAdd(&g_sdf[ev.c0], gw0);
-      atomicAdd(&g_sdf[ev.c1], gw1);
-    }
-  }
-  // vd = sum(ue_e beta_e) / (sum(beta_e) + 1e-8); features alike
   for (int32_t e = o0; e < o1; ++e) {
     EdgeVals<T> ev;
-    edge_vals(g, g.ent_slot[e], f, true, ev);
-    const T w0 = ev.s0 * ev.a0, w1 = ev.s1 * ev.a1;
-    T gw0 = T(0), gw1 = T(0), gb = T(0);
-    for (int d = 0; d < 3; ++d) {
-      const T ue = interp1(w0, w1, ev.x0[d], ev.x1[d]);
-      const T gue = gv[d] * ev.beta / D + (gP ? gP[(int64_t)e * 3 + d] : T(0));
-      gb += gv[d] * (ue - vd[d]) / D;
-      T gx0, gx1;
-      interp1_bwd(w0, w1, ev.x0[d], ev.x1[d], gue, gw0, gw1, gx0, gx1);
-      if (g_deform) {
-        atomicAdd(&g_deform[ev.c0 * 3 + d], gx0);
-        atomicAdd(&g_deform[ev.c1 * 3 + d], gx1);
-      }
-    }
-    if (gvf)
-      for (int c = 0; c < C; ++c) {
-        const T f0 = f.feat[ev.c0 * C + c], f1 = f.feat[ev.c1 * C + c];
-        const T uf = interp1(w0, w1, f0, f1);
-        const T gc = gvf[v * C + c];
-        gb += gc * (uf - vfeat[v * C + c]) / D;
-        T gf0, gf1;
-        interp1_bwd(w0, w1, f0, f1, gc * ev.beta / D, gw0, gw1, gf0, gf1);
-        if (g_feat) {
-          atomicAdd(&g_feat[ev.c0 * C + c], gf0);
-          atomicAdd(&g_feat[ev.c1 * C + c], gf1);
-        }
-      }
-    atomicAdd(&g_sdf[ev.c0], gw0 * ev.a0);
-    atomicAdd(&g_sdf[ev.c1], gw1 * ev.a1);
-    if (g_alpha && f.alpha) {
-      const int k0 = kE0[ev.e], k1 = kE1[ev.e];
-      const T t0 = tanh(f.alpha[ev.cube * 8 + k0]), t1 = tanh(f.alpha[ev.cube * 8 + k1]);
-      atomicAdd(&g_alpha[ev.cube * 8 + k0], gw0 * ev.s0 * f.ws * (T(1) - t0 * t0));
-      atomicAdd(&g_alpha[ev.cube * 8 + k1], gw1 * ev.s1 * f.ws * (T(1) - t1 * t1));
-    }
-    if (g_beta && f.beta) {
-      const T tb = tanh(f.beta[ev.cube * 12 + ev.e]);
-      atomicAdd(&g_beta[ev.cube * 12 + ev.e], gb * f.ws * (T(1) - tb * tb));
-    }
+    edge_vals(g, g.ent_slot[e], f, false, ev);
+    T zc[3];
+    interp(ev.s0, ev.s1, ev.x0, ev.x1, zc);
+    T dd[3] = {zc[0] - vd[0], zc[1] - vd[1], zc[2] - vd[2]};
+    const T dist = norm3(dd);
+    const T gdist = gL[e] * sgn(dist - mean) - sg_sum / n;
+    if (dist > T(0) && gdist != T(0))
+      for (int d = 0; d < 3; ++d) gv[d] -= gdist * dd[d] / dist;
   }
+  for (int d = 0; d < 3; ++d) gvd[v * 3 + d] = gv[d];
+  sgs[v] = sg_sum;
 }
 
-// n_alpha = normalize(sign(ds / el) * ev / el), el = |ev| + 1e-8, ev = x1 - x0:
-// its gradient reaches only the deformation (the sign is piecewise constant).
+// Pass B, one thread per edge-group entry: every gradient path through the
+// entry's edge (its alpha-weighted crossing into the vertex, beta, features,
+// its plain crossing through L_dev, its normal through n_alpha) summed
+// locally, then one atomic per corner component.  Each (cube, edge) slot
+// belongs to one entry, so the beta gradient is a plain store.
 template <typename T>
-__global__ void k_entry_bwd(Geom g, const int32_t* __restrict__ totals, Fields<T> f, const T* __restrict__ gN,
-                            T* __restrict__ g_deform) {
+__global__ void k_entry_bwd(Geom g, const int32_t* __restrict__ totals, Fields<T> f, const int32_t* __restrict__ ent_vd,
+                            const T* __restrict__ aux, const T* __restrict__ gvd, const T* __restrict__ sgs,
+                            const T* __restrict__ vfeat, const T* __restrict__ gvf, const T* __restrict__ gL,
+                            const T* __restrict__ gP, const T* __restrict__ gN, T* __restrict__ g_sdf,
+                            T* __restrict__ g_deform, T* __restrict__ g_alpha, T* __restrict__ g_beta,
+                            T* __restrict__ g_feat) {
   int64_t e = blockIdx.x * (int64_t)kBlk + threadIdx.x;
   if (e >= totals[1]) return;
-  EdgeVals<T> v;
-  edge_vals(g, g.ent_slot[e], f, false, v);
-  T ev[3] = {v.x1[0] - v.x0[0], v.x1[1] - v.x0[1], v.x1[2] - v.x0[2]};
-  const T len = norm3(ev), el = len + T(1e-8);
-  const T sg = sgn((v.s1 - v.s0) / el);
-  T m[3];
-  for (int d = 0; d < 3; ++d) m[d] = (ev[d] / el) * sg;
-  const T ml = norm3(m);
-  T gm[3];
-  const T gn[3] = {gN[e * 3], gN[e * 3 + 1], gN[e * 3 + 2]};
-  if (ml > T(1e-12)) {
-    T n[3] = {m[0] / ml, m[1] / ml, m[2] / ml};
-    T ng = (n[0] * gn[0] + n[1] * gn[1]) + n[2] * gn[2];
-    for (int d = 0; d < 3; ++d) gm[d] = (gn[d] - n[d] * ng) / ml;
-  } else {
-    for (int d = 0; d < 3; ++d) gm[d] = gn[d] / T(1e-12);
-  }
-  // m = ev * (sg / el)
-  const T eg = (ev[0] * gm[0] + ev[1] * gm[1]) + ev[2] * gm[2];
+  const int32_t v = ent_vd[e];
+  const int C = f.C;
+  EdgeVals<T> ev;
+  edge_vals(g, g.ent_slot[e], f, true, ev);
+  const T vd[3] = {aux[v * 5], aux[v * 5 + 1], aux[v * 5 + 2]};
+  const T D = aux[v * 5 + 3];
+  const T gv[3] = {gvd[v * 3], gvd[v * 3 + 1], gvd[v * 3 + 2]};
+  const T w0 = ev.s0 * ev.a0, w1 = ev.s1 * ev.a1;
+  T gw0 = T(0), gw1 = T(0), gs0 = T(0), gs1 = T(0), gb = T(0);
+  T gx0[3] = {T(0), T(0), T(0)}, gx1[3] = {T(0), T(0), T(0)};
+  // vd = sum(ue beta) / D, ue = interp(s a, x); p_alpha = ue
   for (int d = 0; d < 3; ++d) {
-    T gev = gm[d] * (sg / el);
-    if (len > T(0)) gev -= eg * sg / (el * el) * (ev[d] / len);
-    atomicAdd(&g_deform[v.c1 * 3 + d], gev);
-    atomicAdd(&g_deform[v.c0 * 3 + d], -gev);
+    const T ue = interp1(w0, w1, ev.x0[d], ev.x1[d]);
+    const T gue = gv[d] * ev.beta / D + (gP ? gP[e * 3 + d] : T(0));
+    gb += gv[d] * (ue - vd[d]) / D;
+    T t0, t1;
+    interp1_bwd(w0, w1, ev.x0[d], ev.x1[d], gue, gw0, gw1, t0, t1);
+    gx0[d] += t0;
+    gx1[d] += t1;
+  }
+  if (gvf)
+    for (int c = 0; c < C; ++c) {
+      const T f0 = f.feat[ev.c0 * C + c], f1 = f.feat[ev.c1 * C + c];
+      const T gc = gvf[(int64_t)v * C + c];
+      gb += gc * (interp1(w0, w1, f0, f1) - vfeat[(int64_t)v * C + c]) / D;
+      T t0, t1;
+      interp1_bwd(w0, w1, f0, f1, gc * ev.beta / D, gw0, gw1, t0, t1);
+      if (g_feat) {
+        atomicAdd(&g_feat[ev.c0 * C + c], t0);
+        atomicAdd(&g_feat[ev.c1 * C + c], t1);
+      }
+    }
+  // L_dev_e = |dist_e - mean|, dist_e = |zc_e - vd|, zc = interp(s, x)
+  if (gL) {
+    const int32_t n = g.vd_off[v + 1] - g.vd_off[v];
+    T zc[3];
+    interp(ev.s0, ev.s1, ev.x0, ev.x1, zc);
+    T dd[3] = {zc[0] - vd[0], zc[1] - vd[1], zc[2] - vd[2]};
+    const T dist = norm3(dd);
+    const T gdist = gL[e] * sgn(dist - aux[v * 5 + 4]) - sgs[v] / (T)n;
+    if (dist > T(0) && gdist != T(0))
+      for (int d = 0; d < 3; ++d) {
+        T t0, t1;
+        interp1_bwd(ev.s0, ev.s1, ev.x0[d], ev.x1[d], gdist * dd[d] / dist, gs0, gs1, t0, t1);
+        gx0[d] += t0;
+        gx1[d] += t1;
+      }
+  }
+  // n_alpha = normalize(sign(ds / el) ev / el), el = |ev| + 1e-8, ev = x1 - x0:
+  // only the deformation receives a gradient (the sign is piecewise constant)
+  if (gN && g_deform && f.deform) {
+    T evv[3] = {ev.x1[0] - ev.x0[0], ev.x1[1] - ev.x0[1], ev.x1[2] - ev.x0[2]};
+    const T len = norm3(evv), el = len + T(1e-8);
+    const T sg = sgn((ev.s1 - ev.s0) / el);
+    T m[3];
+    for (int d = 0; d < 3; ++d) m[d] = (evv[d] / el) * sg;
+    const T ml = norm3(m);
+    const T gn[3] = {gN[e * 3], gN[e * 3 + 1], gN[e * 3 + 2]};
+    T gm[3];
+    if (ml > T(1e-12)) {
+      const T nn[3] = {m[0] / ml, m[1] / ml, m[2] / ml};
+      const T ng = (nn[0] * gn[0] + nn[1] * gn[1]) + nn[2] * gn[2];
+      for (int d = 0; d < 3; ++d) gm[d] = (gn[d] - nn[d] * ng) / ml;
+    } else {
+      for (int d = 0; d < 3; ++d) gm[d] = gn[d] / T(1e-12);
+    }
+    const T eg = (evv[0] * gm[0] + evv[1] * gm[1]) + evv[2] * gm[2];
+    for (int d = 0; d < 3; ++d) {
+      T gev = gm[d] * (sg / el);
+      if (len > T(0)) gev -= eg * sg / (el * el) * (evv[d] / len);
+      gx1[d] += gev;
+      gx0[d] -= gev;
+    }
+  }
+  atomicAdd(&g_sdf[ev.c0], gw0 * ev.a0 + gs0);
+  atomicAdd(&g_sdf[ev.c1], gw1 * ev.a1 + gs1);
+  if (g_deform)
+    for (int d = 0; d < 3; ++d) {
+      atomicAdd(&g_deform[ev.c0 * 3 + d], gx0[d]);
+      atomicAdd(&g_deform[ev.c1 * 3 + d], gx1[d]);
+    }
+  if (g_alpha && f.alpha) {
+    const int k0 = kE0[ev.e], k1 = kE1[ev.e];
+    const T t0 = tanh(f.alpha[ev.cube * 8 + k0]), t1 = tanh(f.alpha[ev.cube * 8 + k1]);
+    atomicAdd(&g_alpha[ev.cube * 8 + k0], gw0 * ev.s0 * f.ws * (T(1) - t0 * t0));
+    atomicAdd(&g_alpha[ev.cube * 8 + k1], gw1 * ev.s1 * f.ws * (T(1) - t1 * t1));
+  }
+  if (g_beta && f.beta) {
+    const T tb = tanh(f.beta[ev.cube * 12 + ev.e]);
+    g_beta[ev.cube * 12 + ev.e] = gb * f.ws * (T(1) - tb * tb);
   }
 }
 
@@ -935,7 +939,7 @@ int dmc_forward_t(const DmcWs& w, const int64_t* sizes, const Geom& g, const Fie
   hipLaunchKernelGGL(k_entry_fwd<T>, dim3(gridn(E)), dim3(kBlk), 0, s, g, w.totals, f, (T*)p_alpha, (T*)n_alpha);
   A3D_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_vd_fwd<T>, dim3(gridn(Vd)), dim3(kBlk), 0, s, g, w.totals, f, (T*)verts, (T*)L_dev,
-                     (T*)vfeat);
+                     (T*)vfeat, (T*)w.aux);
   A3D_LAUNCH_CHECK();
   if (Q > 0) {
     hipLaunchKernelGGL(k_centers<T>, dim3(gridn(Q)), dim3(kBlk), 0, s, w.quads, Q, Vd, (T*)verts, (T*)vfeat, f.C);
@@ -977,20 +981,21 @@ int dmc_backward_t(const DmcWs& w, const int64_t* sizes, int64_t N, int64_t M, c
       A3D_LAUNCH_CHECK();
     }
   }
-  hipLaunchKernelGGL(k_vd_bwd<T>, dim3(gridn(Vd)), dim3(kBlk), 0, s, g, w.totals, f, (const T*)vfeat, (const T*)gvd,
-                     (const T*)gvf, (const T*)g_L_dev, (const T*)g_p_alpha, (T*)g_sdf, (T*)g_deform, (T*)g_alpha,
-                     (T*)g_beta, (T*)g_feat);
-  A3D_LAUNCH_CHECK();
-  if (g_n_alpha && g_deform && f.deform) {
-    hipLaunchKernelGGL(k_entry_bwd<T>, dim3(gridn(E)), dim3(kBlk), 0, s, g, w.totals, f, (const T*)g_n_alpha,
-                       (T*)g_deform);
+  if (g_L_dev) {
+    hipLaunchKernelGGL(k_vd_bwd_a<T>, dim3(gridn(Vd)), dim3(kBlk), 0, s, g, w.totals, f, (const T*)w.aux,
+                       (const T*)g_L_dev, gvd, (T*)w.sgs);
     A3D_LAUNCH_CHECK();
   }
+  hipLaunchKernelGGL(k_entry_bwd<T>, dim3(gridn(E)), dim3(kBlk), 0, s, g, w.totals, f, w.ent_vd, (const T*)w.aux,
+                     (const T*)gvd, (const T*)w.sgs, (const T*)vfeat, (const T*)gvf, (const T*)g_L_dev,
+                     (const T*)g_p_alpha, (const T*)g_n_alpha, (T*)g_sdf, (T*)g_deform, (T*)g_alpha, (T*)g_beta,
+                     (T*)g_feat);
+  A3D_LAUNCH_CHECK();
   return A3D_OK;
 }
 }  // namespace
 
-extern "C" int a3d_dmc_forward(const void* ws, int64_t N, const int64_t* sizes, const int32_t* vox, const int32_t* ci,
+extern "C" int a3d_dmc_forward(void* ws, int64_t N, const int64_t* sizes, const int32_t* vox, const int32_t* ci,
                                const void* sdf, const void* deform, const void* alpha, const void* beta,
                                const void* feat, int C, int dtype, double isovalue, double weight_scale, int res,
                                void* verts, void* L_dev, void* p_alpha, void* n_alpha, void* vfeat,

@@ -237,9 +237,11 @@ int a3d_flood_dam_mask(const uint32_t* state, int D, int H, int W, const int64_t
  *   optional vfeat [Vd+Q,C].
  * a3d_dmc_backward: gradients of those outputs (each may be NULL = zero) to
  *   g_sdf [M] and, where non-NULL, g_deform [M,3], g_alpha [N,8], g_beta
- *   [N,12], g_feat [M,C] (all overwritten).  The interpolation weights and
- *   dual vertices are recomputed, not stored.  gvf_scratch: [Vd,C] values,
- *   needed with g_vfeat.
+ *   [N,12], g_feat [M,C] (all overwritten).  Follows an a3d_dmc_forward on
+ *   the same workspace with the same inputs (which leaves each dual vertex,
+ *   its weight sum and mean crossing distance there); the interpolation
+ *   weights are recomputed, not stored.  gvf_scratch: [Vd,C] values, needed
+ *   with g_vfeat.
  * a3d_dmc_topology_get: int32 copies of the topology (kind 0 surface cube
  *   ids, 1 case ids, 2 entry -> dual vertex, 3 surface edges [.,2], 4 quads
  *   [.,4]).
@@ -249,7 +251,7 @@ int a3d_dmc_topology(const int32_t* vox, const void* sdf, int dtype, double isov
                      int64_t M, int res, const int8_t* tables_host, void* ws, int64_t ws_bytes, int64_t* sizes_host,
                      a3d_stream_t stream);
 int a3d_dmc_faces(const void* ws, int64_t N, const int64_t* sizes, int64_t* faces, a3d_stream_t stream);
-int a3d_dmc_forward(const void* ws, int64_t N, const int64_t* sizes, const int32_t* vox, const int32_t* ci,
+int a3d_dmc_forward(void* ws, int64_t N, const int64_t* sizes, const int32_t* vox, const int32_t* ci,
                     const void* sdf, const void* deform, const void* alpha, const void* beta, const void* feat, int C,
                     int dtype, double isovalue, double weight_scale, int res, void* verts, void* L_dev,
                     void* p_alpha, void* n_alpha, void* vfeat, a3d_stream_t stream);
