@@ -62,7 +62,7 @@ class BatchGraph:
 
 
 class BatchedUDF:
-    def __init__(self, meshes, device="cuda", leaf: int = 4):
+    def __init__(self, meshes, device="cuda", leaf: int = 8):
         self.device = torch.device(device)
         self.bvhs = [BVHAccelerator(torch.as_tensor(V).to(self.device), torch.as_tensor(F).to(self.device),
                                     n_primitives_per_leaf=leaf) for V, F in meshes]

@@ -64,16 +64,85 @@ __device__ __forceinline__ void write_result(int64_t q, v3 p, float best, int32_
   }
 }
 
+// ------------------------------------------------------------ exact-test queue
+// The exact point-triangle tests of a wave are compacted: at a leaf, every
+// (lane, triangle) pair whose box bound passes the lane's threshold is
+// appended to a per-wave LDS queue, and each batch of 64 queued pairs is
+// tested with one pair per lane.  A lane's best (squared distance, face id)
+// is the 64-bit minimum of its old best and its tested pairs (float bits of
+// a non-negative distance order like the distance, ties to the lowest id:
+// the same rule as testing them one by one).  Deferring the tests only
+// delays the tightening of the pruning threshold, which stays conservative.
+constexpr int kQCap = 128;
+
+struct WaveQ {
+  unsigned long long key[64];  // per-lane best key during a flush
+  int32_t tri[64];             // triangle index of the winning pair
+  int32_t qt[kQCap];           // queued pairs: triangle index
+  uint8_t ql[kQCap];           //               and lane
+};
+
+__device__ __forceinline__ unsigned long long best_key(float d2, int32_t id) {
+  return ((unsigned long long)__float_as_uint(d2) << 32) | (uint32_t)id;
+}
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Tests the first n (1..64) queued pairs and removes them from the queue
+// (qn entries before, qn - n after).
+__device__ __forceinline__ void q_flush(WaveQ* __restrict__ Q, int n, int qn, const Tri48* __restrict__ tris, v3 p,
+                                        int lane, bool valid, float slack, float& best, int32_t& best_id,
+                                        int32_t& best_tri, float& thr) {
+  const unsigned long long mine = best_key(best, best_id);
+  Q->key[lane] = mine;
+  const int32_t t = lane < n ? Q->qt[lane] : 0;
+  const int L = lane < n ? (int)Q->ql[lane] : 0;
+  wave_sync();
+  const float px = __shfl(p.x, L), py = __shfl(p.y, L), pz = __shfl(p.z, L);
+  unsigned long long k = ~0ull;
+  if (lane < n) {
+    v3 ta, tb, tc;
+    int32_t id;
+    load_tri(tris, t, ta, tb, tc, id);
+    k = best_key(tri_dist2(ta, tb, tc, mk(px, py, pz)), id);
+    atomicMin(&Q->key[L], k);
+  }
+  wave_sync();
+  if (lane < n && Q->key[L] == k) Q->tri[L] = t;
+  // move the untested tail to the front
+  const int rest = qn - n;
+  const int32_t tail_t = lane < rest ? Q->qt[n + lane] : 0;
+  const uint8_t tail_l = lane < rest ? Q->ql[n + lane] : (uint8_t)0;
+  wave_sync();
+  if (lane < rest) {
+    Q->qt[lane] = tail_t;
+    Q->ql[lane] = tail_l;
+  }
+  const unsigned long long nk = Q->key[lane];
+  if (nk != mine) {
+    best = __uint_as_float((uint32_t)(nk >> 32));
+    best_id = (int32_t)(uint32_t)(nk & 0xffffffffull);
+    best_tri = Q->tri[lane];
+    thr = valid ? prune_thr(best, slack) : -1.0f;
+  }
+  wave_sync();
+}
+
 // ------------------------------------------------------------ wave-coherent walk
 // The wave-coherent packet walk of one wavefront (see the file comment):
 // each lane's best squared distance / face enter as the seed state and leave
 // as the result.  stk: this wave's LDS stack.  n_*: work counters (STATS).
-template <bool STATS>
+template <bool STATS, bool QUEUE>
 __device__ __forceinline__ void wave_walk(const Node64* __restrict__ nodes, const Tri48* __restrict__ tris, v3 p,
-                                          bool valid, float slack, float4 (*stk)[2], int lane, float& best,
-                                          int32_t& best_id, int32_t& best_tri, uint32_t& n_node,
+                                          bool valid, float slack, float4 (*stk)[2], WaveQ* __restrict__ Q, int lane,
+                                          float& best, int32_t& best_id, int32_t& best_tri, uint32_t& n_node,
                                           uint32_t& n_leaf_tri, uint32_t& n_test, uint32_t& n_lane) {
   float thr = valid ? prune_thr(best, slack) : -1.0f;  // invalid lanes accept nothing
+  int qn = 0;  // queued exact tests (QUEUE)
   int sp = 0;
   int32_t cur = 0;
   while (true) {
@@ -116,7 +185,23 @@ __device__ __forceinline__ void wave_walk(const Node64* __restrict__ nodes, cons
         if (STATS) ++n_leaf_tri;
         bool need = tri_box_d2(ta, tb, tc, tri_pad_of(tris, t), p) <= thr;
         if (STATS) n_lane += need ? 1u : 0u;
-        if (!__ballot(need)) continue;
+        const uint64_t m = __ballot(need);
+        if (!m) continue;
+        if (QUEUE) {
+          if (need) {
+            const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            Q->qt[qn + below] = t;
+            Q->ql[qn + below] = (uint8_t)lane;
+          }
+          qn += __popcll(m);
+          if (qn >= 64) {
+            wave_sync();
+            if (STATS) ++n_test;
+            q_flush(Q, 64, qn, tris, p, lane, valid, slack, best, best_id, best_tri, thr);
+            qn -= 64;
+          }
+          continue;
+        }
         if (STATS) ++n_test;
         float d2 = tri_dist2(ta, tb, tc, p);
         if (d2 < best || (d2 == best && id < best_id)) {
@@ -138,7 +223,21 @@ __device__ __forceinline__ void wave_walk(const Node64* __restrict__ nodes, cons
         break;
       }
     }
-    if (!found) break;
+    if (!found) {
+      if (QUEUE && qn > 0) {  // drain, then re-check the stack with the tightened thresholds
+        wave_sync();
+        if (STATS) ++n_test;
+        const int n = qn < 64 ? qn : 64;
+        q_flush(Q, n, qn, tris, p, lane, valid, slack, best, best_id, best_tri, thr);
+        qn -= n;
+        if (qn > 0) {
+          if (STATS) ++n_test;
+          q_flush(Q, qn, qn, tris, p, lane, valid, slack, best, best_id, best_tri, thr);
+          qn = 0;
+        }
+      }
+      break;
+    }
   }
 }
 
@@ -146,7 +245,7 @@ __device__ __forceinline__ void wave_walk(const Node64* __restrict__ nodes, cons
 // distance; the walk starts with it as the best distance, so it never enters
 // a box farther than the bound.  A query with no face within its bound gets
 // face -1 and dist = its seed.
-template <bool STATS>
+template <bool STATS, bool QUEUE>
 __global__ __launch_bounds__(kBlk) void k_udf_wave(const void* __restrict__ blob, const float* __restrict__ P,
                                                     const uint32_t* __restrict__ perm, int64_t K,
                                                     const float* __restrict__ seed,
@@ -154,6 +253,7 @@ __global__ __launch_bounds__(kBlk) void k_udf_wave(const void* __restrict__ blob
                                                     float* __restrict__ closest, float* __restrict__ uvw,
                                                     unsigned long long* __restrict__ counters) {
   __shared__ float4 s_stk[kWaves][kStackTotal][2];  // (lo.xyz, ref bits), (hi.xyz, -)
+  __shared__ WaveQ s_q[QUEUE ? kWaves : 1];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlk + threadIdx.x;
@@ -173,8 +273,8 @@ __global__ __launch_bounds__(kBlk) void k_udf_wave(const void* __restrict__ blob
   }
   int32_t best_id = 0x7fffffff, best_tri = -1;
   uint32_t n_node = 0, n_leaf_tri = 0, n_test = 0, n_lane = 0;
-  wave_walk<STATS>(nodes, tris, p, valid, slack, s_stk[wv], lane, best, best_id, best_tri, n_node, n_leaf_tri,
-                   n_test, n_lane);
+  wave_walk<STATS, QUEUE>(nodes, tris, p, valid, slack, s_stk[wv], QUEUE ? &s_q[wv] : nullptr, lane, best, best_id,
+                          best_tri, n_node, n_leaf_tri, n_test, n_lane);
   if (STATS) {
     if (!valid) n_lane = 0;
     for (int o = 32; o > 0; o >>= 1) n_lane += __shfl_xor(n_lane, o);
@@ -195,7 +295,8 @@ __global__ __launch_bounds__(kBlk) void k_udf_wave(const void* __restrict__ blob
 
 // One packet of the batched walk: lane's query q (if valid) on `blob`.
 __device__ __forceinline__ void batch_walk_tile(const void* __restrict__ blob, const float* __restrict__ P, int64_t q,
-                                                bool valid, float4 (*stk)[2], int lane, float* __restrict__ dist,
+                                                bool valid, float4 (*stk)[2], WaveQ* __restrict__ Q, int lane,
+                                                float* __restrict__ dist,
                                                 int32_t* __restrict__ face, float* __restrict__ closest,
                                                 float* __restrict__ uvw) {
   const BvhHeader* H = header(blob);
@@ -206,7 +307,7 @@ __device__ __forceinline__ void batch_walk_tile(const void* __restrict__ blob, c
   float best = 1e30f;
   int32_t best_id = 0x7fffffff, best_tri = -1;
   uint32_t n0 = 0, n1 = 0, n2 = 0, n3 = 0;
-  wave_walk<false>(nodes, tris, p, valid, slack, stk, lane, best, best_id, best_tri, n0, n1, n2, n3);
+  wave_walk<false, true>(nodes, tris, p, valid, slack, stk, Q, lane, best, best_id, best_tri, n0, n1, n2, n3);
   if (valid) write_result(q, p, best, best_id, best_tri, tris, dist, face, closest, uvw);
 }
 
@@ -437,7 +538,7 @@ int udf_launch(const void* blob, const float* P, int64_t K, const uint32_t* perm
   if (!blob) return fail(A3D_ERR_ARG, "a3d_bvh_udf: null BVH blob");
   if (K < 0 || K > (int64_t(1) << 40)) return fail(A3D_ERR_ARG, "a3d_bvh_udf: K=%lld out of range", (long long)K);
   if (perm && K > 0xffffffffll) return fail(A3D_ERR_ARG, "a3d_bvh_udf: perm needs K < 2^32");
-  if (flags & ~A3D_UDF_LANE) return fail(A3D_ERR_ARG, "a3d_bvh_udf: unknown flags 0x%x", flags);
+  if (flags & ~(A3D_UDF_LANE | A3D_UDF_DIRECT)) return fail(A3D_ERR_ARG, "a3d_bvh_udf: unknown flags 0x%x", flags);
   if (counters && (flags & A3D_UDF_LANE)) return fail(A3D_ERR_ARG, "a3d_bvh_udf: counters need the wave walk");
   if (K == 0) return A3D_OK;
   if (!P || !dist || !face || !closest) return fail(A3D_ERR_ARG, "a3d_bvh_udf: null input/output");
@@ -446,14 +547,21 @@ int udf_launch(const void* blob, const float* P, int64_t K, const uint32_t* perm
   if (flags & A3D_UDF_LANE)
     hipLaunchKernelGGL(k_udf_lane, dim3(grid_for(K)), dim3(kBlk), 0, s, blob, P, perm, K, seed, dist, face, closest,
                        uvw);
-  else if (counters)
-    hipLaunchKernelGGL(k_udf_wave<true>, dim3(grid_for(K)), dim3(kBlk), 0, s, blob, P, perm, K, seed, dist, face,
-                       closest,
-                       uvw, c);
-  else
-    hipLaunchKernelGGL(k_udf_wave<false>, dim3(grid_for(K)), dim3(kBlk), 0, s, blob, P, perm, K, seed, dist, face,
-                       closest,
-                       uvw, c);
+  else if (flags & A3D_UDF_DIRECT) {
+    if (counters)
+      hipLaunchKernelGGL((k_udf_wave<true, false>), dim3(grid_for(K)), dim3(kBlk), 0, s, blob, P, perm, K, seed, dist,
+                         face, closest, uvw, c);
+    else
+      hipLaunchKernelGGL((k_udf_wave<false, false>), dim3(grid_for(K)), dim3(kBlk), 0, s, blob, P, perm, K, seed, dist,
+                         face, closest, uvw, c);
+  } else {
+    if (counters)
+      hipLaunchKernelGGL((k_udf_wave<true, true>), dim3(grid_for(K)), dim3(kBlk), 0, s, blob, P, perm, K, seed, dist,
+                         face, closest, uvw, c);
+    else
+      hipLaunchKernelGGL((k_udf_wave<false, true>), dim3(grid_for(K)), dim3(kBlk), 0, s, blob, P, perm, K, seed, dist,
+                         face, closest, uvw, c);
+  }
   A3D_LAUNCH_CHECK();
   return A3D_OK;
 }
@@ -536,7 +644,9 @@ __global__ void k_udf_batch(const void* const* __restrict__ blobs, const float* 
   extern __shared__ float4 s_dyn[];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
   float4(*stk)[2] = reinterpret_cast<float4(*)[2]>(s_dyn + (size_t)wv * kStackTotal * 2);
+  WaveQ* Q = reinterpret_cast<WaveQ*>(s_dyn + (size_t)nw * kStackTotal * 2) + wv;
   while (true) {
     unsigned int t = 0;
     if (lane == 0) t = atomicAdd(counter, 1u);
@@ -547,7 +657,7 @@ __global__ void k_udf_batch(const void* const* __restrict__ blobs, const float* 
     const int count = (int)(w0 & 0xff);
     const bool valid = lane < count;
     const int64_t q = valid ? (int64_t)perm[start + lane] : 0;
-    batch_walk_tile(blobs[mesh], P, q, valid, stk, lane, dist, face, closest, uvw);
+    batch_walk_tile(blobs[mesh], P, q, valid, stk, Q, lane, dist, face, closest, uvw);
   }
 }
 
@@ -607,7 +717,7 @@ extern "C" int a3d_batch_udf(const void* const* blobs, const float* P, const uin
   hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, counter);
   A3D_LAUNCH_CHECK();
-  const size_t lds = (size_t)waves_per_block * kStackTotal * 2 * sizeof(float4);
+  const size_t lds = (size_t)waves_per_block * (kStackTotal * 2 * sizeof(float4) + sizeof(WaveQ));
   hipLaunchKernelGGL(k_udf_batch, dim3(blocks), dim3(64 * waves_per_block), lds, s, blobs, P, perm, tiles, n_tiles,
                      counter, dist, face, closest, uvw);
   A3D_LAUNCH_CHECK();

@@ -11,7 +11,9 @@ method names and return tuples).  Differences, all deliberate:
   reference leaves it uninitialised);
 * ``aabb_intersect`` never truncates: it counts pairs first, then writes them
   sorted by (box, face) (the reference appends with an atomic counter, capped
-  at min(100*K, 5e7) pairs, in nondeterministic order).
+  at min(100*K, 5e7) pairs, in nondeterministic order);
+* ``n_primitives_per_leaf`` defaults to 8 (reference: 4): with tree-independent
+  results it only trades node visits against leaf tests, and 8 is faster here.
 """
 from __future__ import annotations
 
@@ -23,7 +25,7 @@ from . import _lib
 
 
 class BVHAccelerator:
-    def __init__(self, vertices: torch.Tensor, faces: torch.Tensor, n_primitives_per_leaf: int = 4):
+    def __init__(self, vertices: torch.Tensor, faces: torch.Tensor, n_primitives_per_leaf: int = 8):
         _lib.require_cuda(vertices, "vertices")
         self.device = vertices.device
         self.vertices = vertices.contiguous().float()

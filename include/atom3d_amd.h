@@ -81,6 +81,7 @@ int a3d_bvh_info(const void* blob, int64_t* out4, a3d_stream_t stream);
  * Workspace of a3d_query_order_ws_bytes(K) bytes, 256-byte aligned; K < 2^31.
  */
 #define A3D_UDF_LANE 1
+#define A3D_UDF_DIRECT 2 /* wave walk testing each needed triangle at once (no exact-test queue) */
 int a3d_query_order_ws_bytes(int64_t K, int64_t* bytes);
 int a3d_query_order(const float* P, int64_t K, uint32_t* perm, void* ws, int64_t ws_bytes,
                     a3d_stream_t stream);

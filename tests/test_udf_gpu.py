@@ -23,7 +23,8 @@ def _gpu_udf(V, F, P, **kw):
     return d.cpu().numpy(), f.cpu().numpy(), c.cpu().numpy(), w.cpu().numpy()
 
 
-VARIANTS = [dict(order=True), dict(order=False), dict(order=True, flags=1), dict(order=False, flags=1)]
+VARIANTS = [dict(order=True), dict(order=False), dict(order=True, flags=1), dict(order=False, flags=1),
+            dict(order=True, flags=2)]
 
 
 def _assert_bitexact(V, F, P, label):
