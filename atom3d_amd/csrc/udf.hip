@@ -431,12 +431,24 @@ __global__ __launch_bounds__(kBlk) void k_qbounds(const float* __restrict__ P, i
       hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], o));
     }
   }
-  if ((threadIdx.x & 63) == 0) {
-#pragma unroll
+  // one atomic per block and bound: same-address atomics serialise
+  __shared__ float s_lo[kWaves][3], s_hi[kWaves][3];
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
     for (int k = 0; k < 3; ++k) {
-      atomicMin(&red[k], f2ord(lo[k]));
-      atomicMax(&red[3 + k], f2ord(hi[k]));
+      s_lo[wv][k] = lo[k];
+      s_hi[wv][k] = hi[k];
     }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int k = threadIdx.x;
+    float a = s_lo[0][k], b = s_hi[0][k];
+    for (int w = 1; w < kWaves; ++w) {
+      a = fminf(a, s_lo[w][k]);
+      b = fmaxf(b, s_hi[w][k]);
+    }
+    atomicMin(&red[k], f2ord(a));
+    atomicMax(&red[3 + k], f2ord(b));
   }
 }
 
@@ -523,7 +535,7 @@ extern "C" int a3d_query_order(const float* P, int64_t K, uint32_t* perm, void* 
   hipLaunchKernelGGL(k_qinit, dim3(1), dim3(64), 0, s, w.red);
   A3D_LAUNCH_CHECK();
   unsigned g = grid_for(K);
-  hipLaunchKernelGGL(k_qbounds, dim3(g < 2048 ? g : 2048), dim3(kBlk), 0, s, P, K, w.red);
+  hipLaunchKernelGGL(k_qbounds, dim3(g < 256 ? g : 256), dim3(kBlk), 0, s, P, K, w.red);
   A3D_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_qmorton, dim3(g), dim3(kBlk), 0, s, P, K, w.red, w.k0, w.v0);
   A3D_LAUNCH_CHECK();

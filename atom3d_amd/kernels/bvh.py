@@ -111,7 +111,9 @@ class BVHAccelerator:
         return points.contiguous().float()
 
     UDF_LANE = 1
+    UDF_DIRECT = 2
     ORDER_MIN_QUERIES = 4096
+    LANE_MAX_QUERIES = 1 << 17
 
     def query_order(self, points: torch.Tensor) -> torch.Tensor:
         """Morton order of the query points (uint32 permutation stored in an
@@ -127,19 +129,27 @@ class BVHAccelerator:
                       _lib.stream(self.device))
         return perm
 
-    def udf(self, points: torch.Tensor, flags: int = 0, order: bool | None = None,
+    def udf(self, points: torch.Tensor, flags: int | None = None, order: bool | None = None,
             counters: torch.Tensor | None = None):
         """Closest point on the mesh: (distances [K], face_ids [K] int32,
-        closest_points [K,3], uvw [K,3]).  ``order`` (default: K >= 4096)
-        walks the queries in Morton order; ``flags`` (UDF_LANE) picks the
-        traversal variant; ``counters`` (int64[5] on the device) accumulates
-        the wave walk's work counts.  Results are identical either way."""
+        closest_points [K,3], uvw [K,3]).  ``flags`` picks the traversal
+        (default: the per-lane walk in input order up to 2^17 queries, else
+        the packet walk in Morton order; UDF_LANE, UDF_DIRECT or 0 force one);
+        ``order`` (default: K >= 4096) walks the queries in Morton order;
+        ``counters`` (int64[5] on the device) accumulates the packet walk's
+        work counts.  Results are identical either way."""
         p = self._points(points)
         K = p.shape[0]
         dist = torch.empty(K, dtype=torch.float32, device=self.device)
         face = torch.empty(K, dtype=torch.int32, device=self.device)
         closest = torch.empty(K, 3, dtype=torch.float32, device=self.device)
         uvw = torch.empty(K, 3, dtype=torch.float32, device=self.device)
+        if flags is None:
+            # few queries: the per-lane walk in input order is latency-bound
+            # at a lower floor than the packet walk (whose packets are sparse)
+            flags = self.UDF_LANE if K <= self.LANE_MAX_QUERIES and counters is None else 0
+            if flags == self.UDF_LANE and order is None:
+                order = False
         if order is None:
             order = self.ORDER_MIN_QUERIES <= K < 2 ** 31
         perm = self.query_order(p) if order and K > 0 else None
