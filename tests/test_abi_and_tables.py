@@ -88,3 +88,33 @@ def test_octree_cpu_geometry_helpers():
     idx = oc.ijk_to_cube_level(torch.tensor([[1, 2, 3]]), 2)
     assert idx.item() == (1 * 4 + 2) * 4 + 3
     assert oc.cube_to_ijk_level(idx, 2).tolist() == [[1, 2, 3]]
+
+
+def test_new_entry_points_validate_arguments():
+    """Argument errors of the DMC, surrogate, batch, graph and flood entry
+    points come back as messages before any device work."""
+    n = ctypes.c_int64()
+    with pytest.raises(RuntimeError, match="out of range"):
+        _lib.call("a3d_dmc_ws_bytes", -1, ctypes.byref(n))
+    with pytest.raises(RuntimeError, match="dtype"):
+        _lib.call("a3d_dmc_topology", None, None, 7, 0.0, None, 10, 10, 4, None, None, 0, None, None)
+    sizes = (ctypes.c_int64 * 5)()
+    with pytest.raises(RuntimeError, match="null"):
+        _lib.call("a3d_dmc_forward", None, 1, sizes, None, None, None, None, None, None, None, 0, 0, 0.0, 0.99, 4,
+                  None, None, None, None, None, None)
+    with pytest.raises(RuntimeError, match="null"):
+        _lib.call("a3d_sg_init", None, None, None, None, 0, None)
+    with pytest.raises(RuntimeError, match="bad batch"):
+        _lib.call("a3d_sg_grad", 1, 1, None, None, 0, None, None)
+    with pytest.raises(RuntimeError, match="launch shape"):
+        _lib.call("a3d_batch_udf", None, None, None, None, 5, None, 0, 1, None, None, None, None, None)
+    with pytest.raises(RuntimeError, match="null handle"):
+        _lib.call("a3d_graph_launch", 0, None)
+    with pytest.raises(RuntimeError, match="bad arguments"):
+        _lib.call("a3d_ijk_bounds", None, 5, None, None)
+    with pytest.raises(RuntimeError, match="null seed"):
+        _lib.call("a3d_bvh_udf_seeded", 1, None, 5, None, None, None, None, None, None, 0, None)
+    pb, sb = ctypes.c_int64(), ctypes.c_int64()
+    _lib.call("a3d_sg_sizes", ctypes.byref(pb), ctypes.byref(sb))
+    assert pb.value == 512 + 16 * 2 ** 19 * 4 + 64 * 32 * 2 + 64 * 64 * 2 + 64 * 4
+    assert sb.value > 4 * 16 * 2 ** 19 * 8
