@@ -60,7 +60,8 @@ struct BvhHeader {
   float tri_pad_max;    // max per-triangle distance error bound (see bvh_build)
   int32_t leaf_max;
   int32_t depth;        // max internal-node depth (root = 0), for stack sizing
-  int32_t pad[46];
+  int64_t boxes_off;    // per-triangle padded boxes (TriBox32), leaf order
+  int32_t pad[44];
 };
 static_assert(sizeof(BvhHeader) == 256, "header is 256 bytes");
 constexpr int32_t kBvhMagic = 0x41334442;  // 'A3DB'
@@ -89,6 +90,15 @@ struct Tri48 {
 };
 static_assert(sizeof(Tri48) == 48, "triangle is 48 bytes");
 
+// Triangle bounding box widened by the triangle's distance-error pad, in leaf
+// order: lo = min(a, b, c) - pad, hi = max(a, b, c) + pad (w unused).  The
+// walks bound-test a triangle from these 32 bytes without its vertices.
+struct TriBox32 {
+  float4 lo;
+  float4 hi;
+};
+static_assert(sizeof(TriBox32) == 32, "triangle box is 32 bytes");
+
 __device__ __forceinline__ void load_tri(const Tri48* __restrict__ t, int32_t i, v3& a, v3& b, v3& c,
                                          int32_t& id) {
   float4 p = t[i].p, q = t[i].q, r = t[i].r;
@@ -99,6 +109,15 @@ __device__ __forceinline__ void load_tri(const Tri48* __restrict__ t, int32_t i,
 }
 
 __device__ __forceinline__ float tri_pad_of(const Tri48* __restrict__ t, int32_t i) { return t[i].r.z; }
+
+__host__ __device__ __forceinline__ TriBox32 tri_box_of(v3 a, v3 b, v3 c, float pad) {
+  TriBox32 t;
+  t.lo = make_float4(fminf(fminf(a.x, b.x), c.x) - pad, fminf(fminf(a.y, b.y), c.y) - pad,
+                     fminf(fminf(a.z, b.z), c.z) - pad, 0.0f);
+  t.hi = make_float4(fmaxf(fmaxf(a.x, b.x), c.x) + pad, fmaxf(fmaxf(a.y, b.y), c.y) + pad,
+                     fmaxf(fmaxf(a.z, b.z), c.z) + pad, 0.0f);
+  return t;
+}
 
 __host__ __device__ __forceinline__ const BvhHeader* header(const void* blob) {
   return reinterpret_cast<const BvhHeader*>(blob);
@@ -255,6 +274,9 @@ __device__ __forceinline__ const Node64* nodes_of(const void* blob) {
 }
 __device__ __forceinline__ const Tri48* tris_of(const void* blob) {
   return reinterpret_cast<const Tri48*>((const char*)blob + header(blob)->tris_off);
+}
+__device__ __forceinline__ const TriBox32* boxes_of(const void* blob) {
+  return reinterpret_cast<const TriBox32*>((const char*)blob + header(blob)->boxes_off);
 }
 
 // Squared-distance threshold above which a box cannot hold a face the

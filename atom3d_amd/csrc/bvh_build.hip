@@ -97,7 +97,7 @@ Ws carve(void* base, int64_t n) {
 
 int64_t blob_size(int64_t nF) {
   int64_t cap = nF > 1 ? nF - 1 : 1;
-  return 256 + 64 * cap + 48 * nF;
+  return 256 + 64 * cap + 48 * nF + 32 * nF;
 }
 
 __device__ __forceinline__ v3 vtx(const float* __restrict__ V, int32_t i) {
@@ -347,7 +347,8 @@ __global__ __launch_bounds__(kBlk) void k_emit_nodes(int64_t m, int leaf_max, co
 }
 
 __global__ __launch_bounds__(kBlk) void k_emit_tris(const float* __restrict__ V, const int32_t* __restrict__ F,
-                                                     const uint32_t* __restrict__ order, int64_t n, Tri48* out) {
+                                                     const uint32_t* __restrict__ order, int64_t n, Tri48* out,
+                                                     TriBox32* boxes) {
   int64_t j = blockIdx.x * (int64_t)kBlk + threadIdx.x;
   if (j >= n) return;
   uint32_t f = order[j];
@@ -355,14 +356,16 @@ __global__ __launch_bounds__(kBlk) void k_emit_tris(const float* __restrict__ V,
   Tri48 t;
   t.p = make_float4(a.x, a.y, a.z, b.x);
   t.q = make_float4(b.y, b.z, c.x, c.y);
-  t.r = make_float4(c.z, __int_as_float((int32_t)f), tri_pad(a, b, c), 0.0f);
+  const float pad = tri_pad(a, b, c);
+  t.r = make_float4(c.z, __int_as_float((int32_t)f), pad, 0.0f);
   out[j] = t;
+  boxes[j] = tri_box_of(a, b, c, pad);
 }
 
 // Meshes with <= leaf_max faces (including none): one root whose left child
 // is a leaf holding every face and whose right child is an empty box.
 __global__ void k_small_root(const float* __restrict__ V, const int32_t* __restrict__ F, int64_t n, Node64* out,
-                             Tri48* tris) {
+                             Tri48* tris, TriBox32* boxes) {
   if (threadIdx.x != 0) return;
   float b[6] = {3.0e38f, 3.0e38f, 3.0e38f, -3.0e38f, -3.0e38f, -3.0e38f};
   for (int64_t j = 0; j < n; ++j) {
@@ -373,8 +376,10 @@ __global__ void k_small_root(const float* __restrict__ V, const int32_t* __restr
     Tri48 t;
     t.p = make_float4(a.x, a.y, a.z, bb.x);
     t.q = make_float4(bb.y, bb.z, c.x, c.y);
-    t.r = make_float4(c.z, __int_as_float((int32_t)j), tri_pad(a, bb, c), 0.0f);
+    const float pad = tri_pad(a, bb, c);
+    t.r = make_float4(c.z, __int_as_float((int32_t)j), pad, 0.0f);
     tris[j] = t;
+    boxes[j] = tri_box_of(a, bb, c, pad);
   }
   const float inf = __int_as_float(0x7f800000);
   Node64 nd;
@@ -391,13 +396,14 @@ __global__ void k_header(BvhHeader* h, int64_t n, int64_t cap, int leaf_max, con
                          const int32_t* __restrict__ kept, const int32_t* __restrict__ remap, int64_t m) {
   if (threadIdx.x != 0) return;
   BvhHeader H;
-  for (int k = 0; k < 46; ++k) H.pad[k] = 0;
+  for (int k = 0; k < 44; ++k) H.pad[k] = 0;
   H.magic = kBvhMagic;
   H.n_tris = (int32_t)n;
   H.n_nodes = (n > leaf_max && m > 0) ? remap[m - 1] + kept[m - 1] : 1;
   H.n_node_cap = (int32_t)cap;
   H.nodes_off = 256;
   H.tris_off = 256 + 64 * cap;
+  H.boxes_off = H.tris_off + 48 * n;
   float cm = 0.0f;
   for (int k = 0; k < 3; ++k) {
     H.scene_lo[k] = n > 0 ? ord2f(red[R_VLO + k]) : 0.0f;
@@ -443,6 +449,7 @@ extern "C" int a3d_bvh_build(const float* V, int64_t nV, const int32_t* F, int64
   char* base = (char*)blob;
   Node64* nodes = (Node64*)(base + 256);
   Tri48* tris = (Tri48*)(base + 256 + 64 * cap);
+  TriBox32* boxes = (TriBox32*)(base + 256 + 64 * cap + 48 * n);
   int64_t m = n - 1;
 
   hipLaunchKernelGGL(k_init, dim3(1), dim3(64), 0, s, w.red);
@@ -454,7 +461,7 @@ extern "C" int a3d_bvh_build(const float* V, int64_t nV, const int32_t* F, int64
     A3D_LAUNCH_CHECK();
   }
   if (n <= leaf_max) {
-    hipLaunchKernelGGL(k_small_root, dim3(1), dim3(64), 0, s, V, F, n, nodes, tris);
+    hipLaunchKernelGGL(k_small_root, dim3(1), dim3(64), 0, s, V, F, n, nodes, tris, boxes);
     A3D_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_header, dim3(1), dim3(64), 0, s, (BvhHeader*)blob, n, cap, leaf_max, w.red, w.kept, w.remap, (int64_t)0);
     A3D_LAUNCH_CHECK();
@@ -475,7 +482,7 @@ extern "C" int a3d_bvh_build(const float* V, int64_t nV, const int32_t* F, int64
   A3D_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(w.cub, cb, w.kept, w.remap, (int)m, s));
   hipLaunchKernelGGL(k_emit_nodes, dim3(gm), dim3(kBlk), 0, s, m, leaf_max, w.kept, w.remap, w.kl, w.kr, w.size, w.first, w.leaf_box, w.node_box, nodes);
   A3D_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_emit_tris, dim3(gn), dim3(kBlk), 0, s, V, F, w.vals_out, n, tris);
+  hipLaunchKernelGGL(k_emit_tris, dim3(gn), dim3(kBlk), 0, s, V, F, w.vals_out, n, tris, boxes);
   A3D_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_header, dim3(1), dim3(64), 0, s, (BvhHeader*)blob, n, cap, leaf_max, w.red, w.kept, w.remap, m);
   A3D_LAUNCH_CHECK();

@@ -137,8 +137,9 @@ __device__ __forceinline__ void q_flush(WaveQ* __restrict__ Q, int n, int qn, co
 // each lane's best squared distance / face enter as the seed state and leave
 // as the result.  stk: this wave's LDS stack.  n_*: work counters (STATS).
 template <bool STATS, bool QUEUE>
-__device__ __forceinline__ void wave_walk(const Node64* __restrict__ nodes, const Tri48* __restrict__ tris, v3 p,
-                                          bool valid, float slack, float4 (*stk)[2], WaveQ* __restrict__ Q, int lane,
+__device__ __forceinline__ void wave_walk(const Node64* __restrict__ nodes, const Tri48* __restrict__ tris,
+                                          const TriBox32* __restrict__ boxes, v3 p, bool valid, float slack,
+                                          float4 (*stk)[2], WaveQ* __restrict__ Q, int lane,
                                           float& best, int32_t& best_id, int32_t& best_tri, uint32_t& n_node,
                                           uint32_t& n_leaf_tri, uint32_t& n_test, uint32_t& n_lane) {
   float thr = valid ? prune_thr(best, slack) : -1.0f;  // invalid lanes accept nothing
@@ -180,10 +181,16 @@ __device__ __forceinline__ void wave_walk(const Node64* __restrict__ nodes, cons
       const int32_t s0 = leaf_start(cur), n = leaf_count(cur);
       for (int32_t t = s0; t < s0 + n; ++t) {
         v3 ta, tb, tc;
-        int32_t id;
-        load_tri(tris, t, ta, tb, tc, id);
+        int32_t id = 0;
+        bool need;
+        if (QUEUE) {  // bound test from the stored box; vertices only in the flush
+          const TriBox32 bx = boxes[t];
+          need = box_d2(p, bx.lo.x, bx.lo.y, bx.lo.z, bx.hi.x, bx.hi.y, bx.hi.z) <= thr;
+        } else {
+          load_tri(tris, t, ta, tb, tc, id);
+          need = tri_box_d2(ta, tb, tc, tri_pad_of(tris, t), p) <= thr;
+        }
         if (STATS) ++n_leaf_tri;
-        bool need = tri_box_d2(ta, tb, tc, tri_pad_of(tris, t), p) <= thr;
         if (STATS) n_lane += need ? 1u : 0u;
         const uint64_t m = __ballot(need);
         if (!m) continue;
@@ -273,7 +280,8 @@ __global__ __launch_bounds__(kBlk) void k_udf_wave(const void* __restrict__ blob
   }
   int32_t best_id = 0x7fffffff, best_tri = -1;
   uint32_t n_node = 0, n_leaf_tri = 0, n_test = 0, n_lane = 0;
-  wave_walk<STATS, QUEUE>(nodes, tris, p, valid, slack, s_stk[wv], QUEUE ? &s_q[wv] : nullptr, lane, best, best_id,
+  wave_walk<STATS, QUEUE>(nodes, tris, boxes_of(blob), p, valid, slack, s_stk[wv], QUEUE ? &s_q[wv] : nullptr, lane,
+                          best, best_id,
                           best_tri, n_node, n_leaf_tri, n_test, n_lane);
   if (STATS) {
     if (!valid) n_lane = 0;
@@ -307,7 +315,8 @@ __device__ __forceinline__ void batch_walk_tile(const void* __restrict__ blob, c
   float best = 1e30f;
   int32_t best_id = 0x7fffffff, best_tri = -1;
   uint32_t n0 = 0, n1 = 0, n2 = 0, n3 = 0;
-  wave_walk<false, true>(nodes, tris, p, valid, slack, stk, Q, lane, best, best_id, best_tri, n0, n1, n2, n3);
+  wave_walk<false, true>(nodes, tris, boxes_of(blob), p, valid, slack, stk, Q, lane, best, best_id, best_tri, n0, n1,
+                         n2, n3);
   if (valid) write_result(q, p, best, best_id, best_tri, tris, dist, face, closest, uvw);
 }
 
