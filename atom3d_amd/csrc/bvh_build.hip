@@ -1,13 +1,22 @@
 // Device BVH build for gfx950: Morton-ordered linear BVH (Karras 2012,
 // "Maximizing Parallelism in the Construction of BVHs, Octrees, and k-d
-// Trees"), bottom-up box fitting with arrival counters, then collapse of
-// subtrees of <= leaf_max triangles into leaves and emission of 64-byte
-// two-child nodes.
+// Trees"), collapse of subtrees of <= leaf_max triangles into leaves, box
+// fitting of the remaining internal nodes in frontier rounds, optional SAH
+// treelet restructuring of those nodes (Karras & Aila 2013, "Fast Parallel
+// Construction of High-Quality Bounding Volume Hierarchies"), and emission of
+// 64-byte two-child nodes.
 //
 // Replaces the reference's single-threaded host SAH build
 // (Atom3D atom3d/kernels/bvh_kernels.cu:616-925), which copies the mesh to
 // the host and back.  Query results never depend on the tree shape: the
-// traversals in bvh_query.hip return the same faces for any valid tree.
+// traversals return the same faces for any valid tree.
+//
+// Bottom-up passes run as frontier rounds, one kernel launch per tree level:
+// a node enters the frontier of round r+1 when the second of its children
+// finishes in round r (relaxed device-scope arrival counter), and the node's
+// children are read only in a later launch.  No box crosses workgroups
+// inside one launch, so no device-scope fence (an L2 write-back on a
+// multi-XCD part) is needed.
 #include <hipcub/hipcub.hpp>
 
 #include <stdarg.h>
@@ -41,7 +50,7 @@ struct Ws {
   uint32_t* vals_in;
   uint32_t* vals_out;
   float* leaf_box;   // [n][6] sorted order
-  float* node_box;   // [n-1][6]
+  float* node_box;   // [n-1][6] leaf units and kept nodes
   int32_t* kl;       // [n-1] child refs: >=0 internal, <0 ~leafprim
   int32_t* kr;
   int32_t* first;    // [n-1]
@@ -51,10 +60,26 @@ struct Ws {
   int32_t* flag;     // [n-1]
   int32_t* kept;     // [n-1]
   int32_t* remap;    // [n-1]
+  float* cost;       // [n-1] SAH cost of the subtree (units and kept nodes)
+  int32_t* height;   // [n-1] kept nodes on the longest path down (units: 0)
+  int32_t* front0;   // [n-1] frontier ping-pong
+  int32_t* front1;
+  int32_t* counts;   // [kRoundsMax + 1] frontier size per round
+  int32_t* kl0;      // [n-1] fitted Karras tree, kept if a refinement is rejected
+  int32_t* kr0;
+  float* node_box0;  // [n-1][6]
   void* cub;
   size_t cub_bytes;
   int64_t total;
 };
+
+constexpr int kRoundsMax = 4096;
+// Deepest kept-node chain a walk stack is sized for (the Karras tree of
+// distinct 95-bit keys is at most 96 deep).
+constexpr int kMaxDepth = 96;
+// SAH constants: one node visit (two child boxes) vs one triangle.
+constexpr float kCi = 1.2f, kCt = 1.0f;
+constexpr int kTreeletLeaves = 7;
 
 inline int64_t al(int64_t x) { return (x + 255) & ~int64_t(255); }
 
@@ -89,6 +114,14 @@ Ws carve(void* base, int64_t n) {
   w.flag = (int32_t*)take(m * 4);
   w.kept = (int32_t*)take(m * 4);
   w.remap = (int32_t*)take(m * 4);
+  w.cost = (float*)take(m * 4);
+  w.height = (int32_t*)take(m * 4);
+  w.front0 = (int32_t*)take(m * 4);
+  w.front1 = (int32_t*)take(m * 4);
+  w.counts = (int32_t*)take((kRoundsMax + 1) * 4);
+  w.kl0 = (int32_t*)take(m * 4);
+  w.kr0 = (int32_t*)take(m * 4);
+  w.node_box0 = (float*)take(m * 24);
   w.cub_bytes = cub_bytes_for(n);
   w.cub = take((int64_t)w.cub_bytes);
   w.total = off;
@@ -139,9 +172,12 @@ __device__ __forceinline__ float wave_max(float v) {
 }
 
 // Per face: centroid (reference formula, bvh_kernels.cu:898-902), bounds
-// reductions.  One atomic per wave per slot.
+// reductions: wave shuffles, then the block's waves through LDS, then one
+// atomic per block per slot (the grid is capped, so a few thousand atomics
+// in all).
 __global__ __launch_bounds__(kBlk) void k_prep(const float* __restrict__ V, const int32_t* __restrict__ F,
                                                 int64_t n, uint32_t* red) {
+  __shared__ float s_r[kBlk / 64][R_N];
   float r[R_N];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -161,17 +197,22 @@ __global__ __launch_bounds__(kBlk) void k_prep(const float* __restrict__ V, cons
     r[R_VHI + 2] = fmaxf(r[R_VHI + 2], fmaxf(fmaxf(a.z, b.z), c.z));
     r[R_PAD] = fmaxf(r[R_PAD], tri_pad(a, b, c));
   }
-  int lane = threadIdx.x & 63;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-  for (int k = 0; k < R_N; ++k) {
-    if (k > R_PAD) continue;
-    bool is_min = (k >= R_CLO && k < R_CHI) || (k >= R_VLO && k < R_VHI);
+  for (int k = 0; k <= R_PAD; ++k) {
     if (k == R_CMAX) continue;
-    float v = is_min ? wave_min(r[k]) : wave_max(r[k]);
-    if (lane == 0) {
-      if (is_min) atomicMin(&red[k], f2ord(v));
-      else atomicMax(&red[k], f2ord(v));
-    }
+    const bool is_min = (k >= R_CLO && k < R_CHI) || (k >= R_VLO && k < R_VHI);
+    const float v = is_min ? wave_min(r[k]) : wave_max(r[k]);
+    if (lane == 0) s_r[wv][k] = v;
+  }
+  __syncthreads();
+  const int k = threadIdx.x;
+  if (k <= R_PAD && k != R_CMAX) {
+    const bool is_min = (k >= R_CLO && k < R_CHI) || (k >= R_VLO && k < R_VHI);
+    float v = s_r[0][k];
+    for (int w = 1; w < kBlk / 64; ++w) v = is_min ? fminf(v, s_r[w][k]) : fmaxf(v, s_r[w][k]);
+    if (is_min) atomicMin(&red[k], f2ord(v));
+    else atomicMax(&red[k], f2ord(v));
   }
 }
 
@@ -267,39 +308,15 @@ __device__ __forceinline__ void leaf_box_of(const float* __restrict__ V, const i
   o[5] = fmaxf(fmaxf(a.z, b.z), c.z) + pad;
 }
 
-// Leaf boxes, then climb: the second child to arrive at a node fits its box.
-__global__ __launch_bounds__(kBlk) void k_fit(const float* __restrict__ V, const int32_t* __restrict__ F,
-                                               const uint32_t* __restrict__ order, int64_t n,
-                                               const int32_t* __restrict__ kl, const int32_t* __restrict__ kr,
-                                               const int32_t* __restrict__ par_int,
-                                               const int32_t* __restrict__ par_leaf, int32_t* flag,
-                                               float* leaf_box, float* node_box) {
+// Padded box of each triangle in sorted order.
+__global__ __launch_bounds__(kBlk) void k_leaf_boxes(const float* __restrict__ V, const int32_t* __restrict__ F,
+                                                      const uint32_t* __restrict__ order, int64_t n,
+                                                      float* leaf_box) {
   int64_t j = blockIdx.x * (int64_t)kBlk + threadIdx.x;
   if (j >= n) return;
   float b[6];
   leaf_box_of(V, F, order[j], b);
   for (int k = 0; k < 6; ++k) leaf_box[j * 6 + k] = b[k];
-  if (n == 1) return;
-  int32_t p = par_leaf[j];
-  while (p >= 0) {
-    __threadfence();
-    int old = atomicAdd(&flag[p], 1);
-    if (old == 0) return;
-    __threadfence();
-    int32_t c[2] = {kl[p], kr[p]};
-    float r[6] = {3.0e38f, 3.0e38f, 3.0e38f, -3.0e38f, -3.0e38f, -3.0e38f};
-    for (int s = 0; s < 2; ++s) {
-      const float* src = c[s] < 0 ? leaf_box + (int64_t)(~c[s]) * 6 : node_box + (int64_t)c[s] * 6;
-      // The sibling's box was written by another CU (plain stores, then a
-      // device-scope fence before its counter increment): read it with
-      // device-scope loads so no stale per-CU cache line is used.
-      float q[6];
-      for (int k = 0; k < 6; ++k) q[k] = __hip_atomic_load(src + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      for (int k = 0; k < 3; ++k) { r[k] = fminf(r[k], q[k]); r[k + 3] = fmaxf(r[k + 3], q[k + 3]); }
-    }
-    for (int k = 0; k < 6; ++k) node_box[(int64_t)p * 6 + k] = r[k];
-    p = par_int[p];
-  }
 }
 
 __global__ __launch_bounds__(kBlk) void k_kept(const int32_t* __restrict__ size, int64_t m, int leaf_max,
@@ -309,14 +326,284 @@ __global__ __launch_bounds__(kBlk) void k_kept(const int32_t* __restrict__ size,
   kept[i] = size[i] > leaf_max ? 1 : 0;
 }
 
-__device__ __forceinline__ void child_info(int32_t c, const int32_t* __restrict__ size,
+__device__ __forceinline__ float box_area(const float* b) {
+  const float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
+  return 2.0f * (dx * dy + dy * dz + dz * dx);
+}
+
+__device__ __forceinline__ bool ref_kept(int32_t ref, const int32_t* __restrict__ kept) {
+  return ref >= 0 && kept[ref] != 0;
+}
+__device__ __forceinline__ const float* ref_box(int32_t ref, const float* leaf_box, const float* node_box) {
+  return ref < 0 ? leaf_box + (int64_t)(~ref) * 6 : node_box + (int64_t)ref * 6;
+}
+__device__ __forceinline__ float ref_cost(int32_t ref, const float* leaf_box, const float* cost) {
+  return ref < 0 ? kCt * box_area(leaf_box + (int64_t)(~ref) * 6) : cost[ref];
+}
+__device__ __forceinline__ int32_t ref_height(int32_t ref, const int32_t* height) {
+  return ref < 0 ? 0 : height[ref];
+}
+
+// Leaf units: internal nodes of <= leaf_max triangles whose parent is kept.
+// Box = union of their (contiguous) triangles' padded boxes; SAH cost
+// kCt * count * area.
+__global__ __launch_bounds__(kBlk) void k_units(int64_t m, const int32_t* __restrict__ kept,
+                                                 const int32_t* __restrict__ par_int,
+                                                 const int32_t* __restrict__ first,
+                                                 const int32_t* __restrict__ size,
+                                                 const float* __restrict__ leaf_box, float* node_box, float* cost,
+                                                 int32_t* height) {
+  int64_t i = blockIdx.x * (int64_t)kBlk + threadIdx.x;
+  if (i >= m || kept[i]) return;
+  const int32_t p = par_int[i];
+  if (p < 0 || !kept[p]) return;
+  float b[6] = {3.0e38f, 3.0e38f, 3.0e38f, -3.0e38f, -3.0e38f, -3.0e38f};
+  const int32_t f0 = first[i], c = size[i];
+  for (int32_t t = f0; t < f0 + c; ++t) {
+    const float* q = leaf_box + (int64_t)t * 6;
+    for (int k = 0; k < 3; ++k) { b[k] = fminf(b[k], q[k]); b[k + 3] = fmaxf(b[k + 3], q[k + 3]); }
+  }
+  for (int k = 0; k < 6; ++k) node_box[i * 6 + k] = b[k];
+  cost[i] = kCt * (float)c * box_area(b);
+  height[i] = 0;
+}
+
+__global__ void k_zero_i32(int32_t* p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0;
+}
+
+// Start of a bottom-up pass over the kept nodes: flag = number of children
+// already final (units and triangles); nodes with two such children form the
+// first frontier.  counts[] must be zero.
+__global__ __launch_bounds__(kBlk) void k_pass_init(int64_t m, const int32_t* __restrict__ kept,
+                                                     const int32_t* __restrict__ kl, const int32_t* __restrict__ kr,
+                                                     int32_t* flag, int32_t* counts, int32_t* front) {
+  int64_t i = blockIdx.x * (int64_t)kBlk + threadIdx.x;
+  if (i >= m || !kept[i]) return;
+  const int c = (ref_kept(kl[i], kept) ? 0 : 1) + (ref_kept(kr[i], kept) ? 0 : 1);
+  flag[i] = c;
+  if (c == 2) front[atomicAdd(&counts[0], 1)] = (int32_t)i;
+}
+
+// The second child to finish puts the parent into the next round's frontier.
+__device__ __forceinline__ void arrive(int32_t node, int r, const int32_t* __restrict__ par_int, int32_t* flag,
+                                       int32_t* counts, int32_t* fout) {
+  const int32_t p = par_int[node];
+  if (p < 0) return;
+  if (atomicAdd(&flag[p], 1) == 1) fout[atomicAdd(&counts[r + 1], 1)] = p;
+}
+
+__device__ __forceinline__ void fit_node(int32_t node, const int32_t* __restrict__ kl,
+                                         const int32_t* __restrict__ kr, const float* leaf_box, float* node_box,
+                                         float* cost, int32_t* height) {
+  const int32_t a = kl[node], b = kr[node];
+  const float* ba = ref_box(a, leaf_box, node_box);
+  const float* bb = ref_box(b, leaf_box, node_box);
+  float o[6];
+  for (int k = 0; k < 3; ++k) { o[k] = fminf(ba[k], bb[k]); o[k + 3] = fmaxf(ba[k + 3], bb[k + 3]); }
+  for (int k = 0; k < 6; ++k) node_box[(int64_t)node * 6 + k] = o[k];
+  cost[node] = kCi * box_area(o) + ref_cost(a, leaf_box, cost) + ref_cost(b, leaf_box, cost);
+  const int32_t ha = ref_height(a, height), hb = ref_height(b, height);
+  height[node] = 1 + (ha > hb ? ha : hb);
+}
+
+// One fitting round: one thread per frontier node.
+__global__ __launch_bounds__(kBlk) void k_fit_round(int r, int32_t* counts, const int32_t* __restrict__ fin,
+                                                     int32_t* fout, const int32_t* __restrict__ kl,
+                                                     const int32_t* __restrict__ kr,
+                                                     const int32_t* __restrict__ par_int,
+                                                     const float* __restrict__ leaf_box, float* node_box,
+                                                     float* cost, int32_t* height, int32_t* flag) {
+  const int32_t nf = counts[r];
+  for (int64_t i = blockIdx.x * (int64_t)kBlk + threadIdx.x; i < nf; i += (int64_t)gridDim.x * kBlk) {
+    const int32_t node = fin[i];
+    fit_node(node, kl, kr, leaf_box, node_box, cost, height);
+    arrive(node, r, par_int, flag, counts, fout);
+  }
+}
+
+// ------------------------------------------------------------ treelets
+// A treelet of node n: start from n's two children and repeatedly replace
+// the kept child of largest surface area by its two children, up to 7
+// leaves.  The wave finds the SAH-optimal binary tree over those leaves by
+// dynamic programming over the 127 leaf subsets (subsets of one size in
+// parallel, sizes in order) and, when it is cheaper, rebuilds the treelet in
+// place on the same internal nodes.
+struct TreeletLds {
+  float area[128];
+  float cost[128];
+  int32_t part[128];
+  int32_t hsub[128];
+  float lbox[8][6];
+  float larea[8];
+  float lcost[8];
+  int32_t lheight[8];
+  int32_t leaves[8];
+  int32_t inner[8];
+  int32_t st_set[8];
+  int32_t st_slot[8];
+  int32_t nl;
+};
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ void set_box(const TreeletLds& T, int32_t S, float* o) {
+  o[0] = o[1] = o[2] = 3.0e38f;
+  o[3] = o[4] = o[5] = -3.0e38f;
+  for (int j = 0; j < kTreeletLeaves; ++j) {
+    if (!((S >> j) & 1)) continue;
+    for (int k = 0; k < 3; ++k) { o[k] = fminf(o[k], T.lbox[j][k]); o[k + 3] = fmaxf(o[k + 3], T.lbox[j][k + 3]); }
+  }
+}
+
+// Returns true when the treelet was rebuilt (node's box, cost and height are
+// then written too).  Every lane of the wave calls it with the same node.
+__device__ bool treelet(int32_t node, TreeletLds& T, int lane, int32_t* kl, int32_t* kr, int32_t* par_int,
+                        int32_t* par_leaf, const int32_t* __restrict__ kept, const float* __restrict__ leaf_box,
+                        float* node_box, float* cost, int32_t* height) {
+  if (lane == 0) {
+    int nl = 2, ni = 1;
+    T.inner[0] = node;
+    T.leaves[0] = kl[node];
+    T.leaves[1] = kr[node];
+    for (int j = 0; j < 2; ++j) {
+      const float* b = ref_box(T.leaves[j], leaf_box, node_box);
+      for (int k = 0; k < 6; ++k) T.lbox[j][k] = b[k];
+      T.larea[j] = box_area(b);
+    }
+    while (nl < kTreeletLeaves) {
+      int best = -1;
+      float ba = -1.0f;
+      for (int j = 0; j < nl; ++j)
+        if (ref_kept(T.leaves[j], kept) && T.larea[j] > ba) { ba = T.larea[j]; best = j; }
+      if (best < 0) break;
+      const int32_t c = T.leaves[best];
+      T.inner[ni++] = c;
+      T.leaves[best] = kl[c];
+      T.leaves[nl] = kr[c];
+      const int js[2] = {best, nl};
+      for (int q = 0; q < 2; ++q) {
+        const float* b = ref_box(T.leaves[js[q]], leaf_box, node_box);
+        for (int k = 0; k < 6; ++k) T.lbox[js[q]][k] = b[k];
+        T.larea[js[q]] = box_area(b);
+      }
+      ++nl;
+    }
+    for (int j = 0; j < nl; ++j) {
+      T.lcost[j] = ref_cost(T.leaves[j], leaf_box, cost);
+      T.lheight[j] = ref_height(T.leaves[j], height);
+    }
+    T.nl = nl;
+  }
+  wave_sync();
+  const int nl = T.nl;
+  if (nl < 3) return false;
+  const int nsub = 1 << nl;
+  for (int S = lane; S < nsub; S += 64) {
+    if (S == 0) continue;
+    float b[6];
+    set_box(T, S, b);
+    T.area[S] = box_area(b);
+    if (__popc(S) == 1) {
+      const int j = __ffs(S) - 1;
+      T.cost[S] = T.lcost[j];
+      T.hsub[S] = T.lheight[j];
+      T.part[S] = 0;
+    }
+  }
+  wave_sync();
+  for (int sz = 2; sz <= nl; ++sz) {
+    for (int S = lane; S < nsub; S += 64) {
+      if (__popc(S) != sz) continue;
+      const int low = S & -S;
+      float best = 3.0e38f;
+      int bp = 0;
+      for (int P = (S - 1) & S; P; P = (P - 1) & S) {
+        if (!(P & low)) continue;
+        const float c = T.cost[P] + T.cost[S ^ P];
+        if (c < best) { best = c; bp = P; }
+      }
+      T.cost[S] = kCi * T.area[S] + best;
+      T.part[S] = bp;
+      const int32_t h0 = T.hsub[bp], h1 = T.hsub[S ^ bp];
+      T.hsub[S] = 1 + (h0 > h1 ? h0 : h1);
+    }
+    wave_sync();
+  }
+  const int full = nsub - 1;
+  const float cur = kCi * T.area[full] + ref_cost(kl[node], leaf_box, cost) + ref_cost(kr[node], leaf_box, cost);
+  // Non-finite costs (huge or NaN coordinates) leave partitions unset: keep
+  // the treelet as it is.
+  if (!(T.cost[full] < cur * (1.0f - 1e-5f)) || !(cur < 3.0e38f)) return false;
+  if (lane == 0) {
+    int sp = 1, next = 1;
+    T.st_set[0] = full;
+    T.st_slot[0] = node;
+    while (sp > 0) {
+      --sp;
+      const int S = T.st_set[sp];
+      const int32_t slot = T.st_slot[sp];
+      const int sub[2] = {T.part[S], S ^ T.part[S]};
+      int32_t ref[2];
+      for (int q = 0; q < 2; ++q) {
+        if (__popc(sub[q]) == 1) {
+          ref[q] = T.leaves[__ffs(sub[q]) - 1];
+        } else {
+          ref[q] = T.inner[next++];
+          T.st_set[sp] = sub[q];
+          T.st_slot[sp] = ref[q];
+          ++sp;
+        }
+        if (ref[q] < 0) par_leaf[~ref[q]] = slot;
+        else par_int[ref[q]] = slot;
+      }
+      kl[slot] = ref[0];
+      kr[slot] = ref[1];
+      float b[6];
+      set_box(T, S, b);
+      for (int k = 0; k < 6; ++k) node_box[(int64_t)slot * 6 + k] = b[k];
+      cost[slot] = T.cost[S];
+      height[slot] = T.hsub[S];
+    }
+  }
+  return true;
+}
+
+// One restructuring round: one wavefront per frontier node.
+__global__ __launch_bounds__(kBlk) void k_treelet_round(int r, int32_t* counts, const int32_t* __restrict__ fin,
+                                                         int32_t* fout, int32_t* kl, int32_t* kr, int32_t* par_int,
+                                                         int32_t* par_leaf, const int32_t* __restrict__ kept,
+                                                         const float* __restrict__ leaf_box, float* node_box,
+                                                         float* cost, int32_t* height, int32_t* flag) {
+  __shared__ TreeletLds s_t[kBlk / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int32_t nf = counts[r];
+  const int64_t nw = (int64_t)gridDim.x * (kBlk / 64);
+  for (int64_t i = blockIdx.x * (int64_t)(kBlk / 64) + wv; i < nf; i += nw) {
+    const int32_t node = fin[i];
+    const bool rebuilt = treelet(node, s_t[wv], lane, kl, kr, par_int, par_leaf, kept, leaf_box, node_box, cost,
+                                 height);
+    if (lane == 0) {
+      if (!rebuilt) fit_node(node, kl, kr, leaf_box, node_box, cost, height);
+      arrive(node, r, par_int, flag, counts, fout);
+    }
+    wave_sync();
+  }
+}
+
+__device__ __forceinline__ void child_info(int32_t c, const int32_t* __restrict__ kept,
+                                           const int32_t* __restrict__ size,
                                            const int32_t* __restrict__ first, const int32_t* __restrict__ remap,
                                            const float* __restrict__ leaf_box, const float* __restrict__ node_box,
-                                           int leaf_max, int32_t& ref, const float*& box) {
+                                           int32_t& ref, const float*& box) {
   if (c < 0) {
     ref = leaf_ref(~c, 1);
     box = leaf_box + (int64_t)(~c) * 6;
-  } else if (size[c] <= leaf_max) {
+  } else if (!kept[c]) {
     ref = leaf_ref(first[c], size[c]);
     box = node_box + (int64_t)c * 6;
   } else {
@@ -325,7 +612,7 @@ __device__ __forceinline__ void child_info(int32_t c, const int32_t* __restrict_
   }
 }
 
-__global__ __launch_bounds__(kBlk) void k_emit_nodes(int64_t m, int leaf_max, const int32_t* __restrict__ kept,
+__global__ __launch_bounds__(kBlk) void k_emit_nodes(int64_t m, const int32_t* __restrict__ kept,
                                                       const int32_t* __restrict__ remap,
                                                       const int32_t* __restrict__ kl, const int32_t* __restrict__ kr,
                                                       const int32_t* __restrict__ size,
@@ -336,8 +623,8 @@ __global__ __launch_bounds__(kBlk) void k_emit_nodes(int64_t m, int leaf_max, co
   if (i >= m || !kept[i]) return;
   int32_t lr, rr;
   const float *lb, *rb;
-  child_info(kl[i], size, first, remap, leaf_box, node_box, leaf_max, lr, lb);
-  child_info(kr[i], size, first, remap, leaf_box, node_box, leaf_max, rr, rb);
+  child_info(kl[i], kept, size, first, remap, leaf_box, node_box, lr, lb);
+  child_info(kr[i], kept, size, first, remap, leaf_box, node_box, rr, rb);
   Node64 nd;
   nd.a = make_float4(lb[0], lb[1], lb[2], lb[3]);
   nd.b = make_float4(lb[4], lb[5], rb[0], rb[1]);
@@ -392,8 +679,11 @@ __global__ void k_small_root(const float* __restrict__ V, const int32_t* __restr
   out[0] = nd;
 }
 
+// depth: the root's height (kept nodes on the longest path down), or null
+// for the single-leaf tree.
 __global__ void k_header(BvhHeader* h, int64_t n, int64_t cap, int leaf_max, const uint32_t* __restrict__ red,
-                         const int32_t* __restrict__ kept, const int32_t* __restrict__ remap, int64_t m) {
+                         const int32_t* __restrict__ kept, const int32_t* __restrict__ remap, int64_t m,
+                         const int32_t* __restrict__ depth) {
   if (threadIdx.x != 0) return;
   BvhHeader H;
   for (int k = 0; k < 44; ++k) H.pad[k] = 0;
@@ -413,7 +703,7 @@ __global__ void k_header(BvhHeader* h, int64_t n, int64_t cap, int leaf_max, con
   H.coord_max = cm;
   H.tri_pad_max = n > 0 ? ord2f(red[R_PAD]) : 0.0f;
   H.leaf_max = leaf_max;
-  H.depth = -1;
+  H.depth = depth ? depth[0] : 1;
   *h = H;
 }
 
@@ -432,12 +722,47 @@ extern "C" int a3d_bvh_sizes(int64_t nF, int leaf_max, int64_t* blob_bytes, int6
   return A3D_OK;
 }
 
-extern "C" int a3d_bvh_build(const float* V, int64_t nV, const int32_t* F, int64_t nF, int leaf_max, void* blob,
-                             int64_t blob_bytes, void* ws, int64_t ws_bytes, a3d_stream_t stream) {
+namespace {
+
+// Launches `count` frontier rounds starting at round r0.
+template <bool TREELET>
+int launch_rounds(const Ws& w, int r0, int count, int64_t m, hipStream_t s) {
+  const unsigned g_fit = 1024, g_tree = 2048;
+  for (int r = r0; r < r0 + count; ++r) {
+    int32_t* fin = (r & 1) ? w.front1 : w.front0;
+    int32_t* fout = (r & 1) ? w.front0 : w.front1;
+    if (TREELET)
+      hipLaunchKernelGGL(k_treelet_round, dim3(g_tree), dim3(kBlk), 0, s, r, w.counts, fin, fout, w.kl, w.kr,
+                         w.par_int, w.par_leaf, w.kept, w.leaf_box, w.node_box, w.cost, w.height, w.flag);
+    else
+      hipLaunchKernelGGL(k_fit_round, dim3(g_fit), dim3(kBlk), 0, s, r, w.counts, fin, fout, w.kl, w.kr,
+                         w.par_int, w.leaf_box, w.node_box, w.cost, w.height, w.flag);
+    A3D_LAUNCH_CHECK();
+  }
+  (void)m;
+  return A3D_OK;
+}
+
+int pass_init(const Ws& w, int64_t m, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_i32, dim3(16), dim3(kBlk), 0, s, w.counts, (int64_t)(kRoundsMax + 1));
+  A3D_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_pass_init, dim3((unsigned)div_up(m, kBlk)), dim3(kBlk), 0, s, m, w.kept, w.kl, w.kr, w.flag,
+                     w.counts, w.front0);
+  A3D_LAUNCH_CHECK();
+  return A3D_OK;
+}
+
+}  // namespace
+
+extern "C" int a3d_bvh_build_ex(const float* V, int64_t nV, const int32_t* F, int64_t nF, int leaf_max,
+                                int treelet_passes, void* blob, int64_t blob_bytes, void* ws, int64_t ws_bytes,
+                                a3d_stream_t stream) {
   hipStream_t s = (hipStream_t)stream;
   int64_t need_blob = 0, need_ws = 0;
   int rc = a3d_bvh_sizes(nF, leaf_max, &need_blob, &need_ws);
   if (rc) return rc;
+  if (treelet_passes < 0 || treelet_passes > 8)
+    return fail(A3D_ERR_ARG, "a3d_bvh_build: treelet_passes=%d not in [0, 8]", treelet_passes);
   if (!blob || !ws) return fail(A3D_ERR_ARG, "a3d_bvh_build: null blob/workspace");
   if ((nF > 0) && (!V || !F || nV <= 0)) return fail(A3D_ERR_ARG, "a3d_bvh_build: null or empty mesh arrays");
   if (blob_bytes < need_blob) return fail(A3D_ERR_CAPACITY, "a3d_bvh_build: blob %lld < %lld bytes", (long long)blob_bytes, (long long)need_blob);
@@ -456,14 +781,15 @@ extern "C" int a3d_bvh_build(const float* V, int64_t nV, const int32_t* F, int64
   A3D_LAUNCH_CHECK();
   if (n > 0) {
     int64_t g = div_up(n, kBlk);
-    if (g > 4096) g = 4096;
+    if (g > 1024) g = 1024;
     hipLaunchKernelGGL(k_prep, dim3((unsigned)g), dim3(kBlk), 0, s, V, F, n, w.red);
     A3D_LAUNCH_CHECK();
   }
   if (n <= leaf_max) {
     hipLaunchKernelGGL(k_small_root, dim3(1), dim3(64), 0, s, V, F, n, nodes, tris, boxes);
     A3D_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_header, dim3(1), dim3(64), 0, s, (BvhHeader*)blob, n, cap, leaf_max, w.red, w.kept, w.remap, (int64_t)0);
+    hipLaunchKernelGGL(k_header, dim3(1), dim3(64), 0, s, (BvhHeader*)blob, n, cap, leaf_max, w.red, w.kept,
+                       w.remap, (int64_t)0, (const int32_t*)nullptr);
     A3D_LAUNCH_CHECK();
     return A3D_OK;
   }
@@ -474,19 +800,67 @@ extern "C" int a3d_bvh_build(const float* V, int64_t nV, const int32_t* F, int64
   A3D_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(w.cub, cb, w.keys_in, w.keys_out, w.vals_in, w.vals_out, (int)n, 0, 63, s));
   hipLaunchKernelGGL(k_karras, dim3(gm), dim3(kBlk), 0, s, w.keys_out, n, w.kl, w.kr, w.first, w.size, w.par_int, w.par_leaf, w.flag);
   A3D_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_fit, dim3(gn), dim3(kBlk), 0, s, V, F, w.vals_out, n, w.kl, w.kr, w.par_int, w.par_leaf, w.flag, w.leaf_box, w.node_box);
+  hipLaunchKernelGGL(k_leaf_boxes, dim3(gn), dim3(kBlk), 0, s, V, F, w.vals_out, n, w.leaf_box);
   A3D_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_kept, dim3(gm), dim3(kBlk), 0, s, w.size, m, leaf_max, w.kept);
   A3D_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_units, dim3(gm), dim3(kBlk), 0, s, m, w.kept, w.par_int, w.first, w.size, w.leaf_box,
+                     w.node_box, w.cost, w.height);
+  A3D_LAUNCH_CHECK();
+  // Fit the kept nodes of the Karras tree: its depth is at most kMaxDepth, so
+  // kMaxDepth + 1 rounds always finish (the empty ones return at once) and
+  // no host synchronisation is needed.
+  if ((rc = pass_init(w, m, s))) return rc;
+  if ((rc = launch_rounds<false>(w, 0, kMaxDepth + 1, m, s))) return rc;
+  const int32_t* depth_src = w.height;  // root = node 0
+  if (treelet_passes > 0) {
+    A3D_HIP_CHECK(hipMemcpyAsync(w.kl0, w.kl, m * 4, hipMemcpyDeviceToDevice, s));
+    A3D_HIP_CHECK(hipMemcpyAsync(w.kr0, w.kr, m * 4, hipMemcpyDeviceToDevice, s));
+    A3D_HIP_CHECK(hipMemcpyAsync(w.node_box0, w.node_box, m * 24, hipMemcpyDeviceToDevice, s));
+    int32_t h0 = 0;
+    A3D_HIP_CHECK(hipMemcpyAsync(&h0, w.height, 4, hipMemcpyDeviceToHost, s));
+    for (int pass = 0; pass < treelet_passes; ++pass) {
+      if ((rc = pass_init(w, m, s))) return rc;
+      // Restructuring can deepen the tree, so the rounds run until the
+      // frontier is empty, checked on the host every `chunk` rounds.
+      const int chunk = 16;
+      int r = 0;
+      while (true) {
+        if (r + chunk > kRoundsMax) return fail(A3D_ERR_INTERNAL, "a3d_bvh_build: treelet pass did not converge");
+        if ((rc = launch_rounds<true>(w, r, chunk, m, s))) return rc;
+        r += chunk;
+        int32_t left = 0;
+        A3D_HIP_CHECK(hipMemcpyAsync(&left, w.counts + r, 4, hipMemcpyDeviceToHost, s));
+        A3D_HIP_CHECK(hipStreamSynchronize(s));
+        if (left == 0) break;
+      }
+    }
+    int32_t h1 = 0;
+    A3D_HIP_CHECK(hipMemcpyAsync(&h1, w.height, 4, hipMemcpyDeviceToHost, s));
+    A3D_HIP_CHECK(hipStreamSynchronize(s));
+    if (h1 > kMaxDepth) {  // deeper than the walks' stacks allow: keep the fitted Karras tree
+      A3D_HIP_CHECK(hipMemcpyAsync(w.kl, w.kl0, m * 4, hipMemcpyDeviceToDevice, s));
+      A3D_HIP_CHECK(hipMemcpyAsync(w.kr, w.kr0, m * 4, hipMemcpyDeviceToDevice, s));
+      A3D_HIP_CHECK(hipMemcpyAsync(w.node_box, w.node_box0, m * 24, hipMemcpyDeviceToDevice, s));
+      A3D_HIP_CHECK(hipMemcpyAsync(w.height, &h0, 4, hipMemcpyHostToDevice, s));
+      A3D_HIP_CHECK(hipStreamSynchronize(s));
+    }
+  }
   cb = w.cub_bytes;
   A3D_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(w.cub, cb, w.kept, w.remap, (int)m, s));
-  hipLaunchKernelGGL(k_emit_nodes, dim3(gm), dim3(kBlk), 0, s, m, leaf_max, w.kept, w.remap, w.kl, w.kr, w.size, w.first, w.leaf_box, w.node_box, nodes);
+  hipLaunchKernelGGL(k_emit_nodes, dim3(gm), dim3(kBlk), 0, s, m, w.kept, w.remap, w.kl, w.kr, w.size, w.first, w.leaf_box, w.node_box, nodes);
   A3D_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_emit_tris, dim3(gn), dim3(kBlk), 0, s, V, F, w.vals_out, n, tris, boxes);
   A3D_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_header, dim3(1), dim3(64), 0, s, (BvhHeader*)blob, n, cap, leaf_max, w.red, w.kept, w.remap, m);
+  hipLaunchKernelGGL(k_header, dim3(1), dim3(64), 0, s, (BvhHeader*)blob, n, cap, leaf_max, w.red, w.kept, w.remap, m,
+                     depth_src);
   A3D_LAUNCH_CHECK();
   return A3D_OK;
+}
+
+extern "C" int a3d_bvh_build(const float* V, int64_t nV, const int32_t* F, int64_t nF, int leaf_max, void* blob,
+                             int64_t blob_bytes, void* ws, int64_t ws_bytes, a3d_stream_t stream) {
+  return a3d_bvh_build_ex(V, nV, F, nF, leaf_max, A3D_BVH_TREELET_PASSES, blob, blob_bytes, ws, ws_bytes, stream);
 }
 
 extern "C" int a3d_bvh_info(const void* blob, int64_t* out4, a3d_stream_t stream) {

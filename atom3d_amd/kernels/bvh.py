@@ -13,7 +13,9 @@ method names and return tuples).  Differences, all deliberate:
   sorted by (box, face) (the reference appends with an atomic counter, capped
   at min(100*K, 5e7) pairs, in nondeterministic order);
 * ``n_primitives_per_leaf`` defaults to 8 (reference: 4): with tree-independent
-  results it only trades node visits against leaf tests, and 8 is faster here.
+  results it only trades node visits against leaf tests, and 8 is faster here;
+* ``treelet_passes`` (default 0 = plain LBVH, asynchronous build) sets how
+  many SAH treelet-restructuring passes refine the device tree.
 """
 from __future__ import annotations
 
@@ -25,13 +27,17 @@ from . import _lib
 
 
 class BVHAccelerator:
-    def __init__(self, vertices: torch.Tensor, faces: torch.Tensor, n_primitives_per_leaf: int = 8):
+    TREELET_PASSES = 0
+
+    def __init__(self, vertices: torch.Tensor, faces: torch.Tensor, n_primitives_per_leaf: int = 8,
+                 treelet_passes: int | None = None):
         _lib.require_cuda(vertices, "vertices")
         self.device = vertices.device
         self.vertices = vertices.contiguous().float()
         self.faces = faces.to(self.device).contiguous().int()
         self.num_faces = int(self.faces.shape[0])
         self.leaf_max = int(n_primitives_per_leaf)
+        self.treelet_passes = self.TREELET_PASSES if treelet_passes is None else int(treelet_passes)
         if self.num_faces > 0:
             fmin, fmax = int(self.faces.min()), int(self.faces.max())
             if fmin < 0 or fmax >= self.vertices.shape[0]:
@@ -44,9 +50,9 @@ class BVHAccelerator:
             self.blob = torch.empty(blob_b.value, dtype=torch.uint8, device=self.device)
             ws = torch.empty(ws_b.value, dtype=torch.uint8, device=self.device)
             _lib.call(
-                "a3d_bvh_build",
+                "a3d_bvh_build_ex",
                 _lib.ptr(self.vertices), self.vertices.shape[0], _lib.ptr(self.faces), self.num_faces,
-                self.leaf_max, _lib.ptr(self.blob), blob_b.value, _lib.ptr(ws), ws_b.value,
+                self.leaf_max, self.treelet_passes, _lib.ptr(self.blob), blob_b.value, _lib.ptr(ws), ws_b.value,
                 _lib.stream(self.device),
             )
             del ws

@@ -34,7 +34,8 @@ enum {
   A3D_ERR_ARG = -1,      /* invalid argument (null pointer, bad size, bad mode) */
   A3D_ERR_HIP = -2,      /* a HIP runtime call failed */
   A3D_ERR_CAPACITY = -3, /* caller buffer too small */
-  A3D_ERR_FORMAT = -4    /* blob is not a BVH built by this library */
+  A3D_ERR_FORMAT = -4,   /* blob is not a BVH built by this library */
+  A3D_ERR_INTERNAL = -5  /* an internal bound was exceeded (message says which) */
 };
 
 /* Last error message of the calling thread ("" if none). */
@@ -46,18 +47,30 @@ const char* a3d_version(void);
  * Replaces build_bvh (bvh_kernels.cu:860-925, bound at :1245) and
  * BVHAccelerator.__init__ (kernels/bvh.py:66-93).  The reference builds a
  * binned-SAH tree on the host; this builds a Morton LBVH on the device
- * (63-bit Morton keys of face centroids, Karras 2012) with leaves of up to leaf_max <= 8
- * triangles.  The query results of every entry point below do not depend on
- * the tree.
+ * (63-bit Morton keys of face centroids, Karras 2012) with leaves of up to
+ * leaf_max <= 8 triangles, then (treelet_passes > 0) restructures the
+ * internal nodes above the leaves for SAH cost, 7-leaf treelets (Karras &
+ * Aila 2013).  The query results of every entry point below do not depend on
+ * the tree; only their speed does.
  *
  * a3d_bvh_sizes: bytes of the BVH blob and of the build workspace for nF faces.
- * a3d_bvh_build: V float32[nV*3], F int32[nF*3] (device) -> blob.
+ * a3d_bvh_build_ex: V float32[nV*3], F int32[nF*3] (device) -> blob.
+ *   treelet_passes in [0, 8].  With 0 the build is asynchronous; with > 0 it
+ *   synchronises the stream a few times per pass (the number of rounds a
+ *   pass needs is read back).  A refined tree deeper than 96 internal levels
+ *   is discarded for the plain one.
+ * a3d_bvh_build: a3d_bvh_build_ex with A3D_BVH_TREELET_PASSES passes.
  */
+#define A3D_BVH_TREELET_PASSES 0
 int a3d_bvh_sizes(int64_t nF, int leaf_max, int64_t* blob_bytes, int64_t* ws_bytes);
+int a3d_bvh_build_ex(const float* V, int64_t nV, const int32_t* F, int64_t nF, int leaf_max,
+                     int treelet_passes, void* blob, int64_t blob_bytes, void* ws, int64_t ws_bytes,
+                     a3d_stream_t stream);
 int a3d_bvh_build(const float* V, int64_t nV, const int32_t* F, int64_t nF, int leaf_max,
                   void* blob, int64_t blob_bytes, void* ws, int64_t ws_bytes, a3d_stream_t stream);
 /* Copies the blob header fields to HOST memory (synchronises the stream):
- * out[0]=n_tris, out[1]=n_nodes, out[2]=depth, out[3]=leaf_max. */
+ * out[0]=n_tris, out[1]=n_nodes, out[2]=depth (internal nodes on the longest
+ * root-to-leaf path), out[3]=leaf_max. */
 int a3d_bvh_info(const void* blob, int64_t* out4, a3d_stream_t stream);
 
 /* ------------------------------------------------------------------ UDF
