@@ -41,7 +41,7 @@ __device__ __forceinline__ float ord2f(uint32_t u) {
 
 // Reduction slots (ordered-uint encoded): centroid lo[3] hi[3], vertex lo[3]
 // hi[3], coord_max, pad_max.
-enum { R_CLO = 0, R_CHI = 3, R_VLO = 6, R_VHI = 9, R_CMAX = 12, R_PAD = 13, R_N = 16 };
+enum { R_CLO = 0, R_CHI = 3, R_VLO = 6, R_VHI = 9, R_CMAX = 12, R_PAD = 13, R_WALK = 14, R_N = 16 };
 
 struct Ws {
   uint32_t* red;
@@ -68,6 +68,7 @@ struct Ws {
   int32_t* kl0;      // [n-1] fitted Karras tree, kept if a refinement is rejected
   int32_t* kr0;
   float* node_box0;  // [n-1][6]
+  int32_t* leafy;    // [n-1] 1: the cost model makes this subtree one leaf
   void* cub;
   size_t cub_bytes;
   int64_t total;
@@ -77,8 +78,20 @@ constexpr int kRoundsMax = 4096;
 // Deepest kept-node chain a walk stack is sized for (the Karras tree of
 // distinct 95-bit keys is at most 96 deep).
 constexpr int kMaxDepth = 96;
-// SAH constants: one node visit (two child boxes) vs one triangle.
-constexpr float kCi = 1.2f, kCt = 1.0f;
+// Cost model (surface-area heuristic generalised to distance queries): a
+// node is visited by a query whose search ball of radius r meets its box, so
+// its weight is the volume of the box grown by r (Steiner:
+// V + A r + pi W r^2 + 4/3 pi r^3, W = dx + dy + dz), divided by r; r -> 0
+// gives the surface area A.  r = kWalkR x the scene diagonal.  kCi / kCt:
+// cost of one internal-node visit (two child boxes) vs one triangle.
+#ifndef A3D_WALK_R
+#define A3D_WALK_R 0.03f
+#endif
+#ifndef A3D_CI
+#define A3D_CI 1.2f
+#endif
+constexpr float kWalkR = A3D_WALK_R;
+constexpr float kCi = A3D_CI, kCt = 1.0f;
 constexpr int kTreeletLeaves = 7;
 
 inline int64_t al(int64_t x) { return (x + 255) & ~int64_t(255); }
@@ -122,6 +135,7 @@ Ws carve(void* base, int64_t n) {
   w.kl0 = (int32_t*)take(m * 4);
   w.kr0 = (int32_t*)take(m * 4);
   w.node_box0 = (float*)take(m * 24);
+  w.leafy = (int32_t*)take(m * 4);
   w.cub_bytes = cub_bytes_for(n);
   w.cub = take((int64_t)w.cub_bytes);
   w.total = off;
@@ -319,16 +333,22 @@ __global__ __launch_bounds__(kBlk) void k_leaf_boxes(const float* __restrict__ V
   for (int k = 0; k < 6; ++k) leaf_box[j * 6 + k] = b[k];
 }
 
-__global__ __launch_bounds__(kBlk) void k_kept(const int32_t* __restrict__ size, int64_t m, int leaf_max,
-                                                int32_t* kept) {
-  int64_t i = blockIdx.x * (int64_t)kBlk + threadIdx.x;
-  if (i >= m) return;
-  kept[i] = size[i] > leaf_max ? 1 : 0;
+// Search radius of the cost model, from the vertex bounds (after k_prep).
+__global__ void k_walk_radius(uint32_t* red) {
+  if (threadIdx.x != 0) return;
+  float d2 = 0.0f;
+  for (int k = 0; k < 3; ++k) {
+    const float e = ord2f(red[R_VHI + k]) - ord2f(red[R_VLO + k]);
+    d2 += e * e;
+  }
+  red[R_WALK] = __float_as_uint(kWalkR * sqrtf(d2));
 }
 
-__device__ __forceinline__ float box_area(const float* b) {
-  const float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
-  return 2.0f * (dx * dy + dy * dz + dz * dx);
+__device__ __forceinline__ float measure(const float* b, float r) {
+  const float dx = fmaxf(b[3] - b[0], 0.0f), dy = fmaxf(b[4] - b[1], 0.0f), dz = fmaxf(b[5] - b[2], 0.0f);
+  const float a = 2.0f * (dx * dy + dy * dz + dz * dx);
+  if (!(r > 0.0f)) return a;
+  return dx * dy * dz / r + a + 3.14159265f * (dx + dy + dz) * r + 4.18879020f * r * r;
 }
 
 __device__ __forceinline__ bool ref_kept(int32_t ref, const int32_t* __restrict__ kept) {
@@ -337,35 +357,25 @@ __device__ __forceinline__ bool ref_kept(int32_t ref, const int32_t* __restrict_
 __device__ __forceinline__ const float* ref_box(int32_t ref, const float* leaf_box, const float* node_box) {
   return ref < 0 ? leaf_box + (int64_t)(~ref) * 6 : node_box + (int64_t)ref * 6;
 }
-__device__ __forceinline__ float ref_cost(int32_t ref, const float* leaf_box, const float* cost) {
-  return ref < 0 ? kCt * box_area(leaf_box + (int64_t)(~ref) * 6) : cost[ref];
+__device__ __forceinline__ float ref_cost(int32_t ref, const float* leaf_box, const float* cost, float r) {
+  return ref < 0 ? kCt * measure(leaf_box + (int64_t)(~ref) * 6, r) : cost[ref];
 }
 __device__ __forceinline__ int32_t ref_height(int32_t ref, const int32_t* height) {
   return ref < 0 ? 0 : height[ref];
 }
 
-// Leaf units: internal nodes of <= leaf_max triangles whose parent is kept.
-// Box = union of their (contiguous) triangles' padded boxes; SAH cost
-// kCt * count * area.
-__global__ __launch_bounds__(kBlk) void k_units(int64_t m, const int32_t* __restrict__ kept,
-                                                 const int32_t* __restrict__ par_int,
-                                                 const int32_t* __restrict__ first,
-                                                 const int32_t* __restrict__ size,
-                                                 const float* __restrict__ leaf_box, float* node_box, float* cost,
-                                                 int32_t* height) {
+// Kept (emitted) internal nodes: not made a leaf by the cost model and with
+// no ancestor made one.  Only nodes of <= leaf_max triangles can be leaves,
+// so the climb stops at the first larger ancestor.
+__global__ __launch_bounds__(kBlk) void k_kept(int64_t m, int leaf_max, const int32_t* __restrict__ size,
+                                                const int32_t* __restrict__ leafy,
+                                                const int32_t* __restrict__ par_int, int32_t* kept) {
   int64_t i = blockIdx.x * (int64_t)kBlk + threadIdx.x;
-  if (i >= m || kept[i]) return;
-  const int32_t p = par_int[i];
-  if (p < 0 || !kept[p]) return;
-  float b[6] = {3.0e38f, 3.0e38f, 3.0e38f, -3.0e38f, -3.0e38f, -3.0e38f};
-  const int32_t f0 = first[i], c = size[i];
-  for (int32_t t = f0; t < f0 + c; ++t) {
-    const float* q = leaf_box + (int64_t)t * 6;
-    for (int k = 0; k < 3; ++k) { b[k] = fminf(b[k], q[k]); b[k + 3] = fmaxf(b[k + 3], q[k + 3]); }
-  }
-  for (int k = 0; k < 6; ++k) node_box[i * 6 + k] = b[k];
-  cost[i] = kCt * (float)c * box_area(b);
-  height[i] = 0;
+  if (i >= m) return;
+  int k = leafy[i] ? 0 : 1;
+  for (int32_t p = par_int[i]; k && p >= 0 && size[p] <= leaf_max; p = par_int[p])
+    if (leafy[p]) k = 0;
+  kept[i] = k;
 }
 
 __global__ void k_zero_i32(int32_t* p, int64_t n) {
@@ -385,6 +395,17 @@ __global__ __launch_bounds__(kBlk) void k_pass_init(int64_t m, const int32_t* __
   if (c == 2) front[atomicAdd(&counts[0], 1)] = (int32_t)i;
 }
 
+// Start of the bottom-up pass over every internal node: triangles are final.
+__global__ __launch_bounds__(kBlk) void k_pass_init_full(int64_t m, const int32_t* __restrict__ kl,
+                                                          const int32_t* __restrict__ kr, int32_t* flag,
+                                                          int32_t* counts, int32_t* front) {
+  int64_t i = blockIdx.x * (int64_t)kBlk + threadIdx.x;
+  if (i >= m) return;
+  const int c = (kl[i] < 0 ? 1 : 0) + (kr[i] < 0 ? 1 : 0);
+  flag[i] = c;
+  if (c == 2) front[atomicAdd(&counts[0], 1)] = (int32_t)i;
+}
+
 // The second child to finish puts the parent into the next round's frontier.
 __device__ __forceinline__ void arrive(int32_t node, int r, const int32_t* __restrict__ par_int, int32_t* flag,
                                        int32_t* counts, int32_t* fout) {
@@ -393,31 +414,49 @@ __device__ __forceinline__ void arrive(int32_t node, int r, const int32_t* __res
   if (atomicAdd(&flag[p], 1) == 1) fout[atomicAdd(&counts[r + 1], 1)] = p;
 }
 
+// Box, cost and height of `node` from its children.  FULL (first pass over
+// every internal node): a node of <= leaf_max triangles becomes a leaf when
+// that is cheaper (leafy = 1; its cost is the leaf cost, its height 0).
+template <bool FULL>
 __device__ __forceinline__ void fit_node(int32_t node, const int32_t* __restrict__ kl,
-                                         const int32_t* __restrict__ kr, const float* leaf_box, float* node_box,
-                                         float* cost, int32_t* height) {
+                                         const int32_t* __restrict__ kr, const int32_t* __restrict__ size,
+                                         int leaf_max, float r, const float* leaf_box, float* node_box,
+                                         float* cost, int32_t* height, int32_t* leafy) {
   const int32_t a = kl[node], b = kr[node];
   const float* ba = ref_box(a, leaf_box, node_box);
   const float* bb = ref_box(b, leaf_box, node_box);
   float o[6];
   for (int k = 0; k < 3; ++k) { o[k] = fminf(ba[k], bb[k]); o[k + 3] = fmaxf(ba[k + 3], bb[k + 3]); }
   for (int k = 0; k < 6; ++k) node_box[(int64_t)node * 6 + k] = o[k];
-  cost[node] = kCi * box_area(o) + ref_cost(a, leaf_box, cost) + ref_cost(b, leaf_box, cost);
+  const float mu = measure(o, r);
+  const float split = kCi * mu + ref_cost(a, leaf_box, cost, r) + ref_cost(b, leaf_box, cost, r);
   const int32_t ha = ref_height(a, height), hb = ref_height(b, height);
-  height[node] = 1 + (ha > hb ? ha : hb);
+  if (FULL) {
+    const int32_t nt = size[node];
+    const bool leaf = nt <= leaf_max && kCt * (float)nt * mu <= split;
+    leafy[node] = leaf ? 1 : 0;
+    cost[node] = leaf ? kCt * (float)nt * mu : split;
+    height[node] = leaf ? 0 : 1 + (ha > hb ? ha : hb);
+  } else {
+    cost[node] = split;
+    height[node] = 1 + (ha > hb ? ha : hb);
+  }
 }
 
-// One fitting round: one thread per frontier node.
+// One fitting round over every internal node: one thread per frontier node.
 __global__ __launch_bounds__(kBlk) void k_fit_round(int r, int32_t* counts, const int32_t* __restrict__ fin,
                                                      int32_t* fout, const int32_t* __restrict__ kl,
                                                      const int32_t* __restrict__ kr,
                                                      const int32_t* __restrict__ par_int,
+                                                     const int32_t* __restrict__ size, int leaf_max,
+                                                     const uint32_t* __restrict__ red,
                                                      const float* __restrict__ leaf_box, float* node_box,
-                                                     float* cost, int32_t* height, int32_t* flag) {
+                                                     float* cost, int32_t* height, int32_t* flag, int32_t* leafy) {
   const int32_t nf = counts[r];
+  const float wr = __uint_as_float(red[R_WALK]);
   for (int64_t i = blockIdx.x * (int64_t)kBlk + threadIdx.x; i < nf; i += (int64_t)gridDim.x * kBlk) {
     const int32_t node = fin[i];
-    fit_node(node, kl, kr, leaf_box, node_box, cost, height);
+    fit_node<true>(node, kl, kr, size, leaf_max, wr, leaf_box, node_box, cost, height, leafy);
     arrive(node, r, par_int, flag, counts, fout);
   }
 }
@@ -462,9 +501,9 @@ __device__ __forceinline__ void set_box(const TreeletLds& T, int32_t S, float* o
 
 // Returns true when the treelet was rebuilt (node's box, cost and height are
 // then written too).  Every lane of the wave calls it with the same node.
-__device__ bool treelet(int32_t node, TreeletLds& T, int lane, int32_t* kl, int32_t* kr, int32_t* par_int,
-                        int32_t* par_leaf, const int32_t* __restrict__ kept, const float* __restrict__ leaf_box,
-                        float* node_box, float* cost, int32_t* height) {
+__device__ bool treelet(int32_t node, TreeletLds& T, int lane, float wr, int32_t* kl, int32_t* kr,
+                        int32_t* par_int, int32_t* par_leaf, const int32_t* __restrict__ kept,
+                        const float* __restrict__ leaf_box, float* node_box, float* cost, int32_t* height) {
   if (lane == 0) {
     int nl = 2, ni = 1;
     T.inner[0] = node;
@@ -473,7 +512,7 @@ __device__ bool treelet(int32_t node, TreeletLds& T, int lane, int32_t* kl, int3
     for (int j = 0; j < 2; ++j) {
       const float* b = ref_box(T.leaves[j], leaf_box, node_box);
       for (int k = 0; k < 6; ++k) T.lbox[j][k] = b[k];
-      T.larea[j] = box_area(b);
+      T.larea[j] = measure(b, wr);
     }
     while (nl < kTreeletLeaves) {
       int best = -1;
@@ -489,12 +528,12 @@ __device__ bool treelet(int32_t node, TreeletLds& T, int lane, int32_t* kl, int3
       for (int q = 0; q < 2; ++q) {
         const float* b = ref_box(T.leaves[js[q]], leaf_box, node_box);
         for (int k = 0; k < 6; ++k) T.lbox[js[q]][k] = b[k];
-        T.larea[js[q]] = box_area(b);
+        T.larea[js[q]] = measure(b, wr);
       }
       ++nl;
     }
     for (int j = 0; j < nl; ++j) {
-      T.lcost[j] = ref_cost(T.leaves[j], leaf_box, cost);
+      T.lcost[j] = ref_cost(T.leaves[j], leaf_box, cost, wr);
       T.lheight[j] = ref_height(T.leaves[j], height);
     }
     T.nl = nl;
@@ -507,7 +546,7 @@ __device__ bool treelet(int32_t node, TreeletLds& T, int lane, int32_t* kl, int3
     if (S == 0) continue;
     float b[6];
     set_box(T, S, b);
-    T.area[S] = box_area(b);
+    T.area[S] = measure(b, wr);
     if (__popc(S) == 1) {
       const int j = __ffs(S) - 1;
       T.cost[S] = T.lcost[j];
@@ -535,7 +574,7 @@ __device__ bool treelet(int32_t node, TreeletLds& T, int lane, int32_t* kl, int3
     wave_sync();
   }
   const int full = nsub - 1;
-  const float cur = kCi * T.area[full] + ref_cost(kl[node], leaf_box, cost) + ref_cost(kr[node], leaf_box, cost);
+  const float cur = kCi * T.area[full] + ref_cost(kl[node], leaf_box, cost, wr) + ref_cost(kr[node], leaf_box, cost, wr);
   // Non-finite costs (huge or NaN coordinates) leave partitions unset: keep
   // the treelet as it is.
   if (!(T.cost[full] < cur * (1.0f - 1e-5f)) || !(cur < 3.0e38f)) return false;
@@ -577,18 +616,20 @@ __device__ bool treelet(int32_t node, TreeletLds& T, int lane, int32_t* kl, int3
 __global__ __launch_bounds__(kBlk) void k_treelet_round(int r, int32_t* counts, const int32_t* __restrict__ fin,
                                                          int32_t* fout, int32_t* kl, int32_t* kr, int32_t* par_int,
                                                          int32_t* par_leaf, const int32_t* __restrict__ kept,
+                                                         const uint32_t* __restrict__ red,
                                                          const float* __restrict__ leaf_box, float* node_box,
                                                          float* cost, int32_t* height, int32_t* flag) {
   __shared__ TreeletLds s_t[kBlk / 64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int32_t nf = counts[r];
+  const float wr = __uint_as_float(red[R_WALK]);
   const int64_t nw = (int64_t)gridDim.x * (kBlk / 64);
   for (int64_t i = blockIdx.x * (int64_t)(kBlk / 64) + wv; i < nf; i += nw) {
     const int32_t node = fin[i];
-    const bool rebuilt = treelet(node, s_t[wv], lane, kl, kr, par_int, par_leaf, kept, leaf_box, node_box, cost,
-                                 height);
+    const bool rebuilt = treelet(node, s_t[wv], lane, wr, kl, kr, par_int, par_leaf, kept, leaf_box, node_box,
+                                 cost, height);
     if (lane == 0) {
-      if (!rebuilt) fit_node(node, kl, kr, leaf_box, node_box, cost, height);
+      if (!rebuilt) fit_node<false>(node, kl, kr, nullptr, 0, wr, leaf_box, node_box, cost, height, nullptr);
       arrive(node, r, par_int, flag, counts, fout);
     }
     wave_sync();
@@ -724,30 +765,37 @@ extern "C" int a3d_bvh_sizes(int64_t nF, int leaf_max, int64_t* blob_bytes, int6
 
 namespace {
 
-// Launches `count` frontier rounds starting at round r0.
+// Launches `count` frontier rounds starting at round r0: TREELET false fits
+// every internal node (and chooses the leaves), true restructures the kept
+// nodes in treelets.
 template <bool TREELET>
-int launch_rounds(const Ws& w, int r0, int count, int64_t m, hipStream_t s) {
+int launch_rounds(const Ws& w, int r0, int count, int leaf_max, hipStream_t s) {
   const unsigned g_fit = 1024, g_tree = 2048;
   for (int r = r0; r < r0 + count; ++r) {
     int32_t* fin = (r & 1) ? w.front1 : w.front0;
     int32_t* fout = (r & 1) ? w.front0 : w.front1;
     if (TREELET)
       hipLaunchKernelGGL(k_treelet_round, dim3(g_tree), dim3(kBlk), 0, s, r, w.counts, fin, fout, w.kl, w.kr,
-                         w.par_int, w.par_leaf, w.kept, w.leaf_box, w.node_box, w.cost, w.height, w.flag);
+                         w.par_int, w.par_leaf, w.kept, (const uint32_t*)w.red, w.leaf_box, w.node_box, w.cost,
+                         w.height, w.flag);
     else
       hipLaunchKernelGGL(k_fit_round, dim3(g_fit), dim3(kBlk), 0, s, r, w.counts, fin, fout, w.kl, w.kr,
-                         w.par_int, w.leaf_box, w.node_box, w.cost, w.height, w.flag);
+                         w.par_int, w.size, leaf_max, (const uint32_t*)w.red, w.leaf_box, w.node_box, w.cost,
+                         w.height, w.flag, w.leafy);
     A3D_LAUNCH_CHECK();
   }
-  (void)m;
   return A3D_OK;
 }
 
-int pass_init(const Ws& w, int64_t m, hipStream_t s) {
+int pass_init(const Ws& w, int64_t m, bool full, hipStream_t s) {
   hipLaunchKernelGGL(k_zero_i32, dim3(16), dim3(kBlk), 0, s, w.counts, (int64_t)(kRoundsMax + 1));
   A3D_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_pass_init, dim3((unsigned)div_up(m, kBlk)), dim3(kBlk), 0, s, m, w.kept, w.kl, w.kr, w.flag,
-                     w.counts, w.front0);
+  if (full)
+    hipLaunchKernelGGL(k_pass_init_full, dim3((unsigned)div_up(m, kBlk)), dim3(kBlk), 0, s, m, w.kl, w.kr, w.flag,
+                       w.counts, w.front0);
+  else
+    hipLaunchKernelGGL(k_pass_init, dim3((unsigned)div_up(m, kBlk)), dim3(kBlk), 0, s, m, w.kept, w.kl, w.kr, w.flag,
+                       w.counts, w.front0);
   A3D_LAUNCH_CHECK();
   return A3D_OK;
 }
@@ -802,16 +850,15 @@ extern "C" int a3d_bvh_build_ex(const float* V, int64_t nV, const int32_t* F, in
   A3D_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_leaf_boxes, dim3(gn), dim3(kBlk), 0, s, V, F, w.vals_out, n, w.leaf_box);
   A3D_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_kept, dim3(gm), dim3(kBlk), 0, s, w.size, m, leaf_max, w.kept);
+  hipLaunchKernelGGL(k_walk_radius, dim3(1), dim3(64), 0, s, w.red);
   A3D_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_units, dim3(gm), dim3(kBlk), 0, s, m, w.kept, w.par_int, w.first, w.size, w.leaf_box,
-                     w.node_box, w.cost, w.height);
+  // Fit every internal node of the Karras tree and choose the leaves.  The
+  // tree is at most kMaxDepth deep, so kMaxDepth + 1 rounds always finish
+  // (the empty ones return at once) and no host synchronisation is needed.
+  if ((rc = pass_init(w, m, true, s))) return rc;
+  if ((rc = launch_rounds<false>(w, 0, kMaxDepth + 1, leaf_max, s))) return rc;
+  hipLaunchKernelGGL(k_kept, dim3(gm), dim3(kBlk), 0, s, m, leaf_max, w.size, w.leafy, w.par_int, w.kept);
   A3D_LAUNCH_CHECK();
-  // Fit the kept nodes of the Karras tree: its depth is at most kMaxDepth, so
-  // kMaxDepth + 1 rounds always finish (the empty ones return at once) and
-  // no host synchronisation is needed.
-  if ((rc = pass_init(w, m, s))) return rc;
-  if ((rc = launch_rounds<false>(w, 0, kMaxDepth + 1, m, s))) return rc;
   const int32_t* depth_src = w.height;  // root = node 0
   if (treelet_passes > 0) {
     A3D_HIP_CHECK(hipMemcpyAsync(w.kl0, w.kl, m * 4, hipMemcpyDeviceToDevice, s));
@@ -820,14 +867,14 @@ extern "C" int a3d_bvh_build_ex(const float* V, int64_t nV, const int32_t* F, in
     int32_t h0 = 0;
     A3D_HIP_CHECK(hipMemcpyAsync(&h0, w.height, 4, hipMemcpyDeviceToHost, s));
     for (int pass = 0; pass < treelet_passes; ++pass) {
-      if ((rc = pass_init(w, m, s))) return rc;
+      if ((rc = pass_init(w, m, false, s))) return rc;
       // Restructuring can deepen the tree, so the rounds run until the
       // frontier is empty, checked on the host every `chunk` rounds.
       const int chunk = 16;
       int r = 0;
       while (true) {
         if (r + chunk > kRoundsMax) return fail(A3D_ERR_INTERNAL, "a3d_bvh_build: treelet pass did not converge");
-        if ((rc = launch_rounds<true>(w, r, chunk, m, s))) return rc;
+        if ((rc = launch_rounds<true>(w, r, chunk, leaf_max, s))) return rc;
         r += chunk;
         int32_t left = 0;
         A3D_HIP_CHECK(hipMemcpyAsync(&left, w.counts + r, 4, hipMemcpyDeviceToHost, s));

@@ -38,27 +38,32 @@ def test_tree_structure_sphere_union():
     print(info)
 
 
-def _sah_cost(b, ci=1.2, ct=1.0):
-    """SAH cost of the emitted tree, from the blob on the host."""
+def _sah_cost(b, ci=1.2, ct=1.0, walk=0.03):
+    """Cost of the emitted tree under the build's cost model (box measure =
+    Steiner volume of the box grown by r = walk x scene diagonal, over r),
+    from the blob on the host."""
     raw = b.blob.cpu().numpy()
     hdr = raw[:256].view(np.int32)
     n_nodes = int(hdr[2])
     nodes_off = int(raw[16:24].view(np.int64)[0])
+    lo, hi = raw[32:44].view(np.float32).astype(np.float64), raw[44:56].view(np.float32).astype(np.float64)
+    r = walk * np.linalg.norm(hi - lo)
     nodes = raw[nodes_off:nodes_off + 64 * n_nodes].view(np.float32).reshape(n_nodes, 16).astype(np.float64)
     refs = raw[nodes_off:nodes_off + 64 * n_nodes].view(np.int32).reshape(n_nodes, 16)[:, 12:14]
 
-    def area(bx):
+    def measure(bx):
         d = np.maximum(bx[:, 3:6] - bx[:, 0:3], 0.0)
-        return 2.0 * (d[:, 0] * d[:, 1] + d[:, 1] * d[:, 2] + d[:, 2] * d[:, 0])
+        a = 2.0 * (d[:, 0] * d[:, 1] + d[:, 1] * d[:, 2] + d[:, 2] * d[:, 0])
+        return d.prod(1) / r + a + np.pi * d.sum(1) * r + 4.0 / 3.0 * np.pi * r * r
 
     lb, rb = nodes[:, 0:6], nodes[:, 6:12]
     un = np.concatenate([np.minimum(lb[:, :3], rb[:, :3]), np.maximum(lb[:, 3:], rb[:, 3:])], 1)
-    cost = ci * area(un).sum()
+    cost = ci * measure(un).sum()
     for side, bx in ((0, lb), (1, rb)):
-        r = refs[:, side]
-        leaf = (r < 0) & (bx[:, 0] <= bx[:, 3])
-        cnt = ((~r[leaf]) & 7) + 1
-        cost += ct * (cnt * area(bx[leaf])).sum()
+        rr = refs[:, side]
+        leaf = (rr < 0) & (bx[:, 0] <= bx[:, 3])
+        cnt = ((~rr[leaf]) & 7) + 1
+        cost += ct * (cnt * measure(bx[leaf])).sum()
     return cost
 
 
@@ -75,7 +80,7 @@ def test_treelet_refinement_valid_cheaper_same_results(mesh):
     assert i0["n_nodes"] == i3["n_nodes"]
     assert 1 <= i3["depth"] <= 96 and 1 <= i0["depth"] <= 96
     c0, c3 = _sah_cost(b0), _sah_cost(b3)
-    print(f"SAH cost plain {c0:.4g} refined {c3:.4g} ({c3 / c0:.3f}); depth {i0['depth']} -> {i3['depth']}")
+    print(f"tree cost plain {c0:.4g} refined {c3:.4g} ({c3 / c0:.3f}); depth {i0['depth']} -> {i3['depth']}")
     assert c3 < c0
     P = datasets.uniform_points(1 << 16, seed=4).to(DEV)
     for kw in (dict(), dict(flags=1), dict(flags=2)):
