@@ -281,8 +281,10 @@ __device__ __forceinline__ const TriBox32* boxes_of(const void* blob) {
 
 // Squared-distance threshold above which a box cannot hold a face the
 // reference formula would rank at or below `best` (see the UDF kernels).
+// The hardware square root (within 1 ulp, i.e. a factor 1 - 2^-23 at worst)
+// is enough: the (1 + 2^-20) widening covers its error.
 __device__ __forceinline__ float prune_thr(float best, float slack) {
-  float t = (sqrtf(best) + slack) * 1.00000095367431640625f;  // x (1 + 2^-20)
+  float t = (__builtin_amdgcn_sqrtf(best) + slack) * 1.00000095367431640625f;  // x (1 + 2^-20)
   return t * t;
 }
 
@@ -293,15 +295,19 @@ __device__ __forceinline__ float query_slack(v3 p, float coord_max) {
 }
 
 // ------------------------------------------------------------------ launch helpers
-// Blocks are handed to the 8 XCDs round-robin; remap so each XCD gets a
-// contiguous run of logical blocks (neighbouring queries share its L2).
+// Blocks are handed to the 8 XCDs round-robin (block b runs on XCD b % 8).
+// Remap so each XCD walks runs of kXcdChunk consecutive logical blocks
+// (neighbouring queries share its L2) and the runs of all XCDs interleave
+// over the whole index range, so query regions of uneven cost spread evenly
+// over the XCDs.  (One contiguous range per XCD left XCDs idle for up to a
+// third of a walk on clustered workloads.)  A trailing partial round of
+// chunks maps to itself.
+constexpr uint32_t kXcdChunk = 64;
 __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
-  const uint32_t nx = 8;
-  uint32_t per = nb / nx, rem = nb % nx;
-  uint32_t x = b % nx, k = b / nx;
-  // XCD x owns logical blocks [x*per + min(x,rem), ...)
-  uint32_t base = x * per + (x < rem ? x : rem);
-  return base + k;
+  const uint32_t nx = 8, S = nx * kXcdChunk, full = nb / S * S;
+  if (b >= full) return b;
+  const uint32_t x = b % nx, k = b / nx;
+  return ((k / kXcdChunk) * nx + x) * kXcdChunk + (k % kXcdChunk);
 }
 
 inline int64_t div_up(int64_t a, int64_t b) { return (a + b - 1) / b; }

@@ -385,28 +385,60 @@ __global__ void k_zero_i32(int32_t* p, int64_t n) {
 // Start of a bottom-up pass over the kept nodes: flag = number of children
 // already final (units and triangles); nodes with two such children form the
 // first frontier.  counts[] must be zero.
+// Appends `val` for every thread of the block whose `pred` holds: one
+// device-scope atomic per block (atomics on one address serialise at the
+// memory side, so per-wave atomics cost ~10 ns x the wave count).  Every
+// thread of the block must call it.
+__device__ __forceinline__ void block_append(bool pred, int32_t val, int32_t* count, int32_t* out) {
+  __shared__ int32_t s_cnt[kBlk / 64];
+  __shared__ int32_t s_base;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const uint64_t m = __ballot(pred);
+  if (lane == 0) s_cnt[wv] = (int32_t)__popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int32_t t = 0;
+    for (int w = 0; w < kBlk / 64; ++w) { const int32_t c = s_cnt[w]; s_cnt[w] = t; t += c; }
+    s_base = t > 0 ? atomicAdd(count, t) : 0;
+  }
+  __syncthreads();
+  if (pred)
+    out[s_base + s_cnt[wv] +
+        (int32_t)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] = val;
+  __syncthreads();  // s_cnt / s_base are reused by the next call
+}
+
+// Grid-stride loops whose trip count is uniform over each block (for
+// block_append): the block's threads cover [i0, i0 + kBlk).
+#define A3D_BLOCK_LOOP(i0, n) \
+  for (int64_t i0 = (int64_t)blockIdx.x * kBlk; i0 < (n); i0 += (int64_t)gridDim.x * kBlk)
+
 __global__ __launch_bounds__(kBlk) void k_pass_init(int64_t m, const int32_t* __restrict__ kept,
                                                      const int32_t* __restrict__ kl, const int32_t* __restrict__ kr,
                                                      int32_t* flag, int32_t* counts, int32_t* front) {
-  int64_t i = blockIdx.x * (int64_t)kBlk + threadIdx.x;
-  if (i >= m || !kept[i]) return;
-  const int c = (ref_kept(kl[i], kept) ? 0 : 1) + (ref_kept(kr[i], kept) ? 0 : 1);
-  flag[i] = c;
-  if (c == 2) front[atomicAdd(&counts[0], 1)] = (int32_t)i;
+  A3D_BLOCK_LOOP(i0, m) {
+    const int64_t i = i0 + threadIdx.x;
+    bool ready = false;
+    if (i < m && kept[i]) {
+      const int c = (ref_kept(kl[i], kept) ? 0 : 1) + (ref_kept(kr[i], kept) ? 0 : 1);
+      flag[i] = c;
+      ready = c == 2;
+    }
+    block_append(ready, (int32_t)i, &counts[0], front);
+  }
 }
 
 // Start of the bottom-up pass over every internal node: triangles are final.
+// Its first round takes the nodes with two triangle children straight from
+// the node array (no list).
 __global__ __launch_bounds__(kBlk) void k_pass_init_full(int64_t m, const int32_t* __restrict__ kl,
-                                                          const int32_t* __restrict__ kr, int32_t* flag,
-                                                          int32_t* counts, int32_t* front) {
-  int64_t i = blockIdx.x * (int64_t)kBlk + threadIdx.x;
-  if (i >= m) return;
-  const int c = (kl[i] < 0 ? 1 : 0) + (kr[i] < 0 ? 1 : 0);
-  flag[i] = c;
-  if (c == 2) front[atomicAdd(&counts[0], 1)] = (int32_t)i;
+                                                          const int32_t* __restrict__ kr, int32_t* flag) {
+  const int64_t i = blockIdx.x * (int64_t)kBlk + threadIdx.x;
+  if (i < m) flag[i] = (kl[i] < 0 ? 1 : 0) + (kr[i] < 0 ? 1 : 0);
 }
 
-// The second child to finish puts the parent into the next round's frontier.
+// The second child to finish puts the parent into the next round's frontier
+// (single-lane form, for the treelet rounds).
 __device__ __forceinline__ void arrive(int32_t node, int r, const int32_t* __restrict__ par_int, int32_t* flag,
                                        int32_t* counts, int32_t* fout) {
   const int32_t p = par_int[node];
@@ -444,7 +476,7 @@ __device__ __forceinline__ void fit_node(int32_t node, const int32_t* __restrict
 }
 
 // One fitting round over every internal node: one thread per frontier node.
-__global__ __launch_bounds__(kBlk) void k_fit_round(int r, int32_t* counts, const int32_t* __restrict__ fin,
+__global__ __launch_bounds__(kBlk) void k_fit_round(int r, int64_t m, int32_t* counts, const int32_t* __restrict__ fin,
                                                      int32_t* fout, const int32_t* __restrict__ kl,
                                                      const int32_t* __restrict__ kr,
                                                      const int32_t* __restrict__ par_int,
@@ -452,12 +484,19 @@ __global__ __launch_bounds__(kBlk) void k_fit_round(int r, int32_t* counts, cons
                                                      const uint32_t* __restrict__ red,
                                                      const float* __restrict__ leaf_box, float* node_box,
                                                      float* cost, int32_t* height, int32_t* flag, int32_t* leafy) {
-  const int32_t nf = counts[r];
+  // round 0: every node with two triangle children, read from the node array
+  const int64_t nf = r == 0 ? m : counts[r];
   const float wr = __uint_as_float(red[R_WALK]);
-  for (int64_t i = blockIdx.x * (int64_t)kBlk + threadIdx.x; i < nf; i += (int64_t)gridDim.x * kBlk) {
-    const int32_t node = fin[i];
-    fit_node<true>(node, kl, kr, size, leaf_max, wr, leaf_box, node_box, cost, height, leafy);
-    arrive(node, r, par_int, flag, counts, fout);
+  A3D_BLOCK_LOOP(i0, nf) {
+    const int64_t i = i0 + threadIdx.x;
+    int32_t p = -1;
+    if (i < nf && (r > 0 || (kl[i] < 0 && kr[i] < 0))) {
+      const int32_t node = r == 0 ? (int32_t)i : fin[i];
+      fit_node<true>(node, kl, kr, size, leaf_max, wr, leaf_box, node_box, cost, height, leafy);
+      p = par_int[node];
+      if (p >= 0 && atomicAdd(&flag[p], 1) != 1) p = -1;
+    }
+    block_append(p >= 0, p, &counts[r + 1], fout);
   }
 }
 
@@ -769,7 +808,7 @@ namespace {
 // every internal node (and chooses the leaves), true restructures the kept
 // nodes in treelets.
 template <bool TREELET>
-int launch_rounds(const Ws& w, int r0, int count, int leaf_max, hipStream_t s) {
+int launch_rounds(const Ws& w, int64_t m, int r0, int count, int leaf_max, hipStream_t s) {
   const unsigned g_fit = 1024, g_tree = 2048;
   for (int r = r0; r < r0 + count; ++r) {
     int32_t* fin = (r & 1) ? w.front1 : w.front0;
@@ -779,7 +818,7 @@ int launch_rounds(const Ws& w, int r0, int count, int leaf_max, hipStream_t s) {
                          w.par_int, w.par_leaf, w.kept, (const uint32_t*)w.red, w.leaf_box, w.node_box, w.cost,
                          w.height, w.flag);
     else
-      hipLaunchKernelGGL(k_fit_round, dim3(g_fit), dim3(kBlk), 0, s, r, w.counts, fin, fout, w.kl, w.kr,
+      hipLaunchKernelGGL(k_fit_round, dim3(g_fit), dim3(kBlk), 0, s, r, m, w.counts, fin, fout, w.kl, w.kr,
                          w.par_int, w.size, leaf_max, (const uint32_t*)w.red, w.leaf_box, w.node_box, w.cost,
                          w.height, w.flag, w.leafy);
     A3D_LAUNCH_CHECK();
@@ -790,12 +829,11 @@ int launch_rounds(const Ws& w, int r0, int count, int leaf_max, hipStream_t s) {
 int pass_init(const Ws& w, int64_t m, bool full, hipStream_t s) {
   hipLaunchKernelGGL(k_zero_i32, dim3(16), dim3(kBlk), 0, s, w.counts, (int64_t)(kRoundsMax + 1));
   A3D_LAUNCH_CHECK();
+  const unsigned g = (unsigned)(div_up(m, kBlk) < 1024 ? div_up(m, kBlk) : 1024);
   if (full)
-    hipLaunchKernelGGL(k_pass_init_full, dim3((unsigned)div_up(m, kBlk)), dim3(kBlk), 0, s, m, w.kl, w.kr, w.flag,
-                       w.counts, w.front0);
+    hipLaunchKernelGGL(k_pass_init_full, dim3((unsigned)div_up(m, kBlk)), dim3(kBlk), 0, s, m, w.kl, w.kr, w.flag);
   else
-    hipLaunchKernelGGL(k_pass_init, dim3((unsigned)div_up(m, kBlk)), dim3(kBlk), 0, s, m, w.kept, w.kl, w.kr, w.flag,
-                       w.counts, w.front0);
+    hipLaunchKernelGGL(k_pass_init, dim3(g), dim3(kBlk), 0, s, m, w.kept, w.kl, w.kr, w.flag, w.counts, w.front0);
   A3D_LAUNCH_CHECK();
   return A3D_OK;
 }
@@ -856,7 +894,7 @@ extern "C" int a3d_bvh_build_ex(const float* V, int64_t nV, const int32_t* F, in
   // tree is at most kMaxDepth deep, so kMaxDepth + 1 rounds always finish
   // (the empty ones return at once) and no host synchronisation is needed.
   if ((rc = pass_init(w, m, true, s))) return rc;
-  if ((rc = launch_rounds<false>(w, 0, kMaxDepth + 1, leaf_max, s))) return rc;
+  if ((rc = launch_rounds<false>(w, m, 0, kMaxDepth + 1, leaf_max, s))) return rc;
   hipLaunchKernelGGL(k_kept, dim3(gm), dim3(kBlk), 0, s, m, leaf_max, w.size, w.leafy, w.par_int, w.kept);
   A3D_LAUNCH_CHECK();
   const int32_t* depth_src = w.height;  // root = node 0
@@ -874,7 +912,7 @@ extern "C" int a3d_bvh_build_ex(const float* V, int64_t nV, const int32_t* F, in
       int r = 0;
       while (true) {
         if (r + chunk > kRoundsMax) return fail(A3D_ERR_INTERNAL, "a3d_bvh_build: treelet pass did not converge");
-        if ((rc = launch_rounds<true>(w, r, chunk, leaf_max, s))) return rc;
+        if ((rc = launch_rounds<true>(w, m, r, chunk, leaf_max, s))) return rc;
         r += chunk;
         int32_t left = 0;
         A3D_HIP_CHECK(hipMemcpyAsync(&left, w.counts + r, 4, hipMemcpyDeviceToHost, s));
