@@ -7,8 +7,10 @@ atom3d/core/mesh_bvh.py:42-1397 for the hot path:
   (:59-117) — casts, bounds, face normals, Baerentzen-Aanaes pseudo-normal
   tables (:129-232) and a device BVH;
 * ``udf(points, return_grad, return_closest, return_uvw, return_face_ids)`` (:601-700);
-* ``sdf(points, mode="pseudo_normal", ...)`` (:702-827);
-* ``intersect_aabb(aabb_min, aabb_max, mode)`` (:297-479).
+* ``sdf(points, mode="pseudo_normal" | "parity", ...)`` (:702-827, :851-885);
+* ``intersect_aabb(aabb_min, aabb_max, mode)`` (:297-479);
+* ``intersect_ray(rays_o, rays_d, max_t, return_uvw, mode="nearest" | "all")``
+  and ``intersect_segment(seg_start, seg_end)`` (:1076-1357).
 
 There is no silent fallback: the reference catches every native failure and
 falls back to slower code with different maths (:122-127, :352-379, :656-683);
@@ -22,7 +24,8 @@ import torch
 
 from ..kernels import _lib
 from ..kernels.bvh import BVHAccelerator
-from .data_structures import AABBIntersectResult, ClosestPointResult
+from .data_structures import (AABBIntersectResult, ClosestPointResult, RayAllHitResult, RayIntersectResult,
+                              SegmentIntersectResult)
 from .device_utils import resolve_device
 
 
@@ -233,9 +236,12 @@ class MeshBVH:
     def sdf(self, points: torch.Tensor, mode: str = "pseudo_normal", return_grad: bool = False,
             return_closest: bool = False, return_uvw: bool = False, return_face_ids: bool = False,
             **kwargs) -> Union[torch.Tensor, ClosestPointResult]:
-        """Signed distance with the pseudo-normal sign (negative inside)."""
+        """Signed distance, negative inside: mode "pseudo_normal" (sign from the
+        pseudo-normal at the closest point) or "parity" (majority vote of ray
+        crossing parities over random directions; kwargs n_dirs, max_t,
+        coalesce_t, seed as the reference's _sdf_parity)."""
         if mode == "parity":
-            raise NotImplementedError("sdf(mode='parity') (ray-parity sign) is not built yet")
+            return self._sdf_parity(points, **kwargs)
         if mode != "pseudo_normal":
             raise ValueError(f"unknown sdf mode {mode!r}")
         res = self._udf_with_grad(points) if return_grad else self._udf_query(points)
@@ -263,3 +269,94 @@ class MeshBVH:
                 _lib.ptr(signs), _lib.stream(self.device),
             )
         return signs
+
+    # ------------------------------------------------------------------ rays
+    def _sdf_parity(self, points: torch.Tensor, n_dirs: int = 8, max_t: float = 1e6, coalesce_t: float = 1e-4,
+                    seed: int = 42) -> torch.Tensor:
+        """Signed distance with the ray-parity sign (mesh_bvh.py:851-885):
+        a point is inside when more than half of n_dirs random directions
+        cross the surface an odd number of times."""
+        N = points.shape[0]
+        udf = self.udf(points)
+        gen = torch.Generator(device=self.device)
+        gen.manual_seed(seed)
+        dirs = torch.randn(n_dirs, 3, device=self.device, generator=gen)
+        dirs = dirs / dirs.norm(dim=1, keepdim=True)
+        votes = torch.zeros(N, dtype=torch.int32, device=self.device)
+        for i in range(n_dirs):
+            sc = self._signed_crossing_count(points, dirs[i], max_t=max_t, coalesce_t=coalesce_t)
+            votes += ((sc.abs() % 2) == 1).int()
+        inside = votes > (n_dirs // 2)
+        sdf = udf.clone()
+        sdf[inside] = -sdf[inside]
+        return sdf
+
+    def _signed_crossing_count(self, origins: torch.Tensor, direction: torch.Tensor, max_t: float = 1e6,
+                               coalesce_t: float = 1e-4) -> torch.Tensor:
+        """Signed crossing count per origin along one direction
+        (mesh_bvh.py:906-970), on the device in one pass per ray: hits sorted
+        by t exactly (the reference sorts by the float32 key ray * 2 max_t + t,
+        which stops resolving t beyond the first few rays)."""
+        N = origins.shape[0]
+        d = direction.reshape(1, 3).to(self.device).float().expand(N, 3).contiguous()
+        return self._bvh.signed_crossings(origins, d, max_t=max_t, coalesce_t=coalesce_t)
+
+    def intersect_ray(self, rays_o: torch.Tensor, rays_d: torch.Tensor, max_t: float = 1e10,
+                      return_uvw: bool = False, mode: str = "nearest") -> Union[RayIntersectResult, RayAllHitResult]:
+        """Ray-mesh intersection (mesh_bvh.py:1076-1157): "nearest" gives the
+        closest hit per ray (minimum t, ties to the lowest face index), "all"
+        every hit, sorted by (ray, face)."""
+        if mode == "all":
+            return self._intersect_ray_all(rays_o, rays_d, max_t)
+        if mode != "nearest":
+            raise ValueError(f"unknown intersect_ray mode {mode!r}")
+        hit, t, face_ids, hit_points, _ = self._bvh.ray_nearest(rays_o, rays_d, max_t)
+        normals = self._compute_normals_from_faces(face_ids, hit)
+        bary = None
+        if return_uvw and bool(hit.any()):
+            bary = torch.zeros(rays_o.shape[0], 3, device=self.device)
+            bary[hit] = self._bary_from_hit(hit_points[hit], face_ids[hit])
+        return RayIntersectResult(hit=hit, t=t, face_ids=face_ids, hit_points=hit_points, normals=normals,
+                                  bary_coords=bary)
+
+    def _intersect_ray_all(self, origins: torch.Tensor, directions: torch.Tensor,
+                           max_t: float = 1e6) -> RayAllHitResult:
+        ray_idx, t, face_ids, sign, hit_points, uvw = self._bvh.ray_all_hit(origins, directions, max_t)
+        return RayAllHitResult(ray_idx=ray_idx, t=t, face_ids=face_ids, sign=sign, hit_points=hit_points,
+                               bary_coords=uvw)
+
+    def intersect_segment(self, seg_start: torch.Tensor, seg_end: torch.Tensor) -> SegmentIntersectResult:
+        """Segment-mesh intersection (mesh_bvh.py:1329-1357): the nearest hit
+        of the ray along the segment, kept when t <= the segment length."""
+        rays_o = seg_start
+        rays_d = seg_end - seg_start
+        seg_lengths = rays_d.norm(dim=1, keepdim=True)
+        rays_d = rays_d / (seg_lengths + 1e-8)
+        max_t = seg_lengths.squeeze().max().item() if seg_lengths.numel() else 0.0
+        ray_result = self.intersect_ray(rays_o, rays_d, max_t=max_t)
+        hit = ray_result.hit & (ray_result.t <= seg_lengths.squeeze())
+        return SegmentIntersectResult(hit=hit, hit_points=ray_result.hit_points, face_ids=ray_result.face_ids,
+                                      bary_coords=ray_result.bary_coords)
+
+    def _bary_from_hit(self, hit_points: torch.Tensor, face_ids: torch.Tensor) -> torch.Tensor:
+        """Barycentrics of hit points on known faces (mesh_bvh.py:1218-1244)."""
+        fi = face_ids.long()
+        faces_long = self.faces.long()
+        v0 = self.vertices[faces_long[fi, 0]]
+        v1 = self.vertices[faces_long[fi, 1]]
+        v2 = self.vertices[faces_long[fi, 2]]
+        e0, e1, vp = v1 - v0, v2 - v0, hit_points - v0
+        d00, d01, d11 = (e0 * e0).sum(-1), (e0 * e1).sum(-1), (e1 * e1).sum(-1)
+        d20, d21 = (vp * e0).sum(-1), (vp * e1).sum(-1)
+        denom = d00 * d11 - d01 * d01 + 1e-12
+        bv = (d11 * d20 - d01 * d21) / denom
+        bw = (d00 * d21 - d01 * d20) / denom
+        return torch.stack([1.0 - bv - bw, bv, bw], dim=-1)
+
+    def _compute_normals_from_faces(self, face_ids: torch.Tensor, valid_mask: torch.Tensor) -> torch.Tensor:
+        """Cached unit face normals of the hit faces, zeros elsewhere
+        (mesh_bvh.py:1246-1264)."""
+        normals = torch.zeros(face_ids.shape[0], 3, device=self.device)
+        if bool(valid_mask.any()):
+            normals[valid_mask] = self._face_normals[face_ids[valid_mask].long()]
+        return normals

@@ -1,0 +1,404 @@
+// Ray queries for gfx950: nearest hit, all hits, signed crossing counts.
+//
+// Replaces bvh_ray_intersect_kernel / bvh_ray_intersect_cuda and
+// bvh_ray_all_hit_kernel / bvh_ray_all_hit_cuda (Atom3D
+// atom3d/kernels/bvh_kernels.cu:469-550, 935-1022, 1068-1190) and the
+// coalescing of MeshBVH._signed_crossing_count (atom3d/core/mesh_bvh.py:
+// 906-970), on the blob built by bvh_build.hip.
+//
+// Per-triangle maths is the reference's: Moller-Trumbore with its FP64 u/v
+// range tests (Triangle::ray_intersect, :185-202), the unit normal
+// (Triangle::normal, :89-97) for the all-hit grazing filter and sign, and the
+// barycentrics of Triangle::barycentric (:205-222).
+//
+// Results do not depend on the tree:
+//   nearest hit = the minimum t over all faces (t < max_t, and not the
+//   test's no-hit value 1e10), ties to the lowest face index;
+//   all hits = every face passing the reference's filters, listed in (ray,
+//   face) order after a3d_pairs_sort + a3d_ray_eval.
+// Box tests widen every box by 2^-20 x (coordinate scale), which covers the
+// FP32 rounding of the slab arithmetic, so a box holding a hit face is never
+// skipped for rounding; the nearest-hit walk skips a box only when its entry
+// distance exceeds the current best t by more than 1/128 of it.  (The
+// FP32 t of a face can be off by more than that only for rays within ~1e-5
+// rad of the face's plane; those faces could then be missed where the
+// all-faces scan would take them.)
+#include "a3d_common.hpp"
+
+namespace a3d {
+namespace {
+
+constexpr int kBlk = 256;
+constexpr int kRayLds = 16;  // per-lane LDS stack entries
+constexpr int kRayOvf = kStackTotal - kRayLds;
+constexpr float kNoHit = 1e10f;
+
+// bvh_kernels.cu:185-202 (FP64 u/v tests as in the reference)
+__device__ __forceinline__ float ray_tri(v3 a, v3 b, v3 c, v3 ro, v3 rd) {
+  const float eps = 1e-8f;
+  const double eps_edge = 1.19209290e-7;
+  v3 e1 = sub3(b, a);
+  v3 e2 = sub3(c, a);
+  v3 h = cross3(rd, e2);
+  float det = dot3(e1, h);
+  if (fabsf(det) < eps) return kNoHit;
+  float f = 1.0f / det;
+  v3 s = sub3(ro, a);
+  double u = (double)f * (double)dot3(s, h);
+  if (u < -eps_edge || u > 1.0 + eps_edge) return kNoHit;
+  v3 q = cross3(s, e1);
+  double v = (double)f * (double)dot3(rd, q);
+  if (v < -eps_edge || u + v > 1.0 + eps_edge) return kNoHit;
+  float t = f * dot3(e2, q);
+  return (t > eps) ? t : kNoHit;
+}
+
+// bvh_kernels.cu:89-97
+__device__ __forceinline__ v3 tri_normal(v3 a, v3 b, v3 c) {
+  v3 n = cross3(sub3(b, a), sub3(c, a));
+  float len = sqrtf(n.x * n.x + n.y * n.y + n.z * n.z);
+  if (len > 1e-10f) return mul3(n, 1.0f / len);
+  return mk(0.0f, 0.0f, 1.0f);
+}
+
+// All-hit filter and sign (bvh_kernels.cu:975-985): 0 when the face does not
+// count, else sign(dot(normal, d)).
+__device__ __forceinline__ int all_hit_sign(v3 a, v3 b, v3 c, v3 ro, v3 rd, float max_t, float& t) {
+  t = ray_tri(a, b, c, ro, rd);
+  if (!(t > 0.0f && t < max_t)) return 0;
+  const float ndotd = dot3(tri_normal(a, b, c), rd);
+  if (ndotd > -1e-4f && ndotd < 1e-4f) return 0;
+  return ndotd > 0.0f ? 1 : -1;
+}
+
+// Reciprocal direction for the slab test; components below 1e-30 in
+// magnitude are clamped (no 0 x inf).
+__device__ __forceinline__ float safe_inv(float d) {
+  return 1.0f / (fabsf(d) > 1e-30f ? d : (d >= 0.0f ? 1e-30f : -1e-30f));
+}
+
+// Entry distance (>= 0) of the ray into the box widened by w, or +inf
+// (also for an empty, inverted box).
+__device__ __forceinline__ float ray_box(v3 ro, v3 inv, float lx, float ly, float lz, float hx, float hy, float hz,
+                                         float w) {
+  if (!(lx <= hx)) return __int_as_float(0x7f800000);  // the empty child of a one-leaf tree
+  float t1 = (lx - w - ro.x) * inv.x, t2 = (hx + w - ro.x) * inv.x;
+  float tmin = fminf(t1, t2), tmax = fmaxf(t1, t2);
+  t1 = (ly - w - ro.y) * inv.y; t2 = (hy + w - ro.y) * inv.y;
+  tmin = fmaxf(tmin, fminf(t1, t2)); tmax = fminf(tmax, fmaxf(t1, t2));
+  t1 = (lz - w - ro.z) * inv.z; t2 = (hz + w - ro.z) * inv.z;
+  tmin = fmaxf(tmin, fminf(t1, t2)); tmax = fminf(tmax, fmaxf(t1, t2));
+  return (tmax >= tmin && tmax >= 0.0f) ? fmaxf(tmin, 0.0f) : __int_as_float(0x7f800000);
+}
+
+__device__ __forceinline__ float box_widen(v3 ro, float coord_max) {
+  const float m = fmaxf(fabsf(ro.x), fmaxf(fabsf(ro.y), fabsf(ro.z)));
+  return 9.5367431640625e-07f * (coord_max + m);  // 2^-20
+}
+
+__device__ __forceinline__ float prune_t(float best) { return best + best * 0.0078125f; }  // x (1 + 2^-7)
+
+// ------------------------------------------------------------ nearest hit
+__global__ __launch_bounds__(kBlk) void k_ray_nearest(const void* __restrict__ blob, const float* __restrict__ O,
+                                                       const float* __restrict__ D, int64_t K, float max_t,
+                                                       uint8_t* __restrict__ hit, float* __restrict__ t_out,
+                                                       int32_t* __restrict__ face, float* __restrict__ pts,
+                                                       float* __restrict__ uvw) {
+  __shared__ int32_t s_ref[kRayLds * kBlk];
+  __shared__ float s_t[kRayLds * kBlk];
+  const int tid = threadIdx.x;
+  const int64_t r = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlk + tid;
+  if (r >= K) return;
+  const BvhHeader* H = header(blob);
+  const Node64* __restrict__ nodes = nodes_of(blob);
+  const Tri48* __restrict__ tris = tris_of(blob);
+  const v3 ro = mk(O[r * 3], O[r * 3 + 1], O[r * 3 + 2]);
+  const v3 rd = mk(D[r * 3], D[r * 3 + 1], D[r * 3 + 2]);
+  const v3 inv = mk(safe_inv(rd.x), safe_inv(rd.y), safe_inv(rd.z));
+  const float wid = box_widen(ro, H->coord_max);
+
+  float best = max_t;
+  int32_t best_id = 0x7fffffff, best_tri = -1;
+  float thr = prune_t(best);
+  int32_t ovf_ref[kRayOvf];
+  float ovf_t[kRayOvf];
+  int sp = 0;
+  int32_t cur = H->n_tris > 0 ? 0 : ~0;
+  if (H->n_tris == 0) sp = -1;
+  while (sp >= 0) {
+    if (cur >= 0) {
+      const Node64* nd = nodes + cur;
+      const float4 a = nd->a, b = nd->b, c = nd->c;
+      const int4 d = nd->d;
+      const float tl = ray_box(ro, inv, a.x, a.y, a.z, a.w, b.x, b.y, wid);
+      const float tr = ray_box(ro, inv, b.z, b.w, c.x, c.y, c.z, c.w, wid);
+      const bool vl = tl <= thr, vr = tr <= thr;
+      if (vl && vr) {
+        int32_t near_ref = d.x, far_ref = d.y;
+        float far_t = tr;
+        if (tr < tl) { near_ref = d.y; far_ref = d.x; far_t = tl; }
+        if (sp < kRayLds) {
+          s_ref[sp * kBlk + tid] = far_ref;
+          s_t[sp * kBlk + tid] = far_t;
+        } else if (sp < kStackTotal) {
+          ovf_ref[sp - kRayLds] = far_ref;
+          ovf_t[sp - kRayLds] = far_t;
+        }
+        sp = sp < kStackTotal ? sp + 1 : sp;
+        cur = near_ref;
+        continue;
+      }
+      if (vl) { cur = d.x; continue; }
+      if (vr) { cur = d.y; continue; }
+    } else {
+      const int32_t s0 = leaf_start(cur), n = leaf_count(cur);
+      for (int32_t t = s0; t < s0 + n; ++t) {
+        v3 ta, tb, tc;
+        int32_t id;
+        load_tri(tris, t, ta, tb, tc, id);
+        const float th = ray_tri(ta, tb, tc, ro, rd);
+        if (th == kNoHit || !(th < max_t)) continue;
+        if (th < best || (th == best && id < best_id)) {
+          best = th;
+          best_id = id;
+          best_tri = t;
+          thr = prune_t(best);
+        }
+      }
+    }
+    bool found = false;
+    while (sp > 0) {
+      --sp;
+      int32_t rr;
+      float tt;
+      if (sp < kRayLds) {
+        rr = s_ref[sp * kBlk + tid];
+        tt = s_t[sp * kBlk + tid];
+      } else {
+        rr = ovf_ref[sp - kRayLds];
+        tt = ovf_t[sp - kRayLds];
+      }
+      if (tt <= thr) { cur = rr; found = true; break; }
+    }
+    if (!found) break;
+  }
+  hit[r] = best_tri >= 0 ? 1 : 0;
+  t_out[r] = best;
+  face[r] = best_tri >= 0 ? best_id : -1;
+  v3 p = ro, w = mk(0.0f, 0.0f, 0.0f);
+  if (best_tri >= 0) {
+    p = add3(ro, mul3(rd, best));
+    v3 ta, tb, tc;
+    int32_t id;
+    load_tri(tris, best_tri, ta, tb, tc, id);
+    w = tri_bary(ta, tb, tc, p);
+  }
+  if (pts) { pts[r * 3] = p.x; pts[r * 3 + 1] = p.y; pts[r * 3 + 2] = p.z; }
+  if (uvw) { uvw[r * 3] = w.x; uvw[r * 3 + 1] = w.y; uvw[r * 3 + 2] = w.z; }
+}
+
+// ------------------------------------------------------------ all hits
+// offsets == nullptr: counts[r] = number of hits; else the hits of ray r are
+// written from offsets[r] on (walk order): ray index, face, t, sign.
+__global__ __launch_bounds__(kBlk) void k_ray_all(const void* __restrict__ blob, const float* __restrict__ O,
+                                                   const float* __restrict__ D, int64_t K, float max_t,
+                                                   int32_t* __restrict__ counts, const int64_t* __restrict__ offsets,
+                                                   int32_t* __restrict__ ray_out, int32_t* __restrict__ face_out,
+                                                   float* __restrict__ t_out, int32_t* __restrict__ sign_out) {
+  __shared__ int32_t s_ref[kRayLds * kBlk];
+  const int tid = threadIdx.x;
+  const int64_t r = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlk + tid;
+  if (r >= K) return;
+  const BvhHeader* H = header(blob);
+  const Node64* __restrict__ nodes = nodes_of(blob);
+  const Tri48* __restrict__ tris = tris_of(blob);
+  const v3 ro = mk(O[r * 3], O[r * 3 + 1], O[r * 3 + 2]);
+  const v3 rd = mk(D[r * 3], D[r * 3 + 1], D[r * 3 + 2]);
+  const v3 inv = mk(safe_inv(rd.x), safe_inv(rd.y), safe_inv(rd.z));
+  const float wid = box_widen(ro, H->coord_max);
+  const float lim = prune_t(max_t);
+  int64_t pos = offsets ? offsets[r] : 0;
+  int32_t n_hit = 0;
+  int32_t ovf_ref[kRayOvf];
+  int sp = 0;
+  int32_t cur = 0;
+  bool live = H->n_tris > 0;
+  while (live) {
+    if (cur >= 0) {
+      const Node64* nd = nodes + cur;
+      const float4 a = nd->a, b = nd->b, c = nd->c;
+      const int4 d = nd->d;
+      const bool vl = ray_box(ro, inv, a.x, a.y, a.z, a.w, b.x, b.y, wid) <= lim;
+      const bool vr = ray_box(ro, inv, b.z, b.w, c.x, c.y, c.z, c.w, wid) <= lim;
+      if (vl && vr) {
+        if (sp < kRayLds) s_ref[sp * kBlk + tid] = d.y;
+        else if (sp < kStackTotal) ovf_ref[sp - kRayLds] = d.y;
+        sp = sp < kStackTotal ? sp + 1 : sp;
+        cur = d.x;
+        continue;
+      }
+      if (vl) { cur = d.x; continue; }
+      if (vr) { cur = d.y; continue; }
+    } else {
+      const int32_t s0 = leaf_start(cur), n = leaf_count(cur);
+      for (int32_t t = s0; t < s0 + n; ++t) {
+        v3 ta, tb, tc;
+        int32_t id;
+        load_tri(tris, t, ta, tb, tc, id);
+        float th;
+        const int sg = all_hit_sign(ta, tb, tc, ro, rd, max_t, th);
+        if (!sg) continue;
+        if (offsets) {
+          ray_out[pos] = (int32_t)r;
+          face_out[pos] = id;
+          if (t_out) t_out[pos] = th;
+          if (sign_out) sign_out[pos] = sg;
+          ++pos;
+        }
+        ++n_hit;
+      }
+    }
+    if (sp == 0) break;
+    --sp;
+    cur = sp < kRayLds ? s_ref[sp * kBlk + tid] : ovf_ref[sp - kRayLds];
+  }
+  if (!offsets) counts[r] = n_hit;
+}
+
+// t, sign, point and barycentrics of (ray, face) pairs from the original mesh
+// arrays (the face's vertices are the blob's, so t is the walk's t).
+__global__ __launch_bounds__(kBlk) void k_ray_eval(const float* __restrict__ V, const int32_t* __restrict__ F,
+                                                    const float* __restrict__ O, const float* __restrict__ D,
+                                                    int64_t n, const int32_t* __restrict__ ray_idx,
+                                                    const int32_t* __restrict__ face, float max_t,
+                                                    float* __restrict__ t_out, int32_t* __restrict__ sign_out,
+                                                    float* __restrict__ pts, float* __restrict__ uvw) {
+  const int64_t i = blockIdx.x * (int64_t)kBlk + threadIdx.x;
+  if (i >= n) return;
+  const int64_t r = ray_idx[i], f = face[i];
+  const v3 ro = mk(O[r * 3], O[r * 3 + 1], O[r * 3 + 2]);
+  const v3 rd = mk(D[r * 3], D[r * 3 + 1], D[r * 3 + 2]);
+  const int32_t i0 = F[f * 3], i1 = F[f * 3 + 1], i2 = F[f * 3 + 2];
+  const v3 a = mk(V[(int64_t)i0 * 3], V[(int64_t)i0 * 3 + 1], V[(int64_t)i0 * 3 + 2]);
+  const v3 b = mk(V[(int64_t)i1 * 3], V[(int64_t)i1 * 3 + 1], V[(int64_t)i1 * 3 + 2]);
+  const v3 c = mk(V[(int64_t)i2 * 3], V[(int64_t)i2 * 3 + 1], V[(int64_t)i2 * 3 + 2]);
+  float t;
+  const int sg = all_hit_sign(a, b, c, ro, rd, max_t, t);
+  const v3 p = add3(ro, mul3(rd, t));
+  const v3 w = tri_bary(a, b, c, p);
+  if (t_out) t_out[i] = t;
+  if (sign_out) sign_out[i] = sg;
+  if (pts) { pts[i * 3] = p.x; pts[i * 3 + 1] = p.y; pts[i * 3 + 2] = p.z; }
+  if (uvw) { uvw[i * 3] = w.x; uvw[i * 3 + 1] = w.y; uvw[i * 3 + 2] = w.z; }
+}
+
+// Signed crossing count per ray (mesh_bvh.py:906-970): the ray's hits are
+// sorted by t in place (insertion sort of its segment), consecutive hits
+// closer than coalesce_t form a group, an all-same-sign group counts as one
+// crossing of that sign, a mixed group by the parity of its sign sum.
+__global__ __launch_bounds__(kBlk) void k_ray_crossings(float* __restrict__ t, int32_t* __restrict__ sign,
+                                                         const int64_t* __restrict__ offsets, int64_t K,
+                                                         float coalesce_t, int32_t* __restrict__ out) {
+  const int64_t r = blockIdx.x * (int64_t)kBlk + threadIdx.x;
+  if (r >= K) return;
+  const int64_t b = offsets[r], e = offsets[r + 1];
+  for (int64_t i = b + 1; i < e; ++i) {
+    const float ti = t[i];
+    const int32_t si = sign[i];
+    int64_t j = i - 1;
+    while (j >= b && (t[j] > ti || (t[j] == ti && sign[j] > si))) {
+      t[j + 1] = t[j];
+      sign[j + 1] = sign[j];
+      --j;
+    }
+    t[j + 1] = ti;
+    sign[j + 1] = si;
+  }
+  int32_t total = 0;
+  int64_t i = b;
+  while (i < e) {
+    int32_t cnt = 1, sum = sign[i];
+    int64_t j = i + 1;
+    while (j < e && fabsf(t[j] - t[j - 1]) < coalesce_t) { ++cnt; sum += sign[j]; ++j; }
+    const int32_t sg = (sum > 0) - (sum < 0);
+    total += (abs(sum) == cnt) ? sg : (abs(sum) % 2) * sg;
+    i = j;
+  }
+  out[r] = total;
+}
+
+inline unsigned grid_for(int64_t n) { return (unsigned)div_up(n > 0 ? n : 1, kBlk); }
+
+int check_rays(const char* fn, const void* blob, const float* O, const float* D, int64_t K) {
+  if (K < 0 || K > (int64_t(1) << 31) - 1) return fail(A3D_ERR_ARG, "%s: K=%lld out of range [0, 2^31)", fn, (long long)K);
+  if (K > 0 && (!blob || !O || !D)) return fail(A3D_ERR_ARG, "%s: null blob or ray arrays", fn);
+  return A3D_OK;
+}
+
+}  // namespace
+}  // namespace a3d
+
+using namespace a3d;
+
+extern "C" int a3d_ray_nearest(const void* blob, const float* O, const float* D, int64_t K, float max_t,
+                               uint8_t* hit, float* t, int32_t* face, float* points, float* uvw,
+                               a3d_stream_t stream) {
+  int rc = check_rays("a3d_ray_nearest", blob, O, D, K);
+  if (rc) return rc;
+  if (K == 0) return A3D_OK;
+  if (!hit || !t || !face) return fail(A3D_ERR_ARG, "a3d_ray_nearest: null output");
+  hipLaunchKernelGGL(k_ray_nearest, dim3(grid_for(K)), dim3(kBlk), 0, (hipStream_t)stream, blob, O, D, K, max_t, hit,
+                     t, face, points, uvw);
+  A3D_LAUNCH_CHECK();
+  return A3D_OK;
+}
+
+extern "C" int a3d_ray_all_count(const void* blob, const float* O, const float* D, int64_t K, float max_t,
+                                 int32_t* counts, a3d_stream_t stream) {
+  int rc = check_rays("a3d_ray_all_count", blob, O, D, K);
+  if (rc) return rc;
+  if (K == 0) return A3D_OK;
+  if (!counts) return fail(A3D_ERR_ARG, "a3d_ray_all_count: null counts");
+  hipLaunchKernelGGL(k_ray_all, dim3(grid_for(K)), dim3(kBlk), 0, (hipStream_t)stream, blob, O, D, K, max_t, counts,
+                     (const int64_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr, (float*)nullptr,
+                     (int32_t*)nullptr);
+  A3D_LAUNCH_CHECK();
+  return A3D_OK;
+}
+
+extern "C" int a3d_ray_all_hits(const void* blob, const float* O, const float* D, int64_t K, float max_t,
+                                const int64_t* offsets, int32_t* ray_idx, int32_t* face, float* t, int32_t* sign,
+                                a3d_stream_t stream) {
+  int rc = check_rays("a3d_ray_all_hits", blob, O, D, K);
+  if (rc) return rc;
+  if (K == 0) return A3D_OK;
+  if (!offsets || !ray_idx || !face) return fail(A3D_ERR_ARG, "a3d_ray_all_hits: null offsets or outputs");
+  hipLaunchKernelGGL(k_ray_all, dim3(grid_for(K)), dim3(kBlk), 0, (hipStream_t)stream, blob, O, D, K, max_t,
+                     (int32_t*)nullptr, offsets, ray_idx, face, t, sign);
+  A3D_LAUNCH_CHECK();
+  return A3D_OK;
+}
+
+extern "C" int a3d_ray_eval(const float* V, int64_t nV, const int32_t* F, int64_t nF, const float* O,
+                            const float* D, int64_t n, const int32_t* ray_idx, const int32_t* face, float max_t,
+                            float* t, int32_t* sign, float* points, float* uvw, a3d_stream_t stream) {
+  if (n < 0) return fail(A3D_ERR_ARG, "a3d_ray_eval: n=%lld < 0", (long long)n);
+  if (n == 0) return A3D_OK;
+  if (!V || !F || !O || !D || !ray_idx || !face || nV <= 0 || nF <= 0)
+    return fail(A3D_ERR_ARG, "a3d_ray_eval: null or empty input");
+  hipLaunchKernelGGL(k_ray_eval, dim3(grid_for(n)), dim3(kBlk), 0, (hipStream_t)stream, V, F, O, D, n, ray_idx, face,
+                     max_t, t, sign, points, uvw);
+  A3D_LAUNCH_CHECK();
+  return A3D_OK;
+}
+
+extern "C" int a3d_ray_crossings(float* t, int32_t* sign, const int64_t* offsets, int64_t K, float coalesce_t,
+                                 int32_t* out, a3d_stream_t stream) {
+  if (K < 0) return fail(A3D_ERR_ARG, "a3d_ray_crossings: K=%lld < 0", (long long)K);
+  if (K == 0) return A3D_OK;
+  if (!offsets || !out) return fail(A3D_ERR_ARG, "a3d_ray_crossings: null offsets or output");
+  hipLaunchKernelGGL(k_ray_crossings, dim3(grid_for(K)), dim3(kBlk), 0, (hipStream_t)stream, t, sign, offsets, K,
+                     coalesce_t, out);
+  A3D_LAUNCH_CHECK();
+  return A3D_OK;
+}

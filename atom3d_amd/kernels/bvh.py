@@ -1,7 +1,7 @@
 """BVHAccelerator: device BVH build and queries over the HIP library.
 
 Mirrors Atom3D atom3d/kernels/bvh.py:54-207 (same constructor arguments, same
-method names and return tuples).  Differences, all deliberate:
+method names and return tuples; ray_intersect / ray_all_hit :119-181).  Differences, all deliberate:
 
 * the tree is built on the GPU (Morton LBVH) instead of on the host, and is
   held as one opaque byte blob (``self.blob``) instead of ``nodes``/``triangles``
@@ -165,6 +165,97 @@ class BVHAccelerator:
                 _lib.ptr(closest), _lib.ptr(uvw), int(flags), _lib.ptr(counters), _lib.stream(self.device),
             )
         return dist, face, closest, uvw
+
+    # ------------------------------------------------------------ rays
+    def _rays(self, rays_o: torch.Tensor, rays_d: torch.Tensor):
+        o, d = self._points(rays_o), self._points(rays_d)
+        if o.shape != d.shape:
+            raise ValueError(f"rays_o {tuple(o.shape)} and rays_d {tuple(d.shape)} differ")
+        return o, d
+
+    def ray_nearest(self, rays_o: torch.Tensor, rays_d: torch.Tensor, max_t: float = 1e10):
+        """(hit [K] bool, t [K] (max_t when missed), face_ids [K] int32 (-1),
+        hit_points [K, 3] (the origin when missed), uvw [K, 3] (zeros when
+        missed)): the nearest face, minimum t over all faces, ties to the
+        lowest face index."""
+        o, d = self._rays(rays_o, rays_d)
+        K = o.shape[0]
+        dev = self.device
+        hit = torch.empty(K, dtype=torch.uint8, device=dev)
+        t = torch.empty(K, dtype=torch.float32, device=dev)
+        f = torch.empty(K, dtype=torch.int32, device=dev)
+        p = torch.empty(K, 3, dtype=torch.float32, device=dev)
+        w = torch.empty(K, 3, dtype=torch.float32, device=dev)
+        with torch.cuda.device(dev):
+            _lib.call("a3d_ray_nearest", _lib.ptr(self.blob), _lib.ptr(o), _lib.ptr(d), K, float(max_t), _lib.ptr(hit),
+                      _lib.ptr(t), _lib.ptr(f), _lib.ptr(p), _lib.ptr(w), _lib.stream(dev))
+        return hit.view(torch.bool), t, f, p, w
+
+    def ray_intersect(self, rays_o: torch.Tensor, rays_d: torch.Tensor, max_t: float = 1e10):
+        """(hit_mask, hit_t, face_ids, hit_points), as the reference's
+        BVHAccelerator.ray_intersect (kernels/bvh.py:119-147)."""
+        return self.ray_nearest(rays_o, rays_d, max_t)[:4]
+
+    def _all_hits(self, o: torch.Tensor, d: torch.Tensor, max_t: float, with_ts: bool):
+        """Walk-order hit lists: (offsets [K+1] int64, ray_idx, face[, t, sign])."""
+        K = o.shape[0]
+        dev = self.device
+        with torch.cuda.device(dev):
+            s = _lib.stream(dev)
+            counts = torch.empty(K, dtype=torch.int32, device=dev)
+            _lib.call("a3d_ray_all_count", _lib.ptr(self.blob), _lib.ptr(o), _lib.ptr(d), K, float(max_t),
+                      _lib.ptr(counts), s)
+            offsets = torch.zeros(K + 1, dtype=torch.int64, device=dev)
+            torch.cumsum(counts, 0, dtype=torch.int64, out=offsets[1:])
+            H = int(offsets[-1]) if K else 0
+            ray_idx = torch.empty(H, dtype=torch.int32, device=dev)
+            face = torch.empty(H, dtype=torch.int32, device=dev)
+            t = torch.empty(H, dtype=torch.float32, device=dev) if with_ts else None
+            sign = torch.empty(H, dtype=torch.int32, device=dev) if with_ts else None
+            if H:
+                _lib.call("a3d_ray_all_hits", _lib.ptr(self.blob), _lib.ptr(o), _lib.ptr(d), K, float(max_t),
+                          _lib.ptr(offsets), _lib.ptr(ray_idx), _lib.ptr(face), _lib.ptr(t), _lib.ptr(sign), s)
+        return offsets, ray_idx, face, t, sign
+
+    def ray_all_hit(self, rays_o: torch.Tensor, rays_d: torch.Tensor, max_t: float = 1e10):
+        """Every hit of every ray (the reference's filters: 0 < t < max_t, not
+        grazing): (ray_idx, t, face_id, sign, hit_points, uvw), sorted by (ray,
+        face) (the reference lists them in its tree's walk order)."""
+        o, d = self._rays(rays_o, rays_d)
+        dev = self.device
+        _, ray_idx, face, _, _ = self._all_hits(o, d, max_t, with_ts=False)
+        H = ray_idx.shape[0]
+        t = torch.empty(H, dtype=torch.float32, device=dev)
+        sign = torch.empty(H, dtype=torch.int32, device=dev)
+        pts = torch.empty(H, 3, dtype=torch.float32, device=dev)
+        uvw = torch.empty(H, 3, dtype=torch.float32, device=dev)
+        if H:
+            with torch.cuda.device(dev):
+                s = _lib.stream(dev)
+                wsb = ctypes.c_int64()
+                _lib.call("a3d_pairs_sort_ws_bytes", H, ctypes.byref(wsb))
+                ws = torch.empty(wsb.value, dtype=torch.uint8, device=dev)
+                _lib.call("a3d_pairs_sort", _lib.ptr(ray_idx), _lib.ptr(face), H, _lib.ptr(ws), wsb.value, s)
+                _lib.call("a3d_ray_eval", _lib.ptr(self.vertices), self.vertices.shape[0], _lib.ptr(self.faces),
+                          self.num_faces, _lib.ptr(o), _lib.ptr(d), H, _lib.ptr(ray_idx), _lib.ptr(face),
+                          float(max_t), _lib.ptr(t), _lib.ptr(sign), _lib.ptr(pts), _lib.ptr(uvw), s)
+        return ray_idx, t, face, sign, pts, uvw
+
+    def signed_crossings(self, rays_o: torch.Tensor, rays_d: torch.Tensor, max_t: float = 1e6,
+                         coalesce_t: float = 1e-4) -> torch.Tensor:
+        """Signed crossing count per ray (int32 [K]): hits sorted by t,
+        near-coincident hits (closer than coalesce_t) merged, a group of one
+        sign counting once, a mixed group by the parity of its sign sum."""
+        o, d = self._rays(rays_o, rays_d)
+        K = o.shape[0]
+        out = torch.zeros(K, dtype=torch.int32, device=self.device)
+        if K == 0:
+            return out
+        offsets, _, _, t, sign = self._all_hits(o, d, max_t, with_ts=True)
+        with torch.cuda.device(self.device):
+            _lib.call("a3d_ray_crossings", _lib.ptr(t), _lib.ptr(sign), _lib.ptr(offsets), K, float(coalesce_t),
+                      _lib.ptr(out), _lib.stream(self.device))
+        return out
 
     def aabb_hit(self, query_min: torch.Tensor, query_max: torch.Tensor) -> torch.Tensor:
         """Hit mask only (stops at the first intersecting face per box)."""

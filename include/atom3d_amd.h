@@ -133,6 +133,44 @@ int a3d_sdf_sign(const float* P, int64_t K, const float* closest, const int32_t*
                  const float* edge_pn, const int64_t* he_edge, const int32_t* F, float* dist,
                  a3d_stream_t stream);
 
+/* ------------------------------------------------------------------ rays
+ * Replace bvh_ray_intersect / bvh_ray_all_hit (bvh_kernels.cu:469-550,
+ * 935-1022, wrappers :1068-1190, bound at :1247-1248) and the coalescing of
+ * MeshBVH._signed_crossing_count (core/mesh_bvh.py:906-970).  O, D float32
+ * [K*3] ray origins and directions (device), K < 2^31.  Per-face test: the
+ * reference's Moller-Trumbore (bvh_kernels.cu:185-202, FP64 u/v range tests
+ * as there).
+ *
+ * a3d_ray_nearest: hit uint8[K], t float32[K] (max_t when missed), face
+ *   int32[K] (original index, -1), points float32[K*3] (o + d t; o when
+ *   missed) and uvw float32[K*3] (Triangle::barycentric at the point; zeros
+ *   when missed) -- points and uvw nullable.  Nearest = minimum t over all
+ *   faces with t < max_t, ties to the lowest face index.
+ * a3d_ray_all_count: counts int32[K] of the hits the reference's all-hit
+ *   kernel keeps (0 < t < max_t, |dot(unit normal, d)| >= 1e-4).
+ * a3d_ray_all_hits: with offsets int64[K] (exclusive prefix sum of the
+ *   counts) writes each ray's hits from offsets[r] on, in walk order: ray
+ *   index, face, and (nullable) t and sign (+1 / -1, sign of dot(n, d)).
+ *   Sort the (ray, face) pairs with a3d_pairs_sort and call a3d_ray_eval for
+ *   the canonical order.
+ * a3d_ray_eval: t, sign, points, uvw (all nullable) of n (ray, face) pairs,
+ *   from the mesh arrays V float32[nV*3], F int32[nF*3].
+ * a3d_ray_crossings: signed crossing count out int32[K] from ray-major hit
+ *   lists (offsets int64[K+1]); sorts each ray's (t, sign) in place by t.
+ */
+int a3d_ray_nearest(const void* blob, const float* O, const float* D, int64_t K, float max_t,
+                    uint8_t* hit, float* t, int32_t* face, float* points, float* uvw, a3d_stream_t stream);
+int a3d_ray_all_count(const void* blob, const float* O, const float* D, int64_t K, float max_t,
+                      int32_t* counts, a3d_stream_t stream);
+int a3d_ray_all_hits(const void* blob, const float* O, const float* D, int64_t K, float max_t,
+                     const int64_t* offsets, int32_t* ray_idx, int32_t* face, float* t, int32_t* sign,
+                     a3d_stream_t stream);
+int a3d_ray_eval(const float* V, int64_t nV, const int32_t* F, int64_t nF, const float* O, const float* D,
+                 int64_t n, const int32_t* ray_idx, const int32_t* face, float max_t, float* t,
+                 int32_t* sign, float* points, float* uvw, a3d_stream_t stream);
+int a3d_ray_crossings(float* t, int32_t* sign, const int64_t* offsets, int64_t K, float coalesce_t,
+                      int32_t* out, a3d_stream_t stream);
+
 /* ------------------------------------------------------------------ AABB / SAT
  * Replaces bvh_aabb_intersect (bvh_kernels.cu:1192-1242 /
  * bvh_aabb_intersect_kernel :555-610, bound at :1249).  A (box, face) pair

@@ -39,6 +39,12 @@ _SIGS = {
     "orc_brute_aabb_hit": (None, [_f, _i, _i64, _f, _f, _i64, _u8, _vp, _int]),
     "orc_sat_clip": (None, [_f, _f, _f, _l, _l, _i64, _int, ctypes.c_float, _u8, _i, _f, _f, _f]),
     "orc_octree_traverse": (_i64, [_f, _f, _f, _int, _int, _l, _i64, _vp, _vp, _int, _int, _vp]),
+    "orc_ray_tri": (ctypes.c_float, [_f, _f, _f, _f]),
+    "orc_brute_ray": (None, [_f, _i, _i64, _f, _f, _i64, ctypes.c_float, _u8, _f, _i, _f, _f, _int]),
+    "orc_sah_ray": (None, [_f, _f, _f, _f, _i64, ctypes.c_float, _u8, _f, _i, _int, _vp]),
+    "orc_brute_ray_all": (None, [_f, _i, _i64, _f, _f, _i64, ctypes.c_float, _l, _vp, _vp, _vp, _vp, _vp, _vp,
+                                 _vp, _int]),
+    "orc_crossings": (None, [_f, _i, _l, _i64, ctypes.c_float, _i]),
 }
 
 
@@ -112,6 +118,19 @@ class SahBvh:
                           w.reshape(-1), nv.ctypes.data if counts else None, nt.ctypes.data if counts else None,
                           threads)
         return (d, f, c, w, nv, nt) if counts else (d, f, c, w)
+
+    def ray(self, O, D, max_t: float = 1e10, threads: int = 0, work: bool = False):
+        """The reference's nearest-hit walk on this tree: (hit, t, face[,
+        (node fetches, triangle tests) summed over the rays])."""
+        O, D = _f32(O).reshape(-1, 3), _f32(D).reshape(-1, 3)
+        K = O.shape[0]
+        hit = np.zeros(K, np.uint8)
+        t = np.zeros(K, np.float32)
+        f = np.zeros(K, np.int32)
+        w = np.zeros(2, np.int64)
+        lib().orc_sah_ray(self.nodes.reshape(-1), self.tris.reshape(-1), O.reshape(-1), D.reshape(-1), K,
+                          float(max_t), hit, t, f, threads, w.ctypes.data)
+        return (hit.astype(bool), t, f, (int(w[0]), int(w[1]))) if work else (hit.astype(bool), t, f)
 
     def aabb(self, qmin, qmax, canonical: bool = True, cap: int | None = None):
         qmin, qmax = _f32(qmin).reshape(-1, 3), _f32(qmax).reshape(-1, 3)
@@ -202,3 +221,56 @@ def sat_clip(bmin, bmax, tris9, cand_a, cand_t, mode: int = 2, eps: float = 1e-8
     lib().orc_sat_clip(bmin.reshape(-1), bmax.reshape(-1), tris9.reshape(-1), ca, ct, N, mode, eps, hit, cnt,
                        poly.reshape(-1), cen.reshape(-1), area)
     return hit.astype(bool), cnt, poly, cen, area
+
+
+def ray_tri(tri9, ro, rd):
+    """(t, unit normal) of the reference's ray-triangle test; t = 1e10 when
+    there is no hit."""
+    n = np.zeros(3, np.float32)
+    t = lib().orc_ray_tri(_f32(tri9).reshape(9), _f32(ro).reshape(3), _f32(rd).reshape(3), n)
+    return float(np.float32(t)), n
+
+
+def brute_ray(V, F, O, D, max_t: float = 1e10, threads: int = 0):
+    """Tree-independent nearest hit (ties to the lowest face index):
+    (hit, t, face, points, uvw)."""
+    V, F, O, D = _f32(V), _i32(F), _f32(O).reshape(-1, 3), _f32(D).reshape(-1, 3)
+    K = O.shape[0]
+    hit = np.zeros(K, np.uint8)
+    t = np.zeros(K, np.float32)
+    f = np.zeros(K, np.int32)
+    p = np.zeros((K, 3), np.float32)
+    w = np.zeros((K, 3), np.float32)
+    lib().orc_brute_ray(V.reshape(-1), F.reshape(-1), F.shape[0], O.reshape(-1), D.reshape(-1), K, float(max_t),
+                        hit, t, f, p.reshape(-1), w.reshape(-1), threads)
+    return hit.astype(bool), t, f, p, w
+
+
+def brute_ray_all(V, F, O, D, max_t: float = 1e10, threads: int = 0):
+    """Every hit of every ray, per ray in face order: dict of ray_idx, face,
+    t, sign, points, uvw and offsets [K+1]."""
+    V, F, O, D = _f32(V), _i32(F), _f32(O).reshape(-1, 3), _f32(D).reshape(-1, 3)
+    K = O.shape[0]
+    cnt = np.zeros(K, np.int64)
+    args = (V.reshape(-1), F.reshape(-1), F.shape[0], O.reshape(-1), D.reshape(-1), K, float(max_t))
+    lib().orc_brute_ray_all(*args, cnt, None, None, None, None, None, None, None, threads)
+    off = np.zeros(K + 1, np.int64)
+    np.cumsum(cnt, out=off[1:])
+    H = int(off[-1])
+    out = dict(ray_idx=np.zeros(H, np.int32), face=np.zeros(H, np.int32), t=np.zeros(H, np.float32),
+               sign=np.zeros(H, np.int32), points=np.zeros((H, 3), np.float32), uvw=np.zeros((H, 3), np.float32))
+    if H:
+        lib().orc_brute_ray_all(*args, cnt, off.ctypes.data, out["ray_idx"].ctypes.data, out["face"].ctypes.data,
+                                out["t"].ctypes.data, out["sign"].ctypes.data, out["points"].ctypes.data,
+                                out["uvw"].ctypes.data, threads)
+    out["offsets"] = off
+    return out
+
+
+def crossings(t, sign, offsets, coalesce_t: float = 1e-4):
+    """Signed crossing count per ray from ray-major hit lists."""
+    off = np.ascontiguousarray(offsets, np.int64)
+    K = off.shape[0] - 1
+    out = np.zeros(K, np.int32)
+    lib().orc_crossings(_f32(t).reshape(-1), _i32(sign).reshape(-1), off, K, float(coalesce_t), out)
+    return out
