@@ -559,6 +559,270 @@ __global__ void k_centers(const int32_t* __restrict__ quads, int64_t Q, int64_t 
     }
 }
 
+// ------------------------------------------------------------------ QEF mode
+// Eval-mode dual vertices (sparse_diffdmc.py:205-212, 289-389, 450-513,
+// 617-655, 724-747).  Forward only.
+
+// Sort keys of every (cube, edge) slot of all N cubes: (corner0 << 32 |
+// corner1), the value is the slot.  Sorting gives the lexicographic unique
+// edges of torch.unique(dim=0) as runs.
+__global__ void k_cg_keys(const int32_t* __restrict__ ci, int64_t N, uint64_t* __restrict__ key,
+                          int32_t* __restrict__ val) {
+  int64_t t = blockIdx.x * (int64_t)kBlk + threadIdx.x;
+  if (t >= 12 * N) return;
+  const int64_t cube = t / 12;
+  const int e = (int)(t % 12);
+  const uint32_t c0 = (uint32_t)ci[cube * 8 + kE0[e]], c1 = (uint32_t)ci[cube * 8 + kE1[e]];
+  key[t] = ((uint64_t)c0 << 32) | c1;
+  val[t] = (int32_t)t;
+}
+
+// World position of corner k of a cube (sparse_diffdmc.py:151-170).
+template <typename T>
+__device__ __forceinline__ void corner_world(const int32_t* __restrict__ vox, int64_t cube, int k, int64_t c,
+                                             const Fields<T>& f, T* x) {
+  for (int d = 0; d < 3; ++d) {
+    T gp = (T)(vox[cube * 3 + d] + ((k >> d) & 1));
+    x[d] = gp * f.scale - T(1);
+    if (f.deform) x[d] = x[d] + f.deform[c * 3 + d];
+  }
+}
+
+// One thread per sorted slot; the first slot of each run is a unique edge
+// (i0, i1) and adds its direction outer product to both corners' normal
+// matrix and +/- its slope times direction to their right-hand sides
+// (sparse_diffdmc.py:466-502).  acc[m] = xx, xy, xz, yy, yz, zz, bx, by, bz.
+template <typename T>
+__global__ void k_cg_accum(const uint64_t* __restrict__ key, const int32_t* __restrict__ val, int64_t n,
+                           const int32_t* __restrict__ vox, const int32_t* __restrict__ ci, Fields<T> f,
+                           T* __restrict__ acc) {
+  int64_t j = blockIdx.x * (int64_t)kBlk + threadIdx.x;
+  if (j >= n) return;
+  const uint64_t k = key[j];
+  if (j > 0 && key[j - 1] == k) return;
+  const int64_t slot = val[j], cube = slot / 12;
+  const int e = (int)(slot % 12);
+  const int64_t c0 = (int64_t)(k >> 32), c1 = (int64_t)(k & 0xffffffffu);
+  T x0[3], x1[3];
+  corner_world(vox, cube, kE0[e], c0, f, x0);
+  corner_world(vox, cube, kE1[e], c1, f, x1);
+  const T s0 = f.sdf[c0] - f.iso, s1 = f.sdf[c1] - f.iso;
+  T ev[3] = {x1[0] - x0[0], x1[1] - x0[1], x1[2] - x0[2]};
+  T el = norm3(ev);
+  el = el > T(1e-8) ? el : T(1e-8);
+  const T d[3] = {ev[0] / el, ev[1] / el, ev[2] / el};
+  const T gp = (s1 - s0) / el;
+  const T o[6] = {d[0] * d[0], d[0] * d[1], d[0] * d[2], d[1] * d[1], d[1] * d[2], d[2] * d[2]};
+  const T r[3] = {gp * d[0], gp * d[1], gp * d[2]};
+  for (int q = 0; q < 6; ++q) {
+    atomicAdd(&acc[c0 * 9 + q], o[q]);
+    atomicAdd(&acc[c1 * 9 + q], o[q]);
+  }
+  for (int q = 0; q < 3; ++q) {
+    atomicAdd(&acc[c0 * 9 + 6 + q], r[q]);
+    atomicAdd(&acc[c1 * 9 + 6 + q], -r[q]);
+  }
+}
+
+// (DtD + 1e-4 I) g = Dtb per corner, solved in FP64 by the adjugate (the
+// matrix is symmetric positive definite), rounded to T; NaN rows become 0
+// (sparse_diffdmc.py:504-513).
+template <typename T>
+__global__ void k_cg_solve(const T* __restrict__ acc, int64_t M, T* __restrict__ cg) {
+  int64_t m = blockIdx.x * (int64_t)kBlk + threadIdx.x;
+  if (m >= M) return;
+  const T* a = acc + m * 9;
+  const T eps = T(1e-4);
+  const double xx = (double)(a[0] + eps), xy = (double)a[1], xz = (double)a[2];
+  const double yy = (double)(a[3] + eps), yz = (double)a[4], zz = (double)(a[5] + eps);
+  const double bx = (double)a[6], by = (double)a[7], bz = (double)a[8];
+  const double c00 = yy * zz - yz * yz, c01 = xz * yz - xy * zz, c02 = xy * yz - xz * yy;
+  const double c11 = xx * zz - xz * xz, c12 = xy * xz - xx * yz, c22 = xx * yy - xy * xy;
+  const double det = xx * c00 + xy * c01 + xz * c02;
+  T g[3] = {(T)((c00 * bx + c01 * by + c02 * bz) / det), (T)((c01 * bx + c11 * by + c12 * bz) / det),
+            (T)((c02 * bx + c12 * by + c22 * bz) / det)};
+  const bool bad = isnan(g[0]) || isnan(g[1]) || isnan(g[2]);
+  for (int d = 0; d < 3; ++d) cg[m * 3 + d] = bad ? T(0) : g[d];
+}
+
+// One thread per entry: p_alpha as in training mode, and the crossing normal
+// interpolated from the corner gradients with confidence weights |g|, or the
+// edge-direction normal where both corners lack a gradient, normalised
+// (sparse_diffdmc.py:617-655).
+template <typename T>
+__global__ void k_entry_qef(Geom g, const int32_t* __restrict__ totals, Fields<T> f, const T* __restrict__ cg,
+                            T* __restrict__ p_alpha, T* __restrict__ n_alpha) {
+  int64_t e = blockIdx.x * (int64_t)kBlk + threadIdx.x;
+  if (e >= totals[1]) return;
+  EdgeVals<T> v;
+  edge_vals(g, g.ent_slot[e], f, true, v);
+  T ue[3];
+  interp(v.s0 * v.a0, v.s1 * v.a1, v.x0, v.x1, ue);
+  for (int d = 0; d < 3; ++d) p_alpha[e * 3 + d] = ue[d];
+  T t = v.s0 / ((v.s0 - v.s1) + T(1e-8));
+  t = t < T(0) ? T(0) : (t > T(1) ? T(1) : t);
+  const T g0[3] = {cg[v.c0 * 3], cg[v.c0 * 3 + 1], cg[v.c0 * 3 + 2]};
+  const T g1[3] = {cg[v.c1 * 3], cg[v.c1 * 3 + 1], cg[v.c1 * 3 + 2]};
+  const T w0 = norm3(g0) * (T(1) - t), w1 = norm3(g1) * t;
+  const T den = w0 + w1;
+  T gx[3];
+  if (den > T(1e-6)) {
+    const T dc = den > T(1e-8) ? den : T(1e-8);
+    for (int d = 0; d < 3; ++d) gx[d] = (w0 * g0[d] + w1 * g1[d]) / dc;
+  } else {
+    T ev[3] = {v.x1[0] - v.x0[0], v.x1[1] - v.x0[1], v.x1[2] - v.x0[2]};
+    T el = norm3(ev) + T(1e-8);
+    T gm = (v.s1 - v.s0) / el;
+    T sg = gm > T(0) ? T(1) : (gm < T(0) ? T(-1) : T(0));
+    for (int d = 0; d < 3; ++d) gx[d] = (ev[d] / el) * sg;
+    T nl = norm3(gx);
+    nl = nl > T(1e-12) ? nl : T(1e-12);
+    for (int d = 0; d < 3; ++d) gx[d] = gx[d] / nl;
+  }
+  T nl = norm3(gx);
+  nl = nl > T(1e-12) ? nl : T(1e-12);
+  for (int d = 0; d < 3; ++d) n_alpha[e * 3 + d] = gx[d] / nl;
+}
+
+// One thread per dual vertex: the reference's 10 x 3 least-squares system
+// around the centroid c of the vertex's zero crossings p_j, solved by
+// Householder QR in FP64, then the NaN and outside-the-cell fallbacks to c
+// (sparse_diffdmc.py:289-389).  Row placement follows the reference: its
+// per-vertex slot offset comes from a scatter_ over duplicate indices that
+// keeps the last write, so the entry of rank r among c entries lands in
+// row r - (c - 1) counted from row 0 if zero, else from the end (row 10 + r -
+// (c - 1)); rows 7-9 then get sqrt(reg) on the diagonal and the
+// regularisation right-hand side sqrt(reg) (cell centre - c), over whatever
+// constraint landed there.  A rank-deficient system gives the centroid.
+// L_dev and the features follow as in training mode.
+template <typename T>
+__global__ void k_vd_qef(Geom g, const int32_t* __restrict__ totals, Fields<T> f, const T* __restrict__ nrm,
+                         double reg, T* __restrict__ verts, T* __restrict__ L_dev, T* __restrict__ vfeat) {
+  int64_t v = blockIdx.x * (int64_t)kBlk + threadIdx.x;
+  if (v >= totals[0]) return;
+  const int32_t o0 = g.vd_off[v], o1 = g.vd_off[v + 1];
+  const int cnt = o1 - o0;
+  // centroid of the crossings, summed in entry order
+  T cen[3] = {T(0), T(0), T(0)};
+  for (int32_t e = o0; e < o1; ++e) {
+    EdgeVals<T> ev;
+    edge_vals(g, g.ent_slot[e], f, false, ev);
+    T zc[3];
+    interp(ev.s0, ev.s1, ev.x0, ev.x1, zc);
+    for (int d = 0; d < 3; ++d) cen[d] = cen[d] + zc[d];
+  }
+  for (int d = 0; d < 3; ++d) cen[d] = cen[d] / (T)cnt;
+  // cell centre, min, max over the owning cube's 8 corners
+  const int64_t cube = g.surf_cube[g.ent_slot[o0] / 12];
+  T cs[3] = {T(0), T(0), T(0)}, cmin[3], cmax[3];
+  for (int k = 0; k < 8; ++k) {
+    T x[3];
+    corner_world(g.vox, cube, k, (int64_t)g.ci[cube * 8 + k], f, x);
+    for (int d = 0; d < 3; ++d) {
+      cs[d] = cs[d] + x[d];
+      cmin[d] = k ? (x[d] < cmin[d] ? x[d] : cmin[d]) : x[d];
+      cmax[d] = k ? (x[d] > cmax[d] ? x[d] : cmax[d]) : x[d];
+    }
+  }
+  double A[10][3], b[10];
+  for (int i = 0; i < 10; ++i) {
+    b[i] = 0.0;
+    for (int d = 0; d < 3; ++d) A[i][d] = 0.0;
+  }
+  for (int32_t e = o0; e < o1; ++e) {
+    EdgeVals<T> ev;
+    edge_vals(g, g.ent_slot[e], f, false, ev);
+    T zc[3];
+    interp(ev.s0, ev.s1, ev.x0, ev.x1, zc);
+    const T n[3] = {nrm[e * 3], nrm[e * 3 + 1], nrm[e * 3 + 2]};
+    const T rhs = (n[0] * (zc[0] - cen[0]) + n[1] * (zc[1] - cen[1])) + n[2] * (zc[2] - cen[2]);
+    const int local = (e - o0) - (cnt - 1);
+    const int row = local < 0 ? local + 10 : local;
+    if (row < 0) continue;  // a group has at most 7 entries, so rows are 0 and 4..9
+    for (int d = 0; d < 3; ++d) A[row][d] = (double)n[d];
+    b[row] = (double)rhs;
+  }
+  const T sl = (T)sqrt(reg);
+  for (int d = 0; d < 3; ++d) {
+    A[7 + d][d] = (double)sl;
+    b[7 + d] = (double)(((cs[d] / T(8)) - cen[d]) * sl);
+  }
+  // Householder QR of [A | b]
+  bool ok = true;
+  for (int k = 0; k < 3; ++k) {
+    double nn = 0.0;
+    for (int i = k; i < 10; ++i) nn += A[i][k] * A[i][k];
+    const double nr = sqrt(nn);
+    if (!(nr > 0.0)) { ok = false; break; }
+    const double alpha = A[k][k] > 0.0 ? -nr : nr;
+    const double vk = A[k][k] - alpha;
+    const double vv = vk * vk + (nn - A[k][k] * A[k][k]);
+    if (vv > 0.0) {
+      for (int j = k + 1; j < 3; ++j) {
+        double s = vk * A[k][j];
+        for (int i = k + 1; i < 10; ++i) s += A[i][k] * A[i][j];
+        s = 2.0 * s / vv;
+        A[k][j] -= s * vk;
+        for (int i = k + 1; i < 10; ++i) A[i][j] -= s * A[i][k];
+      }
+      double s = vk * b[k];
+      for (int i = k + 1; i < 10; ++i) s += A[i][k] * b[i];
+      s = 2.0 * s / vv;
+      b[k] -= s * vk;
+      for (int i = k + 1; i < 10; ++i) b[i] -= s * A[i][k];
+    }
+    A[k][k] = alpha;
+  }
+  T vd[3] = {cen[0], cen[1], cen[2]};
+  if (ok) {
+    double x[3];
+    for (int k = 2; k >= 0; --k) {
+      double s = b[k];
+      for (int j = k + 1; j < 3; ++j) s -= A[k][j] * x[j];
+      x[k] = s / A[k][k];
+    }
+    T w[3];
+    for (int d = 0; d < 3; ++d) w[d] = (T)x[d] + cen[d];
+    bool bad = isnan(w[0]) || isnan(w[1]) || isnan(w[2]);
+    bool out = false;
+    for (int d = 0; d < 3; ++d) {
+      const T sz = cmax[d] - cmin[d];
+      out = out || (w[d] < cmin[d] - sz) || (w[d] > cmax[d] + sz);
+    }
+    if (!bad && !out)
+      for (int d = 0; d < 3; ++d) vd[d] = w[d];
+  }
+  for (int d = 0; d < 3; ++d) verts[v * 3 + d] = vd[d];
+  if (vfeat) {
+    const int C = f.C;
+    T bs = T(0);
+    for (int c = 0; c < C; ++c) vfeat[v * C + c] = T(0);
+    for (int32_t e = o0; e < o1; ++e) {
+      EdgeVals<T> ev;
+      edge_vals(g, g.ent_slot[e], f, true, ev);
+      const T w0 = ev.s0 * ev.a0, w1 = ev.s1 * ev.a1;
+      for (int c = 0; c < C; ++c)
+        vfeat[v * C + c] += interp1(w0, w1, f.feat[ev.c0 * C + c], f.feat[ev.c1 * C + c]) * ev.beta;
+      bs = bs + ev.beta;
+    }
+    const T D = bs + T(1e-8);
+    for (int c = 0; c < C; ++c) vfeat[v * C + c] = vfeat[v * C + c] / D;
+  }
+  T sum = T(0);
+  for (int32_t e = o0; e < o1; ++e) {
+    EdgeVals<T> ev;
+    edge_vals(g, g.ent_slot[e], f, false, ev);
+    T zc[3];
+    interp(ev.s0, ev.s1, ev.x0, ev.x1, zc);
+    T dd[3] = {zc[0] - vd[0], zc[1] - vd[1], zc[2] - vd[2]};
+    T dist = norm3(dd);
+    L_dev[e] = dist;
+    sum = sum + dist;
+  }
+  const T mean = sum / (T)cnt;
+  for (int32_t e = o0; e < o1; ++e) L_dev[e] = fabs(L_dev[e] - mean);
+}
+
 // ------------------------------------------------------------------ backward
 // Adjoint of the forward above.  Nothing but the topology is kept from the
 // forward pass: the interpolation weights, crossings and dual vertices are
@@ -1016,6 +1280,93 @@ extern "C" int a3d_dmc_forward(void* ws, int64_t N, const int64_t* sizes, const 
   return dmc_forward_t<double>(w, sizes, g, make_fields<double>(sdf, deform, alpha, beta, feat, C, isovalue,
                                                                  weight_scale, res),
                                verts, L_dev, p_alpha, n_alpha, vfeat, s);
+}
+
+namespace {
+template <typename T>
+int dmc_corner_grads_t(const DmcWs& w, int64_t N, int64_t M, const int32_t* vox, const int32_t* ci,
+                       const Fields<T>& f, T* acc, T* cg, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero<T>, dim3(gridn(9 * M)), dim3(kBlk), 0, s, acc, 9 * M);
+  A3D_LAUNCH_CHECK();
+  if (N > 0) {
+    const int64_t n12 = 12 * N;
+    hipLaunchKernelGGL(k_cg_keys, dim3(gridn(n12)), dim3(kBlk), 0, s, ci, N, w.skey0, w.sval0);
+    A3D_LAUNCH_CHECK();
+    size_t tbytes = w.cub_bytes;
+    A3D_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(w.cub, tbytes, w.skey0, w.skey1, w.sval0, w.sval1, (int)n12, 0,
+                                                     64, s));
+    hipLaunchKernelGGL(k_cg_accum<T>, dim3(gridn(n12)), dim3(kBlk), 0, s, w.skey1, w.sval1, n12, vox, ci, f, acc);
+    A3D_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(k_cg_solve<T>, dim3(gridn(M)), dim3(kBlk), 0, s, (const T*)acc, M, cg);
+  A3D_LAUNCH_CHECK();
+  return A3D_OK;
+}
+
+template <typename T>
+int dmc_forward_qef_t(const DmcWs& w, const int64_t* sizes, const Geom& g, const Fields<T>& f, const T* cg,
+                      double reg, void* verts, void* L_dev, void* p_alpha, void* n_alpha, void* vfeat,
+                      hipStream_t s) {
+  const int64_t Vd = sizes[1], E = sizes[2], Q = sizes[4];
+  if (Vd == 0) return A3D_OK;
+  hipLaunchKernelGGL(k_entry_qef<T>, dim3(gridn(E)), dim3(kBlk), 0, s, g, w.totals, f, cg, (T*)p_alpha,
+                     (T*)n_alpha);
+  A3D_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_vd_qef<T>, dim3(gridn(Vd)), dim3(kBlk), 0, s, g, w.totals, f, (const T*)n_alpha, reg,
+                     (T*)verts, (T*)L_dev, (T*)vfeat);
+  A3D_LAUNCH_CHECK();
+  if (Q > 0) {
+    hipLaunchKernelGGL(k_centers<T>, dim3(gridn(Q)), dim3(kBlk), 0, s, w.quads, Q, Vd, (T*)verts, (T*)vfeat, f.C);
+    A3D_LAUNCH_CHECK();
+  }
+  return A3D_OK;
+}
+}  // namespace
+
+extern "C" int a3d_dmc_corner_grads(void* ws, int64_t N, int64_t M, const int32_t* vox, const int32_t* ci,
+                                    const void* sdf, const void* deform, int dtype, double isovalue, int res,
+                                    void* acc, void* cgrad, a3d_stream_t stream) {
+  if (!ws) return fail(A3D_ERR_ARG, "a3d_dmc_corner_grads: null workspace");
+  if (dtype != 0 && dtype != 1) return fail(A3D_ERR_ARG, "a3d_dmc_corner_grads: dtype %d not in {0, 1}", dtype);
+  if (N < 0 || M < 0 || N > 150000000 || M >= ((int64_t)1 << 31))
+    return fail(A3D_ERR_ARG, "a3d_dmc_corner_grads: N=%lld or M=%lld out of range", (long long)N, (long long)M);
+  if (res <= 0) return fail(A3D_ERR_ARG, "a3d_dmc_corner_grads: resolution %d must be positive", res);
+  if (M == 0) return A3D_OK;
+  if (!sdf || !acc || !cgrad || (N > 0 && (!vox || !ci)))
+    return fail(A3D_ERR_ARG, "a3d_dmc_corner_grads: null argument");
+  DmcWs w = carve(ws, N);
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == 0)
+    return dmc_corner_grads_t<float>(w, N, M, vox, ci,
+                                     make_fields<float>(sdf, deform, nullptr, nullptr, nullptr, 0, isovalue, 1.0, res),
+                                     (float*)acc, (float*)cgrad, s);
+  return dmc_corner_grads_t<double>(w, N, M, vox, ci,
+                                    make_fields<double>(sdf, deform, nullptr, nullptr, nullptr, 0, isovalue, 1.0, res),
+                                    (double*)acc, (double*)cgrad, s);
+}
+
+extern "C" int a3d_dmc_forward_qef(void* ws, int64_t N, const int64_t* sizes, const int32_t* vox, const int32_t* ci,
+                                   const void* sdf, const void* deform, const void* alpha, const void* beta,
+                                   const void* feat, int C, int dtype, double isovalue, double weight_scale, int res,
+                                   const void* cgrad, double qef_reg_scale, void* verts, void* L_dev, void* p_alpha,
+                                   void* n_alpha, void* vfeat, a3d_stream_t stream) {
+  if (!ws || !sizes) return fail(A3D_ERR_ARG, "a3d_dmc_forward_qef: null workspace or sizes");
+  if (dtype != 0 && dtype != 1) return fail(A3D_ERR_ARG, "a3d_dmc_forward_qef: dtype %d not in {0, 1}", dtype);
+  if (!(qef_reg_scale >= 0.0)) return fail(A3D_ERR_ARG, "a3d_dmc_forward_qef: qef_reg_scale must be >= 0");
+  if (sizes[1] > 0 && (!vox || !ci || !sdf || !cgrad || !verts || !L_dev || !p_alpha || !n_alpha))
+    return fail(A3D_ERR_ARG, "a3d_dmc_forward_qef: null argument");
+  if ((feat != nullptr) != (vfeat != nullptr) || (feat && C <= 0))
+    return fail(A3D_ERR_ARG, "a3d_dmc_forward_qef: features need both feat and vfeat and C > 0");
+  DmcWs w = carve((void*)ws, N);
+  Geom g{vox, ci, w.surf_cube, w.ent_slot, w.vd_off, w.quads};
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == 0)
+    return dmc_forward_qef_t<float>(w, sizes, g, make_fields<float>(sdf, deform, alpha, beta, feat, C, isovalue,
+                                                                     weight_scale, res),
+                                    (const float*)cgrad, qef_reg_scale, verts, L_dev, p_alpha, n_alpha, vfeat, s);
+  return dmc_forward_qef_t<double>(w, sizes, g, make_fields<double>(sdf, deform, alpha, beta, feat, C, isovalue,
+                                                                     weight_scale, res),
+                                   (const double*)cgrad, qef_reg_scale, verts, L_dev, p_alpha, n_alpha, vfeat, s);
 }
 
 extern "C" int a3d_dmc_backward(const void* ws, int64_t N, int64_t M, const int64_t* sizes, const int32_t* vox,

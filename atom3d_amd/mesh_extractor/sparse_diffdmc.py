@@ -1,7 +1,8 @@
 """SparseDiffDMC: sparse differentiable dual marching cubes on gfx950.
 
 Drop-in for Atom3D atom3d/mesh_extractor/sparse_diffdmc.py:10-232 in
-``weighted_avg`` mode (the training mode).  Topology (surface cubes, case
+``weighted_avg`` mode (the training mode) and ``qef`` mode (the eval-mode
+default).  Topology (surface cubes, case
 ids with the check_table flip, dual-vertex numbering, lexicographic surface
 edges, quads) is computed by HIP kernels and is bit-exact with the
 reference; vertex positions, L_dev, p_alpha, n_alpha and interpolated vertex
@@ -10,11 +11,17 @@ hand-written adjoint (``a3d_dmc_backward``) behind ``torch.autograd.Function``
 that recomputes the interpolation weights and dual vertices instead of
 storing them.
 
+QEF mode (sparse_diffdmc.py:205-212, 289-389, 450-513, 617-655, 724-747)
+runs on HIP kernels too: per-corner least-squares gradients (unless
+``corner_gradients`` is given), confidence-weighted crossing normals, and
+one FP64 Householder solve of the reference's 10x3 regularised system per
+dual vertex, with its NaN and outside-the-cell centroid fallbacks.
+
 Differences from the reference, all loud:
-  * ``inference_mode="qef"`` (the default in eval mode) raises
-    NotImplementedError; pass ``inference_mode="weighted_avg"`` in eval mode
-    to get the eval-mode 2-triangle split (shorter diagonal,
-    sparse_diffdmc.py:802-810) of weighted-average vertices.
+  * QEF mode is forward only: its outputs carry no autograd history (the
+    reference would differentiate through torch.linalg.lstsq).
+  * A rank-deficient QEF system gives the centroid (the reference's CPU
+    lstsq returns the minimum-norm solution there).
   * float16 / bfloat16 inputs are computed in float32 and the outputs cast
     back to the requested dtype.
   * The unused ``tet_table`` buffer is not registered.
@@ -130,6 +137,31 @@ class DMCTopology:
                   _lib.ptr(vf), _lib.stream(dev))
         return verts, L, P, Nn, vf
 
+    def corner_gradients(self, sdf, deform) -> torch.Tensor:
+        """Per-corner SDF gradients [M,3] (sparse_diffdmc.py:450-513)."""
+        dev, T = self.device, self.dtype
+        acc = torch.empty((max(self.M, 1) * 9,), dtype=T, device=dev)
+        cg = torch.empty((self.M, 3), dtype=T, device=dev)
+        _lib.call("a3d_dmc_corner_grads", _lib.ptr(self.ws), self.N, self.M, _lib.ptr(self.vox), _lib.ptr(self.ci),
+                  _lib.ptr(sdf), _lib.ptr(deform), self.code, self.iso, self.res, _lib.ptr(acc), _lib.ptr(cg),
+                  _lib.stream(dev))
+        return cg
+
+    def forward_qef(self, sdf, deform, alpha, beta, feat, cg, reg: float):
+        dev, T = self.device, self.dtype
+        V = self.n_vd + self.n_quad
+        verts = torch.empty((V, 3), dtype=T, device=dev)
+        L = torch.empty((self.n_ent,), dtype=T, device=dev)
+        P = torch.empty((self.n_ent, 3), dtype=T, device=dev)
+        Nn = torch.empty((self.n_ent, 3), dtype=T, device=dev)
+        C = int(feat.shape[1]) if feat is not None else 0
+        vf = torch.empty((V, C), dtype=T, device=dev) if feat is not None else None
+        _lib.call("a3d_dmc_forward_qef", _lib.ptr(self.ws), self.N, self.sizes, _lib.ptr(self.vox),
+                  _lib.ptr(self.ci), _lib.ptr(sdf), _lib.ptr(deform), _lib.ptr(alpha), _lib.ptr(beta), _lib.ptr(feat),
+                  C, self.code, self.iso, self.wscale, self.res, _lib.ptr(cg), float(reg), _lib.ptr(verts),
+                  _lib.ptr(L), _lib.ptr(P), _lib.ptr(Nn), _lib.ptr(vf), _lib.stream(dev))
+        return verts, L, P, Nn, vf
+
     def backward(self, sdf, deform, alpha, beta, feat, vf, gV, gL, gP, gN, gF, need):
         """need = (sdf, deform, alpha, beta, feat) flags; returns their grads."""
         dev, T = self.device, self.dtype
@@ -203,11 +235,7 @@ class SparseDiffDMC(nn.Module):
                 v_features=None, isovalue: float = 0.0, qef_reg_scale: float = 1e-2, weight_scale: float = 0.99,
                 inference_mode: str = "auto", dtype=None, corner_gradients=None) -> SparseDiffDMCOutput:
         mode = ("weighted_avg" if self.training else "qef") if inference_mode == "auto" else inference_mode
-        if mode == "qef":
-            raise NotImplementedError(
-                "SparseDiffDMC: the QEF solver (eval mode with inference_mode='auto' or 'qef') is not part of "
-                "this build; use inference_mode='weighted_avg'")
-        if mode != "weighted_avg":
+        if mode not in ("weighted_avg", "qef"):
             raise ValueError(f"unknown inference_mode {inference_mode!r}")
         if dtype is None:
             dtype = sdf.dtype if sdf.dtype in (torch.float16, torch.bfloat16, torch.float32, torch.float64) \
@@ -228,6 +256,11 @@ class SparseDiffDMC(nn.Module):
             raise ValueError("beta must be [N, 12]")
         if feat_c is not None and (feat_c.dim() != 2 or feat_c.shape[0] != sdf_c.shape[0]):
             raise ValueError("v_features must be [M, C]")
+        cg_c = None
+        if mode == "qef" and corner_gradients is not None:
+            cg_c = corner_gradients.detach().to(dev, cdtype).contiguous()
+            if cg_c.shape != (sdf_c.shape[0], 3):
+                raise ValueError("corner_gradients must be [M, 3]")
 
         topo = DMCTopology(voxel_coords, sdf_c, cube_idx, resolution, isovalue, weight_scale)
         if topo.n_surf == 0:
@@ -235,7 +268,16 @@ class SparseDiffDMC(nn.Module):
             return SparseDiffDMCOutput(vertices=e3, faces=torch.zeros((0, 3), dtype=torch.int64, device=dev),
                                        L_dev=torch.zeros((0,), device=dev, dtype=dtype), p_alpha=e3, n_alpha=e3,
                                        v_features=None)
-        verts, L, P, Nn, vf = _DMCFunction.apply(topo, sdf_c, deform_c, alpha_c, beta_c, feat_c)
+        if mode == "qef":
+            with torch.no_grad():
+                det = [None if t is None else t.detach() for t in (sdf_c, deform_c, alpha_c, beta_c, feat_c)]
+                if cg_c is None:
+                    cg_c = topo.corner_gradients(det[0], det[1])
+                verts, L, P, Nn, vf = topo.forward_qef(*det, cg_c, qef_reg_scale)
+            if vf is None:
+                vf = torch.empty((verts.shape[0], 0), dtype=verts.dtype, device=dev)
+        else:
+            verts, L, P, Nn, vf = _DMCFunction.apply(topo, sdf_c, deform_c, alpha_c, beta_c, feat_c)
         if self.training:
             faces = topo.train_faces()
         else:

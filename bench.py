@@ -304,6 +304,27 @@ def bench_dmc(a):
     bwd_bytes = 12 * V + 4 * M + 48 * n_surf
     achieved = (fwd_bytes + bwd_bytes) / (ms * 1e-3) / 1e9
 
+    # eval mode: the reference's default QEF placement (corner gradients
+    # estimated, one regularised 10x3 least-squares solve per dual vertex)
+    m.eval()
+    det = [t.detach() for t in leaves]
+
+    def qef():
+        return m(vox, det[0], ci, res, deform=det[1])
+
+    for _ in range(max(1, a.warmup)):
+        qef()
+    torch.cuda.synchronize()
+    tq = time.perf_counter()
+    for _ in range(a.steps):
+        oq = qef()
+    torch.cuda.synchronize()
+    qef_ms = (time.perf_counter() - tq) * 1e3 / a.steps
+    qef_out = {"ms": qef_ms, "cubes_per_s": N / (qef_ms * 1e-3), "vertices": int(oq.vertices.shape[0]),
+               "faces": int(oq.faces.shape[0])}
+    del oq
+    m.train()
+
     cpu = None
     if not a.no_cpu:
         from oracle.dmc import dmc_forward
@@ -321,6 +342,10 @@ def bench_dmc(a):
                "sample": f"the same shapes' band at {rs}^3 ({cv.shape[0]} cubes), forward + backward of the "
                          f"reference's weighted-average algorithm restated in torch CPU ops (oracle/dmc.py), "
                          f"float32"}
+        tc = time.perf_counter()
+        dmc_forward(cv.long(), cs, cc.long(), rs, dtype=torch.float32, training=False, mode="qef")
+        qef_out["cpu_port_cubes_per_s"] = cv.shape[0] / (time.perf_counter() - tc)
+        qef_out["cpu_port_sample"] = f"eval-mode QEF forward of the {rs}^3 band ({cv.shape[0]} cubes), torch CPU"
     out = {
         "metric": "SparseDiffDMC cubes/s (forward + backward)",
         "value": N / (ms * 1e-3),
@@ -341,6 +366,7 @@ def bench_dmc(a):
         "forward_ms": fwd_ms,
         "backward_ms": bwd_ms,
         "forward_cubes_per_s": N / (fwd_ms * 1e-3),
+        "qef_eval": qef_out,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "whole step",
                      "bytes_fwd": fwd_bytes, "bytes_bwd": bwd_bytes},

@@ -297,6 +297,18 @@ int a3d_flood_dam_mask(const uint32_t* state, int D, int H, int W, const int64_t
  * a3d_dmc_topology_get: int32 copies of the topology (kind 0 surface cube
  *   ids, 1 case ids, 2 entry -> dual vertex, 3 surface edges [.,2], 4 quads
  *   [.,4]).
+ * QEF mode (the reference's eval-mode default; forward only):
+ * a3d_dmc_corner_grads: replaces SparseDiffDMC._estimate_corner_gradients
+ *   (sparse_diffdmc.py:450-513): per-corner least-squares SDF gradient over
+ *   the unique edges of all N cubes, cgrad [M,3].  acc: caller scratch of
+ *   9*M values; uses the workspace's sort buffers (so it may run before or
+ *   after a3d_dmc_topology, but not concurrently with it).
+ * a3d_dmc_forward_qef: replaces the qef branch of _compute_vd and
+ *   _solve_qef_scatter (sparse_diffdmc.py:617-655, 724-747, 289-389): the
+ *   outputs of a3d_dmc_forward with dual vertices placed by the reference's
+ *   regularised QEF (qef_reg_scale) and n_alpha from cgrad [M,3] (estimated
+ *   or the caller's corner_gradients).  Quad centres are appended as in
+ *   a3d_dmc_forward.  No backward.
  */
 int a3d_dmc_ws_bytes(int64_t N, int64_t* bytes);
 int a3d_dmc_topology(const int32_t* vox, const void* sdf, int dtype, double isovalue, const int32_t* ci, int64_t N,
@@ -315,6 +327,14 @@ int a3d_dmc_backward(const void* ws, int64_t N, int64_t M, const int64_t* sizes,
                      void* g_beta, void* g_feat, a3d_stream_t stream);
 int a3d_dmc_topology_get(const void* ws, int64_t N, const int64_t* sizes, int kind, int32_t* out,
                          a3d_stream_t stream);
+int a3d_dmc_corner_grads(void* ws, int64_t N, int64_t M, const int32_t* vox, const int32_t* ci, const void* sdf,
+                         const void* deform, int dtype, double isovalue, int res, void* acc, void* cgrad,
+                         a3d_stream_t stream);
+int a3d_dmc_forward_qef(void* ws, int64_t N, const int64_t* sizes, const int32_t* vox, const int32_t* ci,
+                        const void* sdf, const void* deform, const void* alpha, const void* beta, const void* feat,
+                        int C, int dtype, double isovalue, double weight_scale, int res, const void* cgrad,
+                        double qef_reg_scale, void* verts, void* L_dev, void* p_alpha, void* n_alpha, void* vfeat,
+                        a3d_stream_t stream);
 
 /* ------------------------------------------------------------------ surrogate
  * Learned UDF surrogate: no counterpart in the reference.  A multiresolution

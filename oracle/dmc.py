@@ -10,6 +10,11 @@ Follows Atom3D atom3d/mesh_extractor/sparse_diffdmc.py:
   dual vertices, weighted average, L_dev   :601-780 (weighted_avg branch), :255-260
   triangulation, training 4-triangle split :782-826
   linear interpolation                     :859-871
+  QEF mode (eval default):
+    corner gradients by least squares      :450-513
+    crossing normals from corner gradients :617-655 (edge-direction fallback :263-287)
+    cell geometry per dual vertex          :724-747
+    Tikhonov QEF + mass-point fallbacks    :289-389
 Summation orders follow a sequential (CPU) evaluation.  Tables come from
 atom3d_amd.mesh_extractor.tables, which tests/test_dmc_tables.py pins to the
 reference's values.
@@ -32,11 +37,96 @@ def _interp(w, x):
     return (x * ww).sum(1) / ww.sum(1)
 
 
+def _normalize(x):
+    """F.normalize(x, dim=-1): x / max(|x|, 1e-12)."""
+    return x / x.norm(dim=-1, keepdim=True).clamp(min=1e-12)
+
+
+def estimate_corner_gradients(world, f, cube_idx, dtype):
+    """Per-corner SDF gradient by least squares over the unique edges of all
+    cubes (sparse_diffdmc.py:450-513), including the reference's rhs sign:
+    an edge adds +grad_proj * dir at its first corner and -grad_proj * dir at
+    its second, so a corner with edges on both sides of it along an axis
+    gets the difference of its forward and backward slopes there."""
+    e = cube_idx.long()[:, _EDGE_FLAT].reshape(-1, 2)
+    ue = torch.unique(e, dim=0)
+    i0, i1 = ue[:, 0], ue[:, 1]
+    p0, p1 = world[i0], world[i1]
+    s0, s1 = f[i0].to(dtype), f[i1].to(dtype)
+    ev = p1 - p0
+    el = ev.norm(dim=-1, keepdim=True).clamp(min=1e-8)
+    d = ev / el
+    gp = (s1 - s0) / el.squeeze(-1)
+    outer = d.unsqueeze(2) * d.unsqueeze(1)
+    fwd = gp.unsqueeze(-1) * d
+    M = world.shape[0]
+    DtD = torch.zeros(M, 3, 3, dtype=dtype)
+    Dtb = torch.zeros(M, 3, dtype=dtype)
+    DtD.index_add_(0, i0, outer)
+    DtD.index_add_(0, i1, outer)
+    Dtb.index_add_(0, i0, fwd)
+    Dtb.index_add_(0, i1, -fwd)
+    DtD = DtD + 1e-4 * torch.eye(3, dtype=dtype).unsqueeze(0)
+    g = torch.linalg.solve(DtD, Dtb.unsqueeze(-1)).squeeze(-1)
+    g[torch.isnan(g).any(-1)] = 0.0
+    return g
+
+
+def qef_rows(rank, count):
+    """Row of the 10-row QEF system that an entry's constraint lands in
+    (sparse_diffdmc.py:338-352).  The reference numbers a dual vertex's
+    entries by subtracting a per-vertex offset written with scatter_ over
+    duplicate indices; sequential scatter_ (its CPU kernel) keeps the last
+    write, so the offset is the position of the vertex's last entry and the
+    local index is rank - (count - 1) <= 0.  Index 0 is row 0; a negative
+    index -k addresses row 10 - k from the end, which for count >= 2 reaches
+    into rows 7-9, where the regularisation then overwrites the diagonal
+    element and the right-hand side (:356-363)."""
+    local = rank - (count - 1)
+    return torch.where(local < 0, local + 10, local)
+
+
+def solve_qef(points, normals, ent_vd, nv, reg, center, cmin, cmax):
+    """Tikhonov-regularised QEF per dual vertex with the reference's NaN and
+    outside-the-cell (one cell of slack) mass-point fallbacks
+    (sparse_diffdmc.py:289-389): the 10x3 least-squares problem with the
+    vertex's constraints n_j . d = n_j . (p_j - c) placed by qef_rows, then
+    sqrt(reg) on the diagonal of rows 7-9 and sqrt(reg) (center - c) as their
+    right-hand side, around the centroid c of the vertex's crossings."""
+    dtype = points.dtype
+    K = len(ent_vd)
+    cnt = torch.zeros(nv, dtype=dtype).index_add_(0, ent_vd, torch.ones(K, dtype=dtype)).clamp_min(1)
+    cen = torch.zeros(nv, 3, dtype=dtype).index_add_(0, ent_vd, points) / cnt.view(-1, 1)
+    rhs = (normals * (points - cen[ent_vd])).sum(-1)
+    A = torch.zeros(nv, 10, 3, dtype=dtype)
+    b = torch.zeros(nv, 10, 1, dtype=dtype)
+    n_in = torch.zeros(nv, dtype=torch.long).index_add_(0, ent_vd, torch.ones(K, dtype=torch.long))
+    first = torch.cumsum(n_in, 0) - n_in  # entries of a dual vertex are consecutive
+    rank = torch.arange(K) - first[ent_vd]
+    row = qef_rows(rank, n_in[ent_vd])
+    A[ent_vd, row] = normals
+    b[ent_vd, row, 0] = rhs
+    sl = reg ** 0.5
+    for k in range(3):
+        A[:, 7 + k, k] = sl
+    b[:, 7:, 0] = (center - cen) * sl
+    vd = torch.linalg.lstsq(A, b).solution.squeeze(-1) + cen
+    bad = torch.isnan(vd).any(-1)
+    vd[bad] = cen[bad]
+    size = cmax - cmin
+    out = ((vd < cmin - size).any(-1)) | ((vd > cmax + size).any(-1))
+    vd[out] = cen[out]
+    return vd
+
+
 def dmc_forward(voxel_coords, sdf, cube_idx, resolution, deform=None, beta=None, alpha=None, gamma=None,
-                isovalue=0.0, weight_scale=0.99, dtype=torch.float64, v_features=None, training=True):
+                isovalue=0.0, weight_scale=0.99, dtype=torch.float64, v_features=None, training=True,
+                mode="weighted_avg", qef_reg_scale=1e-2, corner_gradients=None):
     """dict(vertices, faces, L_dev, p_alpha, n_alpha, v_features); inputs may
-    require grad.  training=False gives the eval-mode 2-triangle split
-    (sparse_diffdmc.py:802-810) of the weighted-average vertices."""
+    require grad (weighted_avg mode).  training=False gives the eval-mode
+    2-triangle split (sparse_diffdmc.py:802-810).  mode="qef" places the dual
+    vertices by the QEF solver with per-corner gradients (estimated when
+    corner_gradients is None) and adds "corner_gradients" to the dict."""
     dev = torch.device("cpu")
     vc = voxel_coords.to(dev).long()
     ci = cube_idx.to(dev).long()
@@ -93,7 +183,20 @@ def dmc_forward(voxel_coords, sdf, cube_idx, resolution, deform=None, beta=None,
     ev = ex[:, 1] - ex[:, 0]
     el = ev.norm(dim=-1, keepdim=True) + 1e-8
     nrm = (ev / el) * torch.sign((es[:, 1] - es[:, 0]) / el)
-    nrm = nrm / nrm.norm(dim=-1, keepdim=True).clamp(min=1e-12)
+    nrm = _normalize(nrm)
+    cg = None
+    if mode == "qef":
+        cg = (estimate_corner_gradients(world, f, ci, dtype) if corner_gradients is None
+              else corner_gradients.to(dev, dtype))
+        s0e, s1e = f[sedge[:, 0]], f[sedge[:, 1]]
+        t = (s0e / (s0e - s1e + 1e-8)).clamp(0, 1).unsqueeze(-1)
+        g0, g1 = cg[sedge[:, 0]], cg[sedge[:, 1]]
+        w0 = g0.norm(dim=-1, keepdim=True) * (1 - t)
+        w1 = g1.norm(dim=-1, keepdim=True) * t
+        den = w0 + w1
+        gx = (w0 * g0 + w1 * g1) / den.clamp(min=1e-8)
+        gx = torch.where((den > 1e-6), gx, nrm)
+        nrm = _normalize(gx)
     # dual vertices grouped by ascending num_vd, cube order, table order
     nvd = torch.as_tensor(num_vd_table)[case]
     tab = torch.as_tensor(dmc_table)
@@ -118,7 +221,14 @@ def dmc_forward(voxel_coords, sdf, cube_idx, resolution, deform=None, beta=None,
     p_alpha = _interp(s_g * a_g, x_g)
     b_g = b[ent_cube, ent_edge].unsqueeze(-1)
     bsum = torch.zeros(nv, 1, dtype=dtype).index_add(0, ent_vd, b_g)
-    vd = torch.zeros(nv, 3, dtype=dtype).index_add(0, ent_vd, p_alpha * b_g) / (bsum + 1e-8)
+    if mode == "qef":
+        cv = world[cs.reshape(-1)].reshape(Ns, 8, 3)
+        vcube = torch.zeros(nv, dtype=torch.long)
+        vcube[ent_vd] = ent_cube
+        vd = solve_qef(zc_g.detach(), n_g.detach(), ent_vd, nv, qef_reg_scale, cv.mean(1)[vcube].detach(),
+                       cv.min(1).values[vcube].detach(), cv.max(1).values[vcube].detach())
+    else:
+        vd = torch.zeros(nv, 3, dtype=dtype).index_add(0, ent_vd, p_alpha * b_g) / (bsum + 1e-8)
     vf = None
     if v_features is not None:
         fe = v_features.to(dev, dtype)[sedge.reshape(-1)].reshape(sedge.shape[0], 2, -1)
@@ -140,7 +250,8 @@ def dmc_forward(voxel_coords, sdf, cube_idx, resolution, deform=None, beta=None,
         q = vd.detach()[quad.reshape(-1)].reshape(-1, 4, 3)
         m = (q[:, 0] - q[:, 2]).norm(dim=1) < (q[:, 1] - q[:, 3]).norm(dim=1)
         faces = torch.where(m[:, None], quad[:, [0, 1, 2, 0, 2, 3]], quad[:, [0, 1, 3, 3, 1, 2]]).reshape(-1, 3)
-        return dict(vertices=vd, faces=faces, L_dev=L_dev, p_alpha=p_alpha, n_alpha=n_g, v_features=vf)
+        return dict(vertices=vd, faces=faces, L_dev=L_dev, p_alpha=p_alpha, n_alpha=n_g, v_features=vf,
+                    corner_gradients=cg)
     centers = vd[quad.reshape(-1)].reshape(-1, 4, 3).mean(1)
     cidx = torch.arange(quad.shape[0]) + nv
     verts = torch.cat([vd, centers])
@@ -148,7 +259,8 @@ def dmc_forward(voxel_coords, sdf, cube_idx, resolution, deform=None, beta=None,
         vf = torch.cat([vf, vf[quad.reshape(-1)].reshape(quad.shape[0], 4, -1).mean(1)])
     pairs = quad[:, [0, 1, 1, 2, 2, 3, 3, 0]].reshape(-1, 4, 2)
     faces = torch.cat([pairs, cidx.reshape(-1, 1, 1).repeat(1, 4, 1)], -1).reshape(-1, 3)
-    return dict(vertices=verts, faces=faces, L_dev=L_dev, p_alpha=p_alpha, n_alpha=n_g, v_features=vf)
+    return dict(vertices=verts, faces=faces, L_dev=L_dev, p_alpha=p_alpha, n_alpha=n_g, v_features=vf,
+                corner_gradients=cg)
 
 
 def grid_band(sdf_fn, resolution: int, band: float, dtype=np.float64):

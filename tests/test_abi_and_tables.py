@@ -102,6 +102,19 @@ def test_new_entry_points_validate_arguments():
     with pytest.raises(RuntimeError, match="null"):
         _lib.call("a3d_dmc_forward", None, 1, sizes, None, None, None, None, None, None, None, 0, 0, 0.0, 0.99, 4,
                   None, None, None, None, None, None)
+    with pytest.raises(RuntimeError, match="dtype"):
+        _lib.call("a3d_dmc_corner_grads", 1, 1, 8, None, None, None, None, 3, 0.0, 4, None, None, None)
+    with pytest.raises(RuntimeError, match="null argument"):
+        _lib.call("a3d_dmc_corner_grads", 1, 1, 8, None, None, None, None, 0, 0.0, 4, None, None, None)
+    with pytest.raises(RuntimeError, match="resolution"):
+        _lib.call("a3d_dmc_corner_grads", 1, 1, 8, None, None, None, None, 0, 0.0, 0, None, None, None)
+    sizes[1] = 1
+    with pytest.raises(RuntimeError, match="null argument"):
+        _lib.call("a3d_dmc_forward_qef", 1, 1, sizes, None, None, None, None, None, None, None, 0, 0, 0.0, 0.99, 4,
+                  None, 0.01, None, None, None, None, None, None)
+    with pytest.raises(RuntimeError, match="qef_reg_scale"):
+        _lib.call("a3d_dmc_forward_qef", 1, 1, sizes, None, None, None, None, None, None, None, 0, 0, 0.0, 0.99, 4,
+                  None, -1.0, None, None, None, None, None, None)
     with pytest.raises(RuntimeError, match="null"):
         _lib.call("a3d_sg_init", None, None, None, None, 0, None)
     with pytest.raises(RuntimeError, match="bad batch"):

@@ -134,3 +134,94 @@ def test_oracle_gradient_matches_central_differences():
             fd = (loss(*ap) - loss(*am)).item() / (2 * h)
             g = a.grad.view(-1)[j].item()
             assert abs(fd - g) <= 1e-5 * max(1.0, abs(g)), (i, j, fd, g)
+
+
+QEF_FIX = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "dmc_qef_fixtures.npz")
+QEF_OUTS = ["vertices", "L_dev", "p_alpha", "n_alpha", "v_features"]
+
+
+def qef_inputs(q, case, variant, dt):
+    """Keyword arguments of one QEF fixture variant (see
+    tests/golden/make_dmc_qef_fixtures.py)."""
+    kw = {}
+    if variant == "deform":
+        kw = dict(deform=torch.tensor(q[f"{case}/deform"], dtype=dt),
+                  v_features=torch.tensor(q[f"{case}/v_features"], dtype=dt))
+    if variant == "given":
+        kw = dict(corner_gradients=torch.tensor(q[f"{case}/corner_gradients"], dtype=dt))
+    return kw
+
+
+@pytest.fixture(scope="module")
+def qfx():
+    return dict(np.load(QEF_FIX))
+
+
+@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("tag", ["f64", "f32"])
+@pytest.mark.parametrize("variant", ["plain", "deform", "given"])
+def test_oracle_qef_matches_reference_vectors(qfx, case, tag, variant):
+    dt = torch.float64 if tag == "f64" else torch.float32
+    key = f"{case}_{tag}_{variant}"
+    o = dmc_forward(torch.tensor(qfx[f"{case}/voxel_coords"]), torch.tensor(qfx[f"{case}/sdf"], dtype=dt),
+                    torch.tensor(qfx[f"{case}/cube_idx"]), int(qfx[f"{case}/res"]), dtype=dt, training=False,
+                    mode="qef", **qef_inputs(qfx, case, variant, dt))
+    assert np.array_equal(o["faces"].numpy(), qfx[f"{key}/out_faces"])
+    atol = 1e-12 if tag == "f64" else 2e-6
+    for k in QEF_OUTS:
+        if f"{key}/out_{k}" not in qfx:
+            assert o[k] is None
+            continue
+        np.testing.assert_allclose(o[k].numpy(), qfx[f"{key}/out_{k}"], rtol=0, atol=atol, err_msg=k)
+
+
+def test_qef_single_crossing_closed_form():
+    """One crossing p with unit normal n: the system is n.d = 0 plus
+    sqrt(reg) d = sqrt(reg) t (t = centre - p), so d = t - (n.t) n / (1 + reg)."""
+    from oracle.dmc import solve_qef
+
+    rng = np.random.default_rng(4)
+    nv, reg = 64, 0.01
+    p = torch.tensor(rng.uniform(-0.1, 0.1, (nv, 3)))
+    n = torch.tensor(rng.standard_normal((nv, 3)))
+    n = n / n.norm(dim=1, keepdim=True)
+    cen = torch.tensor(rng.uniform(-0.05, 0.05, (nv, 3)))
+    big = torch.full((nv, 3), 10.0)
+    vd = solve_qef(p, n, torch.arange(nv), nv, reg, cen, -big, big)
+    t = cen - p
+    want = p + t - (n * t).sum(1, keepdim=True) * n / (1 + reg)
+    np.testing.assert_allclose(vd.numpy(), want.numpy(), rtol=0, atol=1e-14)
+    # outside the cell (one cell of slack) -> the centroid, i.e. p itself
+    vd = solve_qef(p, n, torch.arange(nv), nv, reg, cen, p - 1e-3, p + 1e-3)
+    far = ((want - p).abs() > 2e-3).any(1)
+    assert bool(far.any())
+    np.testing.assert_array_equal(vd[far].numpy(), p[far].numpy())
+
+
+def test_corner_gradients_of_a_linear_field_closed_form():
+    """Axis-aligned edges of a linear field s = g.x all have slope g_a along
+    their direction.  Edges x and z run from the lower to the upper corner,
+    edges y (corners 2->0, 3->1, 7->5, 6->4) from the upper to the lower, and
+    the reference adds +slope*dir at an edge's first corner and -slope*dir at
+    its second, so corner gradient a = g_a * (first-end count - second-end
+    count) / (edges along a + 1e-4): 0 inside, +-g_a / (1 + 1e-4) on faces."""
+    from oracle.dmc import estimate_corner_gradients
+
+    r = 5
+    g = torch.tensor([0.7, -1.3, 0.4], dtype=torch.float64)
+    cubes, s, ci = grid_band(lambda P: P @ g.numpy(), r, 10.0)  # every cube
+    assert len(cubes) == r ** 3
+    pos = np.zeros((len(s), 3))
+    off = np.array([[(k >> 0) & 1, (k >> 1) & 1, (k >> 2) & 1] for k in range(8)])
+    for k in range(8):
+        pos[ci[:, k]] = cubes + off[k]
+    world = torch.tensor(pos * (2.0 / r) - 1.0)
+    cg = estimate_corner_gradients(world, torch.tensor(s), torch.tensor(ci), torch.float64).numpy()
+    for a in range(3):
+        lo, hi = pos[:, a] == 0, pos[:, a] == r
+        n_a = np.where(lo | hi, 1.0, 2.0)
+        first_minus_second = np.where(lo, 1.0, 0.0) - np.where(hi, 1.0, 0.0)
+        if a == 1:
+            first_minus_second = -first_minus_second
+        want = g[a].item() * first_minus_second / (n_a + 1e-4)
+        np.testing.assert_allclose(cg[:, a], want, rtol=0, atol=1e-12)
