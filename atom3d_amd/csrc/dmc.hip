@@ -123,6 +123,9 @@ size_t cub_need(int64_t N) {
   (void)hipcub::DeviceRadixSort::SortPairs(nullptr, t, (uint64_t*)nullptr, (uint64_t*)nullptr, (int32_t*)nullptr,
                                            (int32_t*)nullptr, n12, 0, 64);
   b = t > b ? t : b;
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, t, (uint32_t*)nullptr, (uint32_t*)nullptr, (int32_t*)nullptr,
+                                           (int32_t*)nullptr, n12, 0, 32);
+  b = t > b ? t : b;
   return b;
 }
 
@@ -588,19 +591,52 @@ __device__ __forceinline__ void corner_world(const int32_t* __restrict__ vox, in
   }
 }
 
-// One thread per sorted slot; the first slot of each run is a unique edge
-// (i0, i1) and adds its direction outer product to both corners' normal
-// matrix and +/- its slope times direction to their right-hand sides
-// (sparse_diffdmc.py:466-502).  acc[m] = xx, xy, xz, yy, yz, zz, bx, by, bz.
-template <typename T>
-__global__ void k_cg_accum(const uint64_t* __restrict__ key, const int32_t* __restrict__ val, int64_t n,
-                           const int32_t* __restrict__ vox, const int32_t* __restrict__ ci, Fields<T> f,
-                           T* __restrict__ acc) {
+// Unique edges: flag the first slot of each run of equal sorted keys.
+__global__ void k_cg_heads(const uint64_t* __restrict__ key, int64_t n, int32_t* __restrict__ flag) {
   int64_t j = blockIdx.x * (int64_t)kBlk + threadIdx.x;
-  if (j >= n) return;
-  const uint64_t k = key[j];
-  if (j > 0 && key[j - 1] == k) return;
-  const int64_t slot = val[j], cube = slot / 12;
+  if (j < n) flag[j] = (j == 0 || key[j - 1] != key[j]) ? 1 : 0;
+}
+
+// Compact the unique edges (in sorted = torch.unique(dim=0) order): their
+// key and one slot holding them (for the corner positions); also the sort
+// input for grouping edges by second corner.
+__global__ void k_cg_compact(const uint64_t* __restrict__ key, const int32_t* __restrict__ val,
+                             const int32_t* __restrict__ flag, const int32_t* __restrict__ rank, int64_t n,
+                             uint64_t* __restrict__ ekey, int32_t* __restrict__ eslot, uint32_t* __restrict__ k1,
+                             int32_t* __restrict__ v1) {
+  int64_t j = blockIdx.x * (int64_t)kBlk + threadIdx.x;
+  if (j >= n || !flag[j]) return;
+  const int32_t u = rank[j];
+  ekey[u] = key[j];
+  eslot[u] = val[j];
+  k1[u] = (uint32_t)(key[j] & 0xffffffffu);
+  v1[u] = u;
+}
+
+// Per-corner ranges: [s0, e0) of unique edges whose first corner it is (they
+// are contiguous in sorted order), [s1, e1) of positions in the list sorted
+// stably by second corner.
+__global__ void k_cg_ranges(const uint64_t* __restrict__ ekey, const uint32_t* __restrict__ k1s,
+                            const int32_t* __restrict__ nu, int32_t* __restrict__ s0, int32_t* __restrict__ e0,
+                            int32_t* __restrict__ s1, int32_t* __restrict__ e1) {
+  int64_t u = blockIdx.x * (int64_t)kBlk + threadIdx.x;
+  const int64_t U = *nu;
+  if (u >= U) return;
+  const uint32_t a = (uint32_t)(ekey[u] >> 32);
+  if (u == 0 || (uint32_t)(ekey[u - 1] >> 32) != a) s0[a] = (int32_t)u;
+  if (u == U - 1 || (uint32_t)(ekey[u + 1] >> 32) != a) e0[a] = (int32_t)(u + 1);
+  const uint32_t b = k1s[u];
+  if (u == 0 || k1s[u - 1] != b) s1[b] = (int32_t)u;
+  if (u == U - 1 || k1s[u + 1] != b) e1[b] = (int32_t)(u + 1);
+}
+
+// Unit direction and slope of unique edge u (sparse_diffdmc.py:473-487).
+template <typename T>
+__device__ __forceinline__ void cg_edge(const uint64_t* __restrict__ ekey, const int32_t* __restrict__ eslot,
+                                        int32_t u, const int32_t* __restrict__ vox, const Fields<T>& f, T* d,
+                                        T& gp) {
+  const uint64_t k = ekey[u];
+  const int64_t slot = eslot[u], cube = slot / 12;
   const int e = (int)(slot % 12);
   const int64_t c0 = (int64_t)(k >> 32), c1 = (int64_t)(k & 0xffffffffu);
   T x0[3], x1[3];
@@ -610,28 +646,39 @@ __global__ void k_cg_accum(const uint64_t* __restrict__ key, const int32_t* __re
   T ev[3] = {x1[0] - x0[0], x1[1] - x0[1], x1[2] - x0[2]};
   T el = norm3(ev);
   el = el > T(1e-8) ? el : T(1e-8);
-  const T d[3] = {ev[0] / el, ev[1] / el, ev[2] / el};
-  const T gp = (s1 - s0) / el;
-  const T o[6] = {d[0] * d[0], d[0] * d[1], d[0] * d[2], d[1] * d[1], d[1] * d[2], d[2] * d[2]};
-  const T r[3] = {gp * d[0], gp * d[1], gp * d[2]};
-  for (int q = 0; q < 6; ++q) {
-    atomicAdd(&acc[c0 * 9 + q], o[q]);
-    atomicAdd(&acc[c1 * 9 + q], o[q]);
-  }
-  for (int q = 0; q < 3; ++q) {
-    atomicAdd(&acc[c0 * 9 + 6 + q], r[q]);
-    atomicAdd(&acc[c1 * 9 + 6 + q], -r[q]);
-  }
+  for (int q = 0; q < 3; ++q) d[q] = ev[q] / el;
+  gp = (s1 - s0) / el;
 }
 
-// (DtD + 1e-4 I) g = Dtb per corner, solved in FP64 by the adjugate (the
-// matrix is symmetric positive definite), rounded to T; NaN rows become 0
-// (sparse_diffdmc.py:504-513).
+// One thread per corner: the normal matrix and right-hand side summed over
+// its edges in the reference's CPU order (all edges where it is the first
+// corner, in edge order, then all where it is the second), with +slope*dir
+// on the first-corner side and -slope*dir on the second (:488-502); then
+// (DtD + 1e-4 I) g = Dtb solved in FP64 by the adjugate (the matrix is
+// symmetric positive definite), rounded to T; NaN rows become 0 (:504-513).
 template <typename T>
-__global__ void k_cg_solve(const T* __restrict__ acc, int64_t M, T* __restrict__ cg) {
+__global__ void k_cg_solve(const uint64_t* __restrict__ ekey, const int32_t* __restrict__ eslot,
+                           const int32_t* __restrict__ v1s, const int32_t* __restrict__ s0,
+                           const int32_t* __restrict__ e0, const int32_t* __restrict__ s1,
+                           const int32_t* __restrict__ e1, int64_t M, const int32_t* __restrict__ vox, Fields<T> f,
+                           T* __restrict__ cg) {
   int64_t m = blockIdx.x * (int64_t)kBlk + threadIdx.x;
   if (m >= M) return;
-  const T* a = acc + m * 9;
+  T a[9] = {T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
+  for (int side = 0; side < 2; ++side) {
+    const int32_t lo = side ? s1[m] : s0[m], hi = side ? e1[m] : e0[m];
+    for (int32_t p = lo; p < hi; ++p) {
+      T d[3], gp;
+      cg_edge(ekey, eslot, side ? v1s[p] : p, vox, f, d, gp);
+      a[0] = a[0] + d[0] * d[0];
+      a[1] = a[1] + d[0] * d[1];
+      a[2] = a[2] + d[0] * d[2];
+      a[3] = a[3] + d[1] * d[1];
+      a[4] = a[4] + d[1] * d[2];
+      a[5] = a[5] + d[2] * d[2];
+      for (int q = 0; q < 3; ++q) a[6 + q] = a[6 + q] + (side ? -(gp * d[q]) : gp * d[q]);
+    }
+  }
   const T eps = T(1e-4);
   const double xx = (double)(a[0] + eps), xy = (double)a[1], xz = (double)a[2];
   const double yy = (double)(a[3] + eps), yz = (double)a[4], zz = (double)(a[5] + eps);
@@ -701,15 +748,18 @@ __global__ void k_vd_qef(Geom g, const int32_t* __restrict__ totals, Fields<T> f
   int64_t v = blockIdx.x * (int64_t)kBlk + threadIdx.x;
   if (v >= totals[0]) return;
   const int32_t o0 = g.vd_off[v], o1 = g.vd_off[v + 1];
-  const int cnt = o1 - o0;
-  // centroid of the crossings, summed in entry order
+  const int cnt = o1 - o0;  // 1..7: a table group lists at most 7 edges
+  // the vertex's zero crossings, kept in registers; centroid in entry order
+  T zc[7][3];
   T cen[3] = {T(0), T(0), T(0)};
-  for (int32_t e = o0; e < o1; ++e) {
-    EdgeVals<T> ev;
-    edge_vals(g, g.ent_slot[e], f, false, ev);
-    T zc[3];
-    interp(ev.s0, ev.s1, ev.x0, ev.x1, zc);
-    for (int d = 0; d < 3; ++d) cen[d] = cen[d] + zc[d];
+#pragma unroll
+  for (int q = 0; q < 7; ++q) {
+    if (q < cnt) {
+      EdgeVals<T> ev;
+      edge_vals(g, g.ent_slot[o0 + q], f, false, ev);
+      interp(ev.s0, ev.s1, ev.x0, ev.x1, zc[q]);
+      for (int d = 0; d < 3; ++d) cen[d] = cen[d] + zc[q][d];
+    }
   }
   for (int d = 0; d < 3; ++d) cen[d] = cen[d] / (T)cnt;
   // cell centre, min, max over the owning cube's 8 corners
@@ -729,18 +779,17 @@ __global__ void k_vd_qef(Geom g, const int32_t* __restrict__ totals, Fields<T> f
     b[i] = 0.0;
     for (int d = 0; d < 3; ++d) A[i][d] = 0.0;
   }
-  for (int32_t e = o0; e < o1; ++e) {
-    EdgeVals<T> ev;
-    edge_vals(g, g.ent_slot[e], f, false, ev);
-    T zc[3];
-    interp(ev.s0, ev.s1, ev.x0, ev.x1, zc);
-    const T n[3] = {nrm[e * 3], nrm[e * 3 + 1], nrm[e * 3 + 2]};
-    const T rhs = (n[0] * (zc[0] - cen[0]) + n[1] * (zc[1] - cen[1])) + n[2] * (zc[2] - cen[2]);
-    const int local = (e - o0) - (cnt - 1);
-    const int row = local < 0 ? local + 10 : local;
-    if (row < 0) continue;  // a group has at most 7 entries, so rows are 0 and 4..9
-    for (int d = 0; d < 3; ++d) A[row][d] = (double)n[d];
-    b[row] = (double)rhs;
+#pragma unroll
+  for (int q = 0; q < 7; ++q) {
+    if (q < cnt) {
+      const int64_t e = o0 + q;
+      const T n[3] = {nrm[e * 3], nrm[e * 3 + 1], nrm[e * 3 + 2]};
+      const T rhs = (n[0] * (zc[q][0] - cen[0]) + n[1] * (zc[q][1] - cen[1])) + n[2] * (zc[q][2] - cen[2]);
+      const int local = q - (cnt - 1);
+      const int row = local < 0 ? local + 10 : local;
+      for (int d = 0; d < 3; ++d) A[row][d] = (double)n[d];
+      b[row] = (double)rhs;
+    }
   }
   const T sl = (T)sqrt(reg);
   for (int d = 0; d < 3; ++d) {
@@ -808,19 +857,19 @@ __global__ void k_vd_qef(Geom g, const int32_t* __restrict__ totals, Fields<T> f
     const T D = bs + T(1e-8);
     for (int c = 0; c < C; ++c) vfeat[v * C + c] = vfeat[v * C + c] / D;
   }
-  T sum = T(0);
-  for (int32_t e = o0; e < o1; ++e) {
-    EdgeVals<T> ev;
-    edge_vals(g, g.ent_slot[e], f, false, ev);
-    T zc[3];
-    interp(ev.s0, ev.s1, ev.x0, ev.x1, zc);
-    T dd[3] = {zc[0] - vd[0], zc[1] - vd[1], zc[2] - vd[2]};
-    T dist = norm3(dd);
-    L_dev[e] = dist;
-    sum = sum + dist;
+  T sum = T(0), dist[7];
+#pragma unroll
+  for (int q = 0; q < 7; ++q) {
+    if (q < cnt) {
+      T dd[3] = {zc[q][0] - vd[0], zc[q][1] - vd[1], zc[q][2] - vd[2]};
+      dist[q] = norm3(dd);
+      sum = sum + dist[q];
+    }
   }
   const T mean = sum / (T)cnt;
-  for (int32_t e = o0; e < o1; ++e) L_dev[e] = fabs(L_dev[e] - mean);
+#pragma unroll
+  for (int q = 0; q < 7; ++q)
+    if (q < cnt) L_dev[o0 + q] = fabs(dist[q] - mean);
 }
 
 // ------------------------------------------------------------------ backward
@@ -1285,20 +1334,45 @@ extern "C" int a3d_dmc_forward(void* ws, int64_t N, const int64_t* sizes, const 
 namespace {
 template <typename T>
 int dmc_corner_grads_t(const DmcWs& w, int64_t N, int64_t M, const int32_t* vox, const int32_t* ci,
-                       const Fields<T>& f, T* acc, T* cg, hipStream_t s) {
-  hipLaunchKernelGGL(k_zero<T>, dim3(gridn(9 * M)), dim3(kBlk), 0, s, acc, 9 * M);
+                       const Fields<T>& f, int32_t* ranges, T* cg, hipStream_t s) {
+  int32_t *s0 = ranges, *e0 = ranges + M, *s1 = ranges + 2 * M, *e1 = ranges + 3 * M;
+  hipLaunchKernelGGL(k_zero<int32_t>, dim3(gridn(4 * M)), dim3(kBlk), 0, s, ranges, 4 * M);
   A3D_LAUNCH_CHECK();
+  int32_t U = 0;
   if (N > 0) {
     const int64_t n12 = 12 * N;
+    // unique edges of all cubes, lexicographic (corner0, corner1)
     hipLaunchKernelGGL(k_cg_keys, dim3(gridn(n12)), dim3(kBlk), 0, s, ci, N, w.skey0, w.sval0);
     A3D_LAUNCH_CHECK();
+    int end_bit = 33;
+    while (end_bit < 64 && ((uint64_t)(M - 1) >> (end_bit - 32)) != 0) ++end_bit;
     size_t tbytes = w.cub_bytes;
     A3D_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(w.cub, tbytes, w.skey0, w.skey1, w.sval0, w.sval1, (int)n12, 0,
-                                                     64, s));
-    hipLaunchKernelGGL(k_cg_accum<T>, dim3(gridn(n12)), dim3(kBlk), 0, s, w.skey1, w.sval1, n12, vox, ci, f, acc);
+                                                     end_bit, s));
+    hipLaunchKernelGGL(k_cg_heads, dim3(gridn(n12 + 1)), dim3(kBlk), 0, s, w.skey1, n12, w.uflag);
+    A3D_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_zero<int32_t>, dim3(1), dim3(kBlk), 0, s, w.uflag + n12, (int64_t)1);
+    A3D_LAUNCH_CHECK();
+    tbytes = w.cub_bytes;
+    A3D_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(w.cub, tbytes, w.uflag, w.urank, (int)(n12 + 1), s));
+    uint32_t* k1 = (uint32_t*)w.qflag;
+    uint32_t* k1s = (uint32_t*)w.skey1;
+    hipLaunchKernelGGL(k_cg_compact, dim3(gridn(n12)), dim3(kBlk), 0, s, w.skey1, w.sval1, w.uflag, w.urank, n12,
+                       w.skey0, w.sval0, k1, w.qrank);
+    A3D_LAUNCH_CHECK();
+    A3D_HIP_CHECK(hipMemcpyAsync(&U, w.urank + n12, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    A3D_HIP_CHECK(hipStreamSynchronize(s));
+    // the same edges grouped by second corner, stable (edge order kept)
+    int eb = 1;
+    while (eb < 32 && ((uint64_t)(M - 1) >> eb) != 0) ++eb;
+    tbytes = w.cub_bytes;
+    A3D_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(w.cub, tbytes, k1, k1s, w.qrank, w.sval1, (int)U, 0, eb, s));
+    hipLaunchKernelGGL(k_cg_ranges, dim3(gridn(U)), dim3(kBlk), 0, s, (const uint64_t*)w.skey0, (const uint32_t*)k1s,
+                       w.urank + n12, s0, e0, s1, e1);
     A3D_LAUNCH_CHECK();
   }
-  hipLaunchKernelGGL(k_cg_solve<T>, dim3(gridn(M)), dim3(kBlk), 0, s, (const T*)acc, M, cg);
+  hipLaunchKernelGGL(k_cg_solve<T>, dim3(gridn(M)), dim3(kBlk), 0, s, (const uint64_t*)w.skey0,
+                     (const int32_t*)w.sval0, (const int32_t*)w.sval1, s0, e0, s1, e1, M, vox, f, cg);
   A3D_LAUNCH_CHECK();
   return A3D_OK;
 }
@@ -1325,24 +1399,24 @@ int dmc_forward_qef_t(const DmcWs& w, const int64_t* sizes, const Geom& g, const
 
 extern "C" int a3d_dmc_corner_grads(void* ws, int64_t N, int64_t M, const int32_t* vox, const int32_t* ci,
                                     const void* sdf, const void* deform, int dtype, double isovalue, int res,
-                                    void* acc, void* cgrad, a3d_stream_t stream) {
+                                    void* scratch, void* cgrad, a3d_stream_t stream) {
   if (!ws) return fail(A3D_ERR_ARG, "a3d_dmc_corner_grads: null workspace");
   if (dtype != 0 && dtype != 1) return fail(A3D_ERR_ARG, "a3d_dmc_corner_grads: dtype %d not in {0, 1}", dtype);
   if (N < 0 || M < 0 || N > 150000000 || M >= ((int64_t)1 << 31))
     return fail(A3D_ERR_ARG, "a3d_dmc_corner_grads: N=%lld or M=%lld out of range", (long long)N, (long long)M);
   if (res <= 0) return fail(A3D_ERR_ARG, "a3d_dmc_corner_grads: resolution %d must be positive", res);
   if (M == 0) return A3D_OK;
-  if (!sdf || !acc || !cgrad || (N > 0 && (!vox || !ci)))
+  if (!sdf || !scratch || !cgrad || (N > 0 && (!vox || !ci)))
     return fail(A3D_ERR_ARG, "a3d_dmc_corner_grads: null argument");
   DmcWs w = carve(ws, N);
   hipStream_t s = (hipStream_t)stream;
   if (dtype == 0)
     return dmc_corner_grads_t<float>(w, N, M, vox, ci,
                                      make_fields<float>(sdf, deform, nullptr, nullptr, nullptr, 0, isovalue, 1.0, res),
-                                     (float*)acc, (float*)cgrad, s);
+                                     (int32_t*)scratch, (float*)cgrad, s);
   return dmc_corner_grads_t<double>(w, N, M, vox, ci,
                                     make_fields<double>(sdf, deform, nullptr, nullptr, nullptr, 0, isovalue, 1.0, res),
-                                    (double*)acc, (double*)cgrad, s);
+                                    (int32_t*)scratch, (double*)cgrad, s);
 }
 
 extern "C" int a3d_dmc_forward_qef(void* ws, int64_t N, const int64_t* sizes, const int32_t* vox, const int32_t* ci,

@@ -140,7 +140,7 @@ class DMCTopology:
     def corner_gradients(self, sdf, deform) -> torch.Tensor:
         """Per-corner SDF gradients [M,3] (sparse_diffdmc.py:450-513)."""
         dev, T = self.device, self.dtype
-        acc = torch.empty((max(self.M, 1) * 9,), dtype=T, device=dev)
+        acc = torch.empty((max(self.M, 1) * 4,), dtype=torch.int32, device=dev)
         cg = torch.empty((self.M, 3), dtype=T, device=dev)
         _lib.call("a3d_dmc_corner_grads", _lib.ptr(self.ws), self.N, self.M, _lib.ptr(self.vox), _lib.ptr(self.ci),
                   _lib.ptr(sdf), _lib.ptr(deform), self.code, self.iso, self.res, _lib.ptr(acc), _lib.ptr(cg),

@@ -300,9 +300,10 @@ int a3d_flood_dam_mask(const uint32_t* state, int D, int H, int W, const int64_t
  * QEF mode (the reference's eval-mode default; forward only):
  * a3d_dmc_corner_grads: replaces SparseDiffDMC._estimate_corner_gradients
  *   (sparse_diffdmc.py:450-513): per-corner least-squares SDF gradient over
- *   the unique edges of all N cubes, cgrad [M,3].  acc: caller scratch of
- *   9*M values; uses the workspace's sort buffers (so it may run before or
- *   after a3d_dmc_topology, but not concurrently with it).
+ *   the unique edges of all N cubes, cgrad [M,3], each corner's sums taken
+ *   in the reference's CPU order (no atomics).  scratch: caller memory of
+ *   4*M int32.  Uses the workspace's sort buffers but none of the topology
+ *   that a3d_dmc_topology leaves there; synchronises the stream once.
  * a3d_dmc_forward_qef: replaces the qef branch of _compute_vd and
  *   _solve_qef_scatter (sparse_diffdmc.py:617-655, 724-747, 289-389): the
  *   outputs of a3d_dmc_forward with dual vertices placed by the reference's
@@ -328,7 +329,7 @@ int a3d_dmc_backward(const void* ws, int64_t N, int64_t M, const int64_t* sizes,
 int a3d_dmc_topology_get(const void* ws, int64_t N, const int64_t* sizes, int kind, int32_t* out,
                          a3d_stream_t stream);
 int a3d_dmc_corner_grads(void* ws, int64_t N, int64_t M, const int32_t* vox, const int32_t* ci, const void* sdf,
-                         const void* deform, int dtype, double isovalue, int res, void* acc, void* cgrad,
+                         const void* deform, int dtype, double isovalue, int res, void* scratch, void* cgrad,
                          a3d_stream_t stream);
 int a3d_dmc_forward_qef(void* ws, int64_t N, const int64_t* sizes, const int32_t* vox, const int32_t* ci,
                         const void* sdf, const void* deform, const void* alpha, const void* beta, const void* feat,
