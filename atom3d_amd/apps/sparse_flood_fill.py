@@ -3,7 +3,9 @@ neighbourhood signs, all on the GPU.
 
 Same function, arguments and returned dict as Atom3D
 atom3d/apps/sparse_flood_fill.py:103-166 (with get_dam_neighborhood_signs
-:34-100).  The neighbourhood and its signs are computed on the flood's bit
+:34-100), plus the glue towards DMC: get_dam_wet_faces / _edges / _corners
+(:169-227) and construct_surface_dual_graph (:230-291), whose unique / sort /
+search steps run as HIP kernels (kernels/cells.py).  The neighbourhood and its signs are computed on the flood's bit
 planes (k 26-dilations of the dam plane, then masks) instead of unique/sort/
 searchsorted over coordinate lists; the coordinate lists come out in the same
 lexicographic order.
@@ -20,6 +22,7 @@ import torch
 from ..constants import NEIGHBOR_OFFSETS_26, SIGN_DRY, SIGN_WATER
 from ..core.mesh_bvh import MeshBVH
 from ..grid.octree_indexer import OctreeIndexer
+from ..kernels.cells import cell_neighbors, cell_vertices, unique_cells, wet_mask
 from ..kernels.flood_fill import _out, flood_planes
 
 logger = logging.getLogger(__name__)
@@ -96,4 +99,62 @@ def sparse_flood_fill(bvh: MeshBVH, resolution: int, source: Tuple[int, int, int
     return out
 
 
-__all__ = ["sparse_flood_fill", "get_dam_neighborhood_signs"]
+def _check_cells(t: torch.Tensor, resolution: int, name: str) -> None:
+    if len(t):
+        lo, hi = (int(x) for x in torch.aminmax(t.reshape(-1)))
+        if lo < 0 or hi >= resolution:
+            raise IndexError(f"{name}: coordinates [{lo}, {hi}] outside the {resolution}^3 grid")
+
+
+def _wet(kind: int, dam_ijk: torch.Tensor, collision_ijk: torch.Tensor, resolution: int) -> torch.Tensor:
+    per = (6, 12, 8)[kind]
+    if len(dam_ijk) == 0:
+        return torch.zeros((0, per), dtype=torch.bool, device=dam_ijk.device)
+    _check_cells(collision_ijk, resolution, "collision_ijk")
+    return wet_mask(kind, dam_ijk, collision_ijk, resolution)
+
+
+def get_dam_wet_faces(dam_ijk: torch.Tensor, collision_ijk: torch.Tensor, resolution: int) -> torch.Tensor:
+    """bool [N,6]: dam voxel's 6-neighbour (NEIGHBOR_OFFSETS_6 order, in
+    range) is a collision voxel (sparse_flood_fill.py:169-185)."""
+    return _wet(0, dam_ijk, collision_ijk, resolution)
+
+
+def get_dam_wet_edges(dam_ijk: torch.Tensor, collision_ijk: torch.Tensor, resolution: int) -> torch.Tensor:
+    """bool [N,12]: any of an edge's EDGE_NEIGHBOR_OFFSETS neighbours is a
+    collision voxel (sparse_flood_fill.py:188-206)."""
+    return _wet(1, dam_ijk, collision_ijk, resolution)
+
+
+def get_dam_wet_corners(dam_ijk: torch.Tensor, collision_ijk: torch.Tensor, resolution: int) -> torch.Tensor:
+    """bool [N,8]: any of a corner's CORNER_NEIGHBOR_OFFSETS neighbours is a
+    collision voxel (sparse_flood_fill.py:209-227)."""
+    return _wet(2, dam_ijk, collision_ijk, resolution)
+
+
+def construct_surface_dual_graph(dam_ijk: torch.Tensor, collision_ijk: torch.Tensor,
+                                 dry_adjacent_ijk: Optional[torch.Tensor], octree: OctreeIndexer):
+    """Surface dual graph of the dam, collision and dry-adjacent voxels
+    (sparse_flood_fill.py:230-291).
+
+    Returns (voxelgrid_vertices [V,3] float32 world positions, vertices_ijk
+    [V,3] int64, cube_idx [N,8] int64, adj_idx [N,6] int64 (-1 = none),
+    active_cells_ijk [N,3]): cells and vertices in lexicographic order."""
+    device = dam_ijk.device if len(dam_ijk) > 0 else (collision_ijk.device if len(collision_ijk) > 0
+                                                      else torch.device("cuda"))
+    if dry_adjacent_ijk is None:
+        dry_adjacent_ijk = torch.empty((0, 3), dtype=torch.int32, device=device)
+    R = 2 ** octree.max_level
+    parts = [c for c in (dam_ijk, collision_ijk, dry_adjacent_ijk) if len(c) > 0]
+    if not parts:
+        raise RuntimeError("construct_surface_dual_graph: dam, collision and dry-adjacent voxel lists are all empty")
+    cells = torch.cat(parts, dim=0)
+    active = unique_cells(cells, R).to(cells.dtype)
+    cell = ((octree.bounds[1] - octree.bounds[0]) / float(R)).tolist()
+    cube_idx, _, vijk, world = cell_vertices(active, R, lo=octree.bounds[0].tolist(), cell=cell, want_ijk=True)
+    adj = cell_neighbors(active, R)
+    return world, vijk, cube_idx, adj, active
+
+
+__all__ = ["sparse_flood_fill", "get_dam_neighborhood_signs", "get_dam_wet_faces", "get_dam_wet_edges",
+           "get_dam_wet_corners", "construct_surface_dual_graph"]

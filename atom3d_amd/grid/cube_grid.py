@@ -5,7 +5,9 @@ constructor and attributes (:48-114), coordinate conversions (:158-169),
 ravel/unravel (:171-184), cube indexing, boxes and corners (:190-235) and
 voxel_unique_vertices (:237-260).  Cube coordinates are int64 (``index_dtype``),
 positions float32; boxes are ``ijk.float() * cell + b0`` with two separate
-float32 roundings, exactly like the reference.
+float32 roundings, exactly like the reference.  voxel_unique_vertices runs
+on the GPU (one radix sort of the corner keys, ranks by scan) and needs
+device tensors and cubes inside the grid.
 """
 from __future__ import annotations
 
@@ -106,13 +108,17 @@ class CubeGrid:
         return self.vertex_to_world(self.vertex_ijk_from_indices(vertex_idx))
 
     def voxel_unique_vertices(self, voxels: torch.Tensor):
-        """(unique vertex indices [M], their coordinates [M,3], voxel->vertex [N,8])."""
+        """(unique vertex indices [M] int64, their coordinates [M,3] float32,
+        voxel -> vertex [N,8] int64), vertices in ascending index order."""
+        from ..kernels.cells import cell_vertices
+
         if voxels.dim() == 2 and voxels.shape[1] == 3:
-            cube_idx = self.ijk_to_cube(voxels)
+            ijk = voxels
         elif voxels.dim() == 1:
-            cube_idx = voxels
+            ijk = self.cube_to_ijk(voxels)
         else:
             raise ValueError(f"voxels must be [N,3] or [N], got {tuple(voxels.shape)}")
-        corner = self.cube_corner_vertex_indices(cube_idx)
-        uniq, inv = torch.unique(corner.flatten(), return_inverse=True)
-        return uniq, self.vertex_coords_from_indices(uniq), inv.view(-1, 8)
+        lo = self.bounds[0].tolist()
+        cell = self.cell_size.tolist()
+        inv, keys, _, world = cell_vertices(ijk.to(self.device), self.res, lo=lo, cell=cell, want_keys=True)
+        return keys, world, inv

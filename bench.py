@@ -453,7 +453,8 @@ def bench_octree_flood(a):
         achieved = unit_bytes * crop / (ms * 1e-3) / 1e9
         value, unit = crop / (ms * 1e-3), "crop voxels/s"
         extra = {"crop": [st.D, st.H, st.W], "passes": st.passes, "f_water": f_water,
-                 "counts": {k: int(x.shape[0]) for k, x in zip(names, lists)}}
+                 "counts": {k: int(x.shape[0]) for k, x in zip(names, lists)},
+                 "glue": glue_leg(vox, dict(zip(names, lists)), oc, a)}
         if not a.no_cpu:
             from oracle.flood import sparse_flood_fill_from_voxels
 
@@ -482,6 +483,50 @@ def bench_octree_flood(a):
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def glue_leg(vox, lists, oc, a):
+    """Timings of the formats between the flood fill / octree and the SDF /
+    DMC steps at 1024^3: construct_surface_dual_graph of the flood's dam,
+    collision and dry-adjacent voxels, the wet-corner mask of the dam, and
+    voxel_unique_vertices of the octree's voxels (ms per call, best of
+    steps), with the numpy oracle on the same inputs as a CPU reference."""
+    import torch
+
+    from atom3d_amd.apps import construct_surface_dual_graph, get_dam_wet_corners
+
+    dam, coll, dry = (lists[k].int() for k in ("dam", "collision", "dry_adj"))
+    flat = oc.ijk_to_cube(vox)
+
+    def best(fn):
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(max(1, a.steps)):
+            t0 = time.perf_counter()
+            r = fn()
+            torch.cuda.synchronize()
+            ts.append((time.perf_counter() - t0) * 1e3)
+        return min(ts), r
+
+    dg_ms, g = best(lambda: construct_surface_dual_graph(dam, coll, dry, oc))
+    wc_ms, _ = best(lambda: get_dam_wet_corners(dam, coll, 1024))
+    uv_ms, u = best(lambda: oc.voxel_unique_vertices(flat))
+    out = {"dual_graph_ms": dg_ms, "cells": int(g[4].shape[0]), "dual_vertices": int(g[1].shape[0]),
+           "dual_graph_cells_per_s": int(g[4].shape[0]) / (dg_ms * 1e-3), "wet_corners_ms": wc_ms,
+           "unique_vertices_ms": uv_ms, "unique_vertices_voxels": int(vox.shape[0]),
+           "unique_vertices": int(u[0].shape[0]), "unique_vertices_voxels_per_s": int(vox.shape[0]) / (uv_ms * 1e-3)}
+    if not a.no_cpu:
+        from oracle import cells as ocl
+
+        d, c, y = ((x[(x < 384).all(1)]).cpu().numpy() for x in (dam, coll, dry))
+        t0 = time.perf_counter()
+        r = ocl.construct_surface_dual_graph(d, c, y, 1024, [-1.0] * 3, [1.0] * 3)
+        cpu_s = time.perf_counter() - t0
+        out["dual_graph_cpu_port_cells_per_s"] = len(r[4]) / cpu_s
+        out["cpu_port"] = (f"numpy restatement (oracle/cells.py), single thread, on the cells inside [0,384)^3 "
+                           f"({len(r[4])} cells)")
+    return out
 
 
 def measure_surrogate(mesh, V, P, a, fp64_check: bool):

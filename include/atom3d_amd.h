@@ -337,6 +337,48 @@ int a3d_dmc_forward_qef(void* ws, int64_t N, const int64_t* sizes, const int32_t
                         double qef_reg_scale, void* verts, void* L_dev, void* p_alpha, void* n_alpha, void* vfeat,
                         a3d_stream_t stream);
 
+/* ------------------------------------------------------------------ cells
+ * Voxel-to-dual-graph glue between the flood fill / octree and the SDF /
+ * DMC steps.  Cells are int32 [n,3] (i,j,k) in [0, resolution)^3; an input
+ * outside that range is an A3D_ERR_ARG (the calls that synchronise report it).
+ * a3d_cells_ws_bytes: workspace for up to n cells (n <= 2^27).
+ * a3d_unique_cells: replaces the torch.unique(cat(dam, collision,
+ *   dry_adjacent), dim=0) of construct_surface_dual_graph
+ *   (apps/sparse_flood_fill.py:254-257): sorted (lexicographic) unique cells
+ *   into out_ijk [n,3]; *n_out = their count.  Synchronises the stream.
+ * a3d_cell_vertices: the unique corner vertices of n cells (duplicates
+ *   allowed) in lexicographic order and cube_vert int64 [n,8], each cell
+ *   corner's rank among them (CUBE_CORNERS order): replaces the vertex
+ *   unique + searchsorted of construct_surface_dual_graph (:262-271) and the
+ *   unique/inverse of CubeGrid.voxel_unique_vertices (grid/cube_grid.py:
+ *   237-260).  *n_vert = vertex count; the vertices stay in the workspace.
+ *   Synchronises the stream.
+ * a3d_cell_vertices_get: writes those n_vert vertices (any output may be
+ *   NULL): vkey int64 flat index in the (resolution+1)^3 vertex grid, vijk
+ *   int64 [V,3], world float32 [V,3] = ijk * cell + lo (host float[3] each,
+ *   two float32 roundings, as sparse_flood_fill.py:287-289).
+ * a3d_cell_neighbors: adj int64 [n,6] for sorted unique cells: index of the
+ *   cell at offset +x,-x,+y,-y,+z,-z, else -1, looked up by the unchecked
+ *   flat key as the reference does (:273-283).
+ * a3d_wet_ws_bytes / a3d_wet_mask: replaces get_dam_wet_faces / _edges /
+ *   _corners (:169-227).  kind 0 faces -> out uint8 [n,6], 1 edges -> [n,12]
+ *   (the reference's EDGE_NEIGHBOR_OFFSETS), 2 corners -> [n,8]: whether a
+ *   neighbour (in range) is a collision voxel.  bitmap: caller memory of
+ *   a3d_wet_ws_bytes bytes (resolution^3 bits), overwritten.  Collision
+ *   voxels outside the grid are ignored.
+ */
+int a3d_cells_ws_bytes(int64_t n, int64_t* bytes);
+int a3d_unique_cells(const int32_t* ijk, int64_t n, int resolution, void* ws, int64_t ws_bytes, int32_t* out_ijk,
+                     int64_t* n_out, a3d_stream_t stream);
+int a3d_cell_vertices(const int32_t* ijk, int64_t n, int resolution, void* ws, int64_t ws_bytes, int64_t* cube_vert,
+                      int64_t* n_vert, a3d_stream_t stream);
+int a3d_cell_vertices_get(const void* ws, int64_t n, int resolution, int64_t n_vert, int64_t* vkey, int64_t* vijk,
+                          float* world, const float* lo, const float* cell, a3d_stream_t stream);
+int a3d_cell_neighbors(const int32_t* cells, int64_t n, int resolution, int64_t* adj, a3d_stream_t stream);
+int a3d_wet_ws_bytes(int resolution, int64_t* bytes);
+int a3d_wet_mask(int kind, const int32_t* dam, int64_t n, const int32_t* coll, int64_t c, int resolution,
+                 void* bitmap, int64_t bitmap_bytes, uint8_t* out, a3d_stream_t stream);
+
 /* ------------------------------------------------------------------ surrogate
  * Learned UDF surrogate: no counterpart in the reference.  A multiresolution
  * hash grid (16 levels, 2^19 entries of 2 features per level, resolutions
