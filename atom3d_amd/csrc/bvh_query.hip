@@ -96,6 +96,75 @@ __global__ __launch_bounds__(kBlk) void k_aabb(const void* __restrict__ blob, co
   if (MODE == 1) counts[q] = c;
 }
 
+// ---------------------------------------------------------------- voxelizer
+// Cells of an axis-aligned block of a regular grid, tested against the mesh
+// with the SAT walk (Voxelizer.voxelize_surface / voxelize_with_faces,
+// Atom3D atom3d/apps/voxelizer.py:36-131).  A cell's box is
+// CubeGrid.cube_aabb's: min = float(ijk) * cell + b0 (two roundings), max =
+// min + cell (grid/cube_grid.py:164-203).
+struct VoxGeom {
+  float b0[3];
+  float cell[3];
+  int64_t org[3];
+  int64_t dims[3];
+};
+
+__device__ __forceinline__ void vox_box(const VoxGeom& g, int64_t i, int64_t j, int64_t k, v3& lo, v3& hi) {
+  const float ijk[3] = {(float)i, (float)j, (float)k};
+  float l[3], h[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    float t = ijk[a] * g.cell[a];
+    l[a] = t + g.b0[a];
+    h[a] = l[a] + g.cell[a];
+  }
+  lo = mk(l[0], l[1], l[2]);
+  hi = mk(h[0], h[1], h[2]);
+}
+
+// Candidate c of the block (meshgrid 'ij' order: k fastest) -> hit flag by
+// the first-hit walk.
+__global__ __launch_bounds__(kBlk) void k_vox_test(const void* __restrict__ blob, VoxGeom g, int64_t n,
+                                                    uint8_t* __restrict__ flags) {
+  __shared__ int32_t s_ref[kLds * kBlk];
+  const int tid = threadIdx.x;
+  const int64_t c = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlk + tid;
+  if (c >= n) return;
+  const int64_t i = g.org[0] + c / (g.dims[1] * g.dims[2]);
+  const int64_t j = g.org[1] + (c / g.dims[2]) % g.dims[1];
+  const int64_t k = g.org[2] + c % g.dims[2];
+  v3 lo, hi;
+  vox_box(g, i, j, k, lo, hi);
+  flags[c] = box_walk<0>(nodes_of(blob), tris_of(blob), lo, hi, s_ref, tid, 0, nullptr, nullptr, 0) ? 1 : 0;
+}
+
+__global__ void k_vox_unravel(const int32_t* __restrict__ idx, int64_t n, VoxGeom g, int64_t* __restrict__ ijk) {
+  int64_t t = blockIdx.x * (int64_t)kBlk + threadIdx.x;
+  if (t >= n) return;
+  const int64_t c = idx[t];
+  ijk[t * 3 + 0] = g.org[0] + c / (g.dims[1] * g.dims[2]);
+  ijk[t * 3 + 1] = g.org[1] + (c / g.dims[2]) % g.dims[1];
+  ijk[t * 3 + 2] = g.org[2] + c % g.dims[2];
+}
+
+// Given cells (int64 ijk): MODE 1 SAT pair count, MODE 2 pairs (cell row,
+// face id) written at offsets[q].
+template <int MODE>
+__global__ __launch_bounds__(kBlk) void k_cells_sat(const void* __restrict__ blob, VoxGeom g,
+                                                     const int64_t* __restrict__ ijk, int64_t n,
+                                                     int32_t* __restrict__ counts, const int64_t* __restrict__ offsets,
+                                                     int32_t* __restrict__ pa, int32_t* __restrict__ pf) {
+  __shared__ int32_t s_ref[kLds * kBlk];
+  const int tid = threadIdx.x;
+  const int64_t q = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlk + tid;
+  if (q >= n) return;
+  v3 lo, hi;
+  vox_box(g, ijk[q * 3], ijk[q * 3 + 1], ijk[q * 3 + 2], lo, hi);
+  const int64_t off = MODE == 2 ? offsets[q] : 0;
+  const int32_t c = box_walk<MODE>(nodes_of(blob), tris_of(blob), lo, hi, s_ref, tid, (int32_t)q, pa, pf, off);
+  if (MODE == 1) counts[q] = c;
+}
+
 // ---------------------------------------------------------------- octree
 struct LevelGeom {
   float b0[3];
@@ -405,6 +474,111 @@ extern "C" int a3d_point_dist_backward(const float* P, const float* C, const flo
   if (K == 0) return A3D_OK;
   if (!P || !C || !dist || !grad || !grad_P) return fail(A3D_ERR_ARG, "a3d_point_dist_backward: null argument");
   hipLaunchKernelGGL(k_pdist_bwd, dim3(grid_for(K)), dim3(kBlk), 0, (hipStream_t)stream, P, C, dist, grad, K, grad_P);
+  A3D_LAUNCH_CHECK();
+  return A3D_OK;
+}
+
+// ---------------------------------------------------------------- voxelizer host
+namespace {
+size_t vox_cub_bytes(int64_t n) {
+  size_t t = 0;
+  hipcub::CountingInputIterator<int32_t> it(0);
+  (void)hipcub::DeviceSelect::Flagged(nullptr, t, it, (uint8_t*)nullptr, (int32_t*)nullptr, (int64_t*)nullptr,
+                                      (int)(n > 0 ? n : 1));
+  return t;
+}
+
+int vox_geom(const float* b0, const float* cell, const int64_t* org, const int64_t* dims, VoxGeom& g,
+             const char* who) {
+  if (!b0 || !cell) return fail(A3D_ERR_ARG, "%s: null b0 or cell", who);
+  for (int a = 0; a < 3; ++a) {
+    g.b0[a] = b0[a];
+    g.cell[a] = cell[a];
+    g.org[a] = org ? org[a] : 0;
+    g.dims[a] = dims ? dims[a] : 1;
+    if (g.dims[a] < 0 || g.org[a] < 0) return fail(A3D_ERR_ARG, "%s: negative block origin or size", who);
+  }
+  return A3D_OK;
+}
+}  // namespace
+
+extern "C" int a3d_vox_ws_bytes(int64_t n_cells, int64_t* bytes) {
+  if (n_cells < 0 || n_cells > 0x7fffffff || !bytes)
+    return fail(A3D_ERR_ARG, "a3d_vox_ws_bytes: n_cells=%lld not in [0, 2^31-1]", (long long)n_cells);
+  const int64_t n = n_cells > 0 ? n_cells : 1;
+  *bytes = al256(n) + al256(4 * n) + al256(8) + al256((int64_t)vox_cub_bytes(n));
+  return A3D_OK;
+}
+
+extern "C" int a3d_vox_cells(const void* blob, const float* b0, const float* cell, const int64_t* origin,
+                             const int64_t* dims, void* ws, int64_t ws_bytes, int64_t* n_hit, a3d_stream_t stream) {
+  if (int rc = check_blob(blob, "a3d_vox_cells")) return rc;
+  if (!origin || !dims || !n_hit || !ws) return fail(A3D_ERR_ARG, "a3d_vox_cells: null argument");
+  VoxGeom g;
+  if (int rc = vox_geom(b0, cell, origin, dims, g, "a3d_vox_cells")) return rc;
+  const int64_t n = g.dims[0] * g.dims[1] * g.dims[2];
+  int64_t need = 0;
+  if (int rc = a3d_vox_ws_bytes(n, &need)) return rc;
+  if (ws_bytes < need) return fail(A3D_ERR_CAPACITY, "a3d_vox_cells: workspace %lld < %lld", (long long)ws_bytes,
+                                   (long long)need);
+  *n_hit = 0;
+  if (n == 0) return A3D_OK;
+  hipStream_t s = (hipStream_t)stream;
+  char* w = (char*)ws;
+  uint8_t* flags = (uint8_t*)w;
+  int32_t* idx = (int32_t*)(w + al256(n));
+  int64_t* cnt = (int64_t*)(w + al256(n) + al256(4 * n));
+  void* tmp = w + al256(n) + al256(4 * n) + al256(8);
+  hipLaunchKernelGGL(k_vox_test, dim3(grid_for(n)), dim3(kBlk), 0, s, blob, g, n, flags);
+  A3D_LAUNCH_CHECK();
+  size_t tb = vox_cub_bytes(n);
+  hipcub::CountingInputIterator<int32_t> it(0);
+  A3D_HIP_CHECK(hipcub::DeviceSelect::Flagged(tmp, tb, it, flags, idx, cnt, (int)n, s));
+  A3D_HIP_CHECK(hipMemcpyAsync(n_hit, cnt, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  A3D_HIP_CHECK(hipStreamSynchronize(s));
+  return A3D_OK;
+}
+
+extern "C" int a3d_vox_cells_get(const void* ws, const int64_t* origin, const int64_t* dims, int64_t n_hit,
+                                 int64_t* ijk, a3d_stream_t stream) {
+  if (!ws || !origin || !dims || n_hit < 0 || (n_hit > 0 && !ijk))
+    return fail(A3D_ERR_ARG, "a3d_vox_cells_get: bad arguments");
+  const float z[3] = {0.f, 0.f, 0.f};
+  VoxGeom g;
+  if (int rc = vox_geom(z, z, origin, dims, g, "a3d_vox_cells_get")) return rc;
+  const int64_t n = g.dims[0] * g.dims[1] * g.dims[2];
+  if (n_hit > n) return fail(A3D_ERR_ARG, "a3d_vox_cells_get: n_hit %lld > cells %lld", (long long)n_hit, (long long)n);
+  if (n_hit == 0) return A3D_OK;
+  const int32_t* idx = (const int32_t*)((const char*)ws + al256(n));
+  hipLaunchKernelGGL(k_vox_unravel, dim3(grid_for(n_hit)), dim3(kBlk), 0, (hipStream_t)stream, idx, n_hit, g, ijk);
+  A3D_LAUNCH_CHECK();
+  return A3D_OK;
+}
+
+extern "C" int a3d_cells_sat_count(const void* blob, const int64_t* ijk, int64_t n, const float* b0, const float* cell,
+                                   int32_t* counts, a3d_stream_t stream) {
+  if (int rc = check_blob(blob, "a3d_cells_sat_count")) return rc;
+  if (n < 0 || n > 0x7fffffff || (n > 0 && (!ijk || !counts)))
+    return fail(A3D_ERR_ARG, "a3d_cells_sat_count: bad arguments");
+  VoxGeom g;
+  if (int rc = vox_geom(b0, cell, nullptr, nullptr, g, "a3d_cells_sat_count")) return rc;
+  if (n == 0) return A3D_OK;
+  hipLaunchKernelGGL(k_cells_sat<1>, dim3(grid_for(n)), dim3(kBlk), 0, (hipStream_t)stream, blob, g, ijk, n, counts,
+                     nullptr, nullptr, nullptr);
+  A3D_LAUNCH_CHECK();
+  return A3D_OK;
+}
+
+extern "C" int a3d_cells_sat_pairs(const void* blob, const int64_t* ijk, int64_t n, const float* b0, const float* cell,
+                                   const int64_t* offsets, int32_t* pair_a, int32_t* pair_f, a3d_stream_t stream) {
+  if (int rc = check_blob(blob, "a3d_cells_sat_pairs")) return rc;
+  if (n < 0 || n > 0x7fffffff || (n > 0 && (!ijk || !offsets || !pair_a || !pair_f)))
+    return fail(A3D_ERR_ARG, "a3d_cells_sat_pairs: bad arguments");
+  VoxGeom g;
+  if (int rc = vox_geom(b0, cell, nullptr, nullptr, g, "a3d_cells_sat_pairs")) return rc;
+  if (n == 0) return A3D_OK;
+  hipLaunchKernelGGL(k_cells_sat<2>, dim3(grid_for(n)), dim3(kBlk), 0, (hipStream_t)stream, blob, g, ijk, n, nullptr,
+                     offsets, pair_a, pair_f);
   A3D_LAUNCH_CHECK();
   return A3D_OK;
 }

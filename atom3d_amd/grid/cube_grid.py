@@ -107,6 +107,32 @@ class CubeGrid:
     def vertex_coords_from_indices(self, vertex_idx: torch.Tensor) -> torch.Tensor:
         return self.vertex_to_world(self.vertex_ijk_from_indices(vertex_idx))
 
+    def generate_all_cells(self) -> torch.Tensor:
+        """Every cell's ijk, meshgrid('ij') order (cube_grid.py:525-528)."""
+        x = torch.arange(self.res, device=self.device, dtype=self.index_dtype)
+        xx, yy, zz = torch.meshgrid(x, x, x, indexing="ij")
+        return torch.stack([xx.flatten(), yy.flatten(), zz.flatten()], dim=1)
+
+    def candidate_block(self, aabb_min: torch.Tensor, aabb_max: torch.Tensor):
+        """(origin, dims) of the cell block covering boxes [K,3]: floor / ceil
+        of their grid coordinates, clamped to the grid, min / max over the
+        boxes (cube_grid.py:542-550)."""
+        aabb_min = aabb_min.to(self.device, dtype=self.dtype)
+        aabb_max = aabb_max.to(self.device, dtype=self.dtype)
+        gmin = self.world_to_grid(aabb_min).floor().to(self.index_dtype).clamp(0, self.res - 1)
+        gmax = self.world_to_grid(aabb_max).ceil().to(self.index_dtype).clamp(0, self.res - 1)
+        lo = gmin.min(dim=0)[0].tolist()
+        hi = gmax.max(dim=0)[0].tolist()
+        return lo, [h - l + 1 for l, h in zip(lo, hi)]
+
+    def generate_candidate_cells_from_aabb(self, aabb_min: torch.Tensor, aabb_max: torch.Tensor) -> torch.Tensor:
+        """All cells of the block covering the boxes, meshgrid('ij') order
+        (cube_grid.py:542-557)."""
+        lo, dims = self.candidate_block(aabb_min, aabb_max)
+        ax = [torch.arange(l, l + d, device=self.device, dtype=self.index_dtype) for l, d in zip(lo, dims)]
+        xx, yy, zz = torch.meshgrid(*ax, indexing="ij")
+        return torch.stack([xx.flatten(), yy.flatten(), zz.flatten()], dim=1)
+
     def voxel_unique_vertices(self, voxels: torch.Tensor):
         """(unique vertex indices [M] int64, their coordinates [M,3] float32,
         voxel -> vertex [N,8] int64), vertices in ascending index order."""

@@ -79,6 +79,11 @@ SIGNATURES: dict[str, list] = {
     "a3d_dmc_corner_grads": [_vp, _i64, _i64, _vp, _vp, _vp, _vp, _i32, _f64, _i32, _vp, _vp, _vp],
     "a3d_dmc_forward_qef": [_vp, _i64, _pi64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i32, _f64, _f64, _i32,
                             _vp, _f64, _vp, _vp, _vp, _vp, _vp, _vp],
+    "a3d_vox_ws_bytes": [_i64, _pi64],
+    "a3d_vox_cells": [_vp, _pf32, _pf32, _pi64, _pi64, _vp, _i64, _pi64, _vp],
+    "a3d_vox_cells_get": [_vp, _pi64, _pi64, _i64, _vp, _vp],
+    "a3d_cells_sat_count": [_vp, _vp, _i64, _pf32, _pf32, _vp, _vp],
+    "a3d_cells_sat_pairs": [_vp, _vp, _i64, _pf32, _pf32, _vp, _vp, _vp, _vp],
     "a3d_cells_ws_bytes": [_i64, _pi64],
     "a3d_unique_cells": [_vp, _i64, _i32, _vp, _i64, _vp, _pi64, _vp],
     "a3d_cell_vertices": [_vp, _i64, _i32, _vp, _i64, _vp, _pi64, _vp],

@@ -295,6 +295,58 @@ class BVHAccelerator:
                 _lib.call("a3d_pairs_sort", _lib.ptr(pa), _lib.ptr(pf), total, _lib.ptr(ws), wsb.value, s)
         return hit, pa, pf
 
+    def block_cells(self, b0, cell, origin, dims) -> torch.Tensor:
+        """int64 [K,3]: the cells of the grid block origin + [0, dims) whose
+        box (float(ijk) * cell + b0, + cell) intersects a face by the SAT
+        test, in meshgrid('ij') order (host float[3] b0, cell)."""
+        dev = self.device
+        n = int(dims[0]) * int(dims[1]) * int(dims[2])
+        nb = ctypes.c_int64()
+        _lib.call("a3d_vox_ws_bytes", n, ctypes.byref(nb))
+        f3 = ctypes.c_float * 3
+        i3 = ctypes.c_int64 * 3
+        b0c, cc = f3(*[float(x) for x in b0]), f3(*[float(x) for x in cell])
+        oc, dc = i3(*[int(x) for x in origin]), i3(*[int(x) for x in dims])
+        with torch.cuda.device(dev):
+            ws = torch.empty(max(nb.value, 1), dtype=torch.uint8, device=dev)
+            nh = ctypes.c_int64()
+            s = _lib.stream(dev)
+            _lib.call("a3d_vox_cells", _lib.ptr(self.blob), b0c, cc, oc, dc, _lib.ptr(ws), nb.value,
+                      ctypes.byref(nh), s)
+            out = torch.empty((nh.value, 3), dtype=torch.int64, device=dev)
+            if nh.value:
+                _lib.call("a3d_vox_cells_get", _lib.ptr(ws), oc, dc, nh.value, _lib.ptr(out), s)
+        return out
+
+    def cells_sat_pairs(self, ijk: torch.Tensor, b0, cell):
+        """(counts [K] int32, cell rows [P] int32, face ids [P] int32) of the
+        SAT pairs of cells ijk [K,3] (int64), pairs in (cell, face) order."""
+        dev = self.device
+        ijk = ijk.to(dev, torch.int64).contiguous()
+        K = int(ijk.shape[0])
+        f3 = ctypes.c_float * 3
+        b0c, cc = f3(*[float(x) for x in b0]), f3(*[float(x) for x in cell])
+        with torch.cuda.device(dev):
+            s = _lib.stream(dev)
+            counts = torch.empty(K, dtype=torch.int32, device=dev)
+            if K == 0:
+                e = torch.empty(0, dtype=torch.int32, device=dev)
+                return counts, e, e.clone()
+            _lib.call("a3d_cells_sat_count", _lib.ptr(self.blob), _lib.ptr(ijk), K, b0c, cc, _lib.ptr(counts), s)
+            incl = torch.cumsum(counts, 0, dtype=torch.int64)
+            total = int(incl[-1])
+            offsets = incl - counts
+            pa = torch.empty(total, dtype=torch.int32, device=dev)
+            pf = torch.empty(total, dtype=torch.int32, device=dev)
+            if total > 0:
+                _lib.call("a3d_cells_sat_pairs", _lib.ptr(self.blob), _lib.ptr(ijk), K, b0c, cc, _lib.ptr(offsets),
+                          _lib.ptr(pa), _lib.ptr(pf), s)
+                wsb = ctypes.c_int64()
+                _lib.call("a3d_pairs_sort_ws_bytes", total, ctypes.byref(wsb))
+                ws = torch.empty(wsb.value, dtype=torch.uint8, device=dev)
+                _lib.call("a3d_pairs_sort", _lib.ptr(pa), _lib.ptr(pf), total, _lib.ptr(ws), wsb.value, s)
+        return counts, pa, pf
+
 
 def bvh_available() -> bool:
     """True when the HIP library loads and a GPU is visible."""

@@ -337,6 +337,34 @@ int a3d_dmc_forward_qef(void* ws, int64_t N, const int64_t* sizes, const int32_t
                         double qef_reg_scale, void* verts, void* L_dev, void* p_alpha, void* n_alpha, void* vfeat,
                         a3d_stream_t stream);
 
+/* ------------------------------------------------------------------ voxelizer
+ * Replaces the candidate-cell SAT of Voxelizer.voxelize_surface /
+ * voxelize_with_faces (apps/voxelizer.py:36-131), which materialise every
+ * cell of the faces' bounding block and run MeshBVH.intersect_aabb on all of
+ * them.  A cell (i,j,k) has CubeGrid.cube_aabb's box: min = float(ijk) *
+ * cell + b0, max = min + cell (grid/cube_grid.py:164-203); b0, cell are
+ * host float[3].
+ * a3d_vox_ws_bytes: workspace for a block of n_cells (< 2^31) cells.
+ * a3d_vox_cells: cells of the block origin[3] + [0, dims[3]) (host int64)
+ *   that intersect some face (the SAT test of intersect_aabb mode 0), in
+ *   meshgrid('ij') order; *n_hit = their count, the list stays in the
+ *   workspace.  Synchronises the stream.
+ * a3d_vox_cells_get: writes those cells as int64 ijk [n_hit,3].
+ * a3d_cells_sat_count / a3d_cells_sat_pairs: for given cells (int64 [n,3]),
+ *   the SAT pair count per cell, then the (cell row, face id) pairs at
+ *   offsets[n] (exclusive sums of the counts); a3d_pairs_sort puts them in
+ *   (cell, face) order.
+ */
+int a3d_vox_ws_bytes(int64_t n_cells, int64_t* bytes);
+int a3d_vox_cells(const void* blob, const float* b0, const float* cell, const int64_t* origin, const int64_t* dims,
+                  void* ws, int64_t ws_bytes, int64_t* n_hit, a3d_stream_t stream);
+int a3d_vox_cells_get(const void* ws, const int64_t* origin, const int64_t* dims, int64_t n_hit, int64_t* ijk,
+                      a3d_stream_t stream);
+int a3d_cells_sat_count(const void* blob, const int64_t* ijk, int64_t n, const float* b0, const float* cell,
+                        int32_t* counts, a3d_stream_t stream);
+int a3d_cells_sat_pairs(const void* blob, const int64_t* ijk, int64_t n, const float* b0, const float* cell,
+                        const int64_t* offsets, int32_t* pair_a, int32_t* pair_f, a3d_stream_t stream);
+
 /* ------------------------------------------------------------------ cells
  * Voxel-to-dual-graph glue between the flood fill / octree and the SDF /
  * DMC steps.  Cells are int32 [n,3] (i,j,k) in [0, resolution)^3; an input
