@@ -45,6 +45,8 @@ __device__ __forceinline__ float sgn(float x) { return x >= 0.0f ? 1.0f : -1.0f;
 //   [0, 256)            BvhHeader
 //   nodes               Node64[n_node_cap]   internal nodes, root = node 0
 //   tris                Tri48[n_tris]        triangles in leaf order
+//   boxes               TriBox32[n_tris]     padded triangle boxes, leaf order
+//   obbs                Obb96[n_node_cap]    per node, both children's oriented boxes
 // Node64 keeps BOTH children's boxes so one 64-byte fetch decides both
 // children (4 x 16-byte loads, no dependent second fetch).
 struct BvhHeader {
@@ -61,7 +63,8 @@ struct BvhHeader {
   int32_t leaf_max;
   int32_t depth;        // max internal-node depth (root = 0), for stack sizing
   int64_t boxes_off;    // per-triangle padded boxes (TriBox32), leaf order
-  int32_t pad[44];
+  int64_t obbs_off;     // per-node child oriented boxes (Obb96), 0 = none
+  int32_t pad[42];
 };
 static_assert(sizeof(BvhHeader) == 256, "header is 256 bytes");
 constexpr int32_t kBvhMagic = 0x41334442;  // 'A3DB'
@@ -98,6 +101,41 @@ struct TriBox32 {
   float4 hi;
 };
 static_assert(sizeof(TriBox32) == 32, "triangle box is 32 bytes");
+
+// Oriented boxes of a node's two children, in each child's own frame: n =
+// the normalised sum of its triangles' area vectors, t1 a unit vector
+// orthogonal to n, t2 = obb_t2(n, t1); every triangle of the child lies in
+// {x : lo_a <= a . x <= hi_a} for a = n, t1, t2, each range widened by the
+// largest distance-error pad of those triangles.  The distance from p to that
+// box (the three per-axis gaps combined, shrunk by 2^-18 for the axes being
+// orthonormal only to float precision) bounds the reference-formula distance
+// to each triangle from below, as the padded axis-aligned boxes do, and is
+// much tighter for curved or tilted patches.  A child of more than kObbCap
+// triangles, or whose normals cancel, has axes 0 and infinite ranges (no
+// bound).  Per child 3 float4: (n, lo_n), (t1, lo_1), (hi_n, hi_1, lo_2, hi_2).
+struct Obb96 {
+  float4 c[6];  // left child c[0..2], right child c[3..5]
+};
+static_assert(sizeof(Obb96) == 96, "oriented-box record is 96 bytes");
+constexpr int kObbCap = 256;
+
+__host__ __device__ __forceinline__ v3 obb_t2(v3 n, v3 t1) { return cross3(n, t1); }
+
+// Lower bound on the squared distance from p to a child's oriented box; eq =
+// 4 ulp of |p|_1 covers the FP32 error of each a . p (|a_i| <= 1).
+__device__ __forceinline__ float obb_d2(v3 p, float4 u, float4 v, float4 w, float eq) {
+  const v3 n = mk(u.x, u.y, u.z), t1 = mk(v.x, v.y, v.z), t2 = obb_t2(n, t1);
+  const float pn = (n.x * p.x + n.y * p.y) + n.z * p.z;
+  const float p1 = (t1.x * p.x + t1.y * p.y) + t1.z * p.z;
+  const float p2 = (t2.x * p.x + t2.y * p.y) + t2.z * p.z;
+  const float dn = fmaxf(fmaxf(u.w - pn, pn - w.x) - eq, 0.0f);
+  const float d1 = fmaxf(fmaxf(v.w - p1, p1 - w.y) - eq, 0.0f);
+  const float d2 = fmaxf(fmaxf(w.z - p2, p2 - w.w) - eq, 0.0f);
+  return ((dn * dn + d1 * d1) + d2 * d2) * 0.999996185302734375f;  // x (1 - 2^-18)
+}
+__device__ __forceinline__ float obb_eq(v3 p) {
+  return 4.0f * 5.9604645e-08f * (fabsf(p.x) + fabsf(p.y) + fabsf(p.z));
+}
 
 __device__ __forceinline__ void load_tri(const Tri48* __restrict__ t, int32_t i, v3& a, v3& b, v3& c,
                                          int32_t& id) {
@@ -277,6 +315,10 @@ __device__ __forceinline__ const Tri48* tris_of(const void* blob) {
 }
 __device__ __forceinline__ const TriBox32* boxes_of(const void* blob) {
   return reinterpret_cast<const TriBox32*>((const char*)blob + header(blob)->boxes_off);
+}
+__device__ __forceinline__ const Obb96* obbs_of(const void* blob) {
+  const int64_t o = header(blob)->obbs_off;
+  return o ? reinterpret_cast<const Obb96*>((const char*)blob + o) : nullptr;
 }
 
 // Squared-distance threshold above which a box cannot hold a face the

@@ -144,7 +144,7 @@ Ws carve(void* base, int64_t n) {
 
 int64_t blob_size(int64_t nF) {
   int64_t cap = nF > 1 ? nF - 1 : 1;
-  return 256 + 64 * cap + 48 * nF + 32 * nF;
+  return 256 + 64 * cap + 48 * nF + 32 * nF + 96 * cap;
 }
 
 __device__ __forceinline__ v3 vtx(const float* __restrict__ V, int32_t i) {
@@ -759,6 +759,120 @@ __global__ void k_small_root(const float* __restrict__ V, const int32_t* __restr
   out[0] = nd;
 }
 
+// Oriented box of one child (see Obb96): its triangles are the leaves of the
+// subtree under `ref`, gathered by a DFS over node refs (no more than
+// kObbCap triangles, else no box).  n = the normalised sum of the triangles'
+// area vectors (FP64), t1 = n x (the world axis least aligned with n),
+// normalised; both shrunk by 2^-21 so they stay within unit length after
+// rounding to float; t2 = obb_t2(n, t1) in float exactly as the walks compute
+// it.  The ranges of a . v over the vertices (FP64) are widened by the
+// largest triangle pad and rounded outward to float.
+__device__ void child_obb(const Node64* __restrict__ nodes, const Tri48* __restrict__ tris, int32_t ref,
+                          float4* out) {
+  const float inf = __int_as_float(0x7f800000);
+  out[0] = make_float4(0.0f, 0.0f, 0.0f, -inf);
+  out[1] = make_float4(0.0f, 0.0f, 0.0f, -inf);
+  out[2] = make_float4(inf, inf, -inf, inf);
+  int32_t stk[64];
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  double ax[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300}, pad = 0.0;
+  for (int pass = 0; pass < 2; ++pass) {
+    if (pass == 1) {
+      const double len = sqrt(sx * sx + sy * sy + sz * sz);
+      if (!(len > 0.0)) return;
+      const double k = 1.0 - 4.76837158203125e-07;  // 1 - 2^-21
+      const double n[3] = {sx / len, sy / len, sz / len};
+      const int e = (fabs(n[0]) <= fabs(n[1]) && fabs(n[0]) <= fabs(n[2])) ? 0 : (fabs(n[1]) <= fabs(n[2]) ? 1 : 2);
+      double t[3] = {0.0, 0.0, 0.0};  // n x e_axis
+      t[(e + 1) % 3] = n[(e + 2) % 3];
+      t[(e + 2) % 3] = -n[(e + 1) % 3];
+      const double tl = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+      const v3 nf = mk((float)(n[0] * k), (float)(n[1] * k), (float)(n[2] * k));
+      const v3 tf = mk((float)(t[0] / tl * k), (float)(t[1] / tl * k), (float)(t[2] / tl * k));
+      const v3 uf = obb_t2(nf, tf);
+      const v3 a3[3] = {nf, tf, uf};
+      for (int q = 0; q < 3; ++q) {
+        ax[q][0] = a3[q].x;
+        ax[q][1] = a3[q].y;
+        ax[q][2] = a3[q].z;
+      }
+      out[0] = make_float4(nf.x, nf.y, nf.z, -inf);
+      out[1] = make_float4(tf.x, tf.y, tf.z, -inf);
+    }
+    int sp = 0, total = 0;
+    int32_t cur = ref;
+    while (true) {
+      if (cur >= 0) {
+        if (sp >= 64) return;
+        stk[sp++] = nodes[cur].d.y;
+        cur = nodes[cur].d.x;
+        continue;
+      }
+      const int32_t s0 = leaf_start(cur), cnt = leaf_count(cur);
+      total += cnt;
+      if (total > kObbCap) return;
+      for (int32_t t = s0; t < s0 + cnt; ++t) {
+        v3 a, b, c;
+        int32_t id;
+        load_tri(tris, t, a, b, c, id);
+        if (pass == 0) {
+          const double ux = (double)b.x - a.x, uy = (double)b.y - a.y, uz = (double)b.z - a.z;
+          const double vx = (double)c.x - a.x, vy = (double)c.y - a.y, vz = (double)c.z - a.z;
+          sx += uy * vz - uz * vy;
+          sy += uz * vx - ux * vz;
+          sz += ux * vy - uy * vx;
+        } else {
+          const v3 vv[3] = {a, b, c};
+          for (int q = 0; q < 3; ++q)
+            for (int r = 0; r < 3; ++r) {
+              const double d = ax[r][0] * vv[q].x + ax[r][1] * vv[q].y + ax[r][2] * vv[q].z;
+              lo[r] = d < lo[r] ? d : lo[r];
+              hi[r] = d > hi[r] ? d : hi[r];
+            }
+          const double pd = tri_pad_of(tris, t);
+          pad = pd > pad ? pd : pad;
+        }
+      }
+      if (sp == 0) break;
+      cur = stk[--sp];
+    }
+  }
+  float lf[3], hf[3];
+  for (int r = 0; r < 3; ++r) {
+    const double m = 1e-12 * (fabs(lo[r]) + fabs(hi[r])) + pad;
+    const double l = lo[r] - m, h = hi[r] + m;
+    lf[r] = (float)l;
+    hf[r] = (float)h;
+    if ((double)lf[r] > l) lf[r] = nextafterf(lf[r], -inf);
+    if ((double)hf[r] < h) hf[r] = nextafterf(hf[r], inf);
+  }
+  out[0].w = lf[0];
+  out[1].w = lf[1];
+  out[2] = make_float4(hf[0], hf[1], lf[2], hf[2]);
+}
+
+// One thread per (node, child) of the emitted tree.
+__global__ __launch_bounds__(kBlk) void k_obbs(void* blob) {
+  const BvhHeader* H = header(blob);
+  const int64_t t = blockIdx.x * (int64_t)kBlk + threadIdx.x;
+  if (t >= 2 * (int64_t)H->n_nodes) return;
+  const int64_t node = t >> 1;
+  const int side = (int)(t & 1);
+  const Node64* nodes = nodes_of(blob);
+  Obb96* ob = reinterpret_cast<Obb96*>((char*)blob + H->obbs_off);
+  float4 c[3];
+  if (H->n_tris > 0) {
+    child_obb(nodes, tris_of(blob), side ? nodes[node].d.y : nodes[node].d.x, c);
+  } else {
+    const float inf = __int_as_float(0x7f800000);
+    c[0] = make_float4(0.0f, 0.0f, 0.0f, -inf);
+    c[1] = make_float4(0.0f, 0.0f, 0.0f, -inf);
+    c[2] = make_float4(inf, inf, -inf, inf);
+  }
+  for (int q = 0; q < 3; ++q) ob[node].c[3 * side + q] = c[q];
+}
+
 // depth: the root's height (kept nodes on the longest path down), or null
 // for the single-leaf tree.
 __global__ void k_header(BvhHeader* h, int64_t n, int64_t cap, int leaf_max, const uint32_t* __restrict__ red,
@@ -766,7 +880,7 @@ __global__ void k_header(BvhHeader* h, int64_t n, int64_t cap, int leaf_max, con
                          const int32_t* __restrict__ depth) {
   if (threadIdx.x != 0) return;
   BvhHeader H;
-  for (int k = 0; k < 44; ++k) H.pad[k] = 0;
+  for (int k = 0; k < 42; ++k) H.pad[k] = 0;
   H.magic = kBvhMagic;
   H.n_tris = (int32_t)n;
   H.n_nodes = (n > leaf_max && m > 0) ? remap[m - 1] + kept[m - 1] : 1;
@@ -774,6 +888,7 @@ __global__ void k_header(BvhHeader* h, int64_t n, int64_t cap, int leaf_max, con
   H.nodes_off = 256;
   H.tris_off = 256 + 64 * cap;
   H.boxes_off = H.tris_off + 48 * n;
+  H.obbs_off = H.boxes_off + 32 * n;
   float cm = 0.0f;
   for (int k = 0; k < 3; ++k) {
     H.scene_lo[k] = n > 0 ? ord2f(red[R_VLO + k]) : 0.0f;
@@ -877,6 +992,8 @@ extern "C" int a3d_bvh_build_ex(const float* V, int64_t nV, const int32_t* F, in
     hipLaunchKernelGGL(k_header, dim3(1), dim3(64), 0, s, (BvhHeader*)blob, n, cap, leaf_max, w.red, w.kept,
                        w.remap, (int64_t)0, (const int32_t*)nullptr);
     A3D_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_obbs, dim3(1), dim3(kBlk), 0, s, blob);
+    A3D_LAUNCH_CHECK();
     return A3D_OK;
   }
   unsigned gn = (unsigned)div_up(n, kBlk), gm = (unsigned)div_up(m, kBlk);
@@ -939,6 +1056,8 @@ extern "C" int a3d_bvh_build_ex(const float* V, int64_t nV, const int32_t* F, in
   A3D_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_header, dim3(1), dim3(64), 0, s, (BvhHeader*)blob, n, cap, leaf_max, w.red, w.kept, w.remap, m,
                      depth_src);
+  A3D_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_obbs, dim3((unsigned)div_up(2 * cap, kBlk)), dim3(kBlk), 0, s, blob);
   A3D_LAUNCH_CHECK();
   return A3D_OK;
 }

@@ -138,11 +138,13 @@ __device__ __forceinline__ void q_flush(WaveQ* __restrict__ Q, int n, int qn, co
 // as the result.  stk: this wave's LDS stack.  n_*: work counters (STATS).
 template <bool STATS, bool QUEUE>
 __device__ __forceinline__ void wave_walk(const Node64* __restrict__ nodes, const Tri48* __restrict__ tris,
-                                          const TriBox32* __restrict__ boxes, v3 p, bool valid, float slack,
+                                          const TriBox32* __restrict__ boxes, const Obb96* __restrict__ obbs,
+                                          v3 p, bool valid, float slack,
                                           float4 (*stk)[2], WaveQ* __restrict__ Q, int lane,
                                           float& best, int32_t& best_id, int32_t& best_tri, uint32_t& n_node,
                                           uint32_t& n_leaf_tri, uint32_t& n_test, uint32_t& n_lane) {
   float thr = valid ? prune_thr(best, slack) : -1.0f;  // invalid lanes accept nothing
+  const float eq = obb_eq(p);
   int qn = 0;  // queued exact tests (QUEUE)
   int sp = 0;
   int32_t cur = 0;
@@ -154,6 +156,15 @@ __device__ __forceinline__ void wave_walk(const Node64* __restrict__ nodes, cons
       float dr = box_d2(p, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w);
       bool wl = dl <= thr, wr = dr <= thr;
       uint64_t bl = __ballot(wl), br = __ballot(wr);
+      if (obbs && (bl | br)) {  // tighten by the children's oriented boxes
+        const Obb96* ob = obbs + cur;
+        dl = fmaxf(dl, obb_d2(p, ob->c[0], ob->c[1], ob->c[2], eq));
+        dr = fmaxf(dr, obb_d2(p, ob->c[3], ob->c[4], ob->c[5], eq));
+        wl = dl <= thr;
+        wr = dr <= thr;
+        bl = __ballot(wl);
+        br = __ballot(wr);
+      }
       if (bl && br) {
         int nl = __popcll(__ballot((wl || wr) && dl <= dr));
         int nr = __popcll(bl | br) - nl;
@@ -162,12 +173,13 @@ __device__ __forceinline__ void wave_walk(const Node64* __restrict__ nodes, cons
         int32_t far_ref = left_first ? nd.d.y : nd.d.x;
         if (sp < kStackTotal) {
           if (lane == 0) {
+            // w of the second half: the far child as node * 2 + side, for its oriented box
             if (left_first) {
               stk[sp][0] = make_float4(nd.b.z, nd.b.w, nd.c.x, __int_as_float(far_ref));
-              stk[sp][1] = make_float4(nd.c.y, nd.c.z, nd.c.w, 0.0f);
+              stk[sp][1] = make_float4(nd.c.y, nd.c.z, nd.c.w, __int_as_float(cur * 2 + 1));
             } else {
               stk[sp][0] = make_float4(nd.a.x, nd.a.y, nd.a.z, __int_as_float(far_ref));
-              stk[sp][1] = make_float4(nd.a.w, nd.b.x, nd.b.y, 0.0f);
+              stk[sp][1] = make_float4(nd.a.w, nd.b.x, nd.b.y, __int_as_float(cur * 2));
             }
           }
           ++sp;
@@ -224,6 +236,11 @@ __device__ __forceinline__ void wave_walk(const Node64* __restrict__ nodes, cons
       --sp;
       float4 e0 = stk[sp][0], e1 = stk[sp][1];
       float d = box_d2(p, e0.x, e0.y, e0.z, e1.x, e1.y, e1.z);
+      if (obbs && __ballot(d <= thr)) {
+        const int32_t code = uni(__float_as_int(e1.w));
+        const float4* c = obbs[code >> 1].c + 3 * (code & 1);
+        d = fmaxf(d, obb_d2(p, c[0], c[1], c[2], eq));
+      }
       if (__ballot(d <= thr)) {
         cur = uni(__float_as_int(e0.w));
         found = true;
@@ -258,8 +275,8 @@ __global__ __launch_bounds__(kBlk) void k_udf_wave(const void* __restrict__ blob
                                                     const float* __restrict__ seed,
                                                     float* __restrict__ dist, int32_t* __restrict__ face,
                                                     float* __restrict__ closest, float* __restrict__ uvw,
-                                                    unsigned long long* __restrict__ counters) {
-  __shared__ float4 s_stk[kWaves][kStackTotal][2];  // (lo.xyz, ref bits), (hi.xyz, -)
+                                                    unsigned long long* __restrict__ counters, bool use_obbs) {
+  __shared__ float4 s_stk[kWaves][kStackTotal][2];  // (lo.xyz, ref bits), (hi.xyz, obb code)
   __shared__ WaveQ s_q[QUEUE ? kWaves : 1];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
@@ -280,7 +297,8 @@ __global__ __launch_bounds__(kBlk) void k_udf_wave(const void* __restrict__ blob
   }
   int32_t best_id = 0x7fffffff, best_tri = -1;
   uint32_t n_node = 0, n_leaf_tri = 0, n_test = 0, n_lane = 0;
-  wave_walk<STATS, QUEUE>(nodes, tris, boxes_of(blob), p, valid, slack, s_stk[wv], QUEUE ? &s_q[wv] : nullptr, lane,
+  wave_walk<STATS, QUEUE>(nodes, tris, boxes_of(blob), use_obbs ? obbs_of(blob) : nullptr, p, valid, slack,
+                          s_stk[wv], QUEUE ? &s_q[wv] : nullptr, lane,
                           best, best_id,
                           best_tri, n_node, n_leaf_tri, n_test, n_lane);
   if (STATS) {
@@ -315,8 +333,8 @@ __device__ __forceinline__ void batch_walk_tile(const void* __restrict__ blob, c
   float best = 1e30f;
   int32_t best_id = 0x7fffffff, best_tri = -1;
   uint32_t n0 = 0, n1 = 0, n2 = 0, n3 = 0;
-  wave_walk<false, true>(nodes, tris, boxes_of(blob), p, valid, slack, stk, Q, lane, best, best_id, best_tri, n0, n1,
-                         n2, n3);
+  wave_walk<false, true>(nodes, tris, boxes_of(blob), obbs_of(blob), p, valid, slack, stk, Q, lane, best, best_id,
+                         best_tri, n0, n1, n2, n3);
   if (valid) write_result(q, p, best, best_id, best_tri, tris, dist, face, closest, uvw);
 }
 
@@ -324,7 +342,8 @@ __device__ __forceinline__ void batch_walk_tile(const void* __restrict__ blob, c
 __global__ __launch_bounds__(kBlk) void k_udf_lane(const void* __restrict__ blob, const float* __restrict__ P,
                                                     const uint32_t* __restrict__ perm, int64_t K,
                                                     const float* __restrict__ seed, float* __restrict__ dist, int32_t* __restrict__ face,
-                                                    float* __restrict__ closest, float* __restrict__ uvw) {
+                                                    float* __restrict__ closest, float* __restrict__ uvw,
+                                                    bool use_obbs) {
   __shared__ int32_t s_ref[kLaneLds * kBlk];
   __shared__ float s_d2[kLaneLds * kBlk];
   const int tid = threadIdx.x;
@@ -336,6 +355,8 @@ __global__ __launch_bounds__(kBlk) void k_udf_lane(const void* __restrict__ blob
   const Tri48* __restrict__ tris = tris_of(blob);
   const v3 p = mk(P[q * 3], P[q * 3 + 1], P[q * 3 + 2]);
   const float slack = query_slack(p, H->coord_max);
+  const Obb96* __restrict__ obbs = use_obbs ? obbs_of(blob) : nullptr;
+  const float eq = obb_eq(p);
 
   float best = 1e30f;
   if (seed) {
@@ -356,6 +377,13 @@ __global__ __launch_bounds__(kBlk) void k_udf_lane(const void* __restrict__ blob
       float dl = box_d2(p, a.x, a.y, a.z, a.w, b.x, b.y);
       float dr = box_d2(p, b.z, b.w, c.x, c.y, c.z, c.w);
       bool tl = dl <= thr, tr = dr <= thr;
+      if (obbs && (tl || tr)) {
+        const Obb96* ob = obbs + cur;
+        dl = fmaxf(dl, obb_d2(p, ob->c[0], ob->c[1], ob->c[2], eq));
+        dr = fmaxf(dr, obb_d2(p, ob->c[3], ob->c[4], ob->c[5], eq));
+        tl = dl <= thr;
+        tr = dr <= thr;
+      }
       if (tl && tr) {
         int32_t near_ref = d.x, far_ref = d.y;
         float far_d = dr;
@@ -559,7 +587,9 @@ int udf_launch(const void* blob, const float* P, int64_t K, const uint32_t* perm
   if (!blob) return fail(A3D_ERR_ARG, "a3d_bvh_udf: null BVH blob");
   if (K < 0 || K > (int64_t(1) << 40)) return fail(A3D_ERR_ARG, "a3d_bvh_udf: K=%lld out of range", (long long)K);
   if (perm && K > 0xffffffffll) return fail(A3D_ERR_ARG, "a3d_bvh_udf: perm needs K < 2^32");
-  if (flags & ~(A3D_UDF_LANE | A3D_UDF_DIRECT)) return fail(A3D_ERR_ARG, "a3d_bvh_udf: unknown flags 0x%x", flags);
+  if (flags & ~(A3D_UDF_LANE | A3D_UDF_DIRECT | A3D_UDF_NO_OBBS))
+    return fail(A3D_ERR_ARG, "a3d_bvh_udf: unknown flags 0x%x", flags);
+  const bool use_obbs = !(flags & A3D_UDF_NO_OBBS);
   if (counters && (flags & A3D_UDF_LANE)) return fail(A3D_ERR_ARG, "a3d_bvh_udf: counters need the wave walk");
   if (K == 0) return A3D_OK;
   if (!P || !dist || !face || !closest) return fail(A3D_ERR_ARG, "a3d_bvh_udf: null input/output");
@@ -567,21 +597,21 @@ int udf_launch(const void* blob, const float* P, int64_t K, const uint32_t* perm
   unsigned long long* c = (unsigned long long*)counters;
   if (flags & A3D_UDF_LANE)
     hipLaunchKernelGGL(k_udf_lane, dim3(grid_for(K)), dim3(kBlk), 0, s, blob, P, perm, K, seed, dist, face, closest,
-                       uvw);
+                       uvw, use_obbs);
   else if (flags & A3D_UDF_DIRECT) {
     if (counters)
       hipLaunchKernelGGL((k_udf_wave<true, false>), dim3(grid_for(K)), dim3(kBlk), 0, s, blob, P, perm, K, seed, dist,
-                         face, closest, uvw, c);
+                         face, closest, uvw, c, use_obbs);
     else
       hipLaunchKernelGGL((k_udf_wave<false, false>), dim3(grid_for(K)), dim3(kBlk), 0, s, blob, P, perm, K, seed, dist,
-                         face, closest, uvw, c);
+                         face, closest, uvw, c, use_obbs);
   } else {
     if (counters)
       hipLaunchKernelGGL((k_udf_wave<true, true>), dim3(grid_for(K)), dim3(kBlk), 0, s, blob, P, perm, K, seed, dist,
-                         face, closest, uvw, c);
+                         face, closest, uvw, c, use_obbs);
     else
       hipLaunchKernelGGL((k_udf_wave<false, true>), dim3(grid_for(K)), dim3(kBlk), 0, s, blob, P, perm, K, seed, dist,
-                         face, closest, uvw, c);
+                         face, closest, uvw, c, use_obbs);
   }
   A3D_LAUNCH_CHECK();
   return A3D_OK;

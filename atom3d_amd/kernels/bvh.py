@@ -106,7 +106,48 @@ class BVHAccelerator:
         assert np.all(seen == 1), "every node is referenced exactly once (root by the walk)"
         if n_tris:
             assert np.all(covered[:n_tris] == 1), "leaves do not partition the triangles"
+            self._check_obbs(raw, refs, tv, n_nodes)
         return info
+
+    @staticmethod
+    def _check_obbs(raw, refs, tv, n_nodes):
+        """Every child's oriented box holds its subtree's vertices on each of
+        its three axes (n, t1, t2 = n x t1 in float32): lo <= a . v <= hi in
+        FP64, |a| <= 1, axes orthogonal to float precision; children above
+        256 triangles have none."""
+        import numpy as np
+
+        off = int(raw[80:88].view(np.int64)[0])
+        assert off > 0, "blob has no oriented boxes"
+        ob = raw[off:off + 96 * n_nodes].view(np.float32).reshape(n_nodes, 2, 12)
+        memo = {}
+
+        def tris_of(ref):
+            if ref < 0:
+                enc = ~ref
+                return np.arange(enc >> 3, (enc >> 3) + (enc & 7) + 1)
+            if ref not in memo:
+                memo[ref] = np.concatenate([tris_of(int(refs[ref, 0])), tris_of(int(refs[ref, 1]))])
+            return memo[ref]
+
+        for node in range(n_nodes - 1, -1, -1):  # children have larger indices: memo fills bottom-up
+            for side in range(2):
+                c = ob[node, side]
+                n, t1 = c[0:3], c[4:7]
+                if not np.any(n):
+                    continue
+                t2 = np.array([n[1] * t1[2] - n[2] * t1[1], n[2] * t1[0] - n[0] * t1[2],
+                               n[0] * t1[1] - n[1] * t1[0]], np.float32)
+                axes = np.stack([n, t1, t2]).astype(np.float64)
+                assert (np.linalg.norm(axes, axis=1) <= 1.0).all(), "axis longer than 1"
+                g = axes @ axes.T
+                assert np.abs(g - np.diag(np.diag(g))).max() < 1e-6, "axes not orthogonal"
+                lo = np.array([c[3], c[7], c[10]], np.float64)
+                hi = np.array([c[8], c[9], c[11]], np.float64)
+                idx = tris_of(int(refs[node, side]))
+                assert len(idx) <= 256, "oriented box on a child of more than 256 triangles"
+                d = tv[idx].reshape(-1, 3).astype(np.float64) @ axes.T
+                assert (d.min(0) >= lo).all() and (d.max(0) <= hi).all(), "oriented box does not hold its triangles"
 
     def _points(self, points: torch.Tensor) -> torch.Tensor:
         _lib.require_cuda(points, "points")
@@ -118,6 +159,7 @@ class BVHAccelerator:
 
     UDF_LANE = 1
     UDF_DIRECT = 2
+    UDF_NO_OBBS = 4
     ORDER_MIN_QUERIES = 4096
     LANE_MAX_QUERIES = 1 << 17
 

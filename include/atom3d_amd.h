@@ -84,7 +84,8 @@ int a3d_bvh_info(const void* blob, int64_t* out4, a3d_stream_t stream);
  * perm (nullable, uint32[K]): process the queries in this order (a
  * permutation of 0..K-1, e.g. from a3d_query_order); results are still
  * written at each query's own index.  flags: A3D_UDF_LANE = one independent
- * walk per lane instead of one shared walk per wavefront.  counters (nullable,
+ * walk per lane instead of one shared walk per wavefront; A3D_UDF_NO_OBBS =
+ * skip the children's oriented-box bounds (kept for comparisons).  counters (nullable,
  * device int64[5], accumulated, wave walk only): wavefronts, node visits,
  * leaf triangles seen, exact triangle tests run (all summed over wavefronts)
  * and triangles whose bound admitted them per lane (summed over lanes).
@@ -95,6 +96,7 @@ int a3d_bvh_info(const void* blob, int64_t* out4, a3d_stream_t stream);
  */
 #define A3D_UDF_LANE 1
 #define A3D_UDF_DIRECT 2 /* wave walk testing each needed triangle at once (no exact-test queue) */
+#define A3D_UDF_NO_OBBS 4 /* prune by the axis-aligned boxes only, ignoring the blob's oriented boxes (same results) */
 int a3d_query_order_ws_bytes(int64_t K, int64_t* bytes);
 int a3d_query_order(const float* P, int64_t K, uint32_t* perm, void* ws, int64_t ws_bytes,
                     a3d_stream_t stream);
