@@ -156,14 +156,18 @@ __device__ __forceinline__ void wave_walk(const Node64* __restrict__ nodes, cons
       float dr = box_d2(p, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w);
       bool wl = dl <= thr, wr = dr <= thr;
       uint64_t bl = __ballot(wl), br = __ballot(wr);
-      if (obbs && (bl | br)) {  // tighten by the children's oriented boxes
+      if (obbs && (bl | br)) {  // tighten by the children's oriented boxes (those some lane still wants)
         const Obb96* ob = obbs + cur;
-        dl = fmaxf(dl, obb_d2(p, ob->c[0], ob->c[1], ob->c[2], eq));
-        dr = fmaxf(dr, obb_d2(p, ob->c[3], ob->c[4], ob->c[5], eq));
-        wl = dl <= thr;
-        wr = dr <= thr;
-        bl = __ballot(wl);
-        br = __ballot(wr);
+        if (bl) {
+          dl = fmaxf(dl, obb_d2(p, ob->c[0], ob->c[1], ob->c[2], eq));
+          wl = dl <= thr;
+          bl = __ballot(wl);
+        }
+        if (br) {
+          dr = fmaxf(dr, obb_d2(p, ob->c[3], ob->c[4], ob->c[5], eq));
+          wr = dr <= thr;
+          br = __ballot(wr);
+        }
       }
       if (bl && br) {
         int nl = __popcll(__ballot((wl || wr) && dl <= dr));
