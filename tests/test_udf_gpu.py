@@ -23,8 +23,10 @@ def _gpu_udf(V, F, P, **kw):
     return d.cpu().numpy(), f.cpu().numpy(), c.cpu().numpy(), w.cpu().numpy()
 
 
+# flags: 0 wave walk, 1 per-lane walk, 2 wave walk without the exact-test
+# queue, +4 without the oriented-box bounds (axis-aligned boxes only)
 VARIANTS = [dict(order=True), dict(order=False), dict(order=True, flags=1), dict(order=False, flags=1),
-            dict(order=True, flags=2)]
+            dict(order=True, flags=2), dict(order=True, flags=4), dict(order=False, flags=5)]
 
 
 def _assert_bitexact(V, F, P, label):
