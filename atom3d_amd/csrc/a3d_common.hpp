@@ -117,7 +117,10 @@ struct Obb96 {
   float4 c[6];  // left child c[0..2], right child c[3..5]
 };
 static_assert(sizeof(Obb96) == 96, "oriented-box record is 96 bytes");
-constexpr int kObbCap = 256;
+#ifndef A3D_OBB_CAP
+#define A3D_OBB_CAP 256
+#endif
+constexpr int kObbCap = A3D_OBB_CAP;
 
 __host__ __device__ __forceinline__ v3 obb_t2(v3 n, v3 t1) { return cross3(n, t1); }
 
