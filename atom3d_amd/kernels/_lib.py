@@ -84,11 +84,11 @@ SIGNATURES: dict[str, list] = {
     "a3d_vox_cells_get": [_vp, _pi64, _pi64, _i64, _vp, _vp],
     "a3d_cells_sat_count": [_vp, _vp, _i64, _pf32, _pf32, _vp, _vp],
     "a3d_cells_sat_pairs": [_vp, _vp, _i64, _pf32, _pf32, _vp, _vp, _vp, _vp],
-    "a3d_cells_ws_bytes": [_i64, _pi64],
-    "a3d_unique_cells": [_vp, _i64, _i32, _vp, _i64, _vp, _pi64, _vp],
-    "a3d_cell_vertices": [_vp, _i64, _i32, _vp, _i64, _vp, _pi64, _vp],
-    "a3d_cell_vertices_get": [_vp, _i64, _i32, _i64, _vp, _vp, _vp, _pf32, _pf32, _vp],
-    "a3d_cell_neighbors": [_vp, _i64, _i32, _vp, _vp],
+    "a3d_cells_ws_bytes": [_i64, _i32, _i32, _pi64],
+    "a3d_unique_cells": [_vp, _i64, _i32, _i32, _vp, _i64, _vp, _pi64, _vp],
+    "a3d_cell_vertices": [_vp, _i64, _i32, _i32, _vp, _i64, _vp, _pi64, _vp],
+    "a3d_cell_vertices_get": [_vp, _i64, _i32, _i32, _i64, _vp, _vp, _vp, _pf32, _pf32, _vp],
+    "a3d_cell_neighbors": [_vp, _i64, _i32, _i32, _vp, _i64, _vp, _vp],
     "a3d_wet_ws_bytes": [_i32, _pi64],
     "a3d_wet_mask": [_i32, _vp, _i64, _vp, _i64, _i32, _vp, _i64, _vp, _vp],
     "a3d_sg_sizes": [_pi64, _pi64],

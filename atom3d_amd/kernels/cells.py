@@ -26,13 +26,15 @@ def _ijk32(t: torch.Tensor, name: str) -> torch.Tensor:
     return t.to(torch.int32).contiguous()
 
 
-def _ws(n: int, device) -> Tuple[torch.Tensor, int]:
+# method: 0 = the library chooses (bitmaps up to 2048^3 when the grid is not
+# too sparse, else a radix sort), 1 = sort, 2 = bitmaps; results are the same.
+def _ws(n: int, resolution: int, method: int, device) -> Tuple[torch.Tensor, int]:
     nb = ctypes.c_int64()
-    _lib.call("a3d_cells_ws_bytes", int(n), ctypes.byref(nb))
+    _lib.call("a3d_cells_ws_bytes", int(n), int(resolution), int(method), ctypes.byref(nb))
     return torch.empty(max(nb.value, 1), dtype=torch.uint8, device=device), nb.value
 
 
-def unique_cells(ijk: torch.Tensor, resolution: int) -> torch.Tensor:
+def unique_cells(ijk: torch.Tensor, resolution: int, method: int = 0) -> torch.Tensor:
     """Sorted (lexicographic) unique rows of int cells [N,3] -> int32 [U,3]."""
     c = _ijk32(ijk, "cells")
     dev = c.device
@@ -40,15 +42,16 @@ def unique_cells(ijk: torch.Tensor, resolution: int) -> torch.Tensor:
     out = torch.empty((max(n, 1), 3), dtype=torch.int32, device=dev)
     if n == 0:
         return out[:0]
-    ws, nb = _ws(n, dev)
+    ws, nb = _ws(n, resolution, method, dev)
     cnt = ctypes.c_int64()
-    _lib.call("a3d_unique_cells", _lib.ptr(c), n, int(resolution), _lib.ptr(ws), nb, _lib.ptr(out),
+    _lib.call("a3d_unique_cells", _lib.ptr(c), n, int(resolution), int(method), _lib.ptr(ws), nb, _lib.ptr(out),
               ctypes.byref(cnt), _lib.stream(dev))
     return out[:cnt.value]
 
 
 def cell_vertices(ijk: torch.Tensor, resolution: int, lo: Optional[Sequence[float]] = None,
-                  cell: Optional[Sequence[float]] = None, want_keys: bool = False, want_ijk: bool = False):
+                  cell: Optional[Sequence[float]] = None, want_keys: bool = False, want_ijk: bool = False,
+                  method: int = 0):
     """Unique corner vertices of cells [N,3] (duplicates allowed).
 
     Returns (cube_vert int64 [N,8], vkey int64 [V] | None, vijk int64 [V,3] |
@@ -61,10 +64,10 @@ def cell_vertices(ijk: torch.Tensor, resolution: int, lo: Optional[Sequence[floa
     n = int(c.shape[0])
     cube_vert = torch.empty((n, 8), dtype=torch.int64, device=dev)
     nv = ctypes.c_int64()
-    ws, nb = _ws(max(n, 1), dev)
+    ws, nb = _ws(max(n, 1), resolution, method, dev)
     if n:
-        _lib.call("a3d_cell_vertices", _lib.ptr(c), n, int(resolution), _lib.ptr(ws), nb, _lib.ptr(cube_vert),
-                  ctypes.byref(nv), _lib.stream(dev))
+        _lib.call("a3d_cell_vertices", _lib.ptr(c), n, int(resolution), int(method), _lib.ptr(ws), nb,
+                  _lib.ptr(cube_vert), ctypes.byref(nv), _lib.stream(dev))
     V = nv.value
     vkey = torch.empty((V,), dtype=torch.int64, device=dev) if want_keys else None
     vijk = torch.empty((V, 3), dtype=torch.int64, device=dev) if want_ijk else None
@@ -72,19 +75,21 @@ def cell_vertices(ijk: torch.Tensor, resolution: int, lo: Optional[Sequence[floa
     if V and (vkey is not None or vijk is not None or world is not None):
         lo_c = (ctypes.c_float * 3)(*[float(x) for x in lo]) if world is not None else None
         ce_c = (ctypes.c_float * 3)(*[float(x) for x in cell]) if world is not None else None
-        _lib.call("a3d_cell_vertices_get", _lib.ptr(ws), n, int(resolution), V, _lib.ptr(vkey), _lib.ptr(vijk),
-                  _lib.ptr(world), lo_c, ce_c, _lib.stream(dev))
+        _lib.call("a3d_cell_vertices_get", _lib.ptr(ws), n, int(resolution), int(method), V, _lib.ptr(vkey),
+                  _lib.ptr(vijk), _lib.ptr(world), lo_c, ce_c, _lib.stream(dev))
     return cube_vert, vkey, vijk, world
 
 
-def cell_neighbors(cells_sorted: torch.Tensor, resolution: int) -> torch.Tensor:
+def cell_neighbors(cells_sorted: torch.Tensor, resolution: int, method: int = 0) -> torch.Tensor:
     """int64 [N,6]: index of each sorted unique cell's neighbour at +x,-x,+y,
     -y,+z,-z (unchecked flat key, as the reference looks it up), else -1."""
     c = _ijk32(cells_sorted, "cells")
     n = int(c.shape[0])
     adj = torch.empty((n, 6), dtype=torch.int64, device=c.device)
     if n:
-        _lib.call("a3d_cell_neighbors", _lib.ptr(c), n, int(resolution), _lib.ptr(adj), _lib.stream(c.device))
+        ws, nb = _ws(n, resolution, method, c.device)
+        _lib.call("a3d_cell_neighbors", _lib.ptr(c), n, int(resolution), int(method), _lib.ptr(ws), nb,
+                  _lib.ptr(adj), _lib.stream(c.device))
     return adj
 
 

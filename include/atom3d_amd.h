@@ -371,7 +371,10 @@ int a3d_cells_sat_pairs(const void* blob, const int64_t* ijk, int64_t n, const f
  * Voxel-to-dual-graph glue between the flood fill / octree and the SDF /
  * DMC steps.  Cells are int32 [n,3] (i,j,k) in [0, resolution)^3; an input
  * outside that range is an A3D_ERR_ARG (the calls that synchronise report it).
- * a3d_cells_ws_bytes: workspace for up to n cells (n <= 2^27).
+ * method: 1 = radix sort of flat keys, 2 = bitmaps of the cell and vertex
+ * sets ranked by popcount scans (resolution <= 2048), 0 = choose (bitmaps
+ * when resolution <= 2048 and resolution^3 <= 4096 n).  Same results.
+ * a3d_cells_ws_bytes: workspace for up to n cells (n <= 2^27) on that grid.
  * a3d_unique_cells: replaces the torch.unique(cat(dam, collision,
  *   dry_adjacent), dim=0) of construct_surface_dual_graph
  *   (apps/sparse_flood_fill.py:254-257): sorted (lexicographic) unique cells
@@ -386,10 +389,11 @@ int a3d_cells_sat_pairs(const void* blob, const int64_t* ijk, int64_t n, const f
  * a3d_cell_vertices_get: writes those n_vert vertices (any output may be
  *   NULL): vkey int64 flat index in the (resolution+1)^3 vertex grid, vijk
  *   int64 [V,3], world float32 [V,3] = ijk * cell + lo (host float[3] each,
- *   two float32 roundings, as sparse_flood_fill.py:287-289).
+ *   two float32 roundings, as sparse_flood_fill.py:287-289).  Same n and
+ *   method as the a3d_cell_vertices call.
  * a3d_cell_neighbors: adj int64 [n,6] for sorted unique cells: index of the
  *   cell at offset +x,-x,+y,-y,+z,-z, else -1, looked up by the unchecked
- *   flat key as the reference does (:273-283).
+ *   flat key as the reference does (:273-283).  The bitmap method needs ws.
  * a3d_wet_ws_bytes / a3d_wet_mask: replaces get_dam_wet_faces / _edges /
  *   _corners (:169-227).  kind 0 faces -> out uint8 [n,6], 1 edges -> [n,12]
  *   (the reference's EDGE_NEIGHBOR_OFFSETS), 2 corners -> [n,8]: whether a
@@ -397,14 +401,15 @@ int a3d_cells_sat_pairs(const void* blob, const int64_t* ijk, int64_t n, const f
  *   a3d_wet_ws_bytes bytes (resolution^3 bits), overwritten.  Collision
  *   voxels outside the grid are ignored.
  */
-int a3d_cells_ws_bytes(int64_t n, int64_t* bytes);
-int a3d_unique_cells(const int32_t* ijk, int64_t n, int resolution, void* ws, int64_t ws_bytes, int32_t* out_ijk,
-                     int64_t* n_out, a3d_stream_t stream);
-int a3d_cell_vertices(const int32_t* ijk, int64_t n, int resolution, void* ws, int64_t ws_bytes, int64_t* cube_vert,
-                      int64_t* n_vert, a3d_stream_t stream);
-int a3d_cell_vertices_get(const void* ws, int64_t n, int resolution, int64_t n_vert, int64_t* vkey, int64_t* vijk,
-                          float* world, const float* lo, const float* cell, a3d_stream_t stream);
-int a3d_cell_neighbors(const int32_t* cells, int64_t n, int resolution, int64_t* adj, a3d_stream_t stream);
+int a3d_cells_ws_bytes(int64_t n, int resolution, int method, int64_t* bytes);
+int a3d_unique_cells(const int32_t* ijk, int64_t n, int resolution, int method, void* ws, int64_t ws_bytes,
+                     int32_t* out_ijk, int64_t* n_out, a3d_stream_t stream);
+int a3d_cell_vertices(const int32_t* ijk, int64_t n, int resolution, int method, void* ws, int64_t ws_bytes,
+                      int64_t* cube_vert, int64_t* n_vert, a3d_stream_t stream);
+int a3d_cell_vertices_get(const void* ws, int64_t n, int resolution, int method, int64_t n_vert, int64_t* vkey,
+                          int64_t* vijk, float* world, const float* lo, const float* cell, a3d_stream_t stream);
+int a3d_cell_neighbors(const int32_t* cells, int64_t n, int resolution, int method, void* ws, int64_t ws_bytes,
+                       int64_t* adj, a3d_stream_t stream);
 int a3d_wet_ws_bytes(int resolution, int64_t* bytes);
 int a3d_wet_mask(int kind, const int32_t* dam, int64_t n, const int32_t* coll, int64_t c, int resolution,
                  void* bitmap, int64_t bitmap_bytes, uint8_t* out, a3d_stream_t stream);

@@ -115,14 +115,20 @@ def test_new_entry_points_validate_arguments():
     with pytest.raises(RuntimeError, match="qef_reg_scale"):
         _lib.call("a3d_dmc_forward_qef", 1, 1, sizes, None, None, None, None, None, None, None, 0, 0, 0.0, 0.99, 4,
                   None, -1.0, None, None, None, None, None, None)
-    with pytest.raises(RuntimeError, match="out of range"):
-        _lib.call("a3d_cells_ws_bytes", -1, ctypes.byref(n))
+    with pytest.raises(RuntimeError, match="bad arguments"):
+        _lib.call("a3d_cells_ws_bytes", -1, 8, 0, ctypes.byref(n))
+    with pytest.raises(RuntimeError, match="bitmap method"):
+        _lib.call("a3d_cells_ws_bytes", 10, 4096, 2, ctypes.byref(n))
+    _lib.call("a3d_cells_ws_bytes", 10, 1024, 0, ctypes.byref(n))
+    small = n.value
+    _lib.call("a3d_cells_ws_bytes", 10, 1024, 2, ctypes.byref(n))
+    assert n.value > 1024 ** 3 // 8 > small  # 10 cells on 1024^3: the sort path
     with pytest.raises(RuntimeError, match="null argument"):
-        _lib.call("a3d_unique_cells", None, 4, 8, None, 0, None, ctypes.byref(n), None)
+        _lib.call("a3d_unique_cells", None, 4, 8, 0, None, 0, None, ctypes.byref(n), None)
     with pytest.raises(RuntimeError, match="bad arguments"):
-        _lib.call("a3d_cell_vertices", None, 4, 0, None, 0, None, ctypes.byref(n), None)
-    with pytest.raises(RuntimeError, match="bad arguments"):
-        _lib.call("a3d_cell_neighbors", None, 4, 8, None, None)
+        _lib.call("a3d_cell_vertices", None, 4, 0, 0, None, 0, None, ctypes.byref(n), None)
+    with pytest.raises(RuntimeError, match="null argument"):
+        _lib.call("a3d_cell_neighbors", None, 4, 8, 1, None, 0, None, None)
     with pytest.raises(RuntimeError, match="kind"):
         _lib.call("a3d_wet_mask", 3, None, 1, None, 0, 8, None, 0, None, None)
     with pytest.raises(RuntimeError, match="bitmap"):

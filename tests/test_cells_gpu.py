@@ -19,30 +19,33 @@ pytestmark = pytest.mark.gpu
 DEV = "cuda:0"
 
 
-@pytest.mark.parametrize("R,n", [(16, 3000), (1024, 200000), (1700, 50000)])
-def test_unique_cells_vertices_neighbors_match_oracle(R, n):
-    """R = 1700 takes the 64-bit key path ((R+1)^3 > 2^32)."""
-    print(f"cells R={R} n={n}", flush=True)
+@pytest.mark.parametrize("method", [1, 2])
+@pytest.mark.parametrize("R,n", [(16, 3000), (1024, 200000), (1700, 50000), (100, 5)])
+def test_unique_cells_vertices_neighbors_match_oracle(R, n, method):
+    """Both methods (1 radix sort, 2 bitmaps); R = 1700 takes the 64-bit key
+    path of the sort ((R+1)^3 > 2^32); R = 16 and 100 have partial 32-bit
+    words in the bitmap rows."""
+    print(f"cells R={R} n={n} method={method}", flush=True)
     rng = np.random.default_rng(R)
     base = rng.integers(0, R, (n // 4, 3))
     cells = np.concatenate([base, base[rng.integers(0, len(base), n - len(base))]])  # duplicates
     cells = np.concatenate([cells, np.clip(cells[: n // 8] + rng.integers(-1, 2, (n // 8, 3)), 0, R - 1)])
     t = torch.tensor(cells, dtype=torch.int32, device=DEV)
-    u = kc.unique_cells(t, R)
+    u = kc.unique_cells(t, R, method=method)
     ref_u = oc.unique_cells(cells)
     assert np.array_equal(u.cpu().numpy(), ref_u)
     lo, cell = [-1.0, -0.5, -2.0], [float(np.float32(2.0) / np.float32(R))] * 3
-    cube, vkey, vijk, world = kc.cell_vertices(u, R, lo=lo, cell=cell, want_keys=True, want_ijk=True)
+    cube, vkey, vijk, world = kc.cell_vertices(u, R, lo=lo, cell=cell, want_keys=True, want_ijk=True, method=method)
     ref_v, ref_cube = oc.cell_vertices(ref_u)
     assert np.array_equal(vijk.cpu().numpy(), ref_v)
     assert np.array_equal(cube.cpu().numpy(), ref_cube)
     P = R + 1
     assert np.array_equal(vkey.cpu().numpy(), (ref_v[:, 0] * P + ref_v[:, 1]) * P + ref_v[:, 2])
     assert np.array_equal(world.cpu().numpy(), oc.world(ref_v, lo, cell))
-    adj = kc.cell_neighbors(u, R)
+    adj = kc.cell_neighbors(u, R, method=method)
     assert np.array_equal(adj.cpu().numpy(), oc.cell_neighbors(ref_u, R))
     # unsorted cells with duplicates straight into cell_vertices
-    cube2, _, vijk2, _ = kc.cell_vertices(t, R, want_ijk=True)
+    cube2, _, vijk2, _ = kc.cell_vertices(t, R, want_ijk=True, method=method)
     ref_v2, ref_cube2 = oc.cell_vertices(cells)
     assert np.array_equal(vijk2.cpu().numpy(), ref_v2) and np.array_equal(cube2.cpu().numpy(), ref_cube2)
 
@@ -136,5 +139,6 @@ def test_voxel_unique_vertices_demo_pipeline():
     g = torch.autograd.grad(d.sum(), x)[0]
     far = d.detach() > 1e-4
     assert torch.allclose(g.norm(dim=1)[far], torch.ones_like(d[far]), atol=1e-4)
-    with pytest.raises(RuntimeError, match="outside"):
-        kc.cell_vertices(torch.tensor([[0, 0, 600]], device=DEV, dtype=torch.int32), 512)
+    for method in (1, 2):
+        with pytest.raises(RuntimeError, match="outside"):
+            kc.cell_vertices(torch.tensor([[0, 0, 600]], device=DEV, dtype=torch.int32), 512, method=method)
