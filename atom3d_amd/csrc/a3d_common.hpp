@@ -47,6 +47,7 @@ __device__ __forceinline__ float sgn(float x) { return x >= 0.0f ? 1.0f : -1.0f;
 //   tris                Tri48[n_tris]        triangles in leaf order
 //   boxes               TriBox32[n_tris]     padded triangle boxes, leaf order
 //   obbs                Obb96[n_node_cap]    per node, both children's oriented boxes
+//   tobbs               float4[3 n_tris]     per triangle, its oriented box (leaf order)
 // Node64 keeps BOTH children's boxes so one 64-byte fetch decides both
 // children (4 x 16-byte loads, no dependent second fetch).
 struct BvhHeader {
@@ -64,7 +65,8 @@ struct BvhHeader {
   int32_t depth;        // max internal-node depth (root = 0), for stack sizing
   int64_t boxes_off;    // per-triangle padded boxes (TriBox32), leaf order
   int64_t obbs_off;     // per-node child oriented boxes (Obb96), 0 = none
-  int32_t pad[42];
+  int64_t tobbs_off;    // per-triangle oriented boxes (3 float4 each, leaf order), 0 = none
+  int32_t pad[40];
 };
 static_assert(sizeof(BvhHeader) == 256, "header is 256 bytes");
 constexpr int32_t kBvhMagic = 0x41334442;  // 'A3DB'
@@ -318,6 +320,14 @@ __device__ __forceinline__ const Tri48* tris_of(const void* blob) {
 }
 __device__ __forceinline__ const TriBox32* boxes_of(const void* blob) {
   return reinterpret_cast<const TriBox32*>((const char*)blob + header(blob)->boxes_off);
+}
+// Per-triangle oriented box, same 3-float4 layout as one Obb96 child: axes
+// n = the unit normal, t1 = the unit first edge, t2 = obb_t2(n, t1); ranges
+// of the vertices widened by the triangle's pad.  Degenerate triangles get
+// zero axes and infinite ranges (no bound).
+__device__ __forceinline__ const float4* tobbs_of(const void* blob) {
+  const int64_t o = header(blob)->tobbs_off;
+  return o ? reinterpret_cast<const float4*>((const char*)blob + o) : nullptr;
 }
 __device__ __forceinline__ const Obb96* obbs_of(const void* blob) {
   const int64_t o = header(blob)->obbs_off;

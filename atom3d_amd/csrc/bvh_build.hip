@@ -144,7 +144,7 @@ Ws carve(void* base, int64_t n) {
 
 int64_t blob_size(int64_t nF) {
   int64_t cap = nF > 1 ? nF - 1 : 1;
-  return 256 + 64 * cap + 48 * nF + 32 * nF + 96 * cap;
+  return 256 + 64 * cap + 48 * nF + 32 * nF + 96 * cap + 48 * nF;
 }
 
 __device__ __forceinline__ v3 vtx(const float* __restrict__ V, int32_t i) {
@@ -852,6 +852,61 @@ __device__ void child_obb(const Node64* __restrict__ nodes, const Tri48* __restr
   out[2] = make_float4(hf[0], hf[1], lf[2], hf[2]);
 }
 
+// Oriented box of one triangle (see tobbs_of): FP64 frame from the
+// vertices, float axes shrunk by 2^-21, FP64 ranges widened by `pad` and
+// rounded outward.
+__device__ void tri_obb(v3 a, v3 b, v3 c, float pad, float4* out) {
+  const float inf = __int_as_float(0x7f800000);
+  out[0] = make_float4(0.0f, 0.0f, 0.0f, -inf);
+  out[1] = make_float4(0.0f, 0.0f, 0.0f, -inf);
+  out[2] = make_float4(inf, inf, -inf, inf);
+  const double ux = (double)b.x - a.x, uy = (double)b.y - a.y, uz = (double)b.z - a.z;
+  const double vx = (double)c.x - a.x, vy = (double)c.y - a.y, vz = (double)c.z - a.z;
+  const double nx = uy * vz - uz * vy, ny = uz * vx - ux * vz, nz = ux * vy - uy * vx;
+  const double nl = sqrt(nx * nx + ny * ny + nz * nz), ul = sqrt(ux * ux + uy * uy + uz * uz);
+  if (!(nl > 0.0) || !(ul > 0.0)) return;
+  const double k = 1.0 - 4.76837158203125e-07;  // 1 - 2^-21
+  const v3 nf = mk((float)(nx / nl * k), (float)(ny / nl * k), (float)(nz / nl * k));
+  const v3 tf = mk((float)(ux / ul * k), (float)(uy / ul * k), (float)(uz / ul * k));
+  const v3 af[3] = {nf, tf, obb_t2(nf, tf)};
+  const v3 vv[3] = {a, b, c};
+  float lf[3], hf[3];
+  for (int r = 0; r < 3; ++r) {
+    double lo = 1e300, hi = -1e300;
+    for (int q = 0; q < 3; ++q) {
+      const double d = (double)af[r].x * vv[q].x + (double)af[r].y * vv[q].y + (double)af[r].z * vv[q].z;
+      lo = d < lo ? d : lo;
+      hi = d > hi ? d : hi;
+    }
+    const double m = 1e-12 * (fabs(lo) + fabs(hi)) + (double)pad;
+    lo -= m;
+    hi += m;
+    lf[r] = (float)lo;
+    hf[r] = (float)hi;
+    if ((double)lf[r] > lo) lf[r] = nextafterf(lf[r], -inf);
+    if ((double)hf[r] < hi) hf[r] = nextafterf(hf[r], inf);
+  }
+  out[0] = make_float4(nf.x, nf.y, nf.z, lf[0]);
+  out[1] = make_float4(tf.x, tf.y, tf.z, lf[1]);
+  out[2] = make_float4(hf[0], hf[1], lf[2], hf[2]);
+}
+
+__global__ __launch_bounds__(kBlk) void k_tri_obbs(void* blob) {
+  const BvhHeader* H = header(blob);
+  const int64_t t = blockIdx.x * (int64_t)kBlk + threadIdx.x;
+  if (t >= (int64_t)H->n_tris) return;
+  v3 a, b, c;
+  int32_t id;
+  const Tri48* tris = tris_of(blob);
+  load_tri(tris, (int32_t)t, a, b, c, id);
+  float4* out = reinterpret_cast<float4*>((char*)blob + H->tobbs_off) + 3 * t;
+  float4 o[3];
+  tri_obb(a, b, c, tri_pad_of(tris, (int32_t)t), o);
+  out[0] = o[0];
+  out[1] = o[1];
+  out[2] = o[2];
+}
+
 // One thread per (node, child) of the emitted tree.
 __global__ __launch_bounds__(kBlk) void k_obbs(void* blob) {
   const BvhHeader* H = header(blob);
@@ -880,7 +935,7 @@ __global__ void k_header(BvhHeader* h, int64_t n, int64_t cap, int leaf_max, con
                          const int32_t* __restrict__ depth) {
   if (threadIdx.x != 0) return;
   BvhHeader H;
-  for (int k = 0; k < 42; ++k) H.pad[k] = 0;
+  for (int k = 0; k < 40; ++k) H.pad[k] = 0;
   H.magic = kBvhMagic;
   H.n_tris = (int32_t)n;
   H.n_nodes = (n > leaf_max && m > 0) ? remap[m - 1] + kept[m - 1] : 1;
@@ -889,6 +944,7 @@ __global__ void k_header(BvhHeader* h, int64_t n, int64_t cap, int leaf_max, con
   H.tris_off = 256 + 64 * cap;
   H.boxes_off = H.tris_off + 48 * n;
   H.obbs_off = H.boxes_off + 32 * n;
+  H.tobbs_off = H.obbs_off + 96 * cap;
   float cm = 0.0f;
   for (int k = 0; k < 3; ++k) {
     H.scene_lo[k] = n > 0 ? ord2f(red[R_VLO + k]) : 0.0f;
@@ -994,6 +1050,8 @@ extern "C" int a3d_bvh_build_ex(const float* V, int64_t nV, const int32_t* F, in
     A3D_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_obbs, dim3(1), dim3(kBlk), 0, s, blob);
     A3D_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_tri_obbs, dim3(1), dim3(kBlk), 0, s, blob);
+    A3D_LAUNCH_CHECK();
     return A3D_OK;
   }
   unsigned gn = (unsigned)div_up(n, kBlk), gm = (unsigned)div_up(m, kBlk);
@@ -1058,6 +1116,8 @@ extern "C" int a3d_bvh_build_ex(const float* V, int64_t nV, const int32_t* F, in
                      depth_src);
   A3D_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_obbs, dim3((unsigned)div_up(2 * cap, kBlk)), dim3(kBlk), 0, s, blob);
+  A3D_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_tri_obbs, dim3((unsigned)div_up(n, kBlk)), dim3(kBlk), 0, s, blob);
   A3D_LAUNCH_CHECK();
   return A3D_OK;
 }

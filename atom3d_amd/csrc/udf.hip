@@ -39,6 +39,10 @@ constexpr int kLaneLds = 16;  // per-lane LDS stack entries in the lane path
 constexpr int kLaneOvf = kStackTotal - kLaneLds;
 
 __device__ __forceinline__ int32_t uni(int32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+// lane j's value of x (j uniform)
+__device__ __forceinline__ float lane_f(float x, int j) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), j));
+}
 
 __device__ __forceinline__ void write_result(int64_t q, v3 p, float best, int32_t best_id, int32_t best_tri,
                                              const Tri48* __restrict__ tris, float* __restrict__ dist,
@@ -67,20 +71,33 @@ __device__ __forceinline__ void write_result(int64_t q, v3 p, float best, int32_
 // ------------------------------------------------------------ exact-test queue
 // The exact point-triangle tests of a wave are compacted: at a leaf, every
 // (lane, triangle) pair whose box bound passes the lane's threshold is
-// appended to a per-wave LDS queue, and each batch of 64 queued pairs is
-// tested with one pair per lane.  A lane's best (squared distance, face id)
-// is the 64-bit minimum of its old best and its tested pairs (float bits of
-// a non-negative distance order like the distance, ties to the lowest id:
-// the same rule as testing them one by one).  Deferring the tests only
-// delays the tightening of the pruning threshold, which stays conservative.
-constexpr int kQCap = 128;
+// appended to a per-wave LDS queue.  Pairs are first filtered 64 at a time,
+// one pair per lane, by the triangle's oriented box against the owning
+// lane's current threshold; the survivors stay at the front of the queue and
+// each batch of 64 of them is tested exactly, again one pair per lane.  A
+// lane's best (squared distance, face id) is the 64-bit minimum of its old
+// best and its tested pairs (float bits of a non-negative distance order like
+// the distance, ties to the lowest id: the same rule as testing them one by
+// one).  Deferring the tests only delays the tightening of the pruning
+// threshold, which stays conservative.
+//
+// Queue layout: [0, qf) filtered, [qf, qn) not yet filtered.  Between leaf
+// triangles qf < 64 and qn - qf < 64, and one triangle adds at most 64
+// pairs, so qn < 192.
+constexpr int kQCap = 192;
+constexpr int kWaveStack = 100;  // >= 97 (see kStackTotal); sized so 8 blocks of 4 waves fit in 160 KiB of LDS
 
 struct WaveQ {
+  float4 ob[6];                // the current node's Obb96, copied by the node visit (global -> LDS)
   unsigned long long key[64];  // per-lane best key during a flush
   int32_t tri[64];             // triangle index of the winning pair
   int32_t qt[kQCap];           // queued pairs: triangle index
   uint8_t ql[kQCap];           //               and lane
 };
+
+// s_waitcnt immediate (gfx9 encoding) waiting for every vector memory access: vmcnt 0
+// (bits 3:0 and 15:14), expcnt 7 and lgkmcnt 15 (no wait)
+constexpr int kWaitVm0 = 0x0F70;
 
 __device__ __forceinline__ unsigned long long best_key(float d2, int32_t id) {
   return ((unsigned long long)__float_as_uint(d2) << 32) | (uint32_t)id;
@@ -92,8 +109,54 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Tests the first n (1..64) queued pairs and removes them from the queue
-// (qn entries before, qn - n after).
+// Moves queue entries [from, from + cnt) to [to, to + cnt) (to <= from,
+// cnt <= 128).  Ends with the wave synchronised.
+__device__ __forceinline__ void q_move(WaveQ* __restrict__ Q, int from, int to, int cnt, int lane) {
+  const bool a = lane < cnt, b = lane + 64 < cnt;
+  const int32_t ta = a ? Q->qt[from + lane] : 0, tb = b ? Q->qt[from + 64 + lane] : 0;
+  const uint8_t la = a ? Q->ql[from + lane] : (uint8_t)0, lb = b ? Q->ql[from + 64 + lane] : (uint8_t)0;
+  wave_sync();
+  if (a) {
+    Q->qt[to + lane] = ta;
+    Q->ql[to + lane] = la;
+  }
+  if (b) {
+    Q->qt[to + 64 + lane] = tb;
+    Q->ql[to + 64 + lane] = lb;
+  }
+  wave_sync();
+}
+
+// Filters the n (1..64) unfiltered pairs at [qf, qf + n) by their triangles'
+// oriented boxes against the owning lanes' thresholds; the survivors are
+// appended to the filtered front and the rest of the queue closes up.
+__device__ __forceinline__ void q_filter(WaveQ* __restrict__ Q, int n, int& qf, int& qn,
+                                         const float4* __restrict__ tobbs, v3 p, float thr, int lane) {
+  const int32_t t = lane < n ? Q->qt[qf + lane] : 0;
+  const int L = lane < n ? (int)Q->ql[qf + lane] : 0;
+  const v3 pl = mk(__shfl(p.x, L), __shfl(p.y, L), __shfl(p.z, L));
+  const float tl = __shfl(thr, L);
+  bool keep = false;
+  if (lane < n) {
+    const float4* o = tobbs + 3 * (int64_t)t;
+    keep = obb_d2(pl, o[0], o[1], o[2], obb_eq(pl)) <= tl;
+  }
+  const uint64_t m = __ballot(keep);
+  wave_sync();
+  if (keep) {
+    const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+    Q->qt[qf + below] = t;
+    Q->ql[qf + below] = (uint8_t)L;
+  }
+  const int s = __popcll(m);
+  const int rest = qn - qf - n;
+  q_move(Q, qf + n, qf + s, rest, lane);
+  qf += s;
+  qn = qf + rest;
+}
+
+// Tests the first n (1..64) queued pairs exactly and removes them from the
+// queue (qn entries before, qn - n after).
 __device__ __forceinline__ void q_flush(WaveQ* __restrict__ Q, int n, int qn, const Tri48* __restrict__ tris, v3 p,
                                         int lane, bool valid, float slack, float& best, int32_t& best_id,
                                         int32_t& best_tri, float& thr) {
@@ -113,15 +176,7 @@ __device__ __forceinline__ void q_flush(WaveQ* __restrict__ Q, int n, int qn, co
   }
   wave_sync();
   if (lane < n && Q->key[L] == k) Q->tri[L] = t;
-  // move the untested tail to the front
-  const int rest = qn - n;
-  const int32_t tail_t = lane < rest ? Q->qt[n + lane] : 0;
-  const uint8_t tail_l = lane < rest ? Q->ql[n + lane] : (uint8_t)0;
-  wave_sync();
-  if (lane < rest) {
-    Q->qt[lane] = tail_t;
-    Q->ql[lane] = tail_l;
-  }
+  q_move(Q, n, 0, qn - n, lane);  // the untested tail to the front
   const unsigned long long nk = Q->key[lane];
   if (nk != mine) {
     best = __uint_as_float((uint32_t)(nk >> 32));
@@ -139,32 +194,47 @@ __device__ __forceinline__ void q_flush(WaveQ* __restrict__ Q, int n, int qn, co
 template <bool STATS, bool QUEUE>
 __device__ __forceinline__ void wave_walk(const Node64* __restrict__ nodes, const Tri48* __restrict__ tris,
                                           const TriBox32* __restrict__ boxes, const Obb96* __restrict__ obbs,
-                                          v3 p, bool valid, float slack,
+                                          const float4* __restrict__ tobbs, v3 p, bool valid, float slack,
                                           float4 (*stk)[2], WaveQ* __restrict__ Q, int lane,
                                           float& best, int32_t& best_id, int32_t& best_tri, uint32_t& n_node,
                                           uint32_t& n_leaf_tri, uint32_t& n_test, uint32_t& n_lane) {
   float thr = valid ? prune_thr(best, slack) : -1.0f;  // invalid lanes accept nothing
   const float eq = obb_eq(p);
-  int qn = 0;  // queued exact tests (QUEUE)
+  int qn = 0, qf = 0;  // queued pairs, of which the first qf are filtered (QUEUE)
   int sp = 0;
   int32_t cur = 0;
   while (true) {
     if (cur >= 0) {
       if (STATS) ++n_node;
+      const bool obb_lds = QUEUE && obbs;
+      if (obb_lds) {
+        // start copying the node's oriented boxes (6 x 16 B, lanes 0-5) into LDS so that they
+        // arrive with the node instead of one round trip after its box test; the wait first
+        // retires the previous copy, whose slot this one overwrites
+        __builtin_amdgcn_s_waitcnt(kWaitVm0);
+        if (lane < 6)
+          __builtin_amdgcn_global_load_lds(obbs[cur].c + lane, (__attribute__((address_space(3))) void*)Q->ob, 16, 0,
+                                           0);
+      }
       const Node64 nd = nodes[cur];
       float dl = box_d2(p, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y);
       float dr = box_d2(p, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w);
       bool wl = dl <= thr, wr = dr <= thr;
       uint64_t bl = __ballot(wl), br = __ballot(wr);
       if (obbs && (bl | br)) {  // tighten by the children's oriented boxes (those some lane still wants)
-        const Obb96* ob = obbs + cur;
+        const float4* ob = obbs[cur].c;
+        if (obb_lds) {
+          __builtin_amdgcn_s_waitcnt(kWaitVm0);
+          wave_sync();
+          ob = Q->ob;
+        }
         if (bl) {
-          dl = fmaxf(dl, obb_d2(p, ob->c[0], ob->c[1], ob->c[2], eq));
+          dl = fmaxf(dl, obb_d2(p, ob[0], ob[1], ob[2], eq));
           wl = dl <= thr;
           bl = __ballot(wl);
         }
         if (br) {
-          dr = fmaxf(dr, obb_d2(p, ob->c[3], ob->c[4], ob->c[5], eq));
+          dr = fmaxf(dr, obb_d2(p, ob[3], ob[4], ob[5], eq));
           wr = dr <= thr;
           br = __ballot(wr);
         }
@@ -175,7 +245,7 @@ __device__ __forceinline__ void wave_walk(const Node64* __restrict__ nodes, cons
         bool left_first = nl >= nr;
         int32_t near_ref = left_first ? nd.d.x : nd.d.y;
         int32_t far_ref = left_first ? nd.d.y : nd.d.x;
-        if (sp < kStackTotal) {
+        if (sp < kWaveStack) {
           if (lane == 0) {
             // w of the second half: the far child as node * 2 + side, for its oriented box
             if (left_first) {
@@ -195,13 +265,22 @@ __device__ __forceinline__ void wave_walk(const Node64* __restrict__ nodes, cons
       if (br) { cur = uni(nd.d.y); continue; }
     } else {
       const int32_t s0 = leaf_start(cur), n = leaf_count(cur);
+      // the leaf's (<= 8 < 64) triangle boxes in one load, lane k holding box s0 + k: one
+      // memory round trip per leaf instead of one per triangle
+      float4 blo = make_float4(0.0f, 0.0f, 0.0f, 0.0f), bhi = blo;
+      if (QUEUE) {
+        const int32_t k = s0 + (lane < n ? lane : n - 1);
+        blo = boxes[k].lo;
+        bhi = boxes[k].hi;
+      }
       for (int32_t t = s0; t < s0 + n; ++t) {
         v3 ta, tb, tc;
         int32_t id = 0;
         bool need;
         if (QUEUE) {  // bound test from the stored box; vertices only in the flush
-          const TriBox32 bx = boxes[t];
-          need = box_d2(p, bx.lo.x, bx.lo.y, bx.lo.z, bx.hi.x, bx.hi.y, bx.hi.z) <= thr;
+          const int j = t - s0;
+          need = box_d2(p, lane_f(blo.x, j), lane_f(blo.y, j), lane_f(blo.z, j), lane_f(bhi.x, j), lane_f(bhi.y, j),
+                        lane_f(bhi.z, j)) <= thr;
         } else {
           load_tri(tris, t, ta, tb, tc, id);
           need = tri_box_d2(ta, tb, tc, tri_pad_of(tris, t), p) <= thr;
@@ -217,11 +296,17 @@ __device__ __forceinline__ void wave_walk(const Node64* __restrict__ nodes, cons
             Q->ql[qn + below] = (uint8_t)lane;
           }
           qn += __popcll(m);
-          if (qn >= 64) {
+          if (!tobbs) qf = qn;
+          if (qn - qf >= 64) {
+            wave_sync();
+            q_filter(Q, 64, qf, qn, tobbs, p, thr, lane);
+          }
+          if (qf >= 64) {
             wave_sync();
             if (STATS) ++n_test;
             q_flush(Q, 64, qn, tris, p, lane, valid, slack, best, best_id, best_tri, thr);
             qn -= 64;
+            qf -= 64;
           }
           continue;
         }
@@ -254,15 +339,14 @@ __device__ __forceinline__ void wave_walk(const Node64* __restrict__ nodes, cons
     if (!found) {
       if (QUEUE && qn > 0) {  // drain, then re-check the stack with the tightened thresholds
         wave_sync();
-        if (STATS) ++n_test;
-        const int n = qn < 64 ? qn : 64;
-        q_flush(Q, n, qn, tris, p, lane, valid, slack, best, best_id, best_tri, thr);
-        qn -= n;
-        if (qn > 0) {
+        if (qn > qf) q_filter(Q, qn - qf, qf, qn, tobbs, p, thr, lane);
+        while (qn > 0) {
           if (STATS) ++n_test;
-          q_flush(Q, qn, qn, tris, p, lane, valid, slack, best, best_id, best_tri, thr);
-          qn = 0;
+          const int n = qn < 64 ? qn : 64;
+          q_flush(Q, n, qn, tris, p, lane, valid, slack, best, best_id, best_tri, thr);
+          qn -= n;
         }
+        qf = 0;
       }
       break;
     }
@@ -280,7 +364,7 @@ __global__ __launch_bounds__(kBlk) void k_udf_wave(const void* __restrict__ blob
                                                     float* __restrict__ dist, int32_t* __restrict__ face,
                                                     float* __restrict__ closest, float* __restrict__ uvw,
                                                     unsigned long long* __restrict__ counters, bool use_obbs) {
-  __shared__ float4 s_stk[kWaves][kStackTotal][2];  // (lo.xyz, ref bits), (hi.xyz, obb code)
+  __shared__ float4 s_stk[kWaves][kWaveStack][2];  // (lo.xyz, ref bits), (hi.xyz, obb code)
   __shared__ WaveQ s_q[QUEUE ? kWaves : 1];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
@@ -301,7 +385,8 @@ __global__ __launch_bounds__(kBlk) void k_udf_wave(const void* __restrict__ blob
   }
   int32_t best_id = 0x7fffffff, best_tri = -1;
   uint32_t n_node = 0, n_leaf_tri = 0, n_test = 0, n_lane = 0;
-  wave_walk<STATS, QUEUE>(nodes, tris, boxes_of(blob), use_obbs ? obbs_of(blob) : nullptr, p, valid, slack,
+  wave_walk<STATS, QUEUE>(nodes, tris, boxes_of(blob), use_obbs ? obbs_of(blob) : nullptr,
+                          use_obbs ? tobbs_of(blob) : nullptr, p, valid, slack,
                           s_stk[wv], QUEUE ? &s_q[wv] : nullptr, lane,
                           best, best_id,
                           best_tri, n_node, n_leaf_tri, n_test, n_lane);
@@ -337,7 +422,8 @@ __device__ __forceinline__ void batch_walk_tile(const void* __restrict__ blob, c
   float best = 1e30f;
   int32_t best_id = 0x7fffffff, best_tri = -1;
   uint32_t n0 = 0, n1 = 0, n2 = 0, n3 = 0;
-  wave_walk<false, true>(nodes, tris, boxes_of(blob), obbs_of(blob), p, valid, slack, stk, Q, lane, best, best_id,
+  wave_walk<false, true>(nodes, tris, boxes_of(blob), obbs_of(blob), tobbs_of(blob), p, valid, slack, stk, Q, lane,
+                         best, best_id,
                          best_tri, n0, n1, n2, n3);
   if (valid) write_result(q, p, best, best_id, best_tri, tris, dist, face, closest, uvw);
 }
@@ -700,8 +786,8 @@ __global__ void k_udf_batch(const void* const* __restrict__ blobs, const float* 
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   const int nw = blockDim.x >> 6;
-  float4(*stk)[2] = reinterpret_cast<float4(*)[2]>(s_dyn + (size_t)wv * kStackTotal * 2);
-  WaveQ* Q = reinterpret_cast<WaveQ*>(s_dyn + (size_t)nw * kStackTotal * 2) + wv;
+  float4(*stk)[2] = reinterpret_cast<float4(*)[2]>(s_dyn + (size_t)wv * kWaveStack * 2);
+  WaveQ* Q = reinterpret_cast<WaveQ*>(s_dyn + (size_t)nw * kWaveStack * 2) + wv;
   while (true) {
     unsigned int t = 0;
     if (lane == 0) t = atomicAdd(counter, 1u);
@@ -772,7 +858,7 @@ extern "C" int a3d_batch_udf(const void* const* blobs, const float* P, const uin
   hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, counter);
   A3D_LAUNCH_CHECK();
-  const size_t lds = (size_t)waves_per_block * (kStackTotal * 2 * sizeof(float4) + sizeof(WaveQ));
+  const size_t lds = (size_t)waves_per_block * (kWaveStack * 2 * sizeof(float4) + sizeof(WaveQ));
   hipLaunchKernelGGL(k_udf_batch, dim3(blocks), dim3(64 * waves_per_block), lds, s, blobs, P, perm, tiles, n_tiles,
                      counter, dist, face, closest, uvw);
   A3D_LAUNCH_CHECK();

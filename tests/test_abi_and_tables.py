@@ -44,7 +44,7 @@ def test_version_and_argument_errors():
     with pytest.raises(RuntimeError, match="level"):
         _lib.call("a3d_octree_level", 1, (ctypes.c_float * 6)(), 22, None, 0, 1, 1, 1, 1 << 20, None)
     _lib.call("a3d_bvh_sizes", 4096000, 4, ctypes.byref(b), ctypes.byref(w))
-    assert b.value == 256 + 64 * 4095999 + (48 + 32) * 4096000 + 96 * 4095999
+    assert b.value == 256 + 64 * 4095999 + (48 + 32) * 4096000 + 96 * 4095999 + 48 * 4096000
 
 
 def test_cube_tables_match_reference_values():
