@@ -875,7 +875,14 @@ __global__ void k_vd_qef(Geom g, const int32_t* __restrict__ totals, Fields<T> f
 // ------------------------------------------------------------------ backward
 // Adjoint of the forward above.  Nothing but the topology is kept from the
 // forward pass: the interpolation weights, crossings and dual vertices are
-// recomputed per dual vertex, and gradients are scattered with atomics.
+// recomputed per dual vertex.  Gradients reach the corners and dual vertices
+// without atomics: each entry writes its two per-corner contributions, the
+// (corner, entry side) pairs are radix-sorted (stable, so each corner's
+// contributions stay in entry order) and every corner sums its run; alpha
+// sums per surface cube over its contiguous entries, and quad-centre
+// gradients reach their 4 dual vertices through a sort of (vertex, quad)
+// pairs the same way.  The sums are therefore deterministic.  Feature
+// gradients (optional, [M, C]) still use atomics.
 
 // g_vd = g_vertices[:Vd] (+ a quarter of each incident quad centre's gradient).
 template <typename T>
@@ -884,15 +891,86 @@ __global__ void k_gvd_init(const T* __restrict__ gsrc, int64_t n, T* __restrict_
   if (t < n) gdst[t] = gsrc ? gsrc[t] : T(0);
 }
 
+// (dual vertex, quad) pairs of the quads' corners, for the centre gradients.
+__global__ void k_quad_pairs(const int32_t* __restrict__ quads, int64_t n4, uint32_t* __restrict__ key,
+                             int32_t* __restrict__ val) {
+  int64_t t = blockIdx.x * (int64_t)kBlk + threadIdx.x;
+  if (t >= n4) return;
+  key[t] = (uint32_t)quads[t];
+  val[t] = (int32_t)(t >> 2);
+}
+
+// Each quad centre is the mean of its 4 dual vertices: one thread per run of
+// equal vertices in the sorted pairs adds the run's quarter centre
+// gradients, in quad order, to gdst[v] (which holds g_vertices[v]).
 template <typename T>
-__global__ void k_centers_bwd(const int32_t* __restrict__ quads, int64_t Q, int64_t Vd, int C,
-                              const T* __restrict__ gsrc, T* __restrict__ gdst) {
-  int64_t q = blockIdx.x * (int64_t)kBlk + threadIdx.x;
-  if (q >= Q) return;
+__global__ void k_centers_bwd(const uint32_t* __restrict__ key, const int32_t* __restrict__ val, int64_t n,
+                              int64_t Vd, int C, const T* __restrict__ gsrc, T* __restrict__ gdst) {
+  int64_t j = blockIdx.x * (int64_t)kBlk + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t v = key[j];
+  if (j > 0 && key[j - 1] == v) return;
+  int64_t end = j + 1;
+  while (end < n && key[end] == v) ++end;
   for (int c = 0; c < C; ++c) {
-    T g = gsrc[(Vd + q) * C + c] / T(4);
-    for (int k = 0; k < 4; ++k) atomicAdd(&gdst[(int64_t)quads[4 * q + k] * C + c], g);
+    T acc = gdst[(int64_t)v * C + c];
+    for (int64_t q = j; q < end; ++q) acc += gsrc[(Vd + val[q]) * C + c] / T(4);
+    gdst[(int64_t)v * C + c] = acc;
   }
+}
+
+// One thread per run of equal corners in the sorted (corner, entry side)
+// pairs: g_sdf[c] and g_deform[c] = the sums of the run's contributions
+// (rec: 4 values per entry side, sdf then xyz).
+template <typename T>
+__global__ void k_corner_sum(const uint32_t* __restrict__ key, const int32_t* __restrict__ val, int64_t n,
+                             const T* __restrict__ rec, T* __restrict__ g_sdf, T* __restrict__ g_deform) {
+  int64_t j = blockIdx.x * (int64_t)kBlk + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t c = key[j];
+  if (j > 0 && key[j - 1] == c) return;
+  T a = T(0), x = T(0), y = T(0), z = T(0);
+  for (int64_t q = j; q < n && key[q] == c; ++q) {
+    const T* r = rec + (int64_t)val[q] * 4;
+    a += r[0];
+    x += r[1];
+    y += r[2];
+    z += r[3];
+  }
+  g_sdf[c] = a;
+  if (g_deform) {
+    g_deform[(int64_t)c * 3] = x;
+    g_deform[(int64_t)c * 3 + 1] = y;
+    g_deform[(int64_t)c * 3 + 2] = z;
+  }
+}
+
+// One thread per surface cube (the first of its entries, which are
+// contiguous): g_alpha of its 8 corners = the sums of its entries' alpha
+// contributions (ra: 2 per entry, at the edge's two corners), in entry order.
+template <typename T>
+__global__ void k_alpha_sum(const int32_t* __restrict__ ent_slot, int64_t E, const T* __restrict__ ra,
+                            const int32_t* __restrict__ surf_cube, T* __restrict__ g_alpha) {
+  int64_t e = blockIdx.x * (int64_t)kBlk + threadIdx.x;
+  if (e >= E) return;
+  const int32_t i = ent_slot[e] / 12;
+  if (e > 0 && ent_slot[e - 1] / 12 == i) return;
+  T acc[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) acc[k] = T(0);
+  for (int64_t q = e; q < E && ent_slot[q] / 12 == i; ++q) {
+    const int ed = ent_slot[q] % 12;
+    const int k0 = kE0[ed], k1 = kE1[ed];
+    const T r0 = ra[2 * q], r1 = ra[2 * q + 1];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if (k == k0) acc[k] += r0;
+      if (k == k1) acc[k] += r1;
+    }
+  }
+  const int64_t cube = surf_cube[i];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) g_alpha[cube * 8 + k] = acc[k];
 }
 
 // d/dw0, d/dw1, d/dx0, d/dx1 of u = (x0 w1 - x1 w0) / (w1 - w0) for upstream g.
@@ -954,9 +1032,9 @@ template <typename T>
 __global__ void k_entry_bwd(Geom g, const int32_t* __restrict__ totals, Fields<T> f, const int32_t* __restrict__ ent_vd,
                             const T* __restrict__ aux, const T* __restrict__ gvd, const T* __restrict__ sgs,
                             const T* __restrict__ vfeat, const T* __restrict__ gvf, const T* __restrict__ gL,
-                            const T* __restrict__ gP, const T* __restrict__ gN, T* __restrict__ g_sdf,
-                            T* __restrict__ g_deform, T* __restrict__ g_alpha, T* __restrict__ g_beta,
-                            T* __restrict__ g_feat) {
+                            const T* __restrict__ gP, const T* __restrict__ gN, T* __restrict__ rec,
+                            uint32_t* __restrict__ ckey, int32_t* __restrict__ cval, T* __restrict__ ra,
+                            bool g_deform, T* __restrict__ g_beta, T* __restrict__ g_feat) {
   int64_t e = blockIdx.x * (int64_t)kBlk + threadIdx.x;
   if (e >= totals[1]) return;
   const int32_t v = ent_vd[e];
@@ -1033,18 +1111,23 @@ __global__ void k_entry_bwd(Geom g, const int32_t* __restrict__ totals, Fields<T
       gx0[d] -= gev;
     }
   }
-  atomicAdd(&g_sdf[ev.c0], gw0 * ev.a0 + gs0);
-  atomicAdd(&g_sdf[ev.c1], gw1 * ev.a1 + gs1);
-  if (g_deform)
-    for (int d = 0; d < 3; ++d) {
-      atomicAdd(&g_deform[ev.c0 * 3 + d], gx0[d]);
-      atomicAdd(&g_deform[ev.c1 * 3 + d], gx1[d]);
-    }
-  if (g_alpha && f.alpha) {
+  // per entry side: the corner's contributions (summed per corner after the sort)
+  T* r = rec + e * 8;
+  r[0] = gw0 * ev.a0 + gs0;
+  r[4] = gw1 * ev.a1 + gs1;
+  for (int d = 0; d < 3; ++d) {
+    r[1 + d] = g_deform ? gx0[d] : T(0);
+    r[5 + d] = g_deform ? gx1[d] : T(0);
+  }
+  ckey[2 * e] = (uint32_t)ev.c0;
+  ckey[2 * e + 1] = (uint32_t)ev.c1;
+  cval[2 * e] = (int32_t)(2 * e);
+  cval[2 * e + 1] = (int32_t)(2 * e + 1);
+  if (ra) {
     const int k0 = kE0[ev.e], k1 = kE1[ev.e];
     const T t0 = tanh(f.alpha[ev.cube * 8 + k0]), t1 = tanh(f.alpha[ev.cube * 8 + k1]);
-    atomicAdd(&g_alpha[ev.cube * 8 + k0], gw0 * ev.s0 * f.ws * (T(1) - t0 * t0));
-    atomicAdd(&g_alpha[ev.cube * 8 + k1], gw1 * ev.s1 * f.ws * (T(1) - t1 * t1));
+    ra[2 * e] = gw0 * ev.s0 * f.ws * (T(1) - t0 * t0);
+    ra[2 * e + 1] = gw1 * ev.s1 * f.ws * (T(1) - t1 * t1);
   }
   if (g_beta && f.beta) {
     const T tb = tanh(f.beta[ev.cube * 12 + ev.e]);
@@ -1261,11 +1344,55 @@ int dmc_forward_t(const DmcWs& w, const int64_t* sizes, const Geom& g, const Fie
   return A3D_OK;
 }
 
+// Backward scratch, sized from the topology's sizes: per entry side 4
+// contribution values and 1 alpha value (T), the (corner | dual vertex, index)
+// sort pairs twice (radix sort ping-pong) and the sort's temporary storage.
+struct BwdScratch {
+  void* rec;
+  void* ra;
+  uint32_t* k0;
+  uint32_t* k1;
+  int32_t* v0;
+  int32_t* v1;
+  void* tmp;
+  size_t tmp_bytes;
+  int64_t total;
+};
+
+int key_bits(int64_t n) {  // bits of the largest key n - 1 (at least 1)
+  int b = 1;
+  while (b < 32 && (int64_t(1) << b) < n) ++b;
+  return b;
+}
+
+BwdScratch carve_bwd(void* base, const int64_t* sizes, int tsize) {
+  const int64_t E = sizes[2], Q = sizes[4];
+  const int64_t P = std::max<int64_t>(std::max<int64_t>(2 * E, 4 * Q), 1);
+  size_t t = 0, tb = 0;
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, t, (uint32_t*)nullptr, (uint32_t*)nullptr, (int32_t*)nullptr,
+                                           (int32_t*)nullptr, (int)P, 0, 32);
+  tb = t;
+  BwdScratch b{};
+  char* p = (char*)base;
+  int64_t o = 0;
+  auto take = [&](int64_t bytes) { void* q = p ? p + o : nullptr; o += al256(bytes > 0 ? bytes : 1); return q; };
+  b.rec = take(2 * E * 4 * tsize);
+  b.ra = take(2 * E * tsize);
+  b.k0 = (uint32_t*)take(4 * P);
+  b.k1 = (uint32_t*)take(4 * P);
+  b.v0 = (int32_t*)take(4 * P);
+  b.v1 = (int32_t*)take(4 * P);
+  b.tmp = take((int64_t)tb);
+  b.tmp_bytes = tb;
+  b.total = o;
+  return b;
+}
+
 template <typename T>
-int dmc_backward_t(const DmcWs& w, const int64_t* sizes, int64_t N, int64_t M, const Geom& g, const Fields<T>& f,
-                   const void* vfeat, const void* g_verts, const void* g_L_dev, const void* g_p_alpha,
-                   const void* g_n_alpha, const void* g_vfeat, void* gvf_scratch, void* g_sdf, void* g_deform,
-                   void* g_alpha, void* g_beta, void* g_feat, hipStream_t s) {
+int dmc_backward_t(const DmcWs& w, const BwdScratch& b, const int64_t* sizes, int64_t N, int64_t M, const Geom& g,
+                   const Fields<T>& f, const void* vfeat, const void* g_verts, const void* g_L_dev,
+                   const void* g_p_alpha, const void* g_n_alpha, const void* g_vfeat, void* gvf_scratch, void* g_sdf,
+                   void* g_deform, void* g_alpha, void* g_beta, void* g_feat, hipStream_t s) {
   const int64_t Vd = sizes[1], E = sizes[2], Q = sizes[4];
   const int C = f.C;
   struct Z { void* p; int64_t n; } zs[5] = {{g_sdf, M}, {g_deform, 3 * M}, {g_alpha, 8 * N}, {g_beta, 12 * N},
@@ -1276,11 +1403,19 @@ int dmc_backward_t(const DmcWs& w, const int64_t* sizes, int64_t N, int64_t M, c
       A3D_LAUNCH_CHECK();
     }
   if (Vd == 0) return A3D_OK;
+  size_t tb = b.tmp_bytes;
+  if (Q > 0 && (g_verts || (g_vfeat && C > 0))) {  // (vertex, quad) pairs, sorted by vertex
+    hipLaunchKernelGGL(k_quad_pairs, dim3(gridn(4 * Q)), dim3(kBlk), 0, s, w.quads, 4 * Q, b.k0, b.v0);
+    A3D_LAUNCH_CHECK();
+    A3D_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(b.tmp, tb, b.k0, b.k1, b.v0, b.v1, (int)(4 * Q), 0,
+                                                     key_bits(Vd), s));
+  }
   T* gvd = (T*)w.gvd;
   hipLaunchKernelGGL(k_gvd_init<T>, dim3(gridn(3 * Vd)), dim3(kBlk), 0, s, (const T*)g_verts, 3 * Vd, gvd);
   A3D_LAUNCH_CHECK();
   if (Q > 0 && g_verts) {
-    hipLaunchKernelGGL(k_centers_bwd<T>, dim3(gridn(Q)), dim3(kBlk), 0, s, w.quads, Q, Vd, 3, (const T*)g_verts, gvd);
+    hipLaunchKernelGGL(k_centers_bwd<T>, dim3(gridn(4 * Q)), dim3(kBlk), 0, s, b.k1, b.v1, 4 * Q, Vd, 3,
+                       (const T*)g_verts, gvd);
     A3D_LAUNCH_CHECK();
   }
   T* gvf = nullptr;
@@ -1289,8 +1424,8 @@ int dmc_backward_t(const DmcWs& w, const int64_t* sizes, int64_t N, int64_t M, c
     hipLaunchKernelGGL(k_gvd_init<T>, dim3(gridn(C * Vd)), dim3(kBlk), 0, s, (const T*)g_vfeat, C * Vd, gvf);
     A3D_LAUNCH_CHECK();
     if (Q > 0) {
-      hipLaunchKernelGGL(k_centers_bwd<T>, dim3(gridn(Q)), dim3(kBlk), 0, s, w.quads, Q, Vd, C, (const T*)g_vfeat,
-                         gvf);
+      hipLaunchKernelGGL(k_centers_bwd<T>, dim3(gridn(4 * Q)), dim3(kBlk), 0, s, b.k1, b.v1, 4 * Q, Vd, C,
+                         (const T*)g_vfeat, gvf);
       A3D_LAUNCH_CHECK();
     }
   }
@@ -1299,11 +1434,25 @@ int dmc_backward_t(const DmcWs& w, const int64_t* sizes, int64_t N, int64_t M, c
                        (const T*)g_L_dev, gvd, (T*)w.sgs);
     A3D_LAUNCH_CHECK();
   }
+  T* ra = (g_alpha && f.alpha) ? (T*)b.ra : nullptr;
   hipLaunchKernelGGL(k_entry_bwd<T>, dim3(gridn(E)), dim3(kBlk), 0, s, g, w.totals, f, w.ent_vd, (const T*)w.aux,
                      (const T*)gvd, (const T*)w.sgs, (const T*)vfeat, (const T*)gvf, (const T*)g_L_dev,
-                     (const T*)g_p_alpha, (const T*)g_n_alpha, (T*)g_sdf, (T*)g_deform, (T*)g_alpha, (T*)g_beta,
-                     (T*)g_feat);
+                     (const T*)g_p_alpha, (const T*)g_n_alpha, (T*)b.rec, b.k0, b.v0, ra, g_deform != nullptr,
+                     (T*)g_beta, (T*)g_feat);
   A3D_LAUNCH_CHECK();
+  if (E > 0) {
+    tb = b.tmp_bytes;
+    A3D_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(b.tmp, tb, b.k0, b.k1, b.v0, b.v1, (int)(2 * E), 0,
+                                                     key_bits(M), s));
+    hipLaunchKernelGGL(k_corner_sum<T>, dim3(gridn(2 * E)), dim3(kBlk), 0, s, b.k1, b.v1, 2 * E,
+                       (const T*)b.rec, (T*)g_sdf, (T*)g_deform);
+    A3D_LAUNCH_CHECK();
+    if (ra) {
+      hipLaunchKernelGGL(k_alpha_sum<T>, dim3(gridn(E)), dim3(kBlk), 0, s, w.ent_slot, E, (const T*)ra,
+                         w.surf_cube, (T*)g_alpha);
+      A3D_LAUNCH_CHECK();
+    }
+  }
   return A3D_OK;
 }
 }  // namespace
@@ -1443,27 +1592,46 @@ extern "C" int a3d_dmc_forward_qef(void* ws, int64_t N, const int64_t* sizes, co
                                    (const double*)cgrad, qef_reg_scale, verts, L_dev, p_alpha, n_alpha, vfeat, s);
 }
 
+extern "C" int a3d_dmc_bwd_scratch_bytes(const int64_t* sizes, int dtype, int64_t* bytes) {
+  if (!sizes || !bytes) return fail(A3D_ERR_ARG, "a3d_dmc_bwd_scratch_bytes: null argument");
+  if (dtype != 0 && dtype != 1) return fail(A3D_ERR_ARG, "a3d_dmc_bwd_scratch_bytes: dtype %d not in {0, 1}", dtype);
+  if (sizes[2] < 0 || sizes[4] < 0 || 2 * sizes[2] > INT32_MAX || 4 * sizes[4] > INT32_MAX)
+    return fail(A3D_ERR_ARG, "a3d_dmc_bwd_scratch_bytes: sizes out of range");
+  *bytes = carve_bwd(nullptr, sizes, dtype ? 8 : 4).total;
+  return A3D_OK;
+}
+
 extern "C" int a3d_dmc_backward(const void* ws, int64_t N, int64_t M, const int64_t* sizes, const int32_t* vox,
                                 const int32_t* ci, const void* sdf, const void* deform, const void* alpha,
                                 const void* beta, const void* feat, int C, const void* vfeat, int dtype,
                                 double isovalue, double weight_scale, int res, const void* g_verts,
                                 const void* g_L_dev, const void* g_p_alpha, const void* g_n_alpha,
-                                const void* g_vfeat, void* gvf_scratch, void* g_sdf, void* g_deform, void* g_alpha,
-                                void* g_beta, void* g_feat, a3d_stream_t stream) {
+                                const void* g_vfeat, void* gvf_scratch, void* scratch, int64_t scratch_bytes,
+                                void* g_sdf, void* g_deform, void* g_alpha, void* g_beta, void* g_feat,
+                                a3d_stream_t stream) {
   if (!ws || !sizes || !g_sdf) return fail(A3D_ERR_ARG, "a3d_dmc_backward: null workspace, sizes or g_sdf");
   if (dtype != 0 && dtype != 1) return fail(A3D_ERR_ARG, "a3d_dmc_backward: dtype %d not in {0, 1}", dtype);
   if (sizes[1] > 0 && (!vox || !ci || !sdf)) return fail(A3D_ERR_ARG, "a3d_dmc_backward: null input");
   if (g_vfeat && (!feat || !vfeat || !gvf_scratch || C <= 0))
     return fail(A3D_ERR_ARG, "a3d_dmc_backward: feature gradient needs feat, vfeat, scratch and C > 0");
+  if (M > (int64_t(1) << 32) || sizes[1] > (int64_t(1) << 31) || 2 * sizes[2] > INT32_MAX ||
+      4 * sizes[4] > INT32_MAX)
+    return fail(A3D_ERR_ARG, "a3d_dmc_backward: sizes exceed the 32-bit sort (M %lld, Vd %lld, E %lld, Q %lld)",
+                (long long)M, (long long)sizes[1], (long long)sizes[2], (long long)sizes[4]);
+  const int64_t need = carve_bwd(nullptr, sizes, dtype ? 8 : 4).total;
+  if (sizes[1] > 0 && (!scratch || scratch_bytes < need))
+    return fail(A3D_ERR_ARG, "a3d_dmc_backward: scratch of %lld bytes, need %lld (a3d_dmc_bwd_scratch_bytes)",
+                (long long)scratch_bytes, (long long)need);
+  const BwdScratch b = carve_bwd(scratch, sizes, dtype ? 8 : 4);
   DmcWs w = carve((void*)ws, N);
   Geom g{vox, ci, w.surf_cube, w.ent_slot, w.vd_off, w.quads};
   hipStream_t s = (hipStream_t)stream;
   if (dtype == 0)
-    return dmc_backward_t<float>(w, sizes, N, M, g,
+    return dmc_backward_t<float>(w, b, sizes, N, M, g,
                                  make_fields<float>(sdf, deform, alpha, beta, feat, C, isovalue, weight_scale, res),
                                  vfeat, g_verts, g_L_dev, g_p_alpha, g_n_alpha, g_vfeat, gvf_scratch, g_sdf, g_deform,
                                  g_alpha, g_beta, g_feat, s);
-  return dmc_backward_t<double>(w, sizes, N, M, g,
+  return dmc_backward_t<double>(w, b, sizes, N, M, g,
                                 make_fields<double>(sdf, deform, alpha, beta, feat, C, isovalue, weight_scale, res),
                                 vfeat, g_verts, g_L_dev, g_p_alpha, g_n_alpha, g_vfeat, gvf_scratch, g_sdf, g_deform,
                                 g_alpha, g_beta, g_feat, s);

@@ -74,7 +74,8 @@ SIGNATURES: dict[str, list] = {
     "a3d_dmc_forward": [_vp, _i64, _pi64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i32, _f64, _f64, _i32, _vp,
                         _vp, _vp, _vp, _vp, _vp],
     "a3d_dmc_backward": [_vp, _i64, _i64, _pi64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _i32, _f64, _f64,
-                         _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+                         _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp],
+    "a3d_dmc_bwd_scratch_bytes": [_pi64, _i32, _pi64],
     "a3d_dmc_topology_get": [_vp, _i64, _pi64, _i32, _vp, _vp],
     "a3d_dmc_corner_grads": [_vp, _i64, _i64, _vp, _vp, _vp, _vp, _i32, _f64, _i32, _vp, _vp, _vp],
     "a3d_dmc_forward_qef": [_vp, _i64, _pi64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i32, _f64, _f64, _i32,

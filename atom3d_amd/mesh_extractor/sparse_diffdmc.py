@@ -172,11 +172,15 @@ class DMCTopology:
         g_be = torch.empty((self.N, 12), dtype=T, device=dev) if need[3] else None
         g_ft = torch.empty((self.M, C), dtype=T, device=dev) if need[4] else None
         scratch = torch.empty((max(self.n_vd, 1), C), dtype=T, device=dev) if gF is not None else None
+        nb = ctypes.c_int64()
+        _lib.call("a3d_dmc_bwd_scratch_bytes", self.sizes, self.code, ctypes.byref(nb))
+        work = torch.empty((nb.value,), dtype=torch.uint8, device=dev)
         _lib.call("a3d_dmc_backward", _lib.ptr(self.ws), self.N, self.M, self.sizes, _lib.ptr(self.vox),
                   _lib.ptr(self.ci), _lib.ptr(sdf), _lib.ptr(deform), _lib.ptr(alpha), _lib.ptr(beta),
                   _lib.ptr(feat), C, _lib.ptr(vf), self.code, self.iso, self.wscale, self.res, _lib.ptr(gV),
-                  _lib.ptr(gL), _lib.ptr(gP), _lib.ptr(gN), _lib.ptr(gF), _lib.ptr(scratch), _lib.ptr(g_sdf),
-                  _lib.ptr(g_def), _lib.ptr(g_al), _lib.ptr(g_be), _lib.ptr(g_ft), _lib.stream(dev))
+                  _lib.ptr(gL), _lib.ptr(gP), _lib.ptr(gN), _lib.ptr(gF), _lib.ptr(scratch), _lib.ptr(work),
+                  nb.value, _lib.ptr(g_sdf), _lib.ptr(g_def), _lib.ptr(g_al), _lib.ptr(g_be), _lib.ptr(g_ft),
+                  _lib.stream(dev))
         return (g_sdf if need[0] else None), g_def, g_al, g_be, g_ft
 
 

@@ -295,7 +295,11 @@ int a3d_flood_dam_mask(const uint32_t* state, int D, int H, int W, const int64_t
  *   the same workspace with the same inputs (which leaves each dual vertex,
  *   its weight sum and mean crossing distance there); the interpolation
  *   weights are recomputed, not stored.  gvf_scratch: [Vd,C] values, needed
- *   with g_vfeat.
+ *   with g_vfeat.  scratch: scratch_bytes >= a3d_dmc_bwd_scratch_bytes(sizes,
+ *   dtype) of caller memory (per-entry corner contributions and the sort
+ *   buffers that replace atomics: every gradient except g_feat is summed in
+ *   a fixed order, so repeated calls give identical bits).
+ * a3d_dmc_bwd_scratch_bytes: the backward's scratch for a topology's sizes.
  * a3d_dmc_topology_get: int32 copies of the topology (kind 0 surface cube
  *   ids, 1 case ids, 2 entry -> dual vertex, 3 surface edges [.,2], 4 quads
  *   [.,4]).
@@ -326,8 +330,9 @@ int a3d_dmc_backward(const void* ws, int64_t N, int64_t M, const int64_t* sizes,
                      const int32_t* ci, const void* sdf, const void* deform, const void* alpha, const void* beta,
                      const void* feat, int C, const void* vfeat, int dtype, double isovalue, double weight_scale,
                      int res, const void* g_verts, const void* g_L_dev, const void* g_p_alpha, const void* g_n_alpha,
-                     const void* g_vfeat, void* gvf_scratch, void* g_sdf, void* g_deform, void* g_alpha,
-                     void* g_beta, void* g_feat, a3d_stream_t stream);
+                     const void* g_vfeat, void* gvf_scratch, void* scratch, int64_t scratch_bytes, void* g_sdf,
+                     void* g_deform, void* g_alpha, void* g_beta, void* g_feat, a3d_stream_t stream);
+int a3d_dmc_bwd_scratch_bytes(const int64_t* sizes, int dtype, int64_t* bytes);
 int a3d_dmc_topology_get(const void* ws, int64_t N, const int64_t* sizes, int kind, int32_t* out,
                          a3d_stream_t stream);
 int a3d_dmc_corner_grads(void* ws, int64_t N, int64_t M, const int32_t* vox, const int32_t* ci, const void* sdf,

@@ -5,7 +5,9 @@ reference's vectors (tests/golden/dmc_fixtures.npz), the CPU oracle
 Bars: faces (the topology) bit-exact; float32 values within 1e-6 of the
 [-1,1]^3 bounding-box diagonal; float64 values within 1e-10; gradients
 within 1e-4 (float32) / 1e-9 (float64) of the largest reference gradient
-(the adjoint scatters with atomics, so sums are reordered)."""
+(the adjoint sums in entry order, autograd in its own order).  The adjoint's
+sums run in a fixed order, so repeated backward passes agree bit for bit
+(except feature gradients, which use atomics)."""
 import math
 import os
 
@@ -95,6 +97,49 @@ def test_larger_band_matches_oracle(res, isovalue):
     for k in ["vertices", "L_dev", "p_alpha", "n_alpha"]:
         err = (getattr(o, k).cpu() - ref[k]).abs().max().item()
         assert err <= 1e-6 * DIAG, (k, err)
+
+
+@pytest.mark.parametrize("tag", ["f32", "f64"])
+def test_adjoint_on_larger_band_matches_oracle_and_is_deterministic(tag):
+    """A 64^3 band (thousands of corners shared by up to 8 cubes, so the
+    sorted per-corner and per-vertex sums have long runs): gradients of every
+    input against the oracle's autograd in float64, and two backward passes
+    of the HIP adjoint bit-identical."""
+    print(f"dmc adjoint band tag={tag}", flush=True)
+    dt = torch.float64 if tag == "f64" else torch.float32
+    res = 64
+    cubes, s, ci = grid_band(lambda P: shapes_sdf(P, sdf_shapes(6, 3, seed=7)), res, 2 * 2.0 / res)
+    rng = np.random.default_rng(40)
+    N, M = len(cubes), len(s)
+    host = [s, rng.uniform(-0.01, 0.01, (M, 3)), rng.standard_normal((N, 8)), rng.standard_normal((N, 12))]
+    names = ["sdf", "deform", "alpha", "beta"]
+
+    def run(dev, dtype, mod):
+        a = [torch.tensor(x, dtype=dtype, device=dev, requires_grad=True) for x in host]
+        kw = dict(deform=a[1], alpha=a[2], beta=a[3])
+        if mod is None:
+            o = dmc_forward(torch.tensor(cubes), a[0], torch.tensor(ci), res, dtype=dtype, **kw)
+            outs = {k: o[k] for k in OUTS[:4]}
+        else:
+            o = mod(torch.tensor(cubes, device=dev), a[0], torch.tensor(ci, device=dev), res, **kw)
+            outs = {k: getattr(o, k) for k in OUTS[:4]}
+        return a, outs
+
+    ref_a, ref_o = run("cpu", torch.float64, None)
+    W = {k: torch.tensor(np.random.default_rng(1).standard_normal(tuple(v.shape))) for k, v in ref_o.items()}
+    sum((W[k] * v).sum() for k, v in ref_o.items()).backward()
+    m = SparseDiffDMC(device=DEV, dtype=dt)
+    grads = []
+    for _ in range(2):
+        a, o = run(DEV, dt, m)
+        sum((W[k].to(DEV, dt) * v).sum() for k, v in o.items()).backward()
+        grads.append([x.grad.detach().clone() for x in a])
+    assert len(ref_o["L_dev"]) > 15000
+    for name, g0, g1, r in zip(names, grads[0], grads[1], ref_a):
+        assert torch.equal(g0, g1), name
+        ref = r.grad.numpy()
+        tol = (1e-9 if tag == "f64" else 1e-4) * max(1.0, np.abs(ref).max())
+        np.testing.assert_allclose(g0.cpu().double().numpy(), ref, rtol=0, atol=tol, err_msg=name)
 
 
 def test_adjoint_matches_fp64_central_differences_on_gpu():
