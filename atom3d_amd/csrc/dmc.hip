@@ -105,6 +105,14 @@ struct DmcWs {
 
 inline int64_t al256(int64_t x) { return (x + 255) & ~int64_t(255); }
 
+// Bits of the largest key n - 1 (at least 1, at most 63): radix sorts of keys
+// below n only need that many bits.
+inline int key_bits(int64_t n) {
+  int b = 1;
+  while (b < 63 && (int64_t(1) << b) < n) ++b;
+  return b;
+}
+
 size_t cub_need(int64_t N) {
   int n = (int)(N > 1 ? N : 2), n12 = (int)(12 * (N > 1 ? N : 2)), n4 = (int)(4 * (N > 1 ? N : 2));
   size_t b = 0, t = 0;
@@ -299,7 +307,7 @@ template <typename T>
 __global__ void k_hot_write(const T* __restrict__ sdf, T iso, const int32_t* __restrict__ ci,
                             const int32_t* __restrict__ surf_cube, const int64_t* __restrict__ counters,
                             const int32_t* __restrict__ hot_off, uint64_t* __restrict__ key, int32_t* __restrict__ val,
-                            int32_t* __restrict__ totals) {
+                            int32_t* __restrict__ totals, int kb) {
   int64_t i = blockIdx.x * (int64_t)kBlk + threadIdx.x;
   int64_t Ns = counters[0];
   if (i == 0) totals[2] = hot_off[Ns];
@@ -310,7 +318,7 @@ __global__ void k_hot_write(const T* __restrict__ sdf, T iso, const int32_t* __r
     int64_t c0 = ci[(int64_t)cube * 8 + kE0[e]], c1 = ci[(int64_t)cube * 8 + kE1[e]];
     bool a = field(sdf, c0, iso) < T(0), b = field(sdf, c1, iso) < T(0);
     if (a != b) {
-      key[o] = ((uint64_t)(uint32_t)c0 << 32) | (uint64_t)(uint32_t)c1;
+      key[o] = ((uint64_t)c0 << kb) | (uint64_t)c1;  // corner ids < 2^kb: (c0, c1) order in 2 kb bits
       val[o] = (int32_t)(i * 12 + e);
       ++o;
     }
@@ -329,7 +337,7 @@ __global__ void k_unique_flags(const uint64_t* __restrict__ key, const int32_t* 
 __global__ void k_unique_map(const uint64_t* __restrict__ key, const int32_t* __restrict__ val,
                              const int32_t* __restrict__ urank, int32_t* __restrict__ totals,
                              int32_t* __restrict__ sedge_of_slot, int32_t* __restrict__ sedge,
-                             int32_t* __restrict__ qflag) {
+                             int32_t* __restrict__ qflag, int kb) {
   int64_t j = blockIdx.x * (int64_t)kBlk + threadIdx.x;
   int64_t S = totals[2];
   if (j == 0) totals[3] = urank[S];
@@ -340,8 +348,8 @@ __global__ void k_unique_map(const uint64_t* __restrict__ key, const int32_t* __
   sedge_of_slot[val[j]] = u;
   int is_q = 0;
   if (first) {
-    sedge[2 * u] = (int32_t)(key[j] >> 32);
-    sedge[2 * u + 1] = (int32_t)(key[j] & 0xffffffffu);
+    sedge[2 * u] = (int32_t)(key[j] >> kb);
+    sedge[2 * u + 1] = (int32_t)(key[j] & ((uint64_t(1) << kb) - 1));
     int64_t run = 1;
     while (j + run < S && key[j + run] == key[j] && run < 5) ++run;
     is_q = run == 4;
@@ -570,13 +578,13 @@ __global__ void k_centers(const int32_t* __restrict__ quads, int64_t Q, int64_t 
 // corner1), the value is the slot.  Sorting gives the lexicographic unique
 // edges of torch.unique(dim=0) as runs.
 __global__ void k_cg_keys(const int32_t* __restrict__ ci, int64_t N, uint64_t* __restrict__ key,
-                          int32_t* __restrict__ val) {
+                          int32_t* __restrict__ val, int kb) {
   int64_t t = blockIdx.x * (int64_t)kBlk + threadIdx.x;
   if (t >= 12 * N) return;
   const int64_t cube = t / 12;
   const int e = (int)(t % 12);
   const uint32_t c0 = (uint32_t)ci[cube * 8 + kE0[e]], c1 = (uint32_t)ci[cube * 8 + kE1[e]];
-  key[t] = ((uint64_t)c0 << 32) | c1;
+  key[t] = ((uint64_t)c0 << kb) | c1;  // corner ids < 2^kb: (c0, c1) order in 2 kb bits
   val[t] = (int32_t)t;
 }
 
@@ -603,13 +611,14 @@ __global__ void k_cg_heads(const uint64_t* __restrict__ key, int64_t n, int32_t*
 __global__ void k_cg_compact(const uint64_t* __restrict__ key, const int32_t* __restrict__ val,
                              const int32_t* __restrict__ flag, const int32_t* __restrict__ rank, int64_t n,
                              uint64_t* __restrict__ ekey, int32_t* __restrict__ eslot, uint32_t* __restrict__ k1,
-                             int32_t* __restrict__ v1) {
+                             int32_t* __restrict__ v1, int kb) {
   int64_t j = blockIdx.x * (int64_t)kBlk + threadIdx.x;
   if (j >= n || !flag[j]) return;
   const int32_t u = rank[j];
-  ekey[u] = key[j];
+  const uint64_t lo = key[j] & ((uint64_t(1) << kb) - 1);
+  ekey[u] = ((key[j] >> kb) << 32) | lo;  // from here on: first corner << 32 | second corner
   eslot[u] = val[j];
-  k1[u] = (uint32_t)(key[j] & 0xffffffffu);
+  k1[u] = (uint32_t)lo;
   v1[u] = u;
 }
 
@@ -900,43 +909,94 @@ __global__ void k_quad_pairs(const int32_t* __restrict__ quads, int64_t n4, uint
   val[t] = (int32_t)(t >> 2);
 }
 
-// Each quad centre is the mean of its 4 dual vertices: one thread per run of
-// equal vertices in the sorted pairs adds the run's quarter centre
-// gradients, in quad order, to gdst[v] (which holds g_vertices[v]).
+// Segmented sums over sorted (key, value) pairs, one pair per lane: after
+// seg_suffix, each lane holds the sum of its run's values from itself to the
+// run's last lane in this wavefront (runs are contiguous in sorted order, so
+// equal keys at distance `off` mean the whole span is one run); the run's
+// first lane then holds the in-wave total, summed in a fixed tree order.
 template <typename T>
-__global__ void k_centers_bwd(const uint32_t* __restrict__ key, const int32_t* __restrict__ val, int64_t n,
-                              int64_t Vd, int C, const T* __restrict__ gsrc, T* __restrict__ gdst) {
-  int64_t j = blockIdx.x * (int64_t)kBlk + threadIdx.x;
-  if (j >= n) return;
-  const uint32_t v = key[j];
-  if (j > 0 && key[j - 1] == v) return;
-  int64_t end = j + 1;
-  while (end < n && key[end] == v) ++end;
-  for (int c = 0; c < C; ++c) {
-    T acc = gdst[(int64_t)v * C + c];
-    for (int64_t q = j; q < end; ++q) acc += gsrc[(Vd + val[q]) * C + c] / T(4);
-    gdst[(int64_t)v * C + c] = acc;
+__device__ __forceinline__ void seg_suffix(uint32_t key, int lane, T& v) {
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t ko = __shfl_down(key, off);
+    const T vo = __shfl_down(v, off);
+    if (lane + off < 64 && ko == key) v += vo;
   }
 }
 
-// One thread per run of equal corners in the sorted (corner, entry side)
-// pairs: g_sdf[c] and g_deform[c] = the sums of the run's contributions
-// (rec: 4 values per entry side, sdf then xyz).
+// Whether sorted pair j (lane `lane`, key k, valid) starts a run, and whether
+// that run goes on past this wavefront (then the rest is added serially).
+__device__ __forceinline__ void run_bounds(const uint32_t* __restrict__ key, int64_t j, int64_t n, int lane,
+                                           uint32_t k, bool valid, bool& head, bool& spill) {
+  uint32_t prev = __shfl_up(k, 1);
+  if (lane == 0) prev = (valid && j > 0) ? key[j - 1] : ~k;
+  head = valid && (j == 0 || prev != k);
+  const uint32_t last = __shfl(k, 63);
+  const int64_t next = j - lane + 64;  // first pair of the next wavefront
+  spill = head && last == k && next < n && key[next] == k;
+}
+
+// Each quad centre is the mean of its 4 dual vertices: the sorted (vertex,
+// quad) pairs' quarter centre gradients are summed per vertex run and added
+// to gdst[v] (which holds g_vertices[v]).
 template <typename T>
-__global__ void k_corner_sum(const uint32_t* __restrict__ key, const int32_t* __restrict__ val, int64_t n,
-                             const T* __restrict__ rec, T* __restrict__ g_sdf, T* __restrict__ g_deform) {
-  int64_t j = blockIdx.x * (int64_t)kBlk + threadIdx.x;
-  if (j >= n) return;
-  const uint32_t c = key[j];
-  if (j > 0 && key[j - 1] == c) return;
-  T a = T(0), x = T(0), y = T(0), z = T(0);
-  for (int64_t q = j; q < n && key[q] == c; ++q) {
-    const T* r = rec + (int64_t)val[q] * 4;
-    a += r[0];
-    x += r[1];
-    y += r[2];
-    z += r[3];
+__global__ __launch_bounds__(kBlk) void k_centers_bwd(const uint32_t* __restrict__ key,
+                                                      const int32_t* __restrict__ val, int64_t n, int64_t Vd, int C,
+                                                      const T* __restrict__ gsrc, T* __restrict__ gdst) {
+  const int64_t j = blockIdx.x * (int64_t)kBlk + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  if (j - lane >= n) return;  // whole wavefront past the end
+  const bool valid = j < n;
+  const uint32_t v = valid ? key[j] : 0xffffffffu;  // keys are dual vertex ids < 2^31
+  const int64_t q = valid ? val[j] : 0;
+  bool head, spill;
+  run_bounds(key, j, n, lane, v, valid, head, spill);
+  for (int c = 0; c < C; ++c) {
+    T acc = valid ? gsrc[(Vd + q) * C + c] / T(4) : T(0);
+    seg_suffix(v, lane, acc);
+    if (head) {
+      if (spill)
+        for (int64_t r = j - lane + 64; r < n && key[r] == v; ++r) acc += gsrc[(Vd + val[r]) * C + c] / T(4);
+      gdst[(int64_t)v * C + c] += acc;
+    }
   }
+}
+
+// Sorted (corner, entry side) pairs: g_sdf[c] and g_deform[c] = the sums of
+// each corner run's contributions (rec: 4 values per entry side, sdf then
+// xyz).  Corners with no pair keep the zero written before.
+template <typename T>
+__global__ __launch_bounds__(kBlk) void k_corner_sum(const uint32_t* __restrict__ key,
+                                                     const int32_t* __restrict__ val, int64_t n,
+                                                     const T* __restrict__ rec, T* __restrict__ g_sdf,
+                                                     T* __restrict__ g_deform) {
+  const int64_t j = blockIdx.x * (int64_t)kBlk + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  if (j - lane >= n) return;
+  const bool valid = j < n;
+  const uint32_t c = valid ? key[j] : 0xffffffffu;  // keys are corner ids < 2^31
+  T a = T(0), x = T(0), y = T(0), z = T(0);
+  if (valid) {
+    const T* r = rec + (int64_t)val[j] * 4;
+    a = r[0];
+    x = r[1];
+    y = r[2];
+    z = r[3];
+  }
+  bool head, spill;
+  run_bounds(key, j, n, lane, c, valid, head, spill);
+  seg_suffix(c, lane, a);
+  seg_suffix(c, lane, x);
+  seg_suffix(c, lane, y);
+  seg_suffix(c, lane, z);
+  if (!head) return;
+  if (spill)
+    for (int64_t q = j - lane + 64; q < n && key[q] == c; ++q) {
+      const T* r = rec + (int64_t)val[q] * 4;
+      a += r[0];
+      x += r[1];
+      y += r[2];
+      z += r[3];
+    }
   g_sdf[c] = a;
   if (g_deform) {
     g_deform[(int64_t)c * 3] = x;
@@ -1200,6 +1260,8 @@ extern "C" int a3d_dmc_topology(const int32_t* vox, const void* sdf, int dtype, 
   const int64_t R = res;
   unsigned gN = gridn(N);
   size_t tbytes;
+  const int kb = key_bits(M);                                         // corner ids
+  const int vb = R <= 2097151 ? key_bits(R * R * R) : 64;             // voxel keys (x R + y) R + z
 #define A3D_DMC_T(T)                                                                                              \
   do {                                                                                                            \
     const T* S = (const T*)sdf;                                                                                   \
@@ -1227,7 +1289,7 @@ extern "C" int a3d_dmc_topology(const int32_t* vox, const void* sdf, int dtype, 
   unsigned gS = gridn(Ns + 1);
   if (Np > 0) {
     tbytes = w.cub_bytes;
-    A3D_HIP_CHECK(hipcub::DeviceRadixSort::SortKeys(w.cub, tbytes, w.pkey1, w.pkey0, (int)Np, 0, 64, s));
+    A3D_HIP_CHECK(hipcub::DeviceRadixSort::SortKeys(w.cub, tbytes, w.pkey1, w.pkey0, (int)Np, 0, vb, s));
     hipLaunchKernelGGL(k_check, dim3(gS), dim3(kBlk), 0, s, tb, w.surf_cube, w.counters, w.case0, vox, R, w.pkey0,
                        w.case_s);
     A3D_LAUNCH_CHECK();
@@ -1261,7 +1323,7 @@ extern "C" int a3d_dmc_topology(const int32_t* vox, const void* sdf, int dtype, 
     tbytes = w.cub_bytes;                                                                                           \
     A3D_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(w.cub, tbytes, w.hot_cnt, w.hot_off, (int)(Ns + 1), s));         \
     hipLaunchKernelGGL(k_hot_write<T>, dim3(gS), dim3(kBlk), 0, s, S, iso, ci, w.surf_cube, w.counters, w.hot_off,  \
-                       w.skey0, w.sval0, w.totals);                                                                 \
+                       w.skey0, w.sval0, w.totals, kb);                                                             \
     A3D_LAUNCH_CHECK();                                                                                             \
   } while (0)
   if (dtype == 0) A3D_DMC_T2(float); else A3D_DMC_T2(double);
@@ -1270,14 +1332,15 @@ extern "C" int a3d_dmc_topology(const int32_t* vox, const void* sdf, int dtype, 
   A3D_HIP_CHECK(hipMemcpyAsync(&S_slots, w.totals + 2, 4, hipMemcpyDeviceToHost, s));
   A3D_HIP_CHECK(hipStreamSynchronize(s));
   tbytes = w.cub_bytes;
-  A3D_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(w.cub, tbytes, w.skey0, w.skey1, w.sval0, w.sval1, S_slots, 0, 64, s));
+  A3D_HIP_CHECK(
+      hipcub::DeviceRadixSort::SortPairs(w.cub, tbytes, w.skey0, w.skey1, w.sval0, w.sval1, S_slots, 0, 2 * kb, s));
   unsigned gH = gridn((int64_t)S_slots + 1);
   hipLaunchKernelGGL(k_unique_flags, dim3(gH), dim3(kBlk), 0, s, w.skey1, w.totals, w.uflag);
   A3D_LAUNCH_CHECK();
   tbytes = w.cub_bytes;
   A3D_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(w.cub, tbytes, w.uflag, w.urank, S_slots + 1, s));
   hipLaunchKernelGGL(k_unique_map, dim3(gH), dim3(kBlk), 0, s, w.skey1, w.sval1, w.urank, w.totals, w.sedge_of_slot,
-                     w.sedge, w.qflag);
+                     w.sedge, w.qflag, kb);
   A3D_LAUNCH_CHECK();
   tbytes = w.cub_bytes;
   A3D_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(w.cub, tbytes, w.qflag, w.qrank, S_slots + 1, s));
@@ -1358,12 +1421,6 @@ struct BwdScratch {
   size_t tmp_bytes;
   int64_t total;
 };
-
-int key_bits(int64_t n) {  // bits of the largest key n - 1 (at least 1)
-  int b = 1;
-  while (b < 32 && (int64_t(1) << b) < n) ++b;
-  return b;
-}
 
 BwdScratch carve_bwd(void* base, const int64_t* sizes, int tsize) {
   const int64_t E = sizes[2], Q = sizes[4];
@@ -1491,10 +1548,10 @@ int dmc_corner_grads_t(const DmcWs& w, int64_t N, int64_t M, const int32_t* vox,
   if (N > 0) {
     const int64_t n12 = 12 * N;
     // unique edges of all cubes, lexicographic (corner0, corner1)
-    hipLaunchKernelGGL(k_cg_keys, dim3(gridn(n12)), dim3(kBlk), 0, s, ci, N, w.skey0, w.sval0);
+    const int kb = key_bits(M);
+    hipLaunchKernelGGL(k_cg_keys, dim3(gridn(n12)), dim3(kBlk), 0, s, ci, N, w.skey0, w.sval0, kb);
     A3D_LAUNCH_CHECK();
-    int end_bit = 33;
-    while (end_bit < 64 && ((uint64_t)(M - 1) >> (end_bit - 32)) != 0) ++end_bit;
+    const int end_bit = 2 * kb;
     size_t tbytes = w.cub_bytes;
     A3D_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(w.cub, tbytes, w.skey0, w.skey1, w.sval0, w.sval1, (int)n12, 0,
                                                      end_bit, s));
@@ -1507,7 +1564,7 @@ int dmc_corner_grads_t(const DmcWs& w, int64_t N, int64_t M, const int32_t* vox,
     uint32_t* k1 = (uint32_t*)w.qflag;
     uint32_t* k1s = (uint32_t*)w.skey1;
     hipLaunchKernelGGL(k_cg_compact, dim3(gridn(n12)), dim3(kBlk), 0, s, w.skey1, w.sval1, w.uflag, w.urank, n12,
-                       w.skey0, w.sval0, k1, w.qrank);
+                       w.skey0, w.sval0, k1, w.qrank, kb);
     A3D_LAUNCH_CHECK();
     A3D_HIP_CHECK(hipMemcpyAsync(&U, w.urank + n12, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     A3D_HIP_CHECK(hipStreamSynchronize(s));
