@@ -527,6 +527,174 @@ __global__ __launch_bounds__(kBlk) void k_udf_lane(const void* __restrict__ blob
   write_result(q, p, best, best_id, best_tri, tris, dist, face, closest, uvw);
 }
 
+// ------------------------------------------------------------ split per-lane walk
+// For few queries (flag A3D_UDF_SPLIT): 4 lanes per query, lane s of a group
+// walking the subtree under the root's s-th grandchild (or the root's leaf
+// child), so 4x as many wavefronts hide the walk's latency.  Each lane first
+// descends the whole tree greedily (nearer child, no stack) and tests that
+// leaf exactly, which bounds the query before the sub-walks start.  The
+// lanes of a query publish their best squared distances in LDS and prune
+// with the group's minimum: any lane's best is an upper bound on the
+// query's distance, so the pruning stays conservative.  The result is the group's
+// minimum (distance bits, face id) key: the same closest face, ties to the
+// lowest index, as one walk over the whole tree.
+__global__ __launch_bounds__(kBlk) void k_udf_split(const void* __restrict__ blob, const float* __restrict__ P,
+                                                     const uint32_t* __restrict__ perm, int64_t K,
+                                                     float* __restrict__ dist, int32_t* __restrict__ face,
+                                                     float* __restrict__ closest, float* __restrict__ uvw,
+                                                     bool use_obbs) {
+  __shared__ int32_t s_ref[kLaneLds * kBlk];
+  __shared__ float s_d2[kLaneLds * kBlk];
+  __shared__ float s_best[kBlk];  // lane 4j + s: sub-walk s's best of the block's query j
+  const int tid = threadIdx.x;
+  const int sub = tid & 3;
+  const int64_t i = ((int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlk + tid) >> 2;
+  const bool valid = i < K;
+  const int64_t q = valid ? (perm ? (int64_t)perm[i] : i) : 0;
+  const BvhHeader* H = header(blob);
+  const Node64* __restrict__ nodes = nodes_of(blob);
+  const Tri48* __restrict__ tris = tris_of(blob);
+  const v3 p = valid ? mk(P[q * 3], P[q * 3 + 1], P[q * 3 + 2]) : mk(0.0f, 0.0f, 0.0f);
+  const float slack = query_slack(p, H->coord_max);
+  const Obb96* __restrict__ obbs = use_obbs ? obbs_of(blob) : nullptr;
+  const float eq = obb_eq(p);
+  volatile float* grp = s_best + (tid & ~3);
+  grp[sub] = 1e30f;
+  wave_sync();
+
+  float best = 1e30f;
+  int32_t best_id = 0x7fffffff, best_tri = -1;
+  float thr = valid ? prune_thr(best, slack) : -1.0f;
+  if (valid) {  // greedy descent of the whole tree: an exact upper bound for every sub-walk
+    int32_t g = 0;
+    while (g >= 0) {
+      const Node64 nd = nodes[g];
+      const float dl = box_d2(p, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y);
+      const float dr = box_d2(p, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w);
+      g = dl <= dr ? nd.d.x : nd.d.y;
+      if (!(fminf(dl, dr) <= thr)) { g = 0; break; }
+    }
+    if (g < 0) {
+      const int32_t s0 = leaf_start(g), n = leaf_count(g);
+      for (int32_t t = s0; t < s0 + n; ++t) {
+        v3 ta, tb, tc;
+        int32_t id;
+        load_tri(tris, t, ta, tb, tc, id);
+        const float d2 = tri_dist2(ta, tb, tc, p);
+        if (d2 < best || (d2 == best && id < best_id)) {
+          best = d2;
+          best_id = id;
+          best_tri = t;
+        }
+      }
+      thr = prune_thr(best, slack);
+      grp[sub] = best;
+    }
+  }
+  // start: child (sub >> 1) of the root; if internal, its child (sub & 1)
+  const Node64 root = nodes[0];
+  int32_t cur = (sub >> 1) ? root.d.y : root.d.x;
+  float start_d = (sub >> 1) ? box_d2(p, root.b.z, root.b.w, root.c.x, root.c.y, root.c.z, root.c.w)
+                             : box_d2(p, root.a.x, root.a.y, root.a.z, root.a.w, root.b.x, root.b.y);
+  bool active = valid;
+  if (cur >= 0) {
+    const Node64 c = nodes[cur];
+    start_d = (sub & 1) ? box_d2(p, c.b.z, c.b.w, c.c.x, c.c.y, c.c.z, c.c.w)
+                        : box_d2(p, c.a.x, c.a.y, c.a.z, c.a.w, c.b.x, c.b.y);
+    cur = (sub & 1) ? c.d.y : c.d.x;
+  } else if (sub & 1) {
+    active = false;  // a leaf child of the root belongs to the even lane
+  }
+  if (!(start_d <= thr)) active = false;  // empty boxes (a mesh without faces) included
+
+  int32_t ovf_ref[kLaneOvf];
+  float ovf_d2[kLaneOvf];
+  int sp = 0;
+  while (active) {
+    {
+      const float g = fminf(fminf(grp[0], grp[1]), fminf(grp[2], grp[3]));
+      thr = fminf(thr, prune_thr(g, slack));
+    }
+    if (cur >= 0) {
+      const Node64* nd = nodes + cur;
+      float4 a = nd->a, b = nd->b, c = nd->c;
+      int4 d = nd->d;
+      float dl = box_d2(p, a.x, a.y, a.z, a.w, b.x, b.y);
+      float dr = box_d2(p, b.z, b.w, c.x, c.y, c.z, c.w);
+      bool tl = dl <= thr, tr = dr <= thr;
+      if (obbs && (tl || tr)) {
+        const Obb96* ob = obbs + cur;
+        dl = fmaxf(dl, obb_d2(p, ob->c[0], ob->c[1], ob->c[2], eq));
+        dr = fmaxf(dr, obb_d2(p, ob->c[3], ob->c[4], ob->c[5], eq));
+        tl = dl <= thr;
+        tr = dr <= thr;
+      }
+      if (tl && tr) {
+        int32_t near_ref = d.x, far_ref = d.y;
+        float far_d = dr;
+        if (dr < dl) { near_ref = d.y; far_ref = d.x; far_d = dl; }
+        if (sp < kLaneLds) {
+          s_ref[sp * kBlk + tid] = far_ref;
+          s_d2[sp * kBlk + tid] = far_d;
+        } else if (sp < kStackTotal) {
+          ovf_ref[sp - kLaneLds] = far_ref;
+          ovf_d2[sp - kLaneLds] = far_d;
+        }
+        sp = sp < kStackTotal ? sp + 1 : sp;
+        cur = near_ref;
+        continue;
+      }
+      if (tl) { cur = d.x; continue; }
+      if (tr) { cur = d.y; continue; }
+    } else {
+      const int32_t s0 = leaf_start(cur), n = leaf_count(cur);
+      for (int32_t t = s0; t < s0 + n; ++t) {
+        v3 ta, tb, tc;
+        int32_t id;
+        load_tri(tris, t, ta, tb, tc, id);
+        if (tri_box_d2(ta, tb, tc, tri_pad_of(tris, t), p) > thr) continue;
+        float d2 = tri_dist2(ta, tb, tc, p);
+        if (d2 < best || (d2 == best && id < best_id)) {
+          best = d2;
+          best_id = id;
+          best_tri = t;
+          thr = fminf(thr, prune_thr(best, slack));
+          grp[sub] = best;
+        }
+      }
+    }
+    bool found = false;
+    while (sp > 0) {
+      --sp;
+      int32_t r;
+      float dd;
+      if (sp < kLaneLds) {
+        r = s_ref[sp * kBlk + tid];
+        dd = s_d2[sp * kBlk + tid];
+      } else {
+        r = ovf_ref[sp - kLaneLds];
+        dd = ovf_d2[sp - kLaneLds];
+      }
+      if (dd <= thr) { cur = r; found = true; break; }
+    }
+    if (!found) break;
+  }
+  // the group's minimum key; every lane takes part in the shuffles
+  unsigned long long key = best_key(best, best_id);
+  int32_t bt = best_tri;
+  for (int o = 1; o < 4; o <<= 1) {
+    const unsigned long long ok = __shfl_xor(key, o);
+    const int32_t obt = __shfl_xor(bt, o);
+    if (ok < key) {
+      key = ok;
+      bt = obt;
+    }
+  }
+  if (valid && sub == 0)
+    write_result(q, p, __uint_as_float((uint32_t)(key >> 32)), (int32_t)(uint32_t)(key & 0xffffffffull), bt, tris,
+                 dist, face, closest, uvw);
+}
+
 // ------------------------------------------------------------ query ordering
 __device__ __forceinline__ uint32_t f2ord(float f) {
   uint32_t u = __float_as_uint(f);
@@ -675,17 +843,25 @@ namespace {
 int udf_launch(const void* blob, const float* P, int64_t K, const uint32_t* perm, const float* seed, float* dist,
                int32_t* face, float* closest, float* uvw, int flags, int64_t* counters, a3d_stream_t stream) {
   if (!blob) return fail(A3D_ERR_ARG, "a3d_bvh_udf: null BVH blob");
-  if (K < 0 || K > (int64_t(1) << 40)) return fail(A3D_ERR_ARG, "a3d_bvh_udf: K=%lld out of range", (long long)K);
+  if (K < 0 || K > (int64_t(1) << 38)) return fail(A3D_ERR_ARG, "a3d_bvh_udf: K=%lld out of range", (long long)K);
   if (perm && K > 0xffffffffll) return fail(A3D_ERR_ARG, "a3d_bvh_udf: perm needs K < 2^32");
-  if (flags & ~(A3D_UDF_LANE | A3D_UDF_DIRECT | A3D_UDF_NO_OBBS))
+  if (flags & ~(A3D_UDF_LANE | A3D_UDF_DIRECT | A3D_UDF_NO_OBBS | A3D_UDF_SPLIT))
     return fail(A3D_ERR_ARG, "a3d_bvh_udf: unknown flags 0x%x", flags);
   const bool use_obbs = !(flags & A3D_UDF_NO_OBBS);
-  if (counters && (flags & A3D_UDF_LANE)) return fail(A3D_ERR_ARG, "a3d_bvh_udf: counters need the wave walk");
+  if (counters && (flags & (A3D_UDF_LANE | A3D_UDF_SPLIT)))
+    return fail(A3D_ERR_ARG, "a3d_bvh_udf: counters need the wave walk");
+  if ((flags & A3D_UDF_SPLIT) && (seed || (flags & (A3D_UDF_LANE | A3D_UDF_DIRECT))))
+    return fail(A3D_ERR_ARG, "a3d_bvh_udf: A3D_UDF_SPLIT takes no seed and no other walk flag");
+  if ((flags & A3D_UDF_SPLIT) && K > (int64_t(1) << 36))
+    return fail(A3D_ERR_ARG, "a3d_bvh_udf: A3D_UDF_SPLIT needs K <= 2^36");
   if (K == 0) return A3D_OK;
   if (!P || !dist || !face || !closest) return fail(A3D_ERR_ARG, "a3d_bvh_udf: null input/output");
   hipStream_t s = (hipStream_t)stream;
   unsigned long long* c = (unsigned long long*)counters;
-  if (flags & A3D_UDF_LANE)
+  if (flags & A3D_UDF_SPLIT)
+    hipLaunchKernelGGL(k_udf_split, dim3(grid_for(4 * K)), dim3(kBlk), 0, s, blob, P, perm, K, dist, face, closest,
+                       uvw, use_obbs);
+  else if (flags & A3D_UDF_LANE)
     hipLaunchKernelGGL(k_udf_lane, dim3(grid_for(K)), dim3(kBlk), 0, s, blob, P, perm, K, seed, dist, face, closest,
                        uvw, use_obbs);
   else if (flags & A3D_UDF_DIRECT) {

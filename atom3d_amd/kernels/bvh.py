@@ -160,7 +160,9 @@ class BVHAccelerator:
     UDF_LANE = 1
     UDF_DIRECT = 2
     UDF_NO_OBBS = 4
+    UDF_SPLIT = 8
     ORDER_MIN_QUERIES = 4096
+    SPLIT_MAX_QUERIES = 1 << 15
     LANE_MAX_QUERIES = 1 << 17
 
     def query_order(self, points: torch.Tensor) -> torch.Tensor:
@@ -193,10 +195,15 @@ class BVHAccelerator:
         closest = torch.empty(K, 3, dtype=torch.float32, device=self.device)
         uvw = torch.empty(K, 3, dtype=torch.float32, device=self.device)
         if flags is None:
-            # few queries: the per-lane walk in input order is latency-bound
-            # at a lower floor than the packet walk (whose packets are sparse)
-            flags = self.UDF_LANE if K <= self.LANE_MAX_QUERIES and counters is None else 0
-            if flags == self.UDF_LANE and order is None:
+            # few queries: per-lane walks in input order are latency-bound at
+            # a lower floor than the packet walk (whose packets are sparse);
+            # the fewest get 4 lanes each (the split walk)
+            flags = 0
+            if counters is None and K <= self.SPLIT_MAX_QUERIES:
+                flags = self.UDF_SPLIT
+            elif counters is None and K <= self.LANE_MAX_QUERIES:
+                flags = self.UDF_LANE
+            if flags and order is None:
                 order = False
         if order is None:
             order = self.ORDER_MIN_QUERIES <= K < 2 ** 31

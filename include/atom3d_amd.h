@@ -97,6 +97,7 @@ int a3d_bvh_info(const void* blob, int64_t* out4, a3d_stream_t stream);
 #define A3D_UDF_LANE 1
 #define A3D_UDF_DIRECT 2 /* wave walk testing each needed triangle at once (no exact-test queue) */
 #define A3D_UDF_NO_OBBS 4 /* prune by the axis-aligned boxes only, ignoring the blob's oriented boxes (same results) */
+#define A3D_UDF_SPLIT 8 /* few queries: 4 lanes per query, each walking a quarter of the tree (same results; no seed) */
 int a3d_query_order_ws_bytes(int64_t K, int64_t* bytes);
 int a3d_query_order(const float* P, int64_t K, uint32_t* perm, void* ws, int64_t ws_bytes,
                     a3d_stream_t stream);

@@ -40,7 +40,7 @@ def test_version_and_argument_errors():
     with pytest.raises(RuntimeError, match="null BVH blob"):
         _lib.call("a3d_bvh_udf", None, None, 5, None, None, None, None, None, 0, None, None)
     with pytest.raises(RuntimeError, match="unknown flags"):
-        _lib.call("a3d_bvh_udf", 1, None, 5, None, None, None, None, None, 8, None, None)
+        _lib.call("a3d_bvh_udf", 1, None, 5, None, None, None, None, None, 16, None, None)
     with pytest.raises(RuntimeError, match="level"):
         _lib.call("a3d_octree_level", 1, (ctypes.c_float * 6)(), 22, None, 0, 1, 1, 1, 1 << 20, None)
     _lib.call("a3d_bvh_sizes", 4096000, 4, ctypes.byref(b), ctypes.byref(w))

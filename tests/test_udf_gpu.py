@@ -26,7 +26,8 @@ def _gpu_udf(V, F, P, **kw):
 # flags: 0 wave walk, 1 per-lane walk, 2 wave walk without the exact-test
 # queue, +4 without the oriented-box bounds (axis-aligned boxes only)
 VARIANTS = [dict(order=True), dict(order=False), dict(order=True, flags=1), dict(order=False, flags=1),
-            dict(order=True, flags=2), dict(order=True, flags=4), dict(order=False, flags=5)]
+            dict(order=True, flags=2), dict(order=True, flags=4), dict(order=False, flags=5),
+            dict(order=False, flags=8), dict(order=True, flags=8), dict(order=False, flags=12)]
 
 
 def _assert_bitexact(V, F, P, label):
@@ -94,6 +95,28 @@ def test_udf_empty_mesh_and_empty_queries():
     b = BVHAccelerator(V.to(DEV), F.to(DEV))
     d, f, c, w = b.udf(torch.zeros(0, 3, device=DEV))
     assert d.shape == (0,) and f.shape == (0,) and c.shape == (0, 3)
+
+
+def test_split_walk_flag_checks():
+    """A3D_UDF_SPLIT walks without a seed and alone; other combinations are refused."""
+    from atom3d_amd.kernels import _lib
+
+    V, F = datasets.noisy_icosphere(subdivisions=2)
+    b = BVHAccelerator(V.to(DEV), F.to(DEV))
+    P = datasets.uniform_points(64, seed=1).to(DEV)
+    out = [torch.empty(64, device=DEV), torch.empty(64, dtype=torch.int32, device=DEV),
+           torch.empty(64, 3, device=DEV), torch.empty(64, 3, device=DEV)]
+    st = _lib.stream(torch.device(DEV))
+    for flags in (8 | 1, 8 | 2):
+        with pytest.raises(RuntimeError, match="A3D_UDF_SPLIT"):
+            _lib.call("a3d_bvh_udf", _lib.ptr(b.blob), _lib.ptr(P), 64, None, *[_lib.ptr(t) for t in out], flags,
+                      None, st)
+    with pytest.raises(RuntimeError, match="A3D_UDF_SPLIT"):
+        _lib.call("a3d_bvh_udf_seeded", _lib.ptr(b.blob), _lib.ptr(P), 64, None, _lib.ptr(out[0]),
+                  *[_lib.ptr(t) for t in out], 8, st)
+    with pytest.raises(RuntimeError, match="counters"):
+        _lib.call("a3d_bvh_udf", _lib.ptr(b.blob), _lib.ptr(P), 64, None, *[_lib.ptr(t) for t in out], 8,
+                  _lib.ptr(torch.zeros(5, dtype=torch.int64, device=DEV)), st)
 
 
 def test_udf_matches_reference_tree_and_fp64():
