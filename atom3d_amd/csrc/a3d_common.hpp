@@ -130,13 +130,15 @@ __host__ __device__ __forceinline__ v3 obb_t2(v3 n, v3 t1) { return cross3(n, t1
 // 4 ulp of |p|_1 covers the FP32 error of each a . p (|a_i| <= 1).
 __device__ __forceinline__ float obb_d2(v3 p, float4 u, float4 v, float4 w, float eq) {
   const v3 n = mk(u.x, u.y, u.z), t1 = mk(v.x, v.y, v.z), t2 = obb_t2(n, t1);
-  const float pn = (n.x * p.x + n.y * p.y) + n.z * p.z;
-  const float p1 = (t1.x * p.x + t1.y * p.y) + t1.z * p.z;
-  const float p2 = (t2.x * p.x + t2.y * p.y) + t2.z * p.z;
+  // fused multiply-adds: each dot product errs by at most 2 ulp of |p|_1
+  // (unit axes), within eq; the squared sum errs less than the 2^-18 margin
+  const float pn = fmaf(n.z, p.z, fmaf(n.y, p.y, n.x * p.x));
+  const float p1 = fmaf(t1.z, p.z, fmaf(t1.y, p.y, t1.x * p.x));
+  const float p2 = fmaf(t2.z, p.z, fmaf(t2.y, p.y, t2.x * p.x));
   const float dn = fmaxf(fmaxf(u.w - pn, pn - w.x) - eq, 0.0f);
   const float d1 = fmaxf(fmaxf(v.w - p1, p1 - w.y) - eq, 0.0f);
   const float d2 = fmaxf(fmaxf(w.z - p2, p2 - w.w) - eq, 0.0f);
-  return ((dn * dn + d1 * d1) + d2 * d2) * 0.999996185302734375f;  // x (1 - 2^-18)
+  return fmaf(d2, d2, fmaf(d1, d1, dn * dn)) * 0.999996185302734375f;  // x (1 - 2^-18)
 }
 __device__ __forceinline__ float obb_eq(v3 p) {
   return 4.0f * 5.9604645e-08f * (fabsf(p.x) + fabsf(p.y) + fabsf(p.z));
@@ -242,7 +244,9 @@ __device__ __forceinline__ float box_d2(v3 p, float lx, float ly, float lz, floa
   float dx = fmaxf(fmaxf(lx - p.x, 0.0f), p.x - hx);
   float dy = fmaxf(fmaxf(ly - p.y, 0.0f), p.y - hy);
   float dz = fmaxf(fmaxf(lz - p.z, 0.0f), p.z - hz);
-  return dx * dx + dy * dy + dz * dz;
+  // a pruning bound, not a result: fused multiply-adds (less rounding than
+  // the separate products and sums the slack of prune_thr is sized for)
+  return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
 }
 
 // Lower bound used to skip exact triangle tests: squared distance from p to
