@@ -135,9 +135,10 @@ __device__ __forceinline__ float obb_d2(v3 p, float4 u, float4 v, float4 w, floa
   const float pn = fmaf(n.z, p.z, fmaf(n.y, p.y, n.x * p.x));
   const float p1 = fmaf(t1.z, p.z, fmaf(t1.y, p.y, t1.x * p.x));
   const float p2 = fmaf(t2.z, p.z, fmaf(t2.y, p.y, t2.x * p.x));
-  const float dn = fmaxf(fmaxf(u.w - pn, pn - w.x) - eq, 0.0f);
-  const float d1 = fmaxf(fmaxf(v.w - p1, p1 - w.y) - eq, 0.0f);
-  const float d2 = fmaxf(fmaxf(w.z - p2, p2 - w.w) - eq, 0.0f);
+  // max(max(lo - a.p, a.p - hi), eq) - eq == max(max(lo - a.p, a.p - hi) - eq, 0)
+  const float dn = fmaxf(fmaxf(u.w - pn, pn - w.x), eq) - eq;
+  const float d1 = fmaxf(fmaxf(v.w - p1, p1 - w.y), eq) - eq;
+  const float d2 = fmaxf(fmaxf(w.z - p2, p2 - w.w), eq) - eq;
   return fmaf(d2, d2, fmaf(d1, d1, dn * dn)) * 0.999996185302734375f;  // x (1 - 2^-18)
 }
 __device__ __forceinline__ float obb_eq(v3 p) {
