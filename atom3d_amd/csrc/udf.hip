@@ -429,17 +429,20 @@ __device__ __forceinline__ void batch_walk_tile(const void* __restrict__ blob, c
 }
 
 // ------------------------------------------------------------ per-lane walk
+// redo (the face output, or null): walk only the queries whose face is -1
+// there, without a seed (the second pass of a seeded walk).
 __global__ __launch_bounds__(kBlk) void k_udf_lane(const void* __restrict__ blob, const float* __restrict__ P,
                                                     const uint32_t* __restrict__ perm, int64_t K,
                                                     const float* __restrict__ seed, float* __restrict__ dist, int32_t* __restrict__ face,
                                                     float* __restrict__ closest, float* __restrict__ uvw,
-                                                    bool use_obbs) {
+                                                    bool use_obbs, const int32_t* redo) {
   __shared__ int32_t s_ref[kLaneLds * kBlk];
   __shared__ float s_d2[kLaneLds * kBlk];
   const int tid = threadIdx.x;
   const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlk + tid;
   if (i >= K) return;
   const int64_t q = perm ? (int64_t)perm[i] : i;
+  if (redo && redo[q] >= 0) return;
   const BvhHeader* H = header(blob);
   const Node64* __restrict__ nodes = nodes_of(blob);
   const Tri48* __restrict__ tris = tris_of(blob);
@@ -845,8 +848,9 @@ int udf_launch(const void* blob, const float* P, int64_t K, const uint32_t* perm
   if (!blob) return fail(A3D_ERR_ARG, "a3d_bvh_udf: null BVH blob");
   if (K < 0 || K > (int64_t(1) << 38)) return fail(A3D_ERR_ARG, "a3d_bvh_udf: K=%lld out of range", (long long)K);
   if (perm && K > 0xffffffffll) return fail(A3D_ERR_ARG, "a3d_bvh_udf: perm needs K < 2^32");
-  if (flags & ~(A3D_UDF_LANE | A3D_UDF_DIRECT | A3D_UDF_NO_OBBS | A3D_UDF_SPLIT))
+  if (flags & ~(A3D_UDF_LANE | A3D_UDF_DIRECT | A3D_UDF_NO_OBBS | A3D_UDF_SPLIT | A3D_UDF_REDO))
     return fail(A3D_ERR_ARG, "a3d_bvh_udf: unknown flags 0x%x", flags);
+  if ((flags & A3D_UDF_REDO) && !seed) return fail(A3D_ERR_ARG, "a3d_bvh_udf: A3D_UDF_REDO needs seeds");
   const bool use_obbs = !(flags & A3D_UDF_NO_OBBS);
   if (counters && (flags & (A3D_UDF_LANE | A3D_UDF_SPLIT)))
     return fail(A3D_ERR_ARG, "a3d_bvh_udf: counters need the wave walk");
@@ -863,7 +867,7 @@ int udf_launch(const void* blob, const float* P, int64_t K, const uint32_t* perm
                        uvw, use_obbs);
   else if (flags & A3D_UDF_LANE)
     hipLaunchKernelGGL(k_udf_lane, dim3(grid_for(K)), dim3(kBlk), 0, s, blob, P, perm, K, seed, dist, face, closest,
-                       uvw, use_obbs);
+                       uvw, use_obbs, (const int32_t*)nullptr);
   else if (flags & A3D_UDF_DIRECT) {
     if (counters)
       hipLaunchKernelGGL((k_udf_wave<true, false>), dim3(grid_for(K)), dim3(kBlk), 0, s, blob, P, perm, K, seed, dist,
@@ -880,9 +884,88 @@ int udf_launch(const void* blob, const float* P, int64_t K, const uint32_t* perm
                          face, closest, uvw, c, use_obbs);
   }
   A3D_LAUNCH_CHECK();
+  if (flags & A3D_UDF_REDO) {  // queries whose seed was below their distance: walk them again without one
+    hipLaunchKernelGGL(k_udf_lane, dim3(grid_for(K)), dim3(kBlk), 0, s, blob, P, (const uint32_t*)nullptr, K,
+                       (const float*)nullptr, dist, face, closest, uvw, use_obbs, (const int32_t*)face);
+    A3D_LAUNCH_CHECK();
+  }
   return A3D_OK;
 }
+
+// Distance bounds from a grid of exact distances (see a3d_udf_grid_seed).
+__global__ __launch_bounds__(kBlk) void k_grid_seed(const float* __restrict__ G, const int32_t* __restrict__ GF,
+                                                    const float* __restrict__ V, const int32_t* __restrict__ F,
+                                                    int R, float lx, float ly, float lz,
+                                                    float h, const float* __restrict__ P,
+                                                    const uint32_t* __restrict__ perm, int64_t K, float margin,
+                                                    float* __restrict__ seed) {
+  const int64_t j = blockIdx.x * (int64_t)kBlk + threadIdx.x;
+  if (j >= K) return;
+  const int64_t i = perm ? (int64_t)perm[j] : j;  // the walk's order: neighbours share cells and faces
+  const float q[3] = {P[i * 3], P[i * 3 + 1], P[i * 3 + 2]};
+  const float lo[3] = {lx, ly, lz};
+  int c0[3];
+  for (int d = 0; d < 3; ++d) {
+    const float f = floorf((q[d] - lo[d]) / h);
+    c0[d] = f >= 0.0f ? (f < (float)(R - 1) ? (int)f : R - 1) : 0;  // NaN -> 0
+  }
+  float best = __int_as_float(0x7f800000), best_f2 = best, near_e2 = best;
+  int64_t near_g = 0;
+  for (int k = 0; k < 8; ++k) {
+    const int c[3] = {c0[0] + (k & 1), c0[1] + ((k >> 1) & 1), c0[2] + ((k >> 2) & 1)};
+    float e2 = 0.0f;
+    for (int d = 0; d < 3; ++d) {
+      const float e = q[d] - (lo[d] + (float)c[d] * h);  // the grid point as the caller placed it
+      e2 += e * e;
+    }
+    const int64_t g = ((int64_t)c[0] * (R + 1) + c[1]) * (R + 1) + c[2];
+    best = fminf(best, G[g] + sqrtf(e2));
+    if (e2 < near_e2) {
+      near_e2 = e2;
+      near_g = g;
+    }
+  }
+  if (GF) {  // the nearest corner's closest face, measured from q by the walk's own formula
+    const int32_t f = GF[near_g];
+    if (f >= 0) {
+      const int32_t* fv = F + 3 * (int64_t)f;
+      const v3 a = mk(V[3 * (int64_t)fv[0]], V[3 * (int64_t)fv[0] + 1], V[3 * (int64_t)fv[0] + 2]);
+      const v3 b = mk(V[3 * (int64_t)fv[1]], V[3 * (int64_t)fv[1] + 1], V[3 * (int64_t)fv[1] + 2]);
+      const v3 cc = mk(V[3 * (int64_t)fv[2]], V[3 * (int64_t)fv[2] + 1], V[3 * (int64_t)fv[2] + 2]);
+      best_f2 = tri_dist2(a, b, cc, mk(q[0], q[1], q[2]));
+    }
+  }
+  // x (1 + 2^-12): FP32 rounding of the sum and the grid; a face's own
+  // distance needs only the rounding of the square root
+  seed[i] = fminf(best * 1.000244140625f + margin, sqrtf(best_f2) * 1.00000095367431640625f);
+}
 }  // namespace
+
+// Upper bounds on the closest-point distance of each query from a grid of
+// exact distances: G[(i (R+1) + j) (R+1) + k] = the distance at lo + (i, j,
+// k) h (float: lo + (float)i * h), i, j, k in [0, R].  For a query q, the
+// cell holding q (clamped into the grid) gives seed = min over its 8 corners
+// c of G(c) + |q - c| (the distance is 1-Lipschitz), widened by 2^-12
+// relative and `margin` absolute, and, with grid_face (the closest face of
+// each grid point; V, F the mesh), the walk's own distance from q to the
+// nearest corner's face (a distance some face attains: usually far tighter).
+// perm (nullable): the order the walk takes the queries in.  A seed that still falls below a query's
+// distance only makes the seeded walk report face -1 for it, which
+// A3D_UDF_REDO walks again.
+extern "C" int a3d_udf_grid_seed(const float* grid, const int32_t* grid_face, const float* V, const int32_t* F,
+                                 int R, const float* lo3, float h, const float* P, const uint32_t* perm, int64_t K,
+                                 float margin, float* seed, a3d_stream_t stream) {
+  if (K < 0 || K > (int64_t(1) << 38)) return fail(A3D_ERR_ARG, "a3d_udf_grid_seed: K=%lld out of range", (long long)K);
+  if (R < 1 || R > 2048 || !(h > 0.0f) || !(margin >= 0.0f) || !lo3)
+    return fail(A3D_ERR_ARG, "a3d_udf_grid_seed: bad grid (R=%d, h=%g, margin=%g)", R, (double)h, (double)margin);
+  if (K == 0) return A3D_OK;
+  if (!grid || !P || !seed || (grid_face && (!V || !F)))
+    return fail(A3D_ERR_ARG, "a3d_udf_grid_seed: null argument");
+  hipLaunchKernelGGL(k_grid_seed, dim3(grid_for(K)), dim3(kBlk), 0, (hipStream_t)stream, grid, grid_face, V, F, R,
+                     lo3[0], lo3[1], lo3[2], h, P, perm, K, margin, seed);
+  A3D_LAUNCH_CHECK();
+  return A3D_OK;
+}
 
 extern "C" int a3d_bvh_udf(const void* blob, const float* P, int64_t K, const uint32_t* perm, float* dist,
                            int32_t* face, float* closest, float* uvw, int flags, int64_t* counters,

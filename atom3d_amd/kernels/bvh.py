@@ -161,7 +161,10 @@ class BVHAccelerator:
     UDF_DIRECT = 2
     UDF_NO_OBBS = 4
     UDF_SPLIT = 8
+    UDF_REDO = 16
     ORDER_MIN_QUERIES = 4096
+    SEED_MIN_QUERIES = 1 << 20
+    SEED_GRID_RES = 128
     SPLIT_MAX_QUERIES = 1 << 15
     LANE_MAX_QUERIES = 1 << 17
 
@@ -179,15 +182,41 @@ class BVHAccelerator:
                       _lib.stream(self.device))
         return perm
 
+    def seed_grid(self):
+        """(grid, grid_face, lo, h): exact distances and closest faces at the
+        (R+1)^3 points lo + (i, j, k) h (float32 arithmetic, R =
+        SEED_GRID_RES) of a cube holding the mesh with a 5 % margin, built
+        once per mesh by the plain walk; the source of the grid-seeded walk's
+        per-query distance bounds."""
+        if getattr(self, "_seed_grid", None) is None:
+            import numpy as np
+
+            R = self.SEED_GRID_RES
+            lo_b = self.vertices.min(dim=0).values.double()
+            hi_b = self.vertices.max(dim=0).values.double()
+            ext = float((hi_b - lo_b).max())
+            pad = 0.05 * ext + 1e-6
+            h = float(np.float32((ext + 2 * pad) / R))
+            lo = [float(np.float32(x)) for x in (lo_b - pad).tolist()]
+            i = torch.arange(R + 1, dtype=torch.float32, device=self.device)
+            axes = [torch.full_like(i, lo[d]) + i * torch.full_like(i, h) for d in range(3)]
+            pts = torch.stack(torch.meshgrid(*axes, indexing="ij"), dim=-1).reshape(-1, 3).contiguous()
+            d, f, _, _ = self.udf(pts, seed=False)
+            self._seed_grid = (d, f, lo, h)
+        return self._seed_grid
+
     def udf(self, points: torch.Tensor, flags: int | None = None, order: bool | None = None,
-            counters: torch.Tensor | None = None):
+            counters: torch.Tensor | None = None, seed: bool | None = None):
         """Closest point on the mesh: (distances [K], face_ids [K] int32,
         closest_points [K,3], uvw [K,3]).  ``flags`` picks the traversal
         (default: the per-lane walk in input order up to 2^17 queries, else
         the packet walk in Morton order; UDF_LANE, UDF_DIRECT or 0 force one);
         ``order`` (default: K >= 4096) walks the queries in Morton order;
         ``counters`` (int64[5] on the device) accumulates the packet walk's
-        work counts.  Results are identical either way."""
+        work counts; ``seed`` (default: from 2^20 queries on a non-empty
+        mesh, packet walk) bounds each query's walk from the distance grid of
+        ``seed_grid`` and walks the few queries whose bound fell short again
+        without one.  Results are identical either way."""
         p = self._points(points)
         K = p.shape[0]
         dist = torch.empty(K, dtype=torch.float32, device=self.device)
@@ -207,12 +236,25 @@ class BVHAccelerator:
                 order = False
         if order is None:
             order = self.ORDER_MIN_QUERIES <= K < 2 ** 31
+        if seed is None:
+            seed = K >= self.SEED_MIN_QUERIES and self.num_faces > 0 and flags in (0, self.UDF_LANE) and counters is None
         perm = self.query_order(p) if order and K > 0 else None
         with torch.cuda.device(self.device):
-            _lib.call(
-                "a3d_bvh_udf", _lib.ptr(self.blob), _lib.ptr(p), K, _lib.ptr(perm), _lib.ptr(dist), _lib.ptr(face),
-                _lib.ptr(closest), _lib.ptr(uvw), int(flags), _lib.ptr(counters), _lib.stream(self.device),
-            )
+            st = _lib.stream(self.device)
+            if seed and K > 0:
+                grid, gface, lo, h = self.seed_grid()
+                bound = torch.empty(K, dtype=torch.float32, device=self.device)
+                _lib.call("a3d_udf_grid_seed", _lib.ptr(grid), _lib.ptr(gface), _lib.ptr(self.vertices),
+                          _lib.ptr(self.faces), self.SEED_GRID_RES, (ctypes.c_float * 3)(*lo), h, _lib.ptr(p),
+                          _lib.ptr(perm), K, h / 1024, _lib.ptr(bound), st)
+                _lib.call("a3d_bvh_udf_seeded", _lib.ptr(self.blob), _lib.ptr(p), K, _lib.ptr(perm), _lib.ptr(bound),
+                          _lib.ptr(dist), _lib.ptr(face), _lib.ptr(closest), _lib.ptr(uvw),
+                          int(flags) | self.UDF_REDO, st)
+            else:
+                _lib.call(
+                    "a3d_bvh_udf", _lib.ptr(self.blob), _lib.ptr(p), K, _lib.ptr(perm), _lib.ptr(dist),
+                    _lib.ptr(face), _lib.ptr(closest), _lib.ptr(uvw), int(flags), _lib.ptr(counters), st,
+                )
         return dist, face, closest, uvw
 
     # ------------------------------------------------------------ rays

@@ -135,15 +135,28 @@ def main():
     sp = stream.cuda_stream
     lib = _lib.lib()
     order_args = (_lib.ptr(P), K, _lib.ptr(perm), _lib.ptr(ws), wsb.value, sp)
-    udf_args = (_lib.ptr(bvh.blob), _lib.ptr(P), K, _lib.ptr(perm), _lib.ptr(dist), _lib.ptr(face),
-                _lib.ptr(closest), _lib.ptr(uvw), 0, None, sp)
+    # the distance grid that bounds each walk: built once per mesh, like the tree
+    torch.cuda.synchronize()
+    tg = time.perf_counter()
+    grid, gface, glo, gh = bvh.seed_grid()
+    torch.cuda.synchronize()
+    grid_s = time.perf_counter() - tg
+    bound = torch.empty(K, device=dev)
+    seed_args = (_lib.ptr(grid), _lib.ptr(gface), _lib.ptr(bvh.vertices), _lib.ptr(bvh.faces), bvh.SEED_GRID_RES,
+                 (ctypes.c_float * 3)(*glo), ctypes.c_float(gh), _lib.ptr(P), _lib.ptr(perm), K,
+                 ctypes.c_float(gh / 1024), _lib.ptr(bound), sp)
+    udf_args = (_lib.ptr(bvh.blob), _lib.ptr(P), K, _lib.ptr(perm), _lib.ptr(bound), _lib.ptr(dist), _lib.ptr(face),
+                _lib.ptr(closest), _lib.ptr(uvw), bvh.UDF_REDO, sp)
 
-    # one step = Morton ordering of the queries + the wave-coherent walk
+    # one step = Morton ordering of the queries, the grid bounds and the
+    # bounded wave-coherent walk (plus the walk of any query whose bound fell
+    # short); timed as the walk: bounds + walk
     def step(ev=None):
         _lib.check(lib.a3d_query_order(*order_args), "a3d_query_order")
         if ev is not None:
             ev[0].record(stream)
-        _lib.check(lib.a3d_bvh_udf(*udf_args), "a3d_bvh_udf")
+        _lib.check(lib.a3d_udf_grid_seed(*seed_args), "a3d_udf_grid_seed")
+        _lib.check(lib.a3d_bvh_udf_seeded(*udf_args), "a3d_bvh_udf_seeded")
         if ev is not None:
             ev[1].record(stream)
 
@@ -215,7 +228,9 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "plain BVH UDF: union of 200 icospheres (sub 5, seed 1, 4,096,000 faces), "
-                        f"{K} uniform queries in [-1,1]^3; outputs dist, face, closest point, uvw",
+                        f"{K} uniform queries in [-1,1]^3; outputs dist, face, closest point, uvw; each "
+                        f"walk bounded by a {bvh.SEED_GRID_RES + 1}^3 grid of exact distances built once per "
+                        "mesh, like the tree (exact results)",
             "faces": int(F.shape[0]),
             "queries": K,
             "parallelism": "single GPU",
@@ -229,12 +244,14 @@ def main():
             "traffic": traffic,
             "kernel": "k_udf_wave",
             "kernel_ms": kern_ms,
+            "timed_launches": "k_grid_seed + k_udf_wave + the redo pass (k_udf_lane, queries with short bounds)",
             "bytes_per_query": bq,
             "n_node": n_node,
             "n_tri": n_tri,
         },
         "cpu_baseline": cpu,
         "bvh_build_s": build_s,
+        "seed_grid_build_s": grid_s,
         "surrogate_fallback": sg,
     }
     print(json.dumps(out), flush=True)

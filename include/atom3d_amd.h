@@ -98,6 +98,7 @@ int a3d_bvh_info(const void* blob, int64_t* out4, a3d_stream_t stream);
 #define A3D_UDF_DIRECT 2 /* wave walk testing each needed triangle at once (no exact-test queue) */
 #define A3D_UDF_NO_OBBS 4 /* prune by the axis-aligned boxes only, ignoring the blob's oriented boxes (same results) */
 #define A3D_UDF_SPLIT 8 /* few queries: 4 lanes per query, each walking a quarter of the tree (same results; no seed) */
+#define A3D_UDF_REDO 16 /* a3d_bvh_udf_seeded: walk the queries left at face -1 again without a seed */
 int a3d_query_order_ws_bytes(int64_t K, int64_t* bytes);
 int a3d_query_order(const float* P, int64_t K, uint32_t* perm, void* ws, int64_t ws_bytes,
                     a3d_stream_t stream);
@@ -109,10 +110,25 @@ int a3d_bvh_udf(const void* blob, const float* P, int64_t K, const uint32_t* per
  * with it as its best distance and never enters a box farther away.  Results
  * equal a3d_bvh_udf's wherever the true distance is <= seed[q]; elsewhere
  * face[q] = -1 and dist[q] = seed[q] (the caller re-queries those without a
- * seed).  flags: 0 or A3D_UDF_LANE.  No counterpart in the reference: the
- * surrogate router's exact mode. */
+ * seed, or passes A3D_UDF_REDO to have them walked again in the same call,
+ * which makes every result equal a3d_bvh_udf's).  flags: 0 or A3D_UDF_LANE,
+ * plus A3D_UDF_REDO.  No counterpart in the reference: the surrogate
+ * router's exact mode and the grid-seeded walk below.
+ * a3d_udf_grid_seed: seed[K] for a3d_bvh_udf_seeded from a grid of exact
+ * distances grid[(i (R+1) + j) (R+1) + k] at the points lo3 + (i, j, k) h
+ * (float: lo + (float)i * h; i, j, k in [0, R], R <= 2048): for each query
+ * the minimum over the 8 corners c of its cell (clamped into the grid) of
+ * grid(c) + |q - c|, widened by 2^-12 relative and margin absolute (the
+ * distance is 1-Lipschitz, so this bounds it up to FP32 rounding), and, when
+ * grid_face (int32, each grid point's closest face; V float32[.,3] and F
+ * int32[.,3] the mesh) is given, the distance from q to the nearest
+ * corner's face by the walk's own formula.  perm (nullable): process the
+ * queries in the walk's order (e.g. a3d_query_order's). */
 int a3d_bvh_udf_seeded(const void* blob, const float* P, int64_t K, const uint32_t* perm, const float* seed,
                        float* dist, int32_t* face, float* closest, float* uvw, int flags, a3d_stream_t stream);
+int a3d_udf_grid_seed(const float* grid, const int32_t* grid_face, const float* V, const int32_t* F, int R,
+                      const float* lo3, float h, const float* P, const uint32_t* perm, int64_t K, float margin,
+                      float* seed, a3d_stream_t stream);
 
 /* ------------------------------------------------------------------ UDF adjoint
  * Replaces the autograd re-evaluation in MeshBVH._udf_with_grad

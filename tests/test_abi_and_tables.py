@@ -40,9 +40,14 @@ def test_version_and_argument_errors():
     with pytest.raises(RuntimeError, match="null BVH blob"):
         _lib.call("a3d_bvh_udf", None, None, 5, None, None, None, None, None, 0, None, None)
     with pytest.raises(RuntimeError, match="unknown flags"):
-        _lib.call("a3d_bvh_udf", 1, None, 5, None, None, None, None, None, 16, None, None)
+        _lib.call("a3d_bvh_udf", 1, None, 5, None, None, None, None, None, 32, None, None)
     with pytest.raises(RuntimeError, match="level"):
         _lib.call("a3d_octree_level", 1, (ctypes.c_float * 6)(), 22, None, 0, 1, 1, 1, 1 << 20, None)
+    with pytest.raises(RuntimeError, match="bad grid"):
+        _lib.call("a3d_udf_grid_seed", None, None, None, None, 0, (ctypes.c_float * 3)(), 0.1, None, None, 5, 0.0,
+                  None, None)
+    with pytest.raises(RuntimeError, match="null argument"):
+        _lib.call("a3d_udf_grid_seed", 1, 1, None, None, 8, (ctypes.c_float * 3)(), 0.1, 1, None, 5, 0.0, 1, None)
     _lib.call("a3d_bvh_sizes", 4096000, 4, ctypes.byref(b), ctypes.byref(w))
     assert b.value == 256 + 64 * 4095999 + (48 + 32) * 4096000 + 96 * 4095999 + 48 * 4096000
 

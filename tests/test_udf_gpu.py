@@ -97,6 +97,58 @@ def test_udf_empty_mesh_and_empty_queries():
     assert d.shape == (0,) and f.shape == (0,) and c.shape == (0, 3)
 
 
+def test_grid_seeded_walk_equals_plain_walk():
+    """The default path from 2^20 queries bounds each walk from the exact
+    distance grid: bit-identical to the plain walk, inside and far outside
+    the grid; the grid's points hold the plain walk's distances."""
+    V, F = datasets.noisy_icosphere()
+    b = BVHAccelerator(V.to(DEV), F.to(DEV))
+    rng = np.random.default_rng(21)
+    P = np.concatenate([rng.uniform(-1, 1, (1 << 20, 3)), rng.uniform(-6, 6, (4096, 3)),
+                        V.numpy()[rng.integers(0, len(V), 4096)]]).astype(np.float32)
+    Pd = torch.tensor(P, device=DEV)
+    print(f"[udf grid seed] K={len(P)}", flush=True)
+    plain = b.udf(Pd, seed=False)
+    seeded = b.udf(Pd)
+    assert b.SEED_MIN_QUERIES <= len(P)
+    for x, y in zip(plain, seeded):
+        assert torch.equal(x, y)
+    grid, gface, lo, h = b.seed_grid()
+    R = b.SEED_GRID_RES
+    idx = torch.tensor([0, 1, R, (R + 1) ** 2 + 5, (R + 1) ** 3 - 1], device=DEV)
+    ijk = torch.stack([idx // (R + 1) ** 2, (idx // (R + 1)) % (R + 1), idx % (R + 1)], 1).float()
+    pts = torch.tensor(lo, device=DEV) + ijk * h
+    d, f, _, _ = b.udf(pts, seed=False)
+    assert torch.equal(grid[idx], d) and torch.equal(gface[idx], f)
+
+
+def test_seeded_walk_redo_fixes_short_seeds():
+    """A3D_UDF_REDO: seeds below the true distance are walked again, so every
+    result equals the plain walk's."""
+    from atom3d_amd.kernels import _lib
+
+    V, F = datasets.noisy_icosphere(subdivisions=4, seed=3)
+    b = BVHAccelerator(V.to(DEV), F.to(DEV))
+    P = datasets.uniform_points(1 << 16, seed=5).to(DEV)
+    K = P.shape[0]
+    ref = b.udf(P, seed=False)
+    rng = np.random.default_rng(2)
+    short = torch.tensor(rng.random(K) < 0.4, device=DEV)
+    bound = torch.where(short, ref[0] * 0.5, ref[0] * 1.01 + 1e-6)
+    out = [torch.empty(K, device=DEV), torch.empty(K, dtype=torch.int32, device=DEV),
+           torch.empty(K, 3, device=DEV), torch.empty(K, 3, device=DEV)]
+    perm = b.query_order(P)
+    for flags in (16, 16 | 1):
+        print(f"[udf redo] flags={flags}", flush=True)
+        _lib.call("a3d_bvh_udf_seeded", _lib.ptr(b.blob), _lib.ptr(P), K, _lib.ptr(perm), _lib.ptr(bound),
+                  *[_lib.ptr(t) for t in out], flags, _lib.stream(torch.device(DEV)))
+        for x, y in zip(ref, out):
+            assert torch.equal(x, y)
+    with pytest.raises(RuntimeError, match="REDO"):
+        _lib.call("a3d_bvh_udf", _lib.ptr(b.blob), _lib.ptr(P), K, None, *[_lib.ptr(t) for t in out], 16, None,
+                  _lib.stream(torch.device(DEV)))
+
+
 def test_split_walk_flag_checks():
     """A3D_UDF_SPLIT walks without a seed and alone; other combinations are refused."""
     from atom3d_amd.kernels import _lib
