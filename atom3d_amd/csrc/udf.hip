@@ -353,17 +353,77 @@ __device__ __forceinline__ void wave_walk(const Node64* __restrict__ nodes, cons
   }
 }
 
+// ------------------------------------------------------------ grid bounds
+// A grid of exact distances G and closest faces GF at lo + (i, j, k) h
+// (float: lo + (float)i * h; i, j, k in [0, R]), the mesh V, F for the faces.
+struct GridBound {
+  const float* G;
+  const int32_t* GF;
+  const float* V;
+  const int32_t* F;
+  int R;
+  float lo[3];
+  float h;
+  float margin;
+};
+
+// Upper bound on the distance from q: the minimum of (a) over the 8 corners
+// c of q's cell (clamped into the grid), G(c) + |q - c| (the distance is
+// 1-Lipschitz), widened by 2^-12 relative and `margin` absolute for FP32
+// rounding, and (b) with GF, the walk's own distance from q to the nearest
+// corner's face (a distance some face attains).
+__device__ __forceinline__ float grid_bound(const GridBound& g, const float q[3]) {
+  const int R = g.R;
+  int c0[3];
+  for (int d = 0; d < 3; ++d) {
+    const float f = floorf((q[d] - g.lo[d]) / g.h);
+    c0[d] = f >= 0.0f ? (f < (float)(R - 1) ? (int)f : R - 1) : 0;  // NaN -> 0
+  }
+  float best = __int_as_float(0x7f800000), best_f2 = best, near_e2 = best;
+  int64_t near_g = 0;
+  for (int k = 0; k < 8; ++k) {
+    const int c[3] = {c0[0] + (k & 1), c0[1] + ((k >> 1) & 1), c0[2] + ((k >> 2) & 1)};
+    float e2 = 0.0f;
+    for (int d = 0; d < 3; ++d) {
+      const float e = q[d] - (g.lo[d] + (float)c[d] * g.h);  // the grid point as the caller placed it
+      e2 += e * e;
+    }
+    const int64_t gi = ((int64_t)c[0] * (R + 1) + c[1]) * (R + 1) + c[2];
+    best = fminf(best, g.G[gi] + sqrtf(e2));
+    if (e2 < near_e2) {
+      near_e2 = e2;
+      near_g = gi;
+    }
+  }
+  if (g.GF) {
+    const int32_t f = g.GF[near_g];
+    if (f >= 0) {
+      const int32_t* fv = g.F + 3 * (int64_t)f;
+      const float* va = g.V + 3 * (int64_t)fv[0];
+      const float* vb = g.V + 3 * (int64_t)fv[1];
+      const float* vc = g.V + 3 * (int64_t)fv[2];
+      best_f2 = tri_dist2(mk(va[0], va[1], va[2]), mk(vb[0], vb[1], vb[2]), mk(vc[0], vc[1], vc[2]),
+                          mk(q[0], q[1], q[2]));
+    }
+  }
+  // x (1 + 2^-12): FP32 rounding of the sum and the grid; a face's own
+  // distance needs only the rounding of the square root
+  return fminf(best * 1.000244140625f + g.margin, sqrtf(best_f2) * 1.00000095367431640625f);
+}
+
 // seed (optional, indexed like the outputs): an upper bound on each query's
 // distance; the walk starts with it as the best distance, so it never enters
 // a box farther than the bound.  A query with no face within its bound gets
-// face -1 and dist = its seed.
+// face -1 and dist = its seed.  gb.G (when seed is null): the bound from a
+// grid, computed here (grid_bound).
 template <bool STATS, bool QUEUE>
 __global__ __launch_bounds__(kBlk) void k_udf_wave(const void* __restrict__ blob, const float* __restrict__ P,
                                                     const uint32_t* __restrict__ perm, int64_t K,
                                                     const float* __restrict__ seed,
                                                     float* __restrict__ dist, int32_t* __restrict__ face,
                                                     float* __restrict__ closest, float* __restrict__ uvw,
-                                                    unsigned long long* __restrict__ counters, bool use_obbs) {
+                                                    unsigned long long* __restrict__ counters, bool use_obbs,
+                                                    GridBound gb) {
   __shared__ float4 s_stk[kWaves][kWaveStack][2];  // (lo.xyz, ref bits), (hi.xyz, obb code)
   __shared__ WaveQ s_q[QUEUE ? kWaves : 1];
   const int lane = threadIdx.x & 63;
@@ -379,8 +439,9 @@ __global__ __launch_bounds__(kBlk) void k_udf_wave(const void* __restrict__ blob
   const float slack = query_slack(p, H->coord_max);
 
   float best = 1e30f;
-  if (seed && valid) {
-    const float sd = seed[q];
+  if (valid && (seed || gb.G)) {
+    const float q3[3] = {p.x, p.y, p.z};
+    const float sd = seed ? seed[q] : grid_bound(gb, q3);
     if (sd >= 0.0f && sd < 1e15f) best = fmaxf(sd * sd * (1.0f + 4.8e-7f), 1e-30f);
   }
   int32_t best_id = 0x7fffffff, best_tri = -1;
@@ -844,13 +905,17 @@ extern "C" int a3d_query_order(const float* P, int64_t K, uint32_t* perm, void* 
 
 namespace {
 int udf_launch(const void* blob, const float* P, int64_t K, const uint32_t* perm, const float* seed, float* dist,
-               int32_t* face, float* closest, float* uvw, int flags, int64_t* counters, a3d_stream_t stream) {
+               int32_t* face, float* closest, float* uvw, int flags, int64_t* counters, a3d_stream_t stream,
+               const GridBound* grid = nullptr) {
+  const GridBound gb = grid ? *grid : GridBound{};
   if (!blob) return fail(A3D_ERR_ARG, "a3d_bvh_udf: null BVH blob");
   if (K < 0 || K > (int64_t(1) << 38)) return fail(A3D_ERR_ARG, "a3d_bvh_udf: K=%lld out of range", (long long)K);
   if (perm && K > 0xffffffffll) return fail(A3D_ERR_ARG, "a3d_bvh_udf: perm needs K < 2^32");
   if (flags & ~(A3D_UDF_LANE | A3D_UDF_DIRECT | A3D_UDF_NO_OBBS | A3D_UDF_SPLIT | A3D_UDF_REDO))
     return fail(A3D_ERR_ARG, "a3d_bvh_udf: unknown flags 0x%x", flags);
-  if ((flags & A3D_UDF_REDO) && !seed) return fail(A3D_ERR_ARG, "a3d_bvh_udf: A3D_UDF_REDO needs seeds");
+  if ((flags & A3D_UDF_REDO) && !seed && !grid) return fail(A3D_ERR_ARG, "a3d_bvh_udf: A3D_UDF_REDO needs seeds");
+  if (grid && (flags & (A3D_UDF_LANE | A3D_UDF_SPLIT)))
+    return fail(A3D_ERR_ARG, "a3d_bvh_udf_grid: grid bounds need the wave walk");
   const bool use_obbs = !(flags & A3D_UDF_NO_OBBS);
   if (counters && (flags & (A3D_UDF_LANE | A3D_UDF_SPLIT)))
     return fail(A3D_ERR_ARG, "a3d_bvh_udf: counters need the wave walk");
@@ -871,17 +936,17 @@ int udf_launch(const void* blob, const float* P, int64_t K, const uint32_t* perm
   else if (flags & A3D_UDF_DIRECT) {
     if (counters)
       hipLaunchKernelGGL((k_udf_wave<true, false>), dim3(grid_for(K)), dim3(kBlk), 0, s, blob, P, perm, K, seed, dist,
-                         face, closest, uvw, c, use_obbs);
+                         face, closest, uvw, c, use_obbs, gb);
     else
       hipLaunchKernelGGL((k_udf_wave<false, false>), dim3(grid_for(K)), dim3(kBlk), 0, s, blob, P, perm, K, seed, dist,
-                         face, closest, uvw, c, use_obbs);
+                         face, closest, uvw, c, use_obbs, gb);
   } else {
     if (counters)
       hipLaunchKernelGGL((k_udf_wave<true, true>), dim3(grid_for(K)), dim3(kBlk), 0, s, blob, P, perm, K, seed, dist,
-                         face, closest, uvw, c, use_obbs);
+                         face, closest, uvw, c, use_obbs, gb);
     else
       hipLaunchKernelGGL((k_udf_wave<false, true>), dim3(grid_for(K)), dim3(kBlk), 0, s, blob, P, perm, K, seed, dist,
-                         face, closest, uvw, c, use_obbs);
+                         face, closest, uvw, c, use_obbs, gb);
   }
   A3D_LAUNCH_CHECK();
   if (flags & A3D_UDF_REDO) {  // queries whose seed was below their distance: walk them again without one
@@ -893,51 +958,14 @@ int udf_launch(const void* blob, const float* P, int64_t K, const uint32_t* perm
 }
 
 // Distance bounds from a grid of exact distances (see a3d_udf_grid_seed).
-__global__ __launch_bounds__(kBlk) void k_grid_seed(const float* __restrict__ G, const int32_t* __restrict__ GF,
-                                                    const float* __restrict__ V, const int32_t* __restrict__ F,
-                                                    int R, float lx, float ly, float lz,
-                                                    float h, const float* __restrict__ P,
-                                                    const uint32_t* __restrict__ perm, int64_t K, float margin,
+__global__ __launch_bounds__(kBlk) void k_grid_seed(GridBound gb, const float* __restrict__ P,
+                                                    const uint32_t* __restrict__ perm, int64_t K,
                                                     float* __restrict__ seed) {
   const int64_t j = blockIdx.x * (int64_t)kBlk + threadIdx.x;
   if (j >= K) return;
   const int64_t i = perm ? (int64_t)perm[j] : j;  // the walk's order: neighbours share cells and faces
   const float q[3] = {P[i * 3], P[i * 3 + 1], P[i * 3 + 2]};
-  const float lo[3] = {lx, ly, lz};
-  int c0[3];
-  for (int d = 0; d < 3; ++d) {
-    const float f = floorf((q[d] - lo[d]) / h);
-    c0[d] = f >= 0.0f ? (f < (float)(R - 1) ? (int)f : R - 1) : 0;  // NaN -> 0
-  }
-  float best = __int_as_float(0x7f800000), best_f2 = best, near_e2 = best;
-  int64_t near_g = 0;
-  for (int k = 0; k < 8; ++k) {
-    const int c[3] = {c0[0] + (k & 1), c0[1] + ((k >> 1) & 1), c0[2] + ((k >> 2) & 1)};
-    float e2 = 0.0f;
-    for (int d = 0; d < 3; ++d) {
-      const float e = q[d] - (lo[d] + (float)c[d] * h);  // the grid point as the caller placed it
-      e2 += e * e;
-    }
-    const int64_t g = ((int64_t)c[0] * (R + 1) + c[1]) * (R + 1) + c[2];
-    best = fminf(best, G[g] + sqrtf(e2));
-    if (e2 < near_e2) {
-      near_e2 = e2;
-      near_g = g;
-    }
-  }
-  if (GF) {  // the nearest corner's closest face, measured from q by the walk's own formula
-    const int32_t f = GF[near_g];
-    if (f >= 0) {
-      const int32_t* fv = F + 3 * (int64_t)f;
-      const v3 a = mk(V[3 * (int64_t)fv[0]], V[3 * (int64_t)fv[0] + 1], V[3 * (int64_t)fv[0] + 2]);
-      const v3 b = mk(V[3 * (int64_t)fv[1]], V[3 * (int64_t)fv[1] + 1], V[3 * (int64_t)fv[1] + 2]);
-      const v3 cc = mk(V[3 * (int64_t)fv[2]], V[3 * (int64_t)fv[2] + 1], V[3 * (int64_t)fv[2] + 2]);
-      best_f2 = tri_dist2(a, b, cc, mk(q[0], q[1], q[2]));
-    }
-  }
-  // x (1 + 2^-12): FP32 rounding of the sum and the grid; a face's own
-  // distance needs only the rounding of the square root
-  seed[i] = fminf(best * 1.000244140625f + margin, sqrtf(best_f2) * 1.00000095367431640625f);
+  seed[i] = grid_bound(gb, q);
 }
 }  // namespace
 
@@ -961,8 +989,8 @@ extern "C" int a3d_udf_grid_seed(const float* grid, const int32_t* grid_face, co
   if (K == 0) return A3D_OK;
   if (!grid || !P || !seed || (grid_face && (!V || !F)))
     return fail(A3D_ERR_ARG, "a3d_udf_grid_seed: null argument");
-  hipLaunchKernelGGL(k_grid_seed, dim3(grid_for(K)), dim3(kBlk), 0, (hipStream_t)stream, grid, grid_face, V, F, R,
-                     lo3[0], lo3[1], lo3[2], h, P, perm, K, margin, seed);
+  const GridBound gb{grid, grid_face, V, F, R, {lo3[0], lo3[1], lo3[2]}, h, margin};
+  hipLaunchKernelGGL(k_grid_seed, dim3(grid_for(K)), dim3(kBlk), 0, (hipStream_t)stream, gb, P, perm, K, seed);
   A3D_LAUNCH_CHECK();
   return A3D_OK;
 }
@@ -978,6 +1006,17 @@ extern "C" int a3d_bvh_udf_seeded(const void* blob, const float* P, int64_t K, c
                                   int flags, a3d_stream_t stream) {
   if (!seed && K > 0) return fail(A3D_ERR_ARG, "a3d_bvh_udf_seeded: null seed");
   return udf_launch(blob, P, K, perm, seed, dist, face, closest, uvw, flags, nullptr, stream);
+}
+
+extern "C" int a3d_bvh_udf_grid(const void* blob, const float* P, int64_t K, const uint32_t* perm,
+                                const float* grid, const int32_t* grid_face, const float* V, const int32_t* F, int R,
+                                const float* lo3, float h, float margin, float* dist, int32_t* face, float* closest,
+                                float* uvw, int flags, a3d_stream_t stream) {
+  if (R < 1 || R > 2048 || !(h > 0.0f) || !(margin >= 0.0f) || !lo3)
+    return fail(A3D_ERR_ARG, "a3d_bvh_udf_grid: bad grid (R=%d, h=%g, margin=%g)", R, (double)h, (double)margin);
+  if (K > 0 && (!grid || (grid_face && (!V || !F)))) return fail(A3D_ERR_ARG, "a3d_bvh_udf_grid: null grid argument");
+  const GridBound gb{grid, grid_face, V, F, R, {lo3[0], lo3[1], lo3[2]}, h, margin};
+  return udf_launch(blob, P, K, perm, nullptr, dist, face, closest, uvw, flags | A3D_UDF_REDO, nullptr, stream, &gb);
 }
 
 // ============================================================ batched walk

@@ -49,6 +49,8 @@ SIGNATURES: dict[str, list] = {
     "a3d_bvh_udf": [_vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp],
     "a3d_bvh_udf_seeded": [_vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp],
     "a3d_udf_grid_seed": [_vp, _vp, _vp, _vp, _i32, _vp, _f32, _vp, _vp, _i64, _f32, _vp, _vp],
+    "a3d_bvh_udf_grid": [_vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _f32, _f32, _vp, _vp, _vp, _vp, _i32,
+                         _vp],
     "a3d_point_dist": [_vp, _vp, _i64, _vp, _vp],
     "a3d_point_dist_backward": [_vp, _vp, _vp, _vp, _i64, _vp, _vp],
     "a3d_sdf_sign": [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],

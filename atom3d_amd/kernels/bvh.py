@@ -237,19 +237,16 @@ class BVHAccelerator:
         if order is None:
             order = self.ORDER_MIN_QUERIES <= K < 2 ** 31
         if seed is None:
-            seed = K >= self.SEED_MIN_QUERIES and self.num_faces > 0 and flags in (0, self.UDF_LANE) and counters is None
+            seed = K >= self.SEED_MIN_QUERIES and self.num_faces > 0 and flags == 0 and counters is None
         perm = self.query_order(p) if order and K > 0 else None
         with torch.cuda.device(self.device):
             st = _lib.stream(self.device)
             if seed and K > 0:
                 grid, gface, lo, h = self.seed_grid()
-                bound = torch.empty(K, dtype=torch.float32, device=self.device)
-                _lib.call("a3d_udf_grid_seed", _lib.ptr(grid), _lib.ptr(gface), _lib.ptr(self.vertices),
-                          _lib.ptr(self.faces), self.SEED_GRID_RES, (ctypes.c_float * 3)(*lo), h, _lib.ptr(p),
-                          _lib.ptr(perm), K, h / 1024, _lib.ptr(bound), st)
-                _lib.call("a3d_bvh_udf_seeded", _lib.ptr(self.blob), _lib.ptr(p), K, _lib.ptr(perm), _lib.ptr(bound),
-                          _lib.ptr(dist), _lib.ptr(face), _lib.ptr(closest), _lib.ptr(uvw),
-                          int(flags) | self.UDF_REDO, st)
+                _lib.call("a3d_bvh_udf_grid", _lib.ptr(self.blob), _lib.ptr(p), K, _lib.ptr(perm), _lib.ptr(grid),
+                          _lib.ptr(gface), _lib.ptr(self.vertices), _lib.ptr(self.faces), self.SEED_GRID_RES,
+                          (ctypes.c_float * 3)(*lo), h, h / 1024, _lib.ptr(dist), _lib.ptr(face), _lib.ptr(closest),
+                          _lib.ptr(uvw), int(flags), st)
             else:
                 _lib.call(
                     "a3d_bvh_udf", _lib.ptr(self.blob), _lib.ptr(p), K, _lib.ptr(perm), _lib.ptr(dist),

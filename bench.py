@@ -141,22 +141,19 @@ def main():
     grid, gface, glo, gh = bvh.seed_grid()
     torch.cuda.synchronize()
     grid_s = time.perf_counter() - tg
-    bound = torch.empty(K, device=dev)
-    seed_args = (_lib.ptr(grid), _lib.ptr(gface), _lib.ptr(bvh.vertices), _lib.ptr(bvh.faces), bvh.SEED_GRID_RES,
-                 (ctypes.c_float * 3)(*glo), ctypes.c_float(gh), _lib.ptr(P), _lib.ptr(perm), K,
-                 ctypes.c_float(gh / 1024), _lib.ptr(bound), sp)
-    udf_args = (_lib.ptr(bvh.blob), _lib.ptr(P), K, _lib.ptr(perm), _lib.ptr(bound), _lib.ptr(dist), _lib.ptr(face),
-                _lib.ptr(closest), _lib.ptr(uvw), bvh.UDF_REDO, sp)
+    udf_args = (_lib.ptr(bvh.blob), _lib.ptr(P), K, _lib.ptr(perm), _lib.ptr(grid), _lib.ptr(gface),
+                _lib.ptr(bvh.vertices), _lib.ptr(bvh.faces), bvh.SEED_GRID_RES, (ctypes.c_float * 3)(*glo),
+                ctypes.c_float(gh), ctypes.c_float(gh / 1024), _lib.ptr(dist), _lib.ptr(face), _lib.ptr(closest),
+                _lib.ptr(uvw), 0, sp)
 
-    # one step = Morton ordering of the queries, the grid bounds and the
-    # bounded wave-coherent walk (plus the walk of any query whose bound fell
-    # short); timed as the walk: bounds + walk
+    # one step = Morton ordering of the queries + the wave-coherent walk, each
+    # query's walk bounded from its start by the grid (plus the walk of any
+    # query whose bound fell short)
     def step(ev=None):
         _lib.check(lib.a3d_query_order(*order_args), "a3d_query_order")
         if ev is not None:
             ev[0].record(stream)
-        _lib.check(lib.a3d_udf_grid_seed(*seed_args), "a3d_udf_grid_seed")
-        _lib.check(lib.a3d_bvh_udf_seeded(*udf_args), "a3d_bvh_udf_seeded")
+        _lib.check(lib.a3d_bvh_udf_grid(*udf_args), "a3d_bvh_udf_grid")
         if ev is not None:
             ev[1].record(stream)
 
@@ -244,7 +241,8 @@ def main():
             "traffic": traffic,
             "kernel": "k_udf_wave",
             "kernel_ms": kern_ms,
-            "timed_launches": "k_grid_seed + k_udf_wave + the redo pass (k_udf_lane, queries with short bounds)",
+            "timed_launches": "k_udf_wave (grid bounds in its prologue) + the redo pass (k_udf_lane, queries "
+                              "whose bound fell short)",
             "bytes_per_query": bq,
             "n_node": n_node,
             "n_tri": n_tri,

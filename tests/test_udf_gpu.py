@@ -4,6 +4,8 @@ Bar: bit-exact distances, closest faces (ties to the lowest face index),
 closest points and barycentrics against the oracle's all-faces scan of the
 reference's FP32 maths; at full size, size-independent properties.
 """
+import ctypes
+
 import numpy as np
 import pytest
 import torch
@@ -120,6 +122,21 @@ def test_grid_seeded_walk_equals_plain_walk():
     pts = torch.tensor(lo, device=DEV) + ijk * h
     d, f, _, _ = b.udf(pts, seed=False)
     assert torch.equal(grid[idx], d) and torch.equal(gface[idx], f)
+    # the same bounds as a seed array (a3d_udf_grid_seed + a3d_bvh_udf_seeded with A3D_UDF_REDO)
+    from atom3d_amd.kernels import _lib
+
+    K = len(P)
+    perm = b.query_order(Pd)
+    bound = torch.empty(K, device=DEV)
+    st = _lib.stream(torch.device(DEV))
+    _lib.call("a3d_udf_grid_seed", _lib.ptr(grid), _lib.ptr(gface), _lib.ptr(b.vertices), _lib.ptr(b.faces), R,
+              (ctypes.c_float * 3)(*lo), h, _lib.ptr(Pd), _lib.ptr(perm), K, h / 1024, _lib.ptr(bound), st)
+    assert torch.all(bound >= plain[0] * (1 - 1e-6))
+    out = [torch.empty_like(t) for t in plain]
+    _lib.call("a3d_bvh_udf_seeded", _lib.ptr(b.blob), _lib.ptr(Pd), K, _lib.ptr(perm), _lib.ptr(bound),
+              *[_lib.ptr(t) for t in out], b.UDF_REDO, st)
+    for x, y in zip(plain, out):
+        assert torch.equal(x, y)
 
 
 def test_seeded_walk_redo_fixes_short_seeds():
