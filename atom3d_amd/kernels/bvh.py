@@ -183,18 +183,24 @@ class BVHAccelerator:
         return perm
 
     def seed_grid(self):
-        """(grid, grid_face, lo, h): exact distances and closest faces at the
+        """(grid, grid_face, lo, h), or None for a mesh with non-finite
+        coordinates: exact distances and closest faces at the
         (R+1)^3 points lo + (i, j, k) h (float32 arithmetic, R =
         SEED_GRID_RES) of a cube holding the mesh with a 5 % margin, built
         once per mesh by the plain walk; the source of the grid-seeded walk's
         per-query distance bounds."""
         if getattr(self, "_seed_grid", None) is None:
+            import math
+
             import numpy as np
 
             R = self.SEED_GRID_RES
             lo_b = self.vertices.min(dim=0).values.double()
             hi_b = self.vertices.max(dim=0).values.double()
             ext = float((hi_b - lo_b).max())
+            if not (math.isfinite(ext) and all(math.isfinite(x) for x in lo_b.tolist())):
+                self._seed_grid = False  # non-finite coordinates: the plain walk only
+                return None
             pad = 0.05 * ext + 1e-6
             h = float(np.float32((ext + 2 * pad) / R))
             lo = [float(np.float32(x)) for x in (lo_b - pad).tolist()]
@@ -203,7 +209,7 @@ class BVHAccelerator:
             pts = torch.stack(torch.meshgrid(*axes, indexing="ij"), dim=-1).reshape(-1, 3).contiguous()
             d, f, _, _ = self.udf(pts, seed=False)
             self._seed_grid = (d, f, lo, h)
-        return self._seed_grid
+        return self._seed_grid or None
 
     def udf(self, points: torch.Tensor, flags: int | None = None, order: bool | None = None,
             counters: torch.Tensor | None = None, seed: bool | None = None):
@@ -241,8 +247,9 @@ class BVHAccelerator:
         perm = self.query_order(p) if order and K > 0 else None
         with torch.cuda.device(self.device):
             st = _lib.stream(self.device)
-            if seed and K > 0:
-                grid, gface, lo, h = self.seed_grid()
+            sg = self.seed_grid() if seed and K > 0 else None
+            if sg is not None:
+                grid, gface, lo, h = sg
                 _lib.call("a3d_bvh_udf_grid", _lib.ptr(self.blob), _lib.ptr(p), K, _lib.ptr(perm), _lib.ptr(grid),
                           _lib.ptr(gface), _lib.ptr(self.vertices), _lib.ptr(self.faces), self.SEED_GRID_RES,
                           (ctypes.c_float * 3)(*lo), h, h / 1024, _lib.ptr(dist), _lib.ptr(face), _lib.ptr(closest),

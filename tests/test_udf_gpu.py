@@ -139,6 +139,20 @@ def test_grid_seeded_walk_equals_plain_walk():
         assert torch.equal(x, y)
 
 
+def test_grid_path_declines_non_finite_meshes():
+    """Vertex data with a non-finite coordinate gets no bound grid and udf()
+    walks plainly (the tree itself is built from valid data here)."""
+    V, F = datasets.noisy_icosphere(subdivisions=2)
+    b = BVHAccelerator(V.to(DEV), F.to(DEV))
+    b.vertices = b.vertices.clone()
+    b.vertices[0, 0] = float("nan")
+    assert b.seed_grid() is None
+    P = datasets.uniform_points(1 << 20, seed=8).to(DEV)
+    r = b.udf(P)
+    for x, y in zip(r, b.udf(P, seed=False)):
+        assert torch.equal(x, y)
+
+
 def test_seeded_walk_redo_fixes_short_seeds():
     """A3D_UDF_REDO: seeds below the true distance are walked again, so every
     result equals the plain walk's."""
