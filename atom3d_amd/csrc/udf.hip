@@ -496,7 +496,7 @@ __global__ __launch_bounds__(kBlk) void k_udf_lane(const void* __restrict__ blob
                                                     const uint32_t* __restrict__ perm, int64_t K,
                                                     const float* __restrict__ seed, float* __restrict__ dist, int32_t* __restrict__ face,
                                                     float* __restrict__ closest, float* __restrict__ uvw,
-                                                    bool use_obbs, const int32_t* redo) {
+                                                    bool use_obbs, const int32_t* redo, GridBound gb) {
   __shared__ int32_t s_ref[kLaneLds * kBlk];
   __shared__ float s_d2[kLaneLds * kBlk];
   const int tid = threadIdx.x;
@@ -513,8 +513,9 @@ __global__ __launch_bounds__(kBlk) void k_udf_lane(const void* __restrict__ blob
   const float eq = obb_eq(p);
 
   float best = 1e30f;
-  if (seed) {
-    const float sd = seed[q];
+  if (seed || gb.G) {
+    const float q3[3] = {p.x, p.y, p.z};
+    const float sd = seed ? seed[q] : grid_bound(gb, q3);
     if (sd >= 0.0f && sd < 1e15f) best = fmaxf(sd * sd * (1.0f + 4.8e-7f), 1e-30f);
   }
   int32_t best_id = 0x7fffffff, best_tri = -1;
@@ -914,8 +915,7 @@ int udf_launch(const void* blob, const float* P, int64_t K, const uint32_t* perm
   if (flags & ~(A3D_UDF_LANE | A3D_UDF_DIRECT | A3D_UDF_NO_OBBS | A3D_UDF_SPLIT | A3D_UDF_REDO))
     return fail(A3D_ERR_ARG, "a3d_bvh_udf: unknown flags 0x%x", flags);
   if ((flags & A3D_UDF_REDO) && !seed && !grid) return fail(A3D_ERR_ARG, "a3d_bvh_udf: A3D_UDF_REDO needs seeds");
-  if (grid && (flags & (A3D_UDF_LANE | A3D_UDF_SPLIT)))
-    return fail(A3D_ERR_ARG, "a3d_bvh_udf_grid: grid bounds need the wave walk");
+  if (grid && (flags & A3D_UDF_SPLIT)) return fail(A3D_ERR_ARG, "a3d_bvh_udf_grid: no grid bounds in the split walk");
   const bool use_obbs = !(flags & A3D_UDF_NO_OBBS);
   if (counters && (flags & (A3D_UDF_LANE | A3D_UDF_SPLIT)))
     return fail(A3D_ERR_ARG, "a3d_bvh_udf: counters need the wave walk");
@@ -932,7 +932,7 @@ int udf_launch(const void* blob, const float* P, int64_t K, const uint32_t* perm
                        uvw, use_obbs);
   else if (flags & A3D_UDF_LANE)
     hipLaunchKernelGGL(k_udf_lane, dim3(grid_for(K)), dim3(kBlk), 0, s, blob, P, perm, K, seed, dist, face, closest,
-                       uvw, use_obbs, (const int32_t*)nullptr);
+                       uvw, use_obbs, (const int32_t*)nullptr, gb);
   else if (flags & A3D_UDF_DIRECT) {
     if (counters)
       hipLaunchKernelGGL((k_udf_wave<true, false>), dim3(grid_for(K)), dim3(kBlk), 0, s, blob, P, perm, K, seed, dist,
@@ -951,7 +951,7 @@ int udf_launch(const void* blob, const float* P, int64_t K, const uint32_t* perm
   A3D_LAUNCH_CHECK();
   if (flags & A3D_UDF_REDO) {  // queries whose seed was below their distance: walk them again without one
     hipLaunchKernelGGL(k_udf_lane, dim3(grid_for(K)), dim3(kBlk), 0, s, blob, P, (const uint32_t*)nullptr, K,
-                       (const float*)nullptr, dist, face, closest, uvw, use_obbs, (const int32_t*)face);
+                       (const float*)nullptr, dist, face, closest, uvw, use_obbs, (const int32_t*)face, GridBound{});
     A3D_LAUNCH_CHECK();
   }
   return A3D_OK;

@@ -163,7 +163,7 @@ class BVHAccelerator:
     UDF_SPLIT = 8
     UDF_REDO = 16
     ORDER_MIN_QUERIES = 4096
-    SEED_MIN_QUERIES = 1 << 20
+    SEED_MIN_QUERIES = 1 << 15
     SEED_GRID_RES = 128
     SPLIT_MAX_QUERIES = 1 << 15
     LANE_MAX_QUERIES = 1 << 17
@@ -219,8 +219,8 @@ class BVHAccelerator:
         the packet walk in Morton order; UDF_LANE, UDF_DIRECT or 0 force one);
         ``order`` (default: K >= 4096) walks the queries in Morton order;
         ``counters`` (int64[5] on the device) accumulates the packet walk's
-        work counts; ``seed`` (default: from 2^20 queries on a non-empty
-        mesh, packet walk) bounds each query's walk from the distance grid of
+        work counts; ``seed`` (default: from 2^15 queries on a non-empty
+        mesh, packet or per-lane walk) bounds each query's walk from the distance grid of
         ``seed_grid`` and walks the few queries whose bound fell short again
         without one.  Results are identical either way."""
         p = self._points(points)
@@ -243,7 +243,8 @@ class BVHAccelerator:
         if order is None:
             order = self.ORDER_MIN_QUERIES <= K < 2 ** 31
         if seed is None:
-            seed = K >= self.SEED_MIN_QUERIES and self.num_faces > 0 and flags == 0 and counters is None
+            seed = (K >= self.SEED_MIN_QUERIES and self.num_faces > 0 and flags in (0, self.UDF_LANE)
+                    and counters is None)
         perm = self.query_order(p) if order and K > 0 else None
         with torch.cuda.device(self.device):
             st = _lib.stream(self.device)

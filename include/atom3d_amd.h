@@ -131,8 +131,8 @@ int a3d_udf_grid_seed(const float* grid, const int32_t* grid_face, const float* 
                       float* seed, a3d_stream_t stream);
 /* a3d_bvh_udf_grid: a3d_bvh_udf_seeded with A3D_UDF_REDO where each query's
  * bound is a3d_udf_grid_seed's, computed at the start of its walk (no seed
- * array): results equal a3d_bvh_udf's.  flags: 0 or A3D_UDF_NO_OBBS (the
- * wave walk).  No counterpart in the reference. */
+ * array): results equal a3d_bvh_udf's.  flags: as a3d_bvh_udf's except
+ * A3D_UDF_SPLIT.  No counterpart in the reference. */
 int a3d_bvh_udf_grid(const void* blob, const float* P, int64_t K, const uint32_t* perm, const float* grid,
                      const int32_t* grid_face, const float* V, const int32_t* F, int R, const float* lo3, float h,
                      float margin, float* dist, int32_t* face, float* closest, float* uvw, int flags,

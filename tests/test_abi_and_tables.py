@@ -51,9 +51,9 @@ def test_version_and_argument_errors():
     with pytest.raises(RuntimeError, match="bad grid"):
         _lib.call("a3d_bvh_udf_grid", 1, 1, 5, None, 1, None, None, None, 8, (ctypes.c_float * 3)(), -1.0, 0.0,
                   1, 1, 1, None, 0, None)
-    with pytest.raises(RuntimeError, match="wave walk"):
+    with pytest.raises(RuntimeError, match="split walk"):
         _lib.call("a3d_bvh_udf_grid", 1, 1, 5, None, 1, None, None, None, 8, (ctypes.c_float * 3)(), 0.1, 0.0,
-                  1, 1, 1, None, 1, None)
+                  1, 1, 1, None, 8, None)
     _lib.call("a3d_bvh_sizes", 4096000, 4, ctypes.byref(b), ctypes.byref(w))
     assert b.value == 256 + 64 * 4095999 + (48 + 32) * 4096000 + 96 * 4095999 + 48 * 4096000
 

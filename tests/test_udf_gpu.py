@@ -115,6 +115,11 @@ def test_grid_seeded_walk_equals_plain_walk():
     assert b.SEED_MIN_QUERIES <= len(P)
     for x, y in zip(plain, seeded):
         assert torch.equal(x, y)
+    # the per-lane walk with the same bounds (the default from 2^15 to 2^17 queries), incl. outside the grid
+    sub = torch.cat([Pd[:40000], Pd[-8192:]])
+    assert b.SEED_MIN_QUERIES <= len(sub) <= b.LANE_MAX_QUERIES
+    for x, y in zip(b.udf(sub, flags=b.UDF_LANE, seed=False), b.udf(sub)):
+        assert torch.equal(x, y)
     grid, gface, lo, h = b.seed_grid()
     R = b.SEED_GRID_RES
     idx = torch.tensor([0, 1, R, (R + 1) ** 2 + 5, (R + 1) ** 3 - 1], device=DEV)
