@@ -61,8 +61,16 @@ class BatchGraph:
             pass
 
 
+# one a3d_grid_bound record (64 bytes) per mesh
+_BOUND = np.dtype([("G", "<u8"), ("GF", "<u8"), ("V", "<u8"), ("F", "<u8"), ("R", "<i4"), ("lo", "<f4", 3),
+                   ("h", "<f4"), ("margin", "<f4"), ("pad", "<i4", 2)])
+assert _BOUND.itemsize == 64
+
+
 class BatchedUDF:
-    def __init__(self, meshes, device="cuda", leaf: int = 8):
+    BOUND_GRID_RES = 32  # per-mesh distance-bound grid (33^3 points), built with the trees
+
+    def __init__(self, meshes, device="cuda", leaf: int = 8, bounds: bool = True):
         self.device = torch.device(device)
         self.bvhs = [BVHAccelerator(torch.as_tensor(V).to(self.device), torch.as_tensor(F).to(self.device),
                                     n_primitives_per_leaf=leaf) for V, F in meshes]
@@ -70,6 +78,21 @@ class BatchedUDF:
         self.blobs = torch.tensor([b.blob.data_ptr() for b in self.bvhs], dtype=torch.int64, device=self.device)
         props = torch.cuda.get_device_properties(self.device)
         self.n_cu = int(props.multi_processor_count)
+        self.bounds = self._bounds() if bounds else None
+
+    def _bounds(self):
+        """Device array of one grid-bound record per mesh (see
+        BVHAccelerator.seed_grid), or None when a mesh has none (empty or
+        non-finite)."""
+        rec = np.zeros(len(self.bvhs), _BOUND)
+        for m, b in enumerate(self.bvhs):
+            sg = b.seed_grid(self.BOUND_GRID_RES) if b.num_faces > 0 else None
+            if sg is None:
+                return None
+            g, gf, lo, h = sg
+            rec[m] = (g.data_ptr(), gf.data_ptr(), b.vertices.data_ptr(), b.faces.data_ptr(), self.BOUND_GRID_RES,
+                      lo, h, h / 1024, (0, 0))
+        return torch.from_numpy(rec.view(np.uint8).copy()).to(self.device)
 
     # ------------------------------------------------------------------ plan
     def plan(self, P: torch.Tensor, counts) -> BatchPlan:
@@ -126,7 +149,7 @@ class BatchedUDF:
         d, f, c, w = plan.out
         _lib.call("a3d_batch_udf", _lib.ptr(self.blobs), _lib.ptr(plan.P), _lib.ptr(plan.perm), _lib.ptr(t),
                   int(t.shape[0]), _lib.ptr(plan.counter), wpb, blocks, _lib.ptr(d), _lib.ptr(f), _lib.ptr(c),
-                  _lib.ptr(w), stream.cuda_stream)
+                  _lib.ptr(w), _lib.ptr(self.bounds), stream.cuda_stream)
 
     def run(self, plan: BatchPlan, cfg: dict | None = None, reorder: bool = False):
         """(dist, face, closest, uvw) of all queries, in input order."""

@@ -356,7 +356,7 @@ __device__ __forceinline__ void wave_walk(const Node64* __restrict__ nodes, cons
 // ------------------------------------------------------------ grid bounds
 // A grid of exact distances G and closest faces GF at lo + (i, j, k) h
 // (float: lo + (float)i * h; i, j, k in [0, R]), the mesh V, F for the faces.
-struct GridBound {
+struct GridBound {  // also the 64-byte record of a3d_batch_udf's bounds array (a3d_grid_bound)
   const float* G;
   const int32_t* GF;
   const float* V;
@@ -365,7 +365,9 @@ struct GridBound {
   float lo[3];
   float h;
   float margin;
+  int32_t pad[2];
 };
+static_assert(sizeof(GridBound) == 64, "grid bound record is 64 bytes");
 
 // Upper bound on the distance from q: the minimum of (a) over the 8 corners
 // c of q's cell (clamped into the grid), G(c) + |q - c| (the distance is
@@ -470,17 +472,24 @@ __global__ __launch_bounds__(kBlk) void k_udf_wave(const void* __restrict__ blob
 
 
 // One packet of the batched walk: lane's query q (if valid) on `blob`.
+// gb (nullable): the mesh's grid bound (see grid_bound) for each lane's walk.
 __device__ __forceinline__ void batch_walk_tile(const void* __restrict__ blob, const float* __restrict__ P, int64_t q,
                                                 bool valid, float4 (*stk)[2], WaveQ* __restrict__ Q, int lane,
                                                 float* __restrict__ dist,
                                                 int32_t* __restrict__ face, float* __restrict__ closest,
-                                                float* __restrict__ uvw) {
+                                                float* __restrict__ uvw, const GridBound* __restrict__ gb) {
   const BvhHeader* H = header(blob);
   const Node64* __restrict__ nodes = nodes_of(blob);
   const Tri48* __restrict__ tris = tris_of(blob);
   const v3 p = valid ? mk(P[q * 3], P[q * 3 + 1], P[q * 3 + 2]) : mk(0.0f, 0.0f, 0.0f);
   const float slack = query_slack(p, H->coord_max);
   float best = 1e30f;
+  if (gb && valid) {
+    const GridBound g = *gb;
+    const float q3[3] = {p.x, p.y, p.z};
+    const float sd = grid_bound(g, q3);
+    if (sd >= 0.0f && sd < 1e15f) best = fmaxf(sd * sd * (1.0f + 4.8e-7f), 1e-30f);
+  }
   int32_t best_id = 0x7fffffff, best_tri = -1;
   uint32_t n0 = 0, n1 = 0, n2 = 0, n3 = 0;
   wave_walk<false, true>(nodes, tris, boxes_of(blob), obbs_of(blob), tobbs_of(blob), p, valid, slack, stk, Q, lane,
@@ -989,7 +998,7 @@ extern "C" int a3d_udf_grid_seed(const float* grid, const int32_t* grid_face, co
   if (K == 0) return A3D_OK;
   if (!grid || !P || !seed || (grid_face && (!V || !F)))
     return fail(A3D_ERR_ARG, "a3d_udf_grid_seed: null argument");
-  const GridBound gb{grid, grid_face, V, F, R, {lo3[0], lo3[1], lo3[2]}, h, margin};
+  const GridBound gb{grid, grid_face, V, F, R, {lo3[0], lo3[1], lo3[2]}, h, margin, {0, 0}};
   hipLaunchKernelGGL(k_grid_seed, dim3(grid_for(K)), dim3(kBlk), 0, (hipStream_t)stream, gb, P, perm, K, seed);
   A3D_LAUNCH_CHECK();
   return A3D_OK;
@@ -1015,7 +1024,7 @@ extern "C" int a3d_bvh_udf_grid(const void* blob, const float* P, int64_t K, con
   if (R < 1 || R > 2048 || !(h > 0.0f) || !(margin >= 0.0f) || !lo3)
     return fail(A3D_ERR_ARG, "a3d_bvh_udf_grid: bad grid (R=%d, h=%g, margin=%g)", R, (double)h, (double)margin);
   if (K > 0 && (!grid || (grid_face && (!V || !F)))) return fail(A3D_ERR_ARG, "a3d_bvh_udf_grid: null grid argument");
-  const GridBound gb{grid, grid_face, V, F, R, {lo3[0], lo3[1], lo3[2]}, h, margin};
+  const GridBound gb{grid, grid_face, V, F, R, {lo3[0], lo3[1], lo3[2]}, h, margin, {0, 0}};
   return udf_launch(blob, P, K, perm, nullptr, dist, face, closest, uvw, flags | A3D_UDF_REDO, nullptr, stream, &gb);
 }
 
@@ -1076,10 +1085,13 @@ __global__ void k_batch_reset(unsigned int* counter) {
 }
 
 // tiles: int64 pairs (start << 8 | count, mesh), start = index into perm.
+// bounds (nullable): one GridBound per mesh; redo: walk only the lanes whose
+// face is -1 (the second pass after a bounded one), without bounds.
 __global__ void k_udf_batch(const void* const* __restrict__ blobs, const float* __restrict__ P,
                             const uint32_t* __restrict__ perm, const int64_t* __restrict__ tiles, int64_t n_tiles,
                             unsigned int* __restrict__ counter, float* __restrict__ dist, int32_t* __restrict__ face,
-                            float* __restrict__ closest, float* __restrict__ uvw) {
+                            float* __restrict__ closest, float* __restrict__ uvw,
+                            const GridBound* __restrict__ bounds, bool redo) {
   extern __shared__ float4 s_dyn[];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
@@ -1094,9 +1106,14 @@ __global__ void k_udf_batch(const void* const* __restrict__ blobs, const float* 
     const int64_t w0 = tiles[2 * (int64_t)t], mesh = tiles[2 * (int64_t)t + 1];
     const int64_t start = w0 >> 8;
     const int count = (int)(w0 & 0xff);
-    const bool valid = lane < count;
+    bool valid = lane < count;
     const int64_t q = valid ? (int64_t)perm[start + lane] : 0;
-    batch_walk_tile(blobs[mesh], P, q, valid, stk, Q, lane, dist, face, closest, uvw);
+    if (redo) {
+      valid = valid && face[q] < 0;
+      if (!__any(valid)) continue;
+    }
+    batch_walk_tile(blobs[mesh], P, q, valid, stk, Q, lane, dist, face, closest, uvw,
+                    (bounds && !redo) ? bounds + mesh : nullptr);
   }
 }
 
@@ -1146,7 +1163,7 @@ extern "C" int a3d_batch_order(const void* const* blobs, int n_meshes, const flo
 
 extern "C" int a3d_batch_udf(const void* const* blobs, const float* P, const uint32_t* perm, const int64_t* tiles,
                              int64_t n_tiles, unsigned int* counter, int waves_per_block, int blocks, float* dist,
-                             int32_t* face, float* closest, float* uvw, a3d_stream_t stream) {
+                             int32_t* face, float* closest, float* uvw, const void* bounds, a3d_stream_t stream) {
   if (n_tiles < 0 || n_tiles > 0xffffffffll) return fail(A3D_ERR_ARG, "a3d_batch_udf: n_tiles out of range");
   if (waves_per_block < 1 || waves_per_block > 16 || blocks < 1 || blocks > (1 << 20))
     return fail(A3D_ERR_ARG, "a3d_batch_udf: launch shape %d waves x %d blocks out of range", waves_per_block, blocks);
@@ -1157,9 +1174,17 @@ extern "C" int a3d_batch_udf(const void* const* blobs, const float* P, const uin
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, counter);
   A3D_LAUNCH_CHECK();
   const size_t lds = (size_t)waves_per_block * (kWaveStack * 2 * sizeof(float4) + sizeof(WaveQ));
+  const GridBound* gb = (const GridBound*)bounds;
   hipLaunchKernelGGL(k_udf_batch, dim3(blocks), dim3(64 * waves_per_block), lds, s, blobs, P, perm, tiles, n_tiles,
-                     counter, dist, face, closest, uvw);
+                     counter, dist, face, closest, uvw, gb, false);
   A3D_LAUNCH_CHECK();
+  if (gb) {  // lanes whose bound fell short: walk them again without one
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, counter);
+    A3D_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_udf_batch, dim3(blocks), dim3(64 * waves_per_block), lds, s, blobs, P, perm, tiles, n_tiles,
+                       counter, dist, face, closest, uvw, (const GridBound*)nullptr, true);
+    A3D_LAUNCH_CHECK();
+  }
   return A3D_OK;
 }
 

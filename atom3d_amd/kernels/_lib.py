@@ -106,7 +106,7 @@ SIGNATURES: dict[str, list] = {
     "a3d_sg_udf": [_vp, _vp, _vp, _i64, _vp, _f32, _vp, _vp, _vp, _vp, _vp, _i64, _pi64, _vp],
     "a3d_batch_order_ws_bytes": [_i64, _pi64],
     "a3d_batch_order": [_vp, _i32, _vp, _i64, _vp, _vp, _vp, _i64, _vp],
-    "a3d_batch_udf": [_vp, _vp, _vp, _vp, _i64, _vp, _i32, _i32, _vp, _vp, _vp, _vp, _vp],
+    "a3d_batch_udf": [_vp, _vp, _vp, _vp, _i64, _vp, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp],
     "a3d_graph_begin": [_vp],
     "a3d_graph_end": [_vp, _pi64],
     "a3d_graph_launch": [_i64, _vp],

@@ -182,24 +182,26 @@ class BVHAccelerator:
                       _lib.stream(self.device))
         return perm
 
-    def seed_grid(self):
+    def seed_grid(self, res: int | None = None):
         """(grid, grid_face, lo, h), or None for a mesh with non-finite
         coordinates: exact distances and closest faces at the
         (R+1)^3 points lo + (i, j, k) h (float32 arithmetic, R =
         SEED_GRID_RES) of a cube holding the mesh with a 5 % margin, built
-        once per mesh by the plain walk; the source of the grid-seeded walk's
-        per-query distance bounds."""
-        if getattr(self, "_seed_grid", None) is None:
+        once per mesh (and resolution ``res``, default SEED_GRID_RES) by the
+        plain walk; the source of the grid-seeded walk's per-query distance
+        bounds."""
+        R = int(res or self.SEED_GRID_RES)
+        grids = self.__dict__.setdefault("_seed_grids", {})
+        if R not in grids:
             import math
 
             import numpy as np
 
-            R = self.SEED_GRID_RES
             lo_b = self.vertices.min(dim=0).values.double()
             hi_b = self.vertices.max(dim=0).values.double()
             ext = float((hi_b - lo_b).max())
             if not (math.isfinite(ext) and all(math.isfinite(x) for x in lo_b.tolist())):
-                self._seed_grid = False  # non-finite coordinates: the plain walk only
+                grids[R] = False  # non-finite coordinates: the plain walk only
                 return None
             pad = 0.05 * ext + 1e-6
             h = float(np.float32((ext + 2 * pad) / R))
@@ -208,8 +210,8 @@ class BVHAccelerator:
             axes = [torch.full_like(i, lo[d]) + i * torch.full_like(i, h) for d in range(3)]
             pts = torch.stack(torch.meshgrid(*axes, indexing="ij"), dim=-1).reshape(-1, 3).contiguous()
             d, f, _, _ = self.udf(pts, seed=False)
-            self._seed_grid = (d, f, lo, h)
-        return self._seed_grid or None
+            grids[R] = (d, f, lo, h)
+        return grids[R] or None
 
     def udf(self, points: torch.Tensor, flags: int | None = None, order: bool | None = None,
             counters: torch.Tensor | None = None, seed: bool | None = None):

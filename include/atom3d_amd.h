@@ -501,15 +501,32 @@ int a3d_sg_udf(const void* params, const void* blob, const float* P, int64_t K, 
  *   of `waves_per_block` wavefronts takes tiles from *counter (device uint32,
  *   reset by the call) in table order.  Results equal a3d_bvh_udf's on each
  *   mesh, written at the queries' positions.  Sync-free: capturable.
+ *   bounds (nullable): device array of one a3d_grid_bound per mesh; each
+ *   walk then starts bounded by its mesh's grid (as a3d_bvh_udf_grid) and a
+ *   second pass walks the queries whose bound fell short (same results).
  * a3d_graph_begin / end / launch / destroy: capture the work queued on a
  *   stream into an instantiated HIP graph (opaque handle) and replay it.
  */
 int a3d_batch_order_ws_bytes(int64_t K, int64_t* bytes);
 int a3d_batch_order(const void* const* blobs, int n_meshes, const float* P, int64_t K, const int64_t* q_off,
                     uint32_t* perm, void* ws, int64_t ws_bytes, a3d_stream_t stream);
+/* One mesh's distance-bound grid for a3d_batch_udf (64 bytes): the grid
+ * of exact distances and closest faces of a3d_udf_grid_seed, the mesh's
+ * vertices and faces, R, lo, h and the absolute margin. */
+typedef struct a3d_grid_bound {
+  const float* grid;
+  const int32_t* grid_face;
+  const float* V;
+  const int32_t* F;
+  int32_t R;
+  float lo[3];
+  float h;
+  float margin;
+  int32_t pad[2];
+} a3d_grid_bound;
 int a3d_batch_udf(const void* const* blobs, const float* P, const uint32_t* perm, const int64_t* tiles,
                   int64_t n_tiles, unsigned int* counter, int waves_per_block, int blocks, float* dist, int32_t* face,
-                  float* closest, float* uvw, a3d_stream_t stream);
+                  float* closest, float* uvw, const void* bounds, a3d_stream_t stream);
 int a3d_graph_begin(a3d_stream_t stream);
 int a3d_graph_end(a3d_stream_t stream, int64_t* handle);
 int a3d_graph_launch(int64_t handle, a3d_stream_t stream);

@@ -151,7 +151,7 @@ def test_new_entry_points_validate_arguments():
     with pytest.raises(RuntimeError, match="bad batch"):
         _lib.call("a3d_sg_grad", 1, 1, None, None, 0, None, None)
     with pytest.raises(RuntimeError, match="launch shape"):
-        _lib.call("a3d_batch_udf", None, None, None, None, 5, None, 0, 1, None, None, None, None, None)
+        _lib.call("a3d_batch_udf", None, None, None, None, 5, None, 0, 1, None, None, None, None, None, None)
     with pytest.raises(RuntimeError, match="null handle"):
         _lib.call("a3d_graph_launch", 0, None)
     with pytest.raises(RuntimeError, match="bad arguments"):
