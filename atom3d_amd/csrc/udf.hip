@@ -1098,11 +1098,19 @@ __global__ void k_udf_batch(const void* const* __restrict__ blobs, const float* 
   const int nw = blockDim.x >> 6;
   float4(*stk)[2] = reinterpret_cast<float4(*)[2]>(s_dyn + (size_t)wv * kWaveStack * 2);
   WaveQ* Q = reinterpret_cast<WaveQ*>(s_dyn + (size_t)nw * kWaveStack * 2) + wv;
-  while (true) {
+  // the redo pass finds little to do: tiles split statically (wave w takes w, w + W, ...), no queue
+  const int64_t wave_id = (int64_t)blockIdx.x * nw + wv, n_waves = (int64_t)gridDim.x * nw;
+  for (int64_t it = 0;; ++it) {
     unsigned int t = 0;
-    if (lane == 0) t = atomicAdd(counter, 1u);
-    t = __builtin_amdgcn_readfirstlane(t);
-    if ((int64_t)t >= n_tiles) break;
+    if (redo) {
+      const int64_t tt = wave_id + it * n_waves;
+      if (tt >= n_tiles) break;
+      t = (unsigned int)tt;
+    } else {
+      if (lane == 0) t = atomicAdd(counter, 1u);
+      t = __builtin_amdgcn_readfirstlane(t);
+      if ((int64_t)t >= n_tiles) break;
+    }
     const int64_t w0 = tiles[2 * (int64_t)t], mesh = tiles[2 * (int64_t)t + 1];
     const int64_t start = w0 >> 8;
     const int count = (int)(w0 & 0xff);
@@ -1179,8 +1187,6 @@ extern "C" int a3d_batch_udf(const void* const* blobs, const float* P, const uin
                      counter, dist, face, closest, uvw, gb, false);
   A3D_LAUNCH_CHECK();
   if (gb) {  // lanes whose bound fell short: walk them again without one
-    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, counter);
-    A3D_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_udf_batch, dim3(blocks), dim3(64 * waves_per_block), lds, s, blobs, P, perm, tiles, n_tiles,
                        counter, dist, face, closest, uvw, (const GridBound*)nullptr, true);
     A3D_LAUNCH_CHECK();
