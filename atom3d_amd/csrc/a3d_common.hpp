@@ -304,16 +304,6 @@ __device__ __forceinline__ bool tri_aabb_sat(v3 ta, v3 tb, v3 tc, v3 lo, v3 hi) 
   return true;
 }
 
-// Tree-independent hit predicate: the triangle's own AABB overlaps the box
-// (inclusive) and the reference SAT passes.  With one triangle per leaf this
-// is exactly what the reference's BVH walk decides.
-__device__ __forceinline__ bool tri_box_hit(v3 a, v3 b, v3 c, v3 lo, v3 hi) {
-  float tlx = fminf(fminf(a.x, b.x), c.x), tly = fminf(fminf(a.y, b.y), c.y), tlz = fminf(fminf(a.z, b.z), c.z);
-  float thx = fmaxf(fmaxf(a.x, b.x), c.x), thy = fmaxf(fmaxf(a.y, b.y), c.y), thz = fmaxf(fmaxf(a.z, b.z), c.z);
-  if (!box_overlap(tlx, tly, tlz, thx, thy, thz, lo, hi)) return false;
-  return tri_aabb_sat(a, b, c, lo, hi);
-}
-
 // ------------------------------------------------------------------ traversal
 constexpr int kStackTotal = 112;  // >= 97: every tree level adds >= 1 bit of Morton prefix
 

@@ -30,6 +30,11 @@ constexpr int kLds = 16;  // LDS stack entries per lane
 constexpr int kOvf = kStackTotal - kLds;
 
 // ---------------------------------------------------------------- AABB walks
+// Box queries run on the reference-layout SAH tree (a3d_sah_build): a leaf
+// is entered only when its box overlaps the query box (tested with the
+// parent's record), then each of its triangles is kept when it passes the
+// SAT test — the reference's rule (bvh_kernels.cu:583-591), so the kept set
+// is the reference's, leaf grouping included.
 // MODE 0: hit byte, stop at the first hit.  MODE 1: pair count.  MODE 2: write
 // pairs at offsets[q].
 template <int MODE>
@@ -62,7 +67,7 @@ __device__ __forceinline__ int32_t box_walk(const Node64* __restrict__ nodes, co
         v3 ta, tb, tc;
         int32_t id;
         load_tri(tris, i, ta, tb, tc, id);
-        if (tri_box_hit(ta, tb, tc, lo, hi)) {
+        if (tri_aabb_sat(ta, tb, tc, lo, hi)) {
           if (MODE == 0) return 1;
           if (MODE == 2) {
             pa[off + cnt] = qid;

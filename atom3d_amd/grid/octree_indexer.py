@@ -99,7 +99,7 @@ class OctreeIndexer(CubeGrid):
     # traversal
     def octree_traverse(self, bvh, min_level: int = 2) -> torch.Tensor:
         """Active cubes [K, 3] int64 at ``max_level``, in the reference's order."""
-        blob = bvh._bvh.blob if hasattr(bvh, "_bvh") else bvh.blob
+        blob = bvh._bvh.box_blob if hasattr(bvh, "_bvh") else bvh.box_blob  # the reference-layout SAH tree
         dev = blob.device
         b = self.bounds.detach().to("cpu", torch.float32).reshape(-1).tolist()
         bounds = (ctypes.c_float * 6)(*b)

@@ -44,6 +44,8 @@ SIGNATURES: dict[str, list] = {
     "a3d_ray_crossings": [_vp, _vp, _vp, _i64, _f32, _vp, _vp],
     "a3d_bvh_build_ex": [_vp, _i64, _vp, _i64, _i32, _i32, _vp, _i64, _vp, _i64, _vp],
     "a3d_bvh_info": [_vp, _pi64, _vp],
+    "a3d_sah_sizes": [_i64, _i32, _pi64, _pi64],
+    "a3d_sah_build": [_vp, _i64, _vp, _i64, _i32, _vp, _i64, _vp, _i64, ctypes.POINTER(ctypes.c_int32), _vp],
     "a3d_query_order_ws_bytes": [_i64, _pi64],
     "a3d_query_order": [_vp, _i64, _vp, _vp, _i64, _vp],
     "a3d_bvh_udf": [_vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp],

@@ -9,10 +9,17 @@ method names and return tuples; ray_intersect / ray_all_hit :119-181).  Differen
 * the closest face is tree independent (minimum FP32 squared distance, ties to
   the lowest face index) and ``uvw`` is zero where no face was found (the
   reference leaves it uninitialised);
+* the box queries (``aabb_hit``, ``aabb_intersect``, ``block_cells``,
+  ``cells_sat_pairs`` and the octree level test) run on a second tree,
+  ``box_blob``: the reference's own binned-SAH tree rebuilt on the device
+  (same leaves), because the reference keeps a face when its LEAF box
+  overlaps the query box and the SAT passes, which depends on the leaves;
+  it is built on the first box query;
 * ``aabb_intersect`` never truncates: it counts pairs first, then writes them
   sorted by (box, face) (the reference appends with an atomic counter, capped
   at min(100*K, 5e7) pairs, in nondeterministic order);
-* ``n_primitives_per_leaf`` defaults to 8 (reference: 4): with tree-independent
+* ``n_primitives_per_leaf`` sets the box tree's leaves (reference default 4);
+  the distance/ray tree uses 8 unless it is given: with tree-independent
   results it only trades node visits against leaf tests, and 8 is faster here;
 * ``treelet_passes`` (default 0 = plain LBVH, asynchronous build) sets how
   many SAH treelet-restructuring passes refine the device tree.
@@ -29,14 +36,17 @@ from . import _lib
 class BVHAccelerator:
     TREELET_PASSES = 0
 
-    def __init__(self, vertices: torch.Tensor, faces: torch.Tensor, n_primitives_per_leaf: int = 8,
+    def __init__(self, vertices: torch.Tensor, faces: torch.Tensor, n_primitives_per_leaf: int | None = None,
                  treelet_passes: int | None = None):
         _lib.require_cuda(vertices, "vertices")
         self.device = vertices.device
         self.vertices = vertices.contiguous().float()
         self.faces = faces.to(self.device).contiguous().int()
         self.num_faces = int(self.faces.shape[0])
-        self.leaf_max = int(n_primitives_per_leaf)
+        self.leaf_max = 8 if n_primitives_per_leaf is None else int(n_primitives_per_leaf)
+        self.box_leaf = 4 if n_primitives_per_leaf is None else int(n_primitives_per_leaf)
+        self._box_blob = None
+        self.box_depth = None
         self.treelet_passes = self.TREELET_PASSES if treelet_passes is None else int(treelet_passes)
         if self.num_faces > 0:
             fmin, fmax = int(self.faces.min()), int(self.faces.max())
@@ -56,6 +66,67 @@ class BVHAccelerator:
                 _lib.stream(self.device),
             )
             del ws
+
+    @property
+    def box_blob(self) -> torch.Tensor:
+        """The box queries' tree: the reference's binned-SAH tree with
+        ``box_leaf`` triangles per leaf (kernels/bvh_kernels.cu:616-925),
+        built on the device on first use."""
+        if self._box_blob is None:
+            blob_b, ws_b = ctypes.c_int64(), ctypes.c_int64()
+            _lib.call("a3d_sah_sizes", self.num_faces, self.box_leaf, ctypes.byref(blob_b), ctypes.byref(ws_b))
+            depth = ctypes.c_int32()
+            with torch.cuda.device(self.device):
+                blob = torch.empty(blob_b.value, dtype=torch.uint8, device=self.device)
+                ws = torch.empty(ws_b.value, dtype=torch.uint8, device=self.device)
+                _lib.call("a3d_sah_build", _lib.ptr(self.vertices), self.vertices.shape[0], _lib.ptr(self.faces),
+                          self.num_faces, self.box_leaf, _lib.ptr(blob), blob_b.value, _lib.ptr(ws), ws_b.value,
+                          ctypes.byref(depth), _lib.stream(self.device))
+                del ws
+            self._box_blob = blob
+            self.box_depth = int(depth.value)
+        return self._box_blob
+
+    def box_tree(self):
+        """The box tree on the host, for comparisons with the reference's:
+        (order int32 [F] — original face ids in the tree's triangle order,
+        leaves int64 [L, 2] — (start, end) triangle ranges, leaf boxes
+        float32 [L, 6] (min, max), internal-node boxes float32 [N, 6] (the
+        root's is the union of its children's: records hold child boxes
+        only), depth)."""
+        import numpy as np
+
+        raw = self.box_blob.cpu().numpy()
+        hdr = raw[:256].view(np.int32)
+        n_tris, n_nodes, cap = int(hdr[1]), int(hdr[2]), int(hdr[3])
+        nodes_off, tris_off = int(raw[16:24].view(np.int64)[0]), int(raw[24:32].view(np.int64)[0])
+        rec = raw[nodes_off:nodes_off + 64 * cap].view(np.float32).reshape(cap, 16)
+        refs = rec[:, 12:14].view(np.int32)
+        order = raw[tris_off:tris_off + 48 * n_tris].view(np.float32).reshape(n_tris, 12)[:, 9].view(np.int32).copy()
+        depth = int(hdr[17])
+        if n_tris == 0:
+            return order, np.zeros((0, 2), np.int64), np.zeros((0, 6), np.float32), np.zeros((0, 6), np.float32), depth
+        # records reachable from the root (reserved records of small subtrees may be unused)
+        seen = np.zeros(cap, bool)
+        frontier = np.array([0])
+        seen[0] = True
+        while len(frontier):
+            kids = refs[frontier].reshape(-1)
+            kids = kids[kids >= 0]
+            seen[kids] = True
+            frontier = kids
+        live = np.nonzero(seen)[0]
+        boxes = rec[live, :12].reshape(-1, 2, 6)
+        r = refs[live]
+        leaf = (r < 0) & (boxes[:, :, 0] <= boxes[:, :, 3])
+        enc = ~r[leaf].astype(np.int64)
+        leaves = np.stack([enc >> 3, (enc >> 3) + (enc & 7) + 1], 1)
+        root = np.concatenate([np.minimum(boxes[0, 0, :3], boxes[0, 1, :3]),
+                               np.maximum(boxes[0, 0, 3:], boxes[0, 1, 3:])]) if n_tris > self.box_leaf else None
+        inner = boxes[r >= 0]
+        if root is not None:
+            inner = np.concatenate([root[None], inner])
+        return order, leaves, boxes[leaf], inner, depth
 
     def info(self) -> dict:
         out = (ctypes.c_int64 * 4)()
@@ -361,7 +432,7 @@ class BVHAccelerator:
         K = lo.shape[0]
         hit = torch.empty(K, dtype=torch.uint8, device=self.device)
         with torch.cuda.device(self.device):
-            _lib.call("a3d_bvh_aabb_hit", _lib.ptr(self.blob), _lib.ptr(lo), _lib.ptr(hi), K, _lib.ptr(hit),
+            _lib.call("a3d_bvh_aabb_hit", _lib.ptr(self.box_blob), _lib.ptr(lo), _lib.ptr(hi), K, _lib.ptr(hit),
                       _lib.stream(self.device))
         return hit.view(torch.bool)
 
@@ -374,7 +445,8 @@ class BVHAccelerator:
         with torch.cuda.device(dev):
             counts = torch.empty(K, dtype=torch.int32, device=dev)
             s = _lib.stream(dev)
-            _lib.call("a3d_bvh_aabb_count", _lib.ptr(self.blob), _lib.ptr(lo), _lib.ptr(hi), K, _lib.ptr(counts), s)
+            _lib.call("a3d_bvh_aabb_count", _lib.ptr(self.box_blob), _lib.ptr(lo), _lib.ptr(hi), K, _lib.ptr(counts),
+                      s)
             hit = counts > 0
             if K == 0:
                 e = torch.empty(0, dtype=torch.int32, device=dev)
@@ -385,7 +457,7 @@ class BVHAccelerator:
             pa = torch.empty(total, dtype=torch.int32, device=dev)
             pf = torch.empty(total, dtype=torch.int32, device=dev)
             if total > 0:
-                _lib.call("a3d_bvh_aabb_pairs", _lib.ptr(self.blob), _lib.ptr(lo), _lib.ptr(hi), K,
+                _lib.call("a3d_bvh_aabb_pairs", _lib.ptr(self.box_blob), _lib.ptr(lo), _lib.ptr(hi), K,
                           _lib.ptr(offsets), _lib.ptr(pa), _lib.ptr(pf), s)
                 wsb = ctypes.c_int64()
                 _lib.call("a3d_pairs_sort_ws_bytes", total, ctypes.byref(wsb))
@@ -409,7 +481,7 @@ class BVHAccelerator:
             ws = torch.empty(max(nb.value, 1), dtype=torch.uint8, device=dev)
             nh = ctypes.c_int64()
             s = _lib.stream(dev)
-            _lib.call("a3d_vox_cells", _lib.ptr(self.blob), b0c, cc, oc, dc, _lib.ptr(ws), nb.value,
+            _lib.call("a3d_vox_cells", _lib.ptr(self.box_blob), b0c, cc, oc, dc, _lib.ptr(ws), nb.value,
                       ctypes.byref(nh), s)
             out = torch.empty((nh.value, 3), dtype=torch.int64, device=dev)
             if nh.value:
@@ -430,14 +502,16 @@ class BVHAccelerator:
             if K == 0:
                 e = torch.empty(0, dtype=torch.int32, device=dev)
                 return counts, e, e.clone()
-            _lib.call("a3d_cells_sat_count", _lib.ptr(self.blob), _lib.ptr(ijk), K, b0c, cc, _lib.ptr(counts), s)
+            _lib.call("a3d_cells_sat_count", _lib.ptr(self.box_blob), _lib.ptr(ijk), K, b0c, cc, _lib.ptr(counts),
+                      s)
             incl = torch.cumsum(counts, 0, dtype=torch.int64)
             total = int(incl[-1])
             offsets = incl - counts
             pa = torch.empty(total, dtype=torch.int32, device=dev)
             pf = torch.empty(total, dtype=torch.int32, device=dev)
             if total > 0:
-                _lib.call("a3d_cells_sat_pairs", _lib.ptr(self.blob), _lib.ptr(ijk), K, b0c, cc, _lib.ptr(offsets),
+                _lib.call("a3d_cells_sat_pairs", _lib.ptr(self.box_blob), _lib.ptr(ijk), K, b0c, cc,
+                          _lib.ptr(offsets),
                           _lib.ptr(pa), _lib.ptr(pf), s)
                 wsb = ctypes.c_int64()
                 _lib.call("a3d_pairs_sort_ws_bytes", total, ctypes.byref(wsb))

@@ -198,11 +198,34 @@ int a3d_ray_eval(const float* V, int64_t nV, const int32_t* F, int64_t nF, const
 int a3d_ray_crossings(float* t, int32_t* sign, const int64_t* offsets, int64_t K, float coalesce_t,
                       int32_t* out, a3d_stream_t stream);
 
+/* ------------------------------------------------------------------ box-query tree
+ * Replaces build_bvh (bvh_kernels.cu:860-925, bound at :1245) for the box
+ * queries below: the reference's own binned-SAH tree (build_bvh_recursive
+ * :667-854 — 16 bins, strict-minimum SAH split, std::partition, median
+ * fallbacks by std::nth_element; leaf_max triangles per leaf, 4 in
+ * kernels/bvh.py:69), built on the device with the same leaf sets, node
+ * boxes and triangle order.  The reference's box query keeps a face when its
+ * LEAF box overlaps the query box and the SAT passes, so its result depends
+ * on this tree; the UDF and ray queries use the LBVH above instead (their
+ * results do not).
+ * a3d_sah_sizes: bytes of the blob and of the build workspace for nF faces.
+ * a3d_sah_build: V float32[nV*3], F int32[nF*3] (device) -> blob (the BVH
+ *   blob layout, without the UDF walk's extra boxes).  Synchronises the
+ *   stream once per tree level above 64-triangle subtrees and at the end;
+ *   *depth_out (host, nullable) = internal nodes on the longest path.  Fails
+ *   with A3D_ERR_CAPACITY when the depth exceeds the box walks' stack (112;
+ *   the reference's walk drops subtrees past 127).
+ */
+int a3d_sah_sizes(int64_t nF, int leaf_max, int64_t* blob_bytes, int64_t* ws_bytes);
+int a3d_sah_build(const float* V, int64_t nV, const int32_t* F, int64_t nF, int leaf_max, void* blob,
+                  int64_t blob_bytes, void* ws, int64_t ws_bytes, int32_t* depth_out, a3d_stream_t stream);
+
 /* ------------------------------------------------------------------ AABB / SAT
  * Replaces bvh_aabb_intersect (bvh_kernels.cu:1192-1242 /
- * bvh_aabb_intersect_kernel :555-610, bound at :1249).  A (box, face) pair
- * hits when the face's AABB overlaps the box (inclusive) and the reference
- * SAT (eps 1e-6) passes.
+ * bvh_aabb_intersect_kernel :555-610, bound at :1249).  blob: from
+ * a3d_sah_build.  A (box, face) pair hits when the box overlaps the face's
+ * leaf box (inclusive) and the reference SAT (eps 1e-6) passes — the
+ * reference's rule (:583-591).
  * a3d_bvh_aabb_hit:   hit uint8[K] only, first-hit early exit (mode 0).
  * a3d_bvh_aabb_count: counts int32[K] = pairs per box.
  * a3d_bvh_aabb_pairs: writes the pairs of box q at offsets[q] (int64[K],
@@ -243,7 +266,8 @@ int a3d_sat_clip(const float* bmin, const float* bmax, const float* tris9, const
  * (cube_aabb_level :302-337, subdivide :191-216).  Cubes are packed uint64
  * keys i<<42 | j<<21 | k (level <= 21).
  * a3d_octree_ws_bytes: workspace for up to max_cubes candidate cubes.
- * a3d_octree_level: candidates are all res^3 cubes of `level` in meshgrid('ij')
+ * a3d_octree_level (blob from a3d_sah_build; a cube is kept by the box
+ *   query's rule above): candidates are all res^3 cubes of `level` in meshgrid('ij')
  *   order when parents == NULL, else the 8 children (CUBE_CORNERS order) of
  *   each parent key (level-1).  Writes the hit candidates, order preserved,
  *   to kept (capacity: the candidate count) and their number to n_kept
@@ -375,7 +399,7 @@ int a3d_dmc_forward_qef(void* ws, int64_t N, const int64_t* sizes, const int32_t
  * cell of the faces' bounding block and run MeshBVH.intersect_aabb on all of
  * them.  A cell (i,j,k) has CubeGrid.cube_aabb's box: min = float(ijk) *
  * cell + b0, max = min + cell (grid/cube_grid.py:164-203); b0, cell are
- * host float[3].
+ * host float[3]; blob from a3d_sah_build (the box query's rule above).
  * a3d_vox_ws_bytes: workspace for a block of n_cells (< 2^31) cells.
  * a3d_vox_cells: cells of the block origin[3] + [0, dims[3]) (host int64)
  *   that intersect some face (the SAT test of intersect_aabb mode 0), in

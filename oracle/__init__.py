@@ -30,7 +30,7 @@ _SIGS = {
     "orc_num_threads": (_int, []),
     "orc_tri_query": (None, [_f, _f, _f, _f, _f]),
     "orc_tri_aabb_sat": (_int, [_f, _f, _f]),
-    "orc_sah_build": (_i64, [_f, _i64, _i, _i64, _int, _f, _f]),
+    "orc_sah_build": (_i64, [_f, _i64, _i, _i64, _int, _f, _f, _vp]),
     "orc_bvh_udf": (None, [_f, _f, _f, _i64, _f, _i, _f, _f, _vp, _vp, _int]),
     "orc_brute_udf": (None, [_f, _i, _i64, _f, _i64, _f, _i, _f, _f, _int]),
     "orc_fp64_udf": (None, [_f, _i, _i64, _f, _i64, _d, _d, _int]),
@@ -38,7 +38,8 @@ _SIGS = {
     "orc_bvh_aabb": (_i64, [_f, _f, _f, _f, _i64, _u8, _i, _i, _i64, _int]),
     "orc_brute_aabb_hit": (None, [_f, _i, _i64, _f, _f, _i64, _u8, _vp, _int]),
     "orc_sat_clip": (None, [_f, _f, _f, _l, _l, _i64, _int, ctypes.c_float, _u8, _i, _f, _f, _f]),
-    "orc_octree_traverse": (_i64, [_f, _f, _f, _int, _int, _l, _i64, _vp, _vp, _int, _int, _vp]),
+    "orc_octree_traverse": (_i64, [_f, _f, _f, _int, _int, _l, _i64, _vp, _vp, _int, _int, _vp, _int]),
+    "orc_leaf_brute_pairs": (None, [_f, _i64, _f, _f, _f, _i64, _i, _vp, _vp, _vp, _int]),
     "orc_ray_tri": (ctypes.c_float, [_f, _f, _f, _f]),
     "orc_brute_ray": (None, [_f, _i, _i64, _f, _f, _i64, ctypes.c_float, _u8, _f, _i, _f, _f, _int]),
     "orc_sah_ray": (None, [_f, _f, _f, _f, _i64, ctypes.c_float, _u8, _f, _i, _int, _vp]),
@@ -93,17 +94,77 @@ def tri_aabb_sat(tri9, bmin, bmax) -> bool:
 
 
 class SahBvh:
-    """The reference's host SAH tree (nodes [Nn, 9] f32, tris [F, 10] f32)."""
+    """The reference's host SAH tree (nodes [Nn, 9] f32, tris [F, 10] f32).
+    ``fallbacks`` counts the median splits: (after an empty SAH partition,
+    with no axis to bin)."""
 
     def __init__(self, V, F, leaf: int = 4):
         V, F = _f32(V), _i32(F)
         nF = F.shape[0]
         nodes = np.zeros((2 * max(nF, 1) + 1, 9), np.float32)
         tris = np.zeros((max(nF, 1), 10), np.float32)
-        nn = lib().orc_sah_build(V.reshape(-1), V.shape[0], F.reshape(-1), nF, leaf, nodes.reshape(-1), tris.reshape(-1))
+        st = np.zeros(2, np.int64)
+        nn = lib().orc_sah_build(V.reshape(-1), V.shape[0], F.reshape(-1), nF, leaf, nodes.reshape(-1),
+                                 tris.reshape(-1), st.ctypes.data)
         self.nodes = np.ascontiguousarray(nodes[:nn])
         self.tris = tris
         self.num_faces = nF
+        self.fallbacks = (int(st[0]), int(st[1]))
+
+    def order(self) -> np.ndarray:
+        """Original face ids in the tree's triangle order (int32 [F])."""
+        return np.ascontiguousarray(self.tris[:self.num_faces, 9]).view(np.int32).copy()
+
+    def ranges(self):
+        """{(start, end): box [6] float32} for every node: the triangle range
+        a node covers in the tree's order and its box (min, max)."""
+        nodes = self.nodes
+        refs = nodes[:, 6:8].view(np.int32)
+        out = {}
+        if self.num_faces == 0:
+            return out
+        stack = [0]
+        spans = {}
+        order = []
+        while stack:
+            i = stack.pop()
+            order.append(i)
+            l, r = int(refs[i, 0]), int(refs[i, 1])
+            if l < 0:
+                spans[i] = (-l - 1, -r - 1)
+            else:
+                stack += [l, r]
+        for i in reversed(order):  # children before parents
+            l, r = int(refs[i, 0]), int(refs[i, 1])
+            if l >= 0:
+                spans[i] = (spans[l][0], spans[r][1])
+            out[spans[i]] = nodes[i, :6].copy()
+        return out
+
+    def structure(self):
+        """(leaves int64 [L, 2] — (start, end) triangle ranges, leaf boxes
+        float32 [L, 6], internal-node boxes float32 [N, 6]) of this tree."""
+        refs = self.nodes[:, 6:8].view(np.int32)
+        leaf = refs[:, 0] < 0
+        lv = np.stack([-refs[leaf, 0].astype(np.int64) - 1, -refs[leaf, 1].astype(np.int64) - 1], 1)
+        return lv, self.nodes[leaf, :6].copy(), self.nodes[~leaf, :6].copy()
+
+    def leaf_brute_pairs(self, qmin, qmax, threads: int = 0):
+        """Walk-free reference box query on this tree: (hit [K] bool, box ids,
+        face ids), pairs sorted by (box, face)."""
+        qmin, qmax = _f32(qmin).reshape(-1, 3), _f32(qmax).reshape(-1, 3)
+        K = qmin.shape[0]
+        cnt = np.zeros(K, np.int32)
+        args = (self.nodes.reshape(-1), self.nodes.shape[0], self.tris.reshape(-1), qmin.reshape(-1),
+                qmax.reshape(-1), K, cnt)
+        lib().orc_leaf_brute_pairs(*args, None, None, None, threads)
+        off = np.zeros(K + 1, np.int64)
+        np.cumsum(cnt, out=off[1:])
+        pa = np.zeros(int(off[-1]), np.int32)
+        pf = np.zeros(int(off[-1]), np.int32)
+        if len(pa):
+            lib().orc_leaf_brute_pairs(*args, off.ctypes.data, pa.ctypes.data, pf.ctypes.data, threads)
+        return cnt > 0, pa, pf
 
     def udf(self, P, threads: int = 0, counts: bool = False):
         P = _f32(P).reshape(-1, 3)
@@ -132,7 +193,10 @@ class SahBvh:
                           float(max_t), hit, t, f, threads, w.ctypes.data)
         return (hit.astype(bool), t, f, (int(w[0]), int(w[1]))) if work else (hit.astype(bool), t, f)
 
-    def aabb(self, qmin, qmax, canonical: bool = True, cap: int | None = None):
+    def aabb(self, qmin, qmax, canonical: bool = False, cap: int | None = None):
+        """The reference's box walk on this tree: (hit [K] bool, box ids,
+        face ids) with pairs in one thread's append order.  canonical adds
+        the triangle's own AABB test (a tree-independent variant)."""
         qmin, qmax = _f32(qmin).reshape(-1, 3), _f32(qmax).reshape(-1, 3)
         K = qmin.shape[0]
         hit = np.zeros(K, np.uint8)
@@ -146,10 +210,11 @@ class SahBvh:
         return hit.astype(bool), pa[:n], pf[:n]
 
     def octree(self, bounds, min_level: int, max_level: int, threads: int = 0, full: bool = False,
-               work: bool = False):
+               work: bool = False, canonical: bool = False):
         """Octree level loop on this tree: (ijk int64 [n,3], tested per level,
         hits per level[, (node visits, triangle tests)]).  full walks every
-        cube's whole overlap set as the reference does (no first-hit exit)."""
+        cube's whole overlap set as the reference does (no first-hit exit);
+        canonical adds the triangle's own AABB test."""
         b = _f32(bounds).reshape(6)
         cap = 1 << 20
         while True:
@@ -159,7 +224,7 @@ class SahBvh:
             out = np.zeros((cap, 3), np.int64)
             n = lib().orc_octree_traverse(self.nodes.reshape(-1), self.tris.reshape(-1), b, min_level, max_level,
                                           out.reshape(-1), cap, tested.ctypes.data, hits.ctypes.data, threads,
-                                          int(full), w.ctypes.data)
+                                          int(full), w.ctypes.data, int(canonical))
             if n <= cap:
                 return (out[:n], tested, hits, (int(w[0]), int(w[1]))) if work else (out[:n], tested, hits)
             cap = int(n)

@@ -207,7 +207,7 @@ static inline float caxis(const TriInfo& t, int ax) {
 // bvh_kernels.cu:667-854 (the recursion is kept; depth is bounded by the
 // median fallback to O(log F) for all but adversarial inputs)
 static void build_rec(std::vector<TriInfo>& tris, int start, int end, std::vector<Node>& nodes,
-                      int node_idx, int leaf_n) {
+                      int node_idx, int leaf_n, int64_t* stats) {
   Node& node = nodes[node_idx];
   node.bmin[0] = node.bmin[1] = node.bmin[2] = 1e30f;
   node.bmax[0] = node.bmax[1] = node.bmax[2] = -1e30f;
@@ -266,11 +266,13 @@ static void build_rec(std::vector<TriInfo>& tris, int start, int end, std::vecto
                              [&](const TriInfo& t) { return caxis(t, ax) < split; });
     mid = (int)std::distance(tris.begin(), it);
     if (mid == start || mid == end) {
+      if (stats) stats[0]++;
       std::nth_element(tris.begin() + start, tris.begin() + start + count / 2, tris.begin() + end,
                        [ax](const TriInfo& a, const TriInfo& b) { return caxis(a, ax) < caxis(b, ax); });
       mid = start + count / 2;
     }
   } else {
+    if (stats) stats[1]++;
     int fa = 0;
     if (dims[1] > dims[0]) fa = 1;
     if (dims[2] > dims[fa]) fa = 2;
@@ -283,8 +285,8 @@ static void build_rec(std::vector<TriInfo>& tris, int start, int end, std::vecto
   nodes.emplace_back();
   nodes[node_idx].left = left;
   nodes[node_idx].right = left + 1;
-  build_rec(tris, start, mid, nodes, left, leaf_n);
-  build_rec(tris, mid, end, nodes, left + 1, leaf_n);
+  build_rec(tris, start, mid, nodes, left, leaf_n, stats);
+  build_rec(tris, mid, end, nodes, left + 1, leaf_n, stats);
 }
 
 static inline f3 ld3(const float* p) { return mk(p[0], p[1], p[2]); }
@@ -312,9 +314,11 @@ int orc_tri_aabb_sat(const float* tri9, const float* bmin, const float* bmax) {
 
 // Reference SAH build (bvh_kernels.cu:860-925).  nodes_out must hold
 // 2*max(F,1) nodes of 9 floats; tris_out holds F rows of 10 floats (a, b, c,
-// original id bit-cast).  Returns the node count.
+// original id bit-cast).  Returns the node count.  stats (nullable, 2
+// entries) counts the median fallbacks: [0] after an empty SAH partition,
+// [1] with no axis to bin.
 int64_t orc_sah_build(const float* V, int64_t nV, const int32_t* F, int64_t nF, int leaf_n,
-                      float* nodes_out, float* tris_out) {
+                      float* nodes_out, float* tris_out, int64_t* stats) {
   (void)nV;
   std::vector<TriInfo> ti((size_t)nF);
   for (int64_t i = 0; i < nF; i++) {
@@ -328,7 +332,7 @@ int64_t orc_sah_build(const float* V, int64_t nV, const int32_t* F, int64_t nF, 
   std::vector<Node> nodes;
   nodes.emplace_back();
   nodes.reserve((size_t)nF * 2 + 1);
-  build_rec(ti, 0, (int)nF, nodes, 0, leaf_n);
+  build_rec(ti, 0, (int)nF, nodes, 0, leaf_n, stats);
   std::memcpy(nodes_out, nodes.data(), nodes.size() * sizeof(Node));
   for (int64_t i = 0; i < nF; i++) {
     const TriInfo& t = ti[(size_t)i];
@@ -519,9 +523,9 @@ void orc_tri_closest_fp64(const double* tri9, const double* p, double* out) {
 // Reference AABB walk (bvh_kernels.cu:555-610) run sequentially: hit mask and
 // the (box, face) pairs in the order one thread would append them.  Returns the
 // total pair count; writes at most cap pairs.  canonical=1 additionally
-// requires the triangle's own AABB to overlap the box (the tree-independent
-// definition the HIP library implements; identical to the reference with
-// one triangle per leaf).
+// requires the triangle's own AABB to overlap the box (a tree-independent
+// variant, used only to measure how far the reference's leaf-box rule
+// reaches beyond the triangles' own boxes).
 int64_t orc_bvh_aabb(const float* nodes_f, const float* tris, const float* qmin, const float* qmax,
                      int64_t K, uint8_t* hit, int32_t* pair_a, int32_t* pair_f, int64_t cap,
                      int canonical) {
@@ -575,6 +579,48 @@ void orc_brute_aabb_hit(const float* V, const int32_t* F, int64_t nF, const floa
   }
 }
 
+// Walk-free statement of the reference's box query on a given tree: a
+// (box, face) pair is a hit when the box overlaps the face's LEAF box and the
+// face passes the SAT (bvh_kernels.cu:583-591).  Every leaf is scanned (no
+// traversal, so a pruning mistake in a walk cannot hide here); node boxes
+// contain their children's, so any leaf overlap implies the walk reaches it.
+// Pass 1 (offsets == nullptr) writes counts[K]; pass 2 writes the pairs of box
+// q in ascending face order at offsets[q].
+void orc_leaf_brute_pairs(const float* nodes_f, int64_t n_nodes, const float* tris, const float* qmin,
+                          const float* qmax, int64_t K, int32_t* counts, const int64_t* offsets, int32_t* pair_a,
+                          int32_t* pair_f, int nthreads) {
+  const Node* nodes = reinterpret_cast<const Node*>(nodes_f);
+  std::vector<int64_t> leaves;
+  for (int64_t i = 0; i < n_nodes; ++i)
+    if (nodes[i].left < 0) leaves.push_back(i);
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t q = 0; q < K; q++) {
+    f3 lo = ld3(qmin + q * 3), hi = ld3(qmax + q * 3);
+    std::vector<int32_t> fs;
+    for (int64_t li : leaves) {
+      const Node& nd = nodes[li];
+      if (!aabb_overlap(nd.bmin, nd.bmax, lo, hi)) continue;
+      for (int i = -nd.left - 1; i < -nd.right - 1; i++) {
+        const float* t = tris + (int64_t)i * 10;
+        if (tri_aabb_sat(ld3(t), ld3(t + 3), ld3(t + 6), lo, hi)) {
+          int32_t id;
+          std::memcpy(&id, t + 9, 4);
+          fs.push_back(id);
+        }
+      }
+    }
+    counts[q] = (int32_t)fs.size();
+    if (offsets) {
+      std::sort(fs.begin(), fs.end());
+      for (size_t k = 0; k < fs.size(); ++k) {
+        pair_a[offsets[q] + (int64_t)k] = (int32_t)q;
+        pair_f[offsets[q] + (int64_t)k] = fs[k];
+      }
+    }
+  }
+}
+
 // Cube boxes exactly as cube_aabb_level computes them in fp32
 // (octree_indexer.py:331-336): min = float(ijk) * cell + b0, max = min + cell,
 // cell = (b1 - b0) / 2^level.
@@ -589,20 +635,23 @@ static inline void cube_box(const int64_t* ijk, const float* b0, const float* ce
   *hi = mk(M[0], M[1], M[2]);
 }
 
-// Octree level loop (octree_indexer.py:470-509) on a reference SAH tree with
-// the canonical leaf test.  bounds = b0[3], b1[3].  Writes up to cap ijk rows
-// (int64) at max_level in the reference order; returns the count.
-// tested_out (nullable) receives the number of cube tests done per level,
-// hits_out the number of hits per level (max_level+1 entries each).
+// Octree level loop (octree_indexer.py:470-509) on a reference SAH tree.  A
+// cube is kept when some leaf whose box overlaps the cube holds a triangle
+// passing the SAT (bvh_kernels.cu:583-591); canonical = 1 additionally
+// requires the triangle's own AABB to overlap (a tree-independent variant,
+// kept only to measure the difference).  bounds = b0[3], b1[3].  Writes up
+// to cap ijk rows (int64) at max_level in the reference order; returns the
+// count.  tested_out (nullable) receives the number of cube tests done per
+// level, hits_out the number of hits per level (max_level+1 entries each).
 // full = 1 walks each cube's whole overlap set as the reference does (its
 // octree calls intersect_aabb(mode=0), which enumerates every pair,
 // bvh_kernels.cu:555-610) instead of stopping at the first hit; work_out
 // (nullable, 2 entries) then receives the total node visits and triangle
-// SAT tests of all cube tests.
+// SAT tests of all cube tests.  Either way the kept set is the same.
 int64_t orc_octree_traverse(const float* nodes_f, const float* tris, const float* bounds,
                             int min_level, int max_level, int64_t* out, int64_t cap,
                             int64_t* tested_out, int64_t* hits_out, int nthreads, int full,
-                            int64_t* work_out) {
+                            int64_t* work_out, int canonical) {
   const Node* nodes = reinterpret_cast<const Node*>(nodes_f);
   if (nthreads > 0) omp_set_num_threads(nthreads);
   const float* b0 = bounds;
@@ -646,11 +695,7 @@ int64_t orc_octree_traverse(const float* nodes_f, const float* tris, const float
             const float* t = tris + (int64_t)i * 10;
             f3 a = ld3(t), b = ld3(t + 3), c = ld3(t + 6);
             ++w_tri;
-            if (full) {
-              if (tri_aabb_sat(a, b, c, lo, hi) && tri_box_overlap(a, b, c, lo, hi)) any = true;
-            } else if (tri_box_overlap(a, b, c, lo, hi) && tri_aabb_sat(a, b, c, lo, hi)) {
-              any = true;
-            }
+            if ((!canonical || tri_box_overlap(a, b, c, lo, hi)) && tri_aabb_sat(a, b, c, lo, hi)) any = true;
           }
         } else if (sp < 127) {
           stack[sp++] = nd.right;

@@ -27,3 +27,18 @@ def pytest_collection_modifyitems(config, items):
     for it in items:
         if "gpu" in it.keywords:
             it.add_marker(skip)
+
+
+_CACHE = {}
+
+
+def sphere_union_oracle():
+    """(V, F, oracle.SahBvh) of the 4,096,000-face sphere union, built once
+    per session (the oracle's single-threaded SAH build takes ~30 s)."""
+    if "c2" not in _CACHE:
+        import oracle
+        from atom3d_amd import datasets
+
+        V, F = datasets.sphere_union()
+        _CACHE["c2"] = (V, F, oracle.SahBvh(V.numpy(), F.numpy()))
+    return _CACHE["c2"]

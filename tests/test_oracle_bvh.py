@@ -183,6 +183,10 @@ def _octree_order_key(ijk, min_level, max_level):
 
 @pytest.mark.parametrize("min_level,max_level", [(1, 4), (2, 5), (3, 3)])
 def test_octree_oracle_vs_brute_force(min_level, max_level):
+    """The octree's kept set equals the walk-free leaf-box statement of the
+    reference's box query on every cube of max_level (a cube whose parent
+    was dropped has no hit either: node boxes contain their children's, and
+    the parent cube contains the child cube)."""
     V, F = _mesh(2, 5)
     sah = oracle.SahBvh(V, F)
     got, tested, hits = sah.octree([-1, -1, -1, 1, 1, 1], min_level, max_level)
@@ -191,25 +195,57 @@ def test_octree_oracle_vs_brute_force(min_level, max_level):
     cell = np.float32(2.0) / np.float32(r)
     lo = (g.astype(np.float32) * cell + np.float32(-1)).astype(np.float32)
     hi = (lo + cell).astype(np.float32)
-    hit, _ = oracle.brute_aabb_hit(V, F, lo, hi)
+    hit, _, _ = sah.leaf_brute_pairs(lo, hi)
     brute = g[hit]
     assert {tuple(x) for x in got} == {tuple(x) for x in brute}
     keys = _octree_order_key(got, min_level, max_level)
     assert keys == sorted(keys), "octree output is not in parent-major x CUBE_CORNERS order"
     assert hits[max_level] == len(got)
+    full, _, _ = sah.octree([-1, -1, -1, 1, 1, 1], min_level, max_level, full=True)
+    np.testing.assert_array_equal(full, got)
 
 
-def test_canonical_pairs_subset_of_reference_walk():
+def test_reference_walk_equals_leaf_brute_force():
+    """The reference walk (node-box overlap descent, SAT at the leaves) keeps
+    exactly the pairs whose leaf box overlaps and whose SAT passes, found by
+    scanning every leaf; its canonical variant (the triangle's own box
+    overlapping as well) is a subset, equal to the all-faces brute force."""
     V, F = _mesh(3, 6)
     sah = oracle.SahBvh(V, F)
     rng = np.random.default_rng(9)
     c = rng.uniform(-1, 1, (1000, 3))
     h = rng.uniform(0.005, 0.1, (1000, 3))
     lo, hi = (c - h).astype(np.float32), (c + h).astype(np.float32)
+    h0, a0, f0 = sah.aabb(lo, hi)
+    hb, ab, fb = sah.leaf_brute_pairs(lo, hi)
+    np.testing.assert_array_equal(h0, hb)
+    o = np.lexsort((f0, a0))
+    np.testing.assert_array_equal(a0[o], ab)
+    np.testing.assert_array_equal(f0[o], fb)
     h1, a1, f1 = sah.aabb(lo, hi, canonical=True)
-    h0, a0, f0 = sah.aabb(lo, hi, canonical=False)
     s1, s0 = set(zip(a1, f1)), set(zip(a0, f0))
     assert s1 <= s0
-    hb, cb = oracle.brute_aabb_hit(V, F, lo, hi)
-    np.testing.assert_array_equal(h1, hb)
-    np.testing.assert_array_equal(np.bincount(a1, minlength=1000), cb)
+    hc, cc = oracle.brute_aabb_hit(V, F, lo, hi)
+    np.testing.assert_array_equal(h1, hc)
+    np.testing.assert_array_equal(np.bincount(a1, minlength=1000), cc)
+
+
+def test_reference_walk_keeps_eps_band_faces():
+    """A box 5e-7 beyond a face's own box along x (within the SAT slack):
+    the face is kept iff its leaf box reaches the query box."""
+    tri = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    far = np.array([[5, 5, 5], [6, 5, 5], [5, 6, 5]], np.float32)
+    wide = np.array([[0, 2, 0], [2.0, 2, 0], [0, 2, 0.1]], np.float32)  # extends the leaf box past x = 1
+    V = np.concatenate([tri, wide, far]).astype(np.float32)
+    F = np.arange(9, dtype=np.int32).reshape(3, 3)
+    lo = np.array([[1.0 + 5e-7, 0.0, -0.1]], np.float32)
+    hi = np.array([[1.5, 0.5, 0.1]], np.float32)
+    assert oracle.tri_aabb_sat(tri.reshape(-1), lo[0], hi[0])
+    sah = oracle.SahBvh(V, F, leaf=4)  # one leaf holds all three faces
+    hit, pa, pf = sah.aabb(lo, hi)
+    assert hit[0] and 0 in set(pf.tolist())
+    _, _, cf = sah.aabb(lo, hi, canonical=True)
+    assert 0 not in set(cf.tolist())
+    sah1 = oracle.SahBvh(V, F, leaf=1)  # the face alone in its leaf: its own box misses
+    _, _, pf1 = sah1.aabb(lo, hi)
+    assert 0 not in set(pf1.tolist())
