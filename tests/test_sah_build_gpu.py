@@ -106,10 +106,9 @@ def test_icosphere_c1():
 def test_unrotated_icosphere_ties():
     """Exact icosphere (no noise, axis-aligned): many equal centroid
     coordinates."""
-    V, F = datasets.icosphere(subdivisions=1)
-    _compare(V.numpy(), F.numpy())
-    V, F = datasets.icosphere(subdivisions=4)
-    _compare(V.numpy(), F.numpy())
+    for sub in (1, 4):
+        V, F = datasets.icosphere(subdivisions=sub)
+        _compare(V, F)
 
 
 def test_empty_mesh():
