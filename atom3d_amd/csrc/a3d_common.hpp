@@ -306,6 +306,11 @@ __device__ __forceinline__ bool tri_aabb_sat(v3 ta, v3 tb, v3 tc, v3 lo, v3 hi) 
 
 // ------------------------------------------------------------------ traversal
 constexpr int kStackTotal = 112;  // >= 97: every tree level adds >= 1 bit of Morton prefix
+// Deepest internal node (root = 0) every distance/ray walk's stack holds: a
+// walk keeps at most one deferred child per internal level above it, so a
+// tree of depth D needs D + 1 entries; the wave walk has 100 (udf.hip).
+// a3d_bvh_build fails on a deeper tree instead of letting a walk drop work.
+constexpr int kMaxWalkDepth = 99;
 
 __device__ __forceinline__ const Node64* nodes_of(const void* blob) {
   return reinterpret_cast<const Node64*>((const char*)blob + header(blob)->nodes_off);

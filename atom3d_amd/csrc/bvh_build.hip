@@ -19,6 +19,7 @@
 // multi-XCD part) is needed.
 #include <hipcub/hipcub.hpp>
 
+#include <cstddef>
 #include <stdarg.h>
 #include <stdio.h>
 
@@ -1119,6 +1120,14 @@ extern "C" int a3d_bvh_build_ex(const float* V, int64_t nV, const int32_t* F, in
   A3D_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_tri_obbs, dim3((unsigned)div_up(n, kBlk)), dim3(kBlk), 0, s, blob);
   A3D_LAUNCH_CHECK();
+  // The walks' stacks hold trees up to kMaxWalkDepth internal levels; the
+  // Karras tree of distinct keys stays within kMaxDepth < kMaxWalkDepth,
+  // checked here rather than assumed.
+  int32_t depth = 0;
+  A3D_HIP_CHECK(hipMemcpyAsync(&depth, (const char*)blob + offsetof(BvhHeader, depth), 4, hipMemcpyDeviceToHost, s));
+  A3D_HIP_CHECK(hipStreamSynchronize(s));
+  if (depth > kMaxWalkDepth)
+    return fail(A3D_ERR_CAPACITY, "a3d_bvh_build: tree depth %d exceeds the walks' stacks (%d)", depth, kMaxWalkDepth);
   return A3D_OK;
 }
 

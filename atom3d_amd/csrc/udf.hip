@@ -86,6 +86,8 @@ __device__ __forceinline__ void write_result(int64_t q, v3 p, float best, int32_
 // pairs, so qn < 192.
 constexpr int kQCap = 192;
 constexpr int kWaveStack = 100;  // >= 97 (see kStackTotal); sized so 8 blocks of 4 waves fit in 160 KiB of LDS
+static_assert(kWaveStack >= kMaxWalkDepth + 1 && kStackTotal >= kMaxWalkDepth + 1,
+              "walk stacks must hold every tree a3d_bvh_build accepts");
 
 struct WaveQ {
   float4 ob[6];                // the current node's Obb96, copied by the node visit (global -> LDS)
@@ -503,7 +505,8 @@ __device__ __forceinline__ void batch_walk_tile(const void* __restrict__ blob, c
 // there, without a seed (the second pass of a seeded walk).
 __global__ __launch_bounds__(kBlk) void k_udf_lane(const void* __restrict__ blob, const float* __restrict__ P,
                                                     const uint32_t* __restrict__ perm, int64_t K,
-                                                    const float* __restrict__ seed, float* __restrict__ dist, int32_t* __restrict__ face,
+                                                    const float* __restrict__ seed, float* __restrict__ dist,
+                                                    int32_t* face,  // no __restrict__: redo may alias it
                                                     float* __restrict__ closest, float* __restrict__ uvw,
                                                     bool use_obbs, const int32_t* redo, GridBound gb) {
   __shared__ int32_t s_ref[kLaneLds * kBlk];
@@ -976,7 +979,49 @@ __global__ __launch_bounds__(kBlk) void k_grid_seed(GridBound gb, const float* _
   const float q[3] = {P[i * 3], P[i * 3 + 1], P[i * 3 + 2]};
   seed[i] = grid_bound(gb, q);
 }
+
+// The points of a side^3 grid, lo + (float)i * h per axis, at their own
+// index (i * side + j) * side + k, and a walk order covering them in 4x4x4
+// bricks (one wavefront each), the bricks along a Morton curve.
+__device__ __forceinline__ uint32_t compact3(uint32_t v) {  // every third bit -> low bits
+  v &= 0x09249249u;
+  v = (v ^ (v >> 2)) & 0x030c30c3u;
+  v = (v ^ (v >> 4)) & 0x0300f00fu;
+  v = (v ^ (v >> 8)) & 0xff0000ffu;
+  v = (v ^ (v >> 16)) & 0x000003ffu;
+  return v;
+}
+__global__ __launch_bounds__(kBlk) void k_grid_points(float lo0, float lo1, float lo2, float h, int side,
+                                                      float* __restrict__ P, uint32_t* __restrict__ perm) {
+  const int64_t n = (int64_t)side * side * side;
+  const int64_t i = blockIdx.x * (int64_t)kBlk + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t b = (uint32_t)(i >> 6), l = (uint32_t)(i & 63);
+  const uint32_t x = compact3(b) * 4 + (l & 3), y = compact3(b >> 1) * 4 + ((l >> 2) & 3),
+                 z = compact3(b >> 2) * 4 + (l >> 4);
+  const int64_t q = ((int64_t)x * side + y) * side + z;
+  perm[i] = (uint32_t)q;
+  P[q * 3] = lo0 + (float)x * h;
+  P[q * 3 + 1] = lo1 + (float)y * h;
+  P[q * 3 + 2] = lo2 + (float)z * h;
+}
 }  // namespace
+
+// Grid points for the exact-distance grid of a3d_udf_grid_seed /
+// a3d_bvh_udf_grid: P float32[side^3 * 3] = lo3 + (i, j, k) h (host lo3;
+// float: lo + (float)i * h) at index (i side + j) side + k, and perm
+// uint32[side^3], a walk order for a3d_bvh_udf (4x4x4 bricks along a Morton
+// curve: no sort needed).  side = 4 * 2^m, 4 <= side <= 1024.
+extern "C" int a3d_grid_points(const float* lo3, float h, int side, float* P, uint32_t* perm, a3d_stream_t stream) {
+  if (!lo3 || !P || !perm) return fail(A3D_ERR_ARG, "a3d_grid_points: null argument");
+  if (side < 4 || side > 1024 || (side & (side - 1)) || !(h > 0.0f))
+    return fail(A3D_ERR_ARG, "a3d_grid_points: side=%d must be a power of two in [4, 1024], h=%g > 0", side, (double)h);
+  const int64_t n = (int64_t)side * side * side;
+  hipLaunchKernelGGL(k_grid_points, dim3(grid_for(n)), dim3(kBlk), 0, (hipStream_t)stream, lo3[0], lo3[1], lo3[2], h,
+                     side, P, perm);
+  A3D_LAUNCH_CHECK();
+  return A3D_OK;
+}
 
 // Upper bounds on the closest-point distance of each query from a grid of
 // exact distances: G[(i (R+1) + j) (R+1) + k] = the distance at lo + (i, j,

@@ -235,7 +235,8 @@ class BVHAccelerator:
     UDF_REDO = 16
     ORDER_MIN_QUERIES = 4096
     SEED_MIN_QUERIES = 1 << 15
-    SEED_GRID_RES = 128
+    SEED_GRID_RES = 127  # (R+1)^3 = 2^21 grid points
+    SEED_AMORTIZE = 4
     SPLIT_MAX_QUERIES = 1 << 15
     LANE_MAX_QUERIES = 1 << 17
 
@@ -255,13 +256,18 @@ class BVHAccelerator:
 
     def seed_grid(self, res: int | None = None):
         """(grid, grid_face, lo, h), or None for a mesh with non-finite
-        coordinates: exact distances and closest faces at the
-        (R+1)^3 points lo + (i, j, k) h (float32 arithmetic, R =
-        SEED_GRID_RES) of a cube holding the mesh with a 5 % margin, built
-        once per mesh (and resolution ``res``, default SEED_GRID_RES) by the
-        plain walk; the source of the grid-seeded walk's per-query distance
+        coordinates: exact distances and closest faces at the (R+1)^3 points
+        lo + (i, j, k) h (float32 arithmetic, R = ``res``, default
+        SEED_GRID_RES; R + 1 a power of two) of a cube holding the mesh with
+        a 5 % margin, built once per mesh and resolution: the points are
+        generated on the device in a walk order of 4x4x4 bricks along a
+        Morton curve (a3d_grid_points), then walked once by the plain wave
+        walk.  The source of the grid-bounded walk's per-query distance
         bounds."""
         R = int(res or self.SEED_GRID_RES)
+        side = R + 1
+        if side < 4 or side & (side - 1):
+            raise ValueError(f"seed grid resolution R={R}: R + 1 must be a power of two >= 4")
         grids = self.__dict__.setdefault("_seed_grids", {})
         if R not in grids:
             import math
@@ -277,12 +283,35 @@ class BVHAccelerator:
             pad = 0.05 * ext + 1e-6
             h = float(np.float32((ext + 2 * pad) / R))
             lo = [float(np.float32(x)) for x in (lo_b - pad).tolist()]
-            i = torch.arange(R + 1, dtype=torch.float32, device=self.device)
-            axes = [torch.full_like(i, lo[d]) + i * torch.full_like(i, h) for d in range(3)]
-            pts = torch.stack(torch.meshgrid(*axes, indexing="ij"), dim=-1).reshape(-1, 3).contiguous()
-            d, f, _, _ = self.udf(pts, seed=False)
+            n = side ** 3
+            with torch.cuda.device(self.device):
+                st = _lib.stream(self.device)
+                pts = torch.empty(n, 3, dtype=torch.float32, device=self.device)
+                perm = torch.empty(n, dtype=torch.int32, device=self.device)
+                _lib.call("a3d_grid_points", (ctypes.c_float * 3)(*lo), h, side, _lib.ptr(pts), _lib.ptr(perm), st)
+                d = torch.empty(n, dtype=torch.float32, device=self.device)
+                f = torch.empty(n, dtype=torch.int32, device=self.device)
+                c = torch.empty(n, 3, dtype=torch.float32, device=self.device)
+                _lib.call("a3d_bvh_udf", _lib.ptr(self.blob), _lib.ptr(pts), n, _lib.ptr(perm), _lib.ptr(d),
+                          _lib.ptr(f), _lib.ptr(c), None, 0, None, st)
+                del pts, perm, c
             grids[R] = (d, f, lo, h)
         return grids[R] or None
+
+    def _grid_pays(self, K: int) -> bool:
+        """Whether this call should use the exact-distance grid: built
+        already, or (from SEED_MIN_QUERIES queries) once this mesh has
+        answered SEED_AMORTIZE grids' worth of queries without it, so that
+        one-shot calls never pay for the grid (about a plain walk of
+        (R+1)^3 queries) and repeated calls on a mesh get it."""
+        grids = self.__dict__.get("_seed_grids", {})
+        if self.SEED_GRID_RES in grids:
+            return bool(grids[self.SEED_GRID_RES])
+        if K < self.SEED_MIN_QUERIES or self.num_faces == 0:
+            return False
+        seen = self.__dict__.get("_queries_seen", 0)
+        self._queries_seen = seen + K
+        return seen >= self.SEED_AMORTIZE * (self.SEED_GRID_RES + 1) ** 3
 
     def udf(self, points: torch.Tensor, flags: int | None = None, order: bool | None = None,
             counters: torch.Tensor | None = None, seed: bool | None = None):
@@ -292,10 +321,11 @@ class BVHAccelerator:
         the packet walk in Morton order; UDF_LANE, UDF_DIRECT or 0 force one);
         ``order`` (default: K >= 4096) walks the queries in Morton order;
         ``counters`` (int64[5] on the device) accumulates the packet walk's
-        work counts; ``seed`` (default: from 2^15 queries on a non-empty
-        mesh, packet or per-lane walk) bounds each query's walk from the distance grid of
-        ``seed_grid`` and walks the few queries whose bound fell short again
-        without one.  Results are identical either way."""
+        work counts; ``seed`` bounds each query's walk from the distance
+        grid of ``seed_grid`` and walks the few queries whose bound fell
+        short again without one (default, packet or per-lane walk: once the
+        grid pays for itself, see ``_grid_pays``; True builds it now).
+        Results are identical either way."""
         p = self._points(points)
         K = p.shape[0]
         dist = torch.empty(K, dtype=torch.float32, device=self.device)
@@ -316,8 +346,7 @@ class BVHAccelerator:
         if order is None:
             order = self.ORDER_MIN_QUERIES <= K < 2 ** 31
         if seed is None:
-            seed = (K >= self.SEED_MIN_QUERIES and self.num_faces > 0 and flags in (0, self.UDF_LANE)
-                    and counters is None)
+            seed = flags in (0, self.UDF_LANE) and counters is None and self._grid_pays(K)
         perm = self.query_order(p) if order and K > 0 else None
         with torch.cuda.device(self.device):
             st = _lib.stream(self.device)

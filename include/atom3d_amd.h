@@ -55,10 +55,12 @@ const char* a3d_version(void);
  *
  * a3d_bvh_sizes: bytes of the BVH blob and of the build workspace for nF faces.
  * a3d_bvh_build_ex: V float32[nV*3], F int32[nF*3] (device) -> blob.
- *   treelet_passes in [0, 8].  With 0 the build is asynchronous; with > 0 it
- *   synchronises the stream a few times per pass (the number of rounds a
- *   pass needs is read back).  A refined tree deeper than 96 internal levels
- *   is discarded for the plain one.
+ *   treelet_passes in [0, 8].  Synchronises the stream once at the end to
+ *   check the tree depth against the walks' stacks (A3D_ERR_CAPACITY beyond
+ *   99 internal levels; a Morton tree stays within 96), and with > 0 passes
+ *   a few times per pass (the number of rounds a pass needs is read back).
+ *   A refined tree deeper than 96 internal levels is discarded for the
+ *   plain one.
  * a3d_bvh_build: a3d_bvh_build_ex with A3D_BVH_TREELET_PASSES passes.
  */
 #define A3D_BVH_TREELET_PASSES 0
@@ -126,6 +128,14 @@ int a3d_bvh_udf(const void* blob, const float* P, int64_t K, const uint32_t* per
  * queries in the walk's order (e.g. a3d_query_order's). */
 int a3d_bvh_udf_seeded(const void* blob, const float* P, int64_t K, const uint32_t* perm, const float* seed,
                        float* dist, int32_t* face, float* closest, float* uvw, int flags, a3d_stream_t stream);
+/* a3d_grid_points: the grid's points for a3d_udf_grid_seed / a3d_bvh_udf_grid
+ * (R + 1 = side): P float32[side^3 * 3] = lo3 + (i, j, k) h (host lo3;
+ * float: lo + (float)i * h) at index (i side + j) side + k, and perm
+ * uint32[side^3], a walk order for a3d_bvh_udf in 4x4x4 bricks along a
+ * Morton curve, so the grid's exact distances come from one plain walk with
+ * no sort.  side a power of two in [4, 1024].  No counterpart in the
+ * reference. */
+int a3d_grid_points(const float* lo3, float h, int side, float* P, uint32_t* perm, a3d_stream_t stream);
 int a3d_udf_grid_seed(const float* grid, const int32_t* grid_face, const float* V, const int32_t* F, int R,
                       const float* lo3, float h, const float* P, const uint32_t* perm, int64_t K, float margin,
                       float* seed, a3d_stream_t stream);

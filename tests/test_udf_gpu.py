@@ -13,6 +13,7 @@ import torch
 import oracle
 from atom3d_amd import MeshBVH, datasets
 from atom3d_amd.kernels.bvh import BVHAccelerator
+from conftest import sphere_union_oracle
 
 pytestmark = pytest.mark.gpu
 DEV = "cuda:0"
@@ -25,11 +26,17 @@ def _gpu_udf(V, F, P, **kw):
     return d.cpu().numpy(), f.cpu().numpy(), c.cpu().numpy(), w.cpu().numpy()
 
 
-# flags: 0 wave walk, 1 per-lane walk, 2 wave walk without the exact-test
-# queue, +4 without the oriented-box bounds (axis-aligned boxes only)
-VARIANTS = [dict(order=True), dict(order=False), dict(order=True, flags=1), dict(order=False, flags=1),
+# flags: 0 wave walk (exact-test queue, oriented-box child bounds, per-triangle
+# oriented-box queue filter), 1 per-lane walk, 2 wave walk without the
+# exact-test queue, +4 without the oriented-box bounds (axis-aligned boxes
+# only), 8 split walk; no flags: the size-based default; seed=True: each walk
+# bounded from the exact-distance grid (the headline path).
+VARIANTS = [dict(order=True), dict(order=False), dict(order=True, flags=0), dict(order=False, flags=0),
+            dict(order=True, flags=1), dict(order=False, flags=1),
             dict(order=True, flags=2), dict(order=True, flags=4), dict(order=False, flags=5),
-            dict(order=False, flags=8), dict(order=True, flags=8), dict(order=False, flags=12)]
+            dict(order=False, flags=8), dict(order=True, flags=8), dict(order=False, flags=12),
+            dict(order=True, flags=0, seed=True), dict(order=False, flags=0, seed=True),
+            dict(order=True, flags=1, seed=True)]
 
 
 def _assert_bitexact(V, F, P, label):
@@ -100,9 +107,9 @@ def test_udf_empty_mesh_and_empty_queries():
 
 
 def test_grid_seeded_walk_equals_plain_walk():
-    """The default path from 2^20 queries bounds each walk from the exact
-    distance grid: bit-identical to the plain walk, inside and far outside
-    the grid; the grid's points hold the plain walk's distances."""
+    """The grid-bounded walk (seed=True; the default once the grid pays for
+    itself) is bit-identical to the plain walk, inside and far outside the
+    grid; the grid's points hold the plain walk's distances."""
     V, F = datasets.noisy_icosphere()
     b = BVHAccelerator(V.to(DEV), F.to(DEV))
     rng = np.random.default_rng(21)
@@ -111,7 +118,7 @@ def test_grid_seeded_walk_equals_plain_walk():
     Pd = torch.tensor(P, device=DEV)
     print(f"[udf grid seed] K={len(P)}", flush=True)
     plain = b.udf(Pd, seed=False)
-    seeded = b.udf(Pd)
+    seeded = b.udf(Pd, seed=True)
     assert b.SEED_MIN_QUERIES <= len(P)
     for x, y in zip(plain, seeded):
         assert torch.equal(x, y)
@@ -153,8 +160,30 @@ def test_grid_path_declines_non_finite_meshes():
     b.vertices[0, 0] = float("nan")
     assert b.seed_grid() is None
     P = datasets.uniform_points(1 << 20, seed=8).to(DEV)
-    r = b.udf(P)
+    r = b.udf(P, seed=True)
     for x, y in zip(r, b.udf(P, seed=False)):
+        assert torch.equal(x, y)
+
+
+def test_grid_policy_one_shot_calls_do_not_build():
+    """One call, however large, walks without the grid (building it costs
+    about a plain walk of (R+1)^3 queries); once a mesh has answered
+    SEED_AMORTIZE grids' worth of queries the next call builds and uses it;
+    every call's results are the plain walk's."""
+    V, F = datasets.noisy_icosphere(subdivisions=4, seed=6)
+    b = BVHAccelerator(V.to(DEV), F.to(DEV))
+    P = datasets.uniform_points(1 << 21, seed=12).to(DEV)
+    ref = b.udf(P, seed=False)
+    need = b.SEED_AMORTIZE * (b.SEED_GRID_RES + 1) ** 3
+    calls = 0
+    while "_seed_grids" not in b.__dict__:
+        got = b.udf(P)
+        calls += 1
+        for x, y in zip(ref, got):
+            assert torch.equal(x, y)
+        assert calls <= need // P.shape[0] + 1
+    assert calls == need // P.shape[0] + 1
+    for x, y in zip(ref, b.udf(P)):
         assert torch.equal(x, y)
 
 
@@ -311,3 +340,34 @@ def test_seeded_walk_equals_plain_and_flags_short_seeds(flags):
     assert np.all(f[s] == -1) and np.array_equal(d[s], seed.cpu().numpy()[s])
     ok = ~short.cpu().numpy()
     assert np.array_equal(f[ok], rf[ok]) and np.array_equal(d[ok], rd[ok]) and np.array_equal(c[ok], rc[ok])
+
+
+@pytest.mark.timeout(900)
+def test_udf_sphere_union_full_size_bitexact_sample():
+    """The headline configuration end to end: 4,096,000 faces, all 2^24
+    uniform queries through MeshBVH.udf (default path: Morton order, wave
+    walk bounded by the exact-distance grid once the grid is built) -- a
+    2,048-query sample bit-exact (distance, face, closest point, uvw)
+    against the oracle's scan of all 4,096,000 faces, and the FP64 error of
+    256 of them stated as a fraction of the bounding-box diagonal."""
+    V, F, _ = sphere_union_oracle()
+    m = MeshBVH(V.to(DEV), F.to(DEV))
+    P = datasets.uniform_points(1 << 24, seed=1).to(DEV)
+    print("[udf c2] 2^24 queries, 4.1M faces", flush=True)
+    first = m.udf(P, return_closest=True, return_face_ids=True, return_uvw=True)
+    r = m.udf(P, return_closest=True, return_face_ids=True, return_uvw=True)
+    for x, y in zip((first.distances, first.face_ids, first.closest_points, first.uvw),
+                    (r.distances, r.face_ids, r.closest_points, r.uvw)):
+        assert torch.equal(x, y)  # the first call (plain walk) and a later one (grid-bounded walk) agree
+    idx = torch.randperm(P.shape[0], generator=torch.Generator().manual_seed(3))[:2048].to(DEV)
+    Pn = P[idx].cpu().numpy()
+    rd, rf, rc, rw = oracle.brute_udf(V.numpy(), F.numpy(), Pn)
+    np.testing.assert_array_equal(r.distances[idx].cpu().numpy(), rd)
+    np.testing.assert_array_equal(r.face_ids[idx].cpu().numpy(), rf)
+    np.testing.assert_array_equal(r.closest_points[idx].cpu().numpy(), rc)
+    np.testing.assert_array_equal(r.uvw[idx].cpu().numpy(), rw)
+    diag = float(np.linalg.norm(V.numpy().max(0) - V.numpy().min(0)))
+    dd, _ = oracle.fp64_udf(V.numpy(), F.numpy(), Pn[:256])
+    err = np.abs(rd[:256].astype(np.float64) - dd)
+    print(f"[udf c2] FP64 error: max {err.max() / diag:.3e}, mean {err.mean() / diag:.3e} of the diagonal")
+    assert err.max() <= 1e-6 * diag
