@@ -28,6 +28,8 @@
 // extra nodes a wave visits.
 #include <hipcub/hipcub.hpp>
 
+#include <mutex>
+
 #include "a3d_common.hpp"
 
 namespace a3d {
@@ -914,6 +916,51 @@ extern "C" int a3d_query_order(const float* P, int64_t K, uint32_t* perm, void* 
   size_t tb = w.cub_bytes;
   A3D_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(w.cub, tb, w.k0, w.k1, w.v0, perm, (int)K, 0, 30, s));
   return A3D_OK;
+}
+
+// ------------------------------------------------------------ allocation-free query order
+// a3d_query_order with its workspace in a file-scope device array instead
+// of a caller buffer, for callers that cannot allocate.  Each GPU holds its
+// own copy of the array (kNoAllocScratch bytes) until the process ends; its
+// device address is looked up once per device with the device current, and
+// host code uses only that address (the array's name is a host-side stand-in).
+namespace {
+constexpr int64_t kNoAllocScratch = int64_t(48) << 20;  // 2^21 queries need ~42 MB (keys, values, sort state)
+constexpr int64_t kNoAllocMaxQueries = int64_t(1) << 21;
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) unsigned char scratch[kNoAllocScratch];
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_ptr(void** out) {
+  int dev = 0;
+  A3D_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(A3D_ERR_INTERNAL, "a3d_query_order_noalloc: device %d >= %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    A3D_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(A3D_ERR_INTERNAL, "a3d_query_order_noalloc: no device address for the scratch array");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return A3D_OK;
+}
+}  // namespace
+
+extern "C" int a3d_query_order_noalloc(const float* P, int64_t K, uint32_t* perm, a3d_stream_t stream) {
+  if (K < 0 || K > kNoAllocMaxQueries)
+    return fail(A3D_ERR_CAPACITY, "a3d_query_order_noalloc: K=%lld not in [0, %lld] (fixed scratch of %lld bytes)",
+                (long long)K, (long long)kNoAllocMaxQueries, (long long)kNoAllocScratch);
+  int64_t need = 0;
+  if (int rc = a3d_query_order_ws_bytes(K, &need)) return rc;
+  if (need > kNoAllocScratch)
+    return fail(A3D_ERR_CAPACITY, "a3d_query_order_noalloc: K=%lld needs %lld bytes of scratch > %lld",
+                (long long)K, (long long)need, (long long)kNoAllocScratch);
+  if (K == 0) return A3D_OK;
+  void* ws = nullptr;
+  if (int rc = scratch_ptr(&ws)) return rc;
+  return a3d_query_order(P, K, perm, ws, kNoAllocScratch, stream);
 }
 
 namespace {

@@ -104,6 +104,16 @@ int a3d_bvh_info(const void* blob, int64_t* out4, a3d_stream_t stream);
 int a3d_query_order_ws_bytes(int64_t K, int64_t* bytes);
 int a3d_query_order(const float* P, int64_t K, uint32_t* perm, void* ws, int64_t ws_bytes,
                     a3d_stream_t stream);
+/* a3d_query_order_noalloc: a3d_query_order (same perm, bit for bit) that
+ * allocates nothing and takes no workspace: the workspace is a file-scope
+ * device array of 48 MiB that each GPU holds until the process ends.  For
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc); with a3d_bvh_udf it gives the
+ * ordered closest-point path without allocation.  K <= 2^21 (else
+ * A3D_ERR_CAPACITY).  Every call on a GPU shares that GPU's array: calls
+ * that overlap on one GPU (concurrent streams or threads) race.  No
+ * counterpart in the reference. */
+int a3d_query_order_noalloc(const float* P, int64_t K, uint32_t* perm, a3d_stream_t stream);
 int a3d_bvh_udf(const void* blob, const float* P, int64_t K, const uint32_t* perm, float* dist,
                 int32_t* face, float* closest, float* uvw, int flags, int64_t* counters,
                 a3d_stream_t stream);
