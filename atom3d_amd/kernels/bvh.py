@@ -274,11 +274,16 @@ class BVHAccelerator:
 
             import numpy as np
 
-            lo_b = self.vertices.min(dim=0).values.double()
-            hi_b = self.vertices.max(dim=0).values.double()
-            ext = float((hi_b - lo_b).max())
-            if not (math.isfinite(ext) and all(math.isfinite(x) for x in lo_b.tolist())):
+            # the exact box of the referenced vertices, from the blob header (one 256-byte copy)
+            hdr = self.blob[:256].cpu().numpy()
+            box = hdr[32:56].view(np.float32).astype(np.float64)
+            if not np.all(np.isfinite(box)) or not bool(torch.isfinite(self.vertices).all()):
                 grids[R] = False  # non-finite coordinates: the plain walk only
+                return None
+            lo_b, hi_b = box[:3], box[3:]
+            ext = float((hi_b - lo_b).max())
+            if not math.isfinite(ext):
+                grids[R] = False
                 return None
             pad = 0.05 * ext + 1e-6
             h = float(np.float32((ext + 2 * pad) / R))

@@ -60,16 +60,70 @@ def parse():
     return ap.parse_args()
 
 
-def traffic_from_profiles():
-    """HBM bytes per launch of the traversal kernel, from the committed
-    rocprofv3 counter summary (line ``hbm_bytes_per_launch: N``), if present."""
-    p = os.path.join(ROOT, "profiles", "k_udf_wave_sphere_union_counters.md")
+def profile_figures(name):
+    """``key: value`` figures of a committed rocprofv3 summary
+    (profiles/<name>.md), or {} when absent."""
+    p = os.path.join(ROOT, "profiles", name + ".md")
     if not os.path.exists(p):
-        return None
+        return {}
     import re
 
-    m = re.search(r"hbm_bytes_per_launch:\s*([0-9.eE+]+)", open(p).read())
-    return float(m.group(1)) if m else None
+    out = {}
+    for m in re.finditer(r"^([a-z_0-9]+):\s*([-0-9.eE+]+)\s*$", open(p).read(), re.M):
+        out[m.group(1)] = float(m.group(2))
+    return out
+
+
+def roofline(kernel, kernel_ms, profile, algorithmic=None):
+    """The roofline object of a kernel from its committed counter summary:
+    achieved = the HBM bytes it moves per launch (FETCH_SIZE + WRITE_SIZE,
+    scaled by the copy-kernel calibration) / its HIP-event-timed launch time,
+    frac = achieved / 8 TB/s; plus its VALU busy fraction (rocprofv3's
+    VALUBusy), VALU instructions per wave, L2 hit rate and occupancy where
+    collected.  ``algorithmic`` (bytes per launch of the
+    reference algorithm's work, and a note) is reported beside it, labelled:
+    it counts work this kernel skips, so it is not a rate it moves."""
+    f = profile_figures(profile)
+    sec = kernel_ms * 1e-3
+    traffic = f.get("hbm_bytes_per_launch")
+    r = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None, "traffic": traffic,
+         "kernel": kernel, "kernel_ms": kernel_ms, "profile": f"profiles/{profile}.md" if f else None}
+    if traffic:
+        r["achieved"] = traffic / sec / 1e9
+        r["frac"] = r["achieved"] / HBM_PEAK_GBS
+    if "valu_busy_percent" in f:
+        # rocprofv3's VALUBusy: SQ_ACTIVE_INST_VALU / (CUs x GRBM_GUI_ACTIVE), the share of the kernel's time
+        # the CUs' vector ALUs are issuing: the bound of a kernel that moves little memory
+        r["valu_busy_frac"] = f["valu_busy_percent"] / 100.0
+    for k in ("valu_insts_per_wave", "salu_insts_per_wave", "l2_hit_rate", "occupancy_percent",
+              "valu_utilization_percent", "kernel_ms_profiled"):
+        if k in f:
+            r[k] = f[k]
+    if algorithmic:
+        b, note = algorithmic
+        r["algorithmic"] = {"bytes_per_launch": b, "GBps": b / sec / 1e9, "note": note}
+    return r
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def median_of(fn, runs=5):
+    """One warm-up run, then the median wall time of `runs` runs (s)."""
+    fn()
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
 
 
 def main():
@@ -79,6 +133,14 @@ def main():
               "multi-GPU query sharding is not built.", file=sys.stderr)
         sys.exit(2)
 
+    if a.copy_probe_mb:  # counter calibration: one elementwise kernel reading and writing copy_probe_mb MiB
+        import torch
+
+        src = torch.ones(a.copy_probe_mb << 18, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.mul(src, 2.0, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
     if a.workload == "dmc":
         return bench_dmc(a)
     if a.workload in ("octree", "flood"):
@@ -105,24 +167,22 @@ def main():
     V, F = datasets.sphere_union()
     P = datasets.uniform_points(a.queries, seed=1).to(dev)
     Vd, Fd = V.to(dev), F.to(dev)
+    K = P.shape[0]
     torch.cuda.synchronize()
+    # runtime start-up (kernel loading, allocator pools) outside every timing
+    BVHAccelerator(Vd[:3 * 64], Fd[:64] % 64).udf(P[:1 << 16])
+    torch.cuda.synchronize()
+
+    # cold call: a fresh tree and one closest-point call on it (Morton order +
+    # the plain walk: a mesh's first call never builds the distance grid)
     t0 = time.perf_counter()
     bvh = BVHAccelerator(Vd, Fd)
     torch.cuda.synchronize()
     build_s = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    bvh = BVHAccelerator(Vd, Fd)  # second build: steady-state (allocator warm)
+    bvh.udf(P)
     torch.cuda.synchronize()
-    build_s = min(build_s, time.perf_counter() - t0)
+    cold_s = time.perf_counter() - t0
 
-    if a.copy_probe_mb:
-        src = torch.ones(a.copy_probe_mb << 18, device=dev)
-        dst = torch.empty_like(src)
-        torch.mul(src, 2.0, out=dst)  # one elementwise kernel: reads and writes copy_probe_mb MiB
-        torch.cuda.synchronize()
-        del src, dst
-
-    K = P.shape[0]
     dist = torch.empty(K, device=dev)
     face = torch.empty(K, dtype=torch.int32, device=dev)
     closest = torch.empty(K, 3, device=dev)
@@ -135,8 +195,15 @@ def main():
     sp = stream.cuda_stream
     lib = _lib.lib()
     order_args = (_lib.ptr(P), K, _lib.ptr(perm), _lib.ptr(ws), wsb.value, sp)
-    # the distance grid that bounds each walk: built once per mesh, like the tree
+    # the distance grid that bounds each walk: built once per mesh, like the
+    # tree (what MeshBVH.udf does once a mesh has answered a few grids' worth
+    # of queries); timed twice: the process's first build and a rebuild
     torch.cuda.synchronize()
+    tg = time.perf_counter()
+    bvh.seed_grid()
+    torch.cuda.synchronize()
+    grid_first_s = time.perf_counter() - tg
+    bvh.__dict__.pop("_seed_grids")
     tg = time.perf_counter()
     grid, gface, glo, gh = bvh.seed_grid()
     torch.cuda.synchronize()
@@ -171,7 +238,7 @@ def main():
     ms_per_step = wall * 1e3 / a.steps
     qps = K / (ms_per_step * 1e-3)
 
-    # --- oracle: reference work counts and CPU baseline on a bounded sample
+    # --- oracle: reference work counts and CPU baseline on bounded samples
     cpu = None
     n_node = n_tri = None
     if not a.no_cpu:
@@ -181,26 +248,29 @@ def main():
         tb = time.perf_counter()
         sah = oracle.SahBvh(V.numpy(), F.numpy())
         sah_build_s = time.perf_counter() - tb
-        sample = P[: a.cpu_sample].cpu().numpy()
-        sah.udf(sample[:4096], threads=cores)  # warm-up
-        tq = time.perf_counter()
+        n_all, n_one = min(1 << 20, K), min(1 << 15, K)
+        sample = P[:n_all].cpu().numpy()
+        t_all = median_of(lambda: sah.udf(sample, threads=cores))
+        t_one = median_of(lambda: sah.udf(sample[:n_one], threads=1))
         _, _, _, _, nv, nt = sah.udf(sample, threads=cores, counts=True)
-        cpu_s = time.perf_counter() - tq
         n_node, n_tri = float(nv.mean()), float(nt.mean())
         cpu = {
-            "value": len(sample) / cpu_s,
+            "value": n_all / t_all,
             "unit": "queries/s",
             "cores": cores,
             "kind": "port",
-            "sample": f"{len(sample)} of the {K} queries on the same 4.1M-face mesh; the reference SAH "
-                      f"tree walk restated in C++ (OpenMP, {cores} threads); SAH build {sah_build_s:.1f} s "
-                      f"single-threaded, not included",
+            "single_thread_value": n_one / t_one,
+            "cpu_model": cpu_model(),
+            "protocol": "one warm-up run, then the median of 5",
+            "sample": f"all cores: the first {n_all} of the {K} queries; single thread: the first {n_one}; the "
+                      f"reference's SAH tree walk restated in C++ (OpenMP), on the reference's SAH tree of the "
+                      f"same 4.1M-face mesh, whose single-threaded host build took {sah_build_s:.1f} s (not "
+                      f"included)",
+            "sah_build_s": sah_build_s,
         }
     if n_node is None:
-        n_node, n_tri = 245.06, 212.58  # oracle counts on a 65,536-query sample of this workload
+        n_node, n_tri = 245.06, 212.58  # oracle counts on a 2^20-query sample of this workload
     bq = 44.0 + 32.0 * n_node + 36.0 * n_tri
-    achieved = bq * K / (kern_ms * 1e-3) / 1e9
-    traffic = traffic_from_profiles()
     sg = None
     if not a.no_surrogate:
         r = measure_surrogate(MeshBVH(Vd, Fd), V, P, a, fp64_check=False)
@@ -232,24 +302,17 @@ def main():
             "queries": K,
             "parallelism": "single GPU",
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
-            "kernel": "k_udf_wave",
-            "kernel_ms": kern_ms,
-            "timed_launches": "k_udf_wave (grid bounds in its prologue) + the redo pass (k_udf_lane, queries "
-                              "whose bound fell short)",
-            "bytes_per_query": bq,
-            "n_node": n_node,
-            "n_tri": n_tri,
-        },
+        "roofline": roofline(
+            "k_udf_wave", kern_ms, "k_udf_wave_sphere_union_counters",
+            (bq * K, f"reference algorithm: B_q = 44 + 32 N_node + 36 N_tri = {bq:.0f} B per query with the "
+                     f"reference SAH walk's N_node = {n_node:.1f}, N_tri = {n_tri:.1f} (oracle counts)")),
         "cpu_baseline": cpu,
         "bvh_build_s": build_s,
+        "cold_call": {"queries_per_s": K / cold_s, "s": cold_s,
+                      "what": "a fresh BVHAccelerator on the mesh + one udf() call of all queries (Morton order "
+                              "and the plain walk: no grid on a first call)"},
         "seed_grid_build_s": grid_s,
+        "seed_grid_first_build_s": grid_first_s,
         "surrogate_fallback": sg,
     }
     print(json.dumps(out), flush=True)
@@ -317,7 +380,6 @@ def bench_dmc(a):
     n_surf = int(((sdf.detach()[ci.long()] < 0).sum(1) % 8 != 0).sum())
     fwd_bytes = 48 * N + 4 * M + 12 * V + 24 * Ftri
     bwd_bytes = 12 * V + 4 * M + 48 * n_surf
-    achieved = (fwd_bytes + bwd_bytes) / (ms * 1e-3) / 1e9
 
     # eval mode: the reference's default QEF placement (corner gradients
     # estimated, one regularised 10x3 least-squares solve per dual vertex)
@@ -382,9 +444,10 @@ def bench_dmc(a):
         "backward_ms": bwd_ms,
         "forward_cubes_per_s": N / (fwd_ms * 1e-3),
         "qef_eval": qef_out,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "whole step",
-                     "bytes_fwd": fwd_bytes, "bytes_bwd": bwd_bytes},
+        "roofline": roofline("whole step (forward + backward kernels)", ms, "dmc_band1024_counters",
+                             (fwd_bytes + bwd_bytes,
+                              f"forward 48 N + 4 M + 12 V + 24 F = {fwd_bytes} B, backward 12 V + 4 M + 48 N_surf "
+                              f"= {bwd_bytes} B (inputs read and outputs written once)")),
         "cpu_baseline": cpu,
         "band_build_s": gen_s,
     }
@@ -444,7 +507,7 @@ def bench_octree_flood(a):
             ref, ref_tested, _, (wn, wt) = sah.octree([-1, -1, -1, 1, 1, 1], 4, 10, threads=cores, full=True,
                                                       work=True)
             cpu_s = time.perf_counter() - tq
-            assert len(ref) == n_vox
+            assert np.array_equal(ref, vox.cpu().numpy()), "octree differs from the oracle"
             n_node, n_tri = wn / ref_tested.sum(), wt / ref_tested.sum()
             cpu = {"value": n_vox / cpu_s, "unit": "voxels/s", "cores": cores, "kind": "port",
                    "sample": f"the whole workload (levels 4-10, {int(ref_tested.sum())} cube tests); the "
@@ -453,7 +516,9 @@ def bench_octree_flood(a):
         if n_node is None:
             n_node, n_tri = 29.7, 4.4
         unit_bytes = 13 + 32 * n_node + 36 * n_tri
-        achieved = unit_bytes * tested / (ms * 1e-3) / 1e9
+        units, kernel_name, kernel_profile = tested / len(counts), "k_octree_test", "k_octree_test_sphere_union_counters"
+        alg_note = (f"reference algorithm per cube test: 13 + 32 N_node + 36 N_tri = {unit_bytes:.0f} B (its full pair "
+                    f"walk, oracle counts), times the mean cube tests per level launch")
         value, unit = n_vox / (ms * 1e-3), "voxels/s"
         extra = {"cubes_tested": tested, "cube_tests_per_s": tested / (ms * 1e-3), "levels": counts,
                  "n_node": n_node, "n_tri": n_tri}
@@ -465,7 +530,8 @@ def bench_octree_flood(a):
         crop = st.D * st.H * st.W
         f_water = st.totals[PLANE["water"]] / crop
         unit_bytes = 1 / 8 + 4 + 8 * f_water
-        achieved = unit_bytes * crop / (ms * 1e-3) / 1e9
+        units, kernel_name, kernel_profile = crop, "whole step", "flood_sphere_union_1024_counters"
+        alg_note = "per crop voxel: 1/8 B occupancy + 4 B label + 8 B per water voxel (the reference's BFS)"
         value, unit = crop / (ms * 1e-3), "crop voxels/s"
         extra = {"crop": [st.D, st.H, st.W], "passes": st.passes, "f_water": f_water,
                  "counts": {k: int(x.shape[0]) for k, x in zip(names, lists)},
@@ -487,14 +553,17 @@ def bench_octree_flood(a):
         metric = f"sparse flood fill crop voxels/s (1024^3, conn {a.conn})"
         workload = f"sparse_flood_fill of the union of 200 icospheres' level-10 voxels ({n_vox}), padding 10, " \
                    f"source (0,0,0), connectivity {a.conn}, k-ring 1"
+    prof = profile_figures(kernel_profile)
+    kms = prof.get("kernel_ms_profiled", ms)
+    rl = roofline(kernel_name, kms, kernel_profile, (unit_bytes * units, alg_note))
+    if "kernel_ms_profiled" not in prof:
+        rl["kernel"] = "whole step (no kernel profile)"
     out = {
         "metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
         "data": "synthetic",
         "config": {"workload": workload, "voxels": n_vox, "parallelism": "single GPU", **extra},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "whole step",
-                     "bytes_per_unit": unit_bytes},
+        "roofline": rl,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -718,7 +787,6 @@ def bench_batch(a):
     if n_node is None:
         n_node, n_tri = 60.0, 50.0
     bq = 44 + 32 * n_node + 36 * n_tri
-    achieved = bq * K / (walk_ms * 1e-3) / 1e9
     res = {
         "metric": "batched uneven UDF queries/s (256 meshes, one graph launch)",
         "value": K / (ms * 1e-3), "unit": "queries/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -730,9 +798,9 @@ def bench_batch(a):
                    "parallelism": "single GPU", "autotuned": best, "autotune_s": tune_s, "bvh_build_s": build_s},
         "per_mesh_launches_queries_per_s": K / (per_mesh_ms * 1e-3),
         "autotune_table": [[c, t] for c, t in table],
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_udf_batch",
-                     "kernel_ms": walk_ms, "bytes_per_query": bq, "n_node": n_node, "n_tri": n_tri},
+        "roofline": roofline("k_udf_batch", walk_ms, "k_udf_batch_uneven256_counters",
+                             (bq * K, f"reference algorithm: B_q = 44 + 32 N_node + 36 N_tri = {bq:.0f} B per "
+                                      f"query (oracle counts on a sample of the meshes)")),
         "cpu_baseline": cpu,
     }
     g.close()
@@ -786,7 +854,6 @@ def bench_ray(a):
                          f"(OpenMP, {cores} threads) on the reference SAH tree (build {sah_s:.1f} s, not "
                          f"included); same face for {agree:.6f} of them"}
     unit_bytes = 24 + 36 * n_node + 40 * n_tri
-    achieved = unit_bytes * K / (ms * 1e-3) / 1e9
     out = {
         "metric": "nearest-hit rays/s at 4M tris", "value": K / (ms * 1e-3), "unit": "rays/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
@@ -796,9 +863,9 @@ def bench_ray(a):
                    "rays": K, "hit_fraction": float(hit.float().mean()), "parallelism": "single GPU",
                    "parity_sdf_points_per_s": (1 << 20) / (ms_par * 1e-3), "parity_sdf_ms": ms_par,
                    "n_node": n_node, "n_tri": n_tri},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_ray_nearest",
-                     "bytes_per_unit": unit_bytes},
+        "roofline": roofline("k_ray_nearest", ms, "k_ray_nearest_sphere_union_counters",
+                             (unit_bytes * K, f"reference algorithm per ray: 24 + 36 N_node + 40 N_tri = "
+                                              f"{unit_bytes:.0f} B (oracle counts)")),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -867,7 +934,6 @@ def bench_clip(a):
                          f"SAH pair walk and Sutherland-Hodgman clip restated in C++ (single thread)"}
         _ = out
     per_pair = 24 + 36 + 16 + 1 + 4 + 96 + 12 + 4
-    achieved = per_pair * n_pairs / (ms3 * 1e-3) / 1e9
     res = {
         "metric": "intersect_aabb mode 3 clipped pairs/s (demo icosphere, 512^3)", "value": n_pairs / (ms3 * 1e-3),
         "unit": "pairs/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms3,
@@ -876,9 +942,8 @@ def bench_clip(a):
                                f"voxels, {n_pairs} SAT pairs, {n_poly} non-empty polygons (demo known answer "
                                f"1,172,044)", "parallelism": "single GPU"},
         "mode1_ms": ms1, "mode1_pairs_per_s": n_pairs / (ms1 * 1e-3), "voxelize_with_faces": vox_leg,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "whole mode-3 call",
-                     "bytes_per_pair": per_pair},
+        "roofline": roofline("whole mode-3 call", ms3, "clip_demo512_counters",
+                             (per_pair * n_pairs, f"{per_pair} B per clipped pair (box, triangle, outputs)")),
         "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)
@@ -923,7 +988,6 @@ def bench_c1(a):
     if n_node is None:
         n_node, n_tri = 40.0, 30.0
     bq = 44 + 32 * n_node + 36 * n_tri
-    achieved = bq * K / (ms * 1e-3) / 1e9
     res = {
         "metric": "closest-point queries/s, noisy icosphere (20,480 tris)", "value": K / (ms * 1e-3),
         "unit": "queries/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms,
@@ -932,9 +996,9 @@ def bench_c1(a):
                                "queries in [-1,1]^3 (seed 0)", "parallelism": "single GPU"},
         "sdf_queries_per_s": K / (ms_sdf * 1e-3),
         "gpu_over_cpu": (K / (ms * 1e-3)) / cpu["value"] if cpu else None,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "whole step (launch-bound size)",
-                     "bytes_per_query": bq, "n_node": n_node, "n_tri": n_tri},
+        "roofline": roofline("whole step (launch-bound size)", ms, "udf_icosphere_c1_counters",
+                             (bq * K, f"reference algorithm: B_q = 44 + 32 N_node + 36 N_tri = {bq:.0f} B per "
+                                      f"query (oracle counts)")),
         "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)
