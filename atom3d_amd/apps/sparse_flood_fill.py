@@ -78,25 +78,35 @@ def sparse_flood_fill(bvh: MeshBVH, resolution: int, source: Tuple[int, int, int
     octree = OctreeIndexer(max_level=max_level, bounds=bounds, device=device)
     vox = octree.octree_traverse(bvh, min_level=min_level)
     t0 = time.time()
-    e3 = torch.empty(0, 3, dtype=torch.int32, device=device)
-    st = flood_planes(vox, resolution, source, _PADDING, connectivity, k_ring) if len(vox) else None
-    if st is None:
-        out = {k: e3.clone() for k in ("dam_ijk", "water_ijk", "dry_ijk", "tide_ijk", "extended_tide_ijk",
-                                        "dry_adjacent_ijk")}
-    else:
-        dt = torch.promote_types(torch.int32, vox.dtype)
-        dam = _out(st.coords("dam"), dt)
-        out = {
-            "dam_ijk": dam,
-            "water_ijk": _out(st.coords("water"), dt),
-            "dry_ijk": _out(st.coords("dry"), dt),
-            "tide_ijk": _out(st.coords("collision"), dt),
-            "extended_tide_ijk": st.coords("ext_tide").int() if len(dam) and k_ring > 0 else e3.clone(),
-            "dry_adjacent_ijk": st.coords("dry_adj").int() if len(dam) and k_ring > 0 else e3.clone(),
-        }
+    out = sparse_flood_fill_from_voxels(vox, resolution, source, connectivity, k_ring)
     if verbose:
         logger.info(f"Sparse Flood Fill completed in {time.time() - t0:.4f}s")
     return out
+
+
+def sparse_flood_fill_from_voxels(vox: torch.Tensor, resolution: int, source: Tuple[int, int, int] = (0, 0, 0),
+                                  connectivity: int = 26, k_ring: int = 1) -> Dict[str, torch.Tensor]:
+    """sparse_flood_fill's output dict for given surface voxels ([N, 3] ijk
+    on the device; the octree's level-``log2(resolution)`` voxels):
+    the flood, classification and k-ring steps of
+    apps/sparse_flood_fill.py:134-166 without the octree."""
+    if k_ring > _PADDING:
+        raise NotImplementedError(f"k_ring > {_PADDING} reaches outside the flood crop")
+    e3 = torch.empty(0, 3, dtype=torch.int32, device=vox.device)
+    st = flood_planes(vox, resolution, source, _PADDING, connectivity, k_ring) if len(vox) else None
+    if st is None:
+        return {k: e3.clone() for k in ("dam_ijk", "water_ijk", "dry_ijk", "tide_ijk", "extended_tide_ijk",
+                                         "dry_adjacent_ijk")}
+    dt = torch.promote_types(torch.int32, vox.dtype)
+    dam = _out(st.coords("dam"), dt)
+    return {
+        "dam_ijk": dam,
+        "water_ijk": _out(st.coords("water"), dt),
+        "dry_ijk": _out(st.coords("dry"), dt),
+        "tide_ijk": _out(st.coords("collision"), dt),
+        "extended_tide_ijk": st.coords("ext_tide").int() if len(dam) and k_ring > 0 else e3.clone(),
+        "dry_adjacent_ijk": st.coords("dry_adj").int() if len(dam) and k_ring > 0 else e3.clone(),
+    }
 
 
 def _check_cells(t: torch.Tensor, resolution: int, name: str) -> None:

@@ -206,12 +206,19 @@ __global__ void k_iota(int32_t* p, int64_t n) {
 
 __device__ __forceinline__ uint64_t vkey(int64_t x, int64_t y, int64_t z, int64_t R) { return (uint64_t)((x * R + y) * R + z); }
 
+// Problematic-cube flags and voxel keys of the surface cubes; the flags of
+// [Ns, N) are cleared too, because the selection that follows reads all N
+// (the surface count exists only on the device when it is launched).
 __global__ void k_prob(const Tables* __restrict__ tb, const int32_t* __restrict__ surf_cube,
                        const int64_t* __restrict__ counters, const uint8_t* __restrict__ case0,
-                       const int32_t* __restrict__ vox, int64_t R, uint8_t* __restrict__ prob, uint64_t* __restrict__ key,
-                       uint8_t* __restrict__ case_s) {
+                       const int32_t* __restrict__ vox, int64_t R, int64_t N, uint8_t* __restrict__ prob,
+                       uint64_t* __restrict__ key, uint8_t* __restrict__ case_s) {
   int64_t i = blockIdx.x * (int64_t)kBlk + threadIdx.x;
-  if (i >= counters[0]) return;
+  if (i >= N) return;
+  if (i >= counters[0]) {
+    prob[i] = 0;
+    return;
+  }
   int32_t cube = surf_cube[i];
   uint8_t c = case0[cube];
   case_s[i] = c;
@@ -1272,7 +1279,7 @@ extern "C" int a3d_dmc_topology(const int32_t* vox, const void* sdf, int dtype, 
     A3D_LAUNCH_CHECK();                                                                                           \
     tbytes = w.cub_bytes;                                                                                         \
     A3D_HIP_CHECK(hipcub::DeviceSelect::Flagged(w.cub, tbytes, w.sval0, w.surf, w.surf_cube, w.counters, (int)N, s)); \
-    hipLaunchKernelGGL(k_prob, dim3(gN), dim3(kBlk), 0, s, tb, w.surf_cube, w.counters, w.case0, vox, R, w.prob,  \
+    hipLaunchKernelGGL(k_prob, dim3(gN), dim3(kBlk), 0, s, tb, w.surf_cube, w.counters, w.case0, vox, R, N, w.prob, \
                        w.pkey0, w.case_s);                                                                        \
     A3D_LAUNCH_CHECK();                                                                                           \
     tbytes = w.cub_bytes;                                                                                         \
