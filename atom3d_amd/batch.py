@@ -7,11 +7,18 @@ No counterpart in the reference (one mesh per MeshBVH call).
     cfg, table = bu.autotune(plan)            # waves/block, waves/CU, tile order
     dist, face, closest, uvw = bu.run(plan, cfg)
     g = bu.capture(plan, cfg); g.replay()     # one graph launch per batch
+
+The autotuner's choice is kept in a JSON file keyed by (GPU architecture and
+CU count, kernel, shape bucket), so a process that meets a shape bucket
+already tuned on this machine launches at once.
 """
 from __future__ import annotations
 
 import ctypes
+import json
 import math
+import os
+import tempfile
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -178,13 +185,42 @@ class BatchedUDF:
         return BatchGraph(h.value, stream)
 
     # ------------------------------------------------------------------ autotune
-    def autotune(self, plan: BatchPlan, configs=None, reps: int = 3) -> tuple[dict, list]:
+    def tune_key(self, plan: BatchPlan) -> str:
+        """Cache key of a plan's launch-shape choice: the GPU (architecture
+        and CU count), the kernel, and the shape bucket (log2 of the query
+        count, the face count and the mesh count, rounded; whether per-mesh
+        distance bounds are used)."""
+        props = torch.cuda.get_device_properties(self.device)
+        arch = str(getattr(props, "gcnArchName", "") or props.name).split(":")[0]
+
+        def lg(x):
+            return int(round(math.log2(max(int(x), 1))))
+
+        return (f"{arch}/cu{self.n_cu}/k_udf_batch/K2^{lg(plan.K)}/F2^{lg(sum(self.n_faces))}"
+                f"/M2^{lg(len(self.bvhs))}/bounds{int(self.bounds is not None)}")
+
+    def autotune(self, plan: BatchPlan, configs=None, reps: int = 3,
+                 cache: bool | str | None = True) -> tuple[dict, list]:
         """Time each launch shape on the plan and return the fastest with the
-        table of (config, ms)."""
+        table of (config, ms).  With the default search space and ``cache``
+        true (or a file path), the choice is kept in a JSON file keyed by
+        ``tune_key`` (default file: $ATOM3D_AUTOTUNE_CACHE, else
+        ~/.cache/atom3d_amd/autotune.json) and a later call with the same key
+        returns it without timing, with the stored table."""
+        path = None
+        if configs is None and cache:
+            path = cache if isinstance(cache, str) else autotune_cache_path()
+            hit = load_tuned(path, self.tune_key(plan))
+            if hit is not None:
+                return hit
+        best, table = self._time_configs(plan, configs, reps)
+        if path is not None:
+            store_tuned(path, self.tune_key(plan), best, table)
+        return best, table
+
+    def _time_configs(self, plan: BatchPlan, configs, reps: int) -> tuple[dict, list]:
         if configs is None:
-            configs = [{"waves_per_block": wpb, "waves_per_cu": wpc, "order": order}
-                       for order in ("cost", "mesh") for wpb in (2, 4, 8, 16) for wpc in (8, 16, 32)
-                       if wpb <= wpc]
+            configs = default_tune_space()
         s = torch.cuda.current_stream(self.device)
         table = []
         for cfg in configs:
@@ -201,4 +237,64 @@ class BatchedUDF:
         return best, table
 
 
-__all__ = ["BatchedUDF", "BatchPlan", "BatchGraph", "TILE"]
+# ---------------------------------------------------------------------- autotune cache
+_TUNE_KEYS = {"waves_per_block": int, "waves_per_cu": int, "order": str}
+
+
+def default_tune_space() -> list:
+    """The launch shapes the autotuner times: waves per block x resident waves
+    per CU x tile order."""
+    return [{"waves_per_block": wpb, "waves_per_cu": wpc, "order": order}
+            for order in ("cost", "mesh") for wpb in (2, 4, 8, 16) for wpc in (8, 16, 32) if wpb <= wpc]
+
+
+def autotune_cache_path() -> str:
+    return os.environ.get("ATOM3D_AUTOTUNE_CACHE") or os.path.join(
+        os.path.expanduser("~"), ".cache", "atom3d_amd", "autotune.json")
+
+
+def _valid_cfg(cfg) -> bool:
+    return (isinstance(cfg, dict) and set(cfg) == set(_TUNE_KEYS)
+            and all(isinstance(cfg[k], t) for k, t in _TUNE_KEYS.items())
+            and cfg in default_tune_space())
+
+
+def load_tuned(path: str, key: str):
+    """(config, table) stored under ``key``, or None when the file is absent,
+    unreadable, or holds no valid entry for the key (a damaged cache only
+    costs a re-tune)."""
+    try:
+        with open(path) as fh:
+            ent = json.load(fh).get(key)
+        cfg, table = ent["config"], [(c, float(ms)) for c, ms in ent["table"]]
+    except (OSError, ValueError, KeyError, TypeError, AttributeError):
+        return None
+    return (cfg, table) if _valid_cfg(cfg) else None
+
+
+def store_tuned(path: str, key: str, cfg: dict, table: list) -> None:
+    """Add or replace one entry; the file is rewritten whole through a
+    temporary file and an atomic rename, so concurrent writers never leave a
+    torn file (the last one wins)."""
+    try:
+        with open(path) as fh:
+            data = json.load(fh)
+        if not isinstance(data, dict):
+            data = {}
+    except (OSError, ValueError):
+        data = {}
+    data[key] = {"config": cfg, "table": [[c, ms] for c, ms in table]}
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    fd, tmp = tempfile.mkstemp(prefix=".autotune.", dir=os.path.dirname(os.path.abspath(path)))
+    try:
+        with os.fdopen(fd, "w") as fh:
+            json.dump(data, fh, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+    except BaseException:
+        if os.path.exists(tmp):
+            os.unlink(tmp)
+        raise
+
+
+__all__ = ["BatchedUDF", "BatchPlan", "BatchGraph", "TILE", "default_tune_space", "autotune_cache_path",
+           "load_tuned", "store_tuned"]

@@ -153,8 +153,64 @@ def check(rc: int, what: str) -> None:
         raise RuntimeError(f"atom3d_amd {what} failed (code {rc}): {msg}")
 
 
+_roctx_fns = None  # (push, pop) when ATOM3D_ROCTX is set, else False; None until first asked
+
+
+def _roctx():
+    """roctx range functions when the environment variable ATOM3D_ROCTX is set
+    to anything but "" or "0" (then every C-ABI call below runs inside a
+    range named after the entry point, which `rocprofv3 --marker-trace`
+    records), else False.  Off by default: two extra foreign calls per entry
+    point cost about a microsecond."""
+    global _roctx_fns
+    if _roctx_fns is None:
+        fns = False
+        if os.environ.get("ATOM3D_ROCTX", "0") not in ("", "0"):
+            try:
+                r = ctypes.CDLL("libroctx64.so.4")
+            except OSError:
+                r = ctypes.CDLL("/opt/rocm/lib/libroctx64.so.4")
+            r.roctxRangePushA.argtypes = [ctypes.c_char_p]
+            r.roctxRangePushA.restype = ctypes.c_int
+            r.roctxRangePop.argtypes = []
+            r.roctxRangePop.restype = ctypes.c_int
+            fns = (r.roctxRangePushA, r.roctxRangePop)
+        _roctx_fns = fns
+    return _roctx_fns
+
+
+class trace_range:
+    """``with trace_range("MeshBVH.udf"): ...`` — a roctx range around a block
+    of host code when ATOM3D_ROCTX is set; nothing otherwise."""
+
+    def __init__(self, name: str):
+        self.name = name.encode()
+
+    def __enter__(self):
+        tr = _roctx()
+        if tr:
+            tr[0](self.name)
+        return self
+
+    def __exit__(self, *exc):
+        tr = _roctx()
+        if tr:
+            tr[1]()
+        return False
+
+
 def call(name: str, *args) -> None:
-    check(getattr(lib(), name)(*args), name)
+    fn = getattr(lib(), name)
+    tr = _roctx_fns if _roctx_fns is not None else _roctx()
+    if not tr:
+        check(fn(*args), name)
+        return
+    tr[0](name.encode())
+    try:
+        rc = fn(*args)
+    finally:
+        tr[1]()
+    check(rc, name)
 
 
 def ptr(t: torch.Tensor | None) -> int | None:

@@ -744,6 +744,9 @@ def bench_batch(a):
     K = int(P.shape[0])
     plan = bu.plan(P, counts)
     ta = time.perf_counter()
+    from atom3d_amd.batch import autotune_cache_path, load_tuned
+
+    tune_cached = load_tuned(autotune_cache_path(), bu.tune_key(plan)) is not None
     best, table = bu.autotune(plan, reps=2)
     tune_s = time.perf_counter() - ta
     g = bu.capture(plan, best)
@@ -801,7 +804,8 @@ def bench_batch(a):
         "config": {"workload": f"256 meshes (noisy icospheres / sphere unions, faces log-uniform 1k-200k, seed 3; "
                                f"{sum(bu.n_faces)} faces), query sets log-uniform 2^14-2^20 in each padded box "
                                f"({K} queries)", "faces": sum(bu.n_faces), "queries": K,
-                   "parallelism": "single GPU", "autotuned": best, "autotune_s": tune_s, "bvh_build_s": build_s},
+                   "parallelism": "single GPU", "autotuned": best, "autotune_s": tune_s,
+                   "autotune_from_cache": tune_cached, "bvh_build_s": build_s},
         "per_mesh_launches_queries_per_s": K / (per_mesh_ms * 1e-3),
         "autotune_table": [[c, t] for c, t in table],
         "roofline": roofline("k_udf_batch", walk_ms, "k_udf_batch_uneven256_counters",

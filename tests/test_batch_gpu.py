@@ -67,12 +67,27 @@ def test_graph_replays_match_eager(small_batch):
 def test_autotune_shapes_agree(small_batch):
     meshes, P, counts, bu = small_batch
     plan = bu.plan(P, counts)
-    best, table = bu.autotune(plan, reps=1)
+    best, table = bu.autotune(plan, reps=1, cache=False)
     assert best in [c for c, _ in table] and all(ms > 0 for _, ms in table)
     ref = [t.clone() for t in bu.run(plan, table[0][0])]
     for cfg, _ in table[1:]:
         out = bu.run(plan, cfg)
         assert _same(out[:3], ref[:3]), cfg
+
+
+def test_autotune_choice_persists(small_batch, tmp_path, monkeypatch):
+    """The second autotune of the same shape bucket reads the JSON cache and
+    launches nothing."""
+    meshes, P, counts, bu = small_batch
+    path = str(tmp_path / "autotune.json")
+    plan = bu.plan(P, counts)
+    best, table = bu.autotune(plan, reps=1, cache=path)
+    key = bu.tune_key(plan)
+    assert key.startswith("gfx950/") and "/k_udf_batch/" in key
+    calls = []
+    monkeypatch.setattr(bu, "_launch", lambda *a, **k: calls.append(a))
+    best2, table2 = bu.autotune(plan, reps=1, cache=path)
+    assert (best2, table2) == (best, table) and calls == []
 
 
 def test_plan_rejects_bad_counts(small_batch):
