@@ -93,7 +93,10 @@ struct DmcWs {
   int32_t* sedge;     // [12N][2]
   int32_t* qflag;     // [12N]
   int32_t* qrank;     // [12N]
+  int32_t* quad_of_edge;  // [12N] quad of each surface edge, -1 when fewer than 4 cubes share it
   int32_t* quads;     // [3N][4]
+  // per entry, written by the forward for the backward: crossing zc (3) and beta weight, T
+  void* erec;         // [12N][4] of T
   // gradient scratch (backward): per dual vertex and per quad
   void* gvd;          // [4N][3] of T
   void* aux;          // [4N][5] of T: forward's dual vertex xyz, beta sum + 1e-8, mean crossing distance
@@ -171,7 +174,9 @@ DmcWs carve(void* base, int64_t N) {
   w.sedge = (int32_t*)take(8 * 12 * n);
   w.qflag = (int32_t*)take(4 * (12 * n + 1));
   w.qrank = (int32_t*)take(4 * (12 * n + 1));
+  w.quad_of_edge = (int32_t*)take(4 * 12 * n);
   w.quads = (int32_t*)take(16 * 3 * n);
+  w.erec = take(8 * 4 * 12 * n);
   w.gvd = take(8 * 3 * 4 * n);
   w.aux = take(8 * 5 * 4 * n);
   w.sgs = take(8 * 4 * n);
@@ -340,11 +345,12 @@ __global__ void k_unique_flags(const uint64_t* __restrict__ key, const int32_t* 
   uflag[j] = (j < S && (j == 0 || key[j] != key[j - 1])) ? 1 : 0;
 }
 
-// Rank of each sorted slot's edge; unique edge endpoints; run length == 4 flag.
+// Rank of each sorted slot's edge; unique edge endpoints; run length == 4
+// flag; quad_of_edge cleared to -1 (k_quads sets the edges that get a quad).
 __global__ void k_unique_map(const uint64_t* __restrict__ key, const int32_t* __restrict__ val,
                              const int32_t* __restrict__ urank, int32_t* __restrict__ totals,
                              int32_t* __restrict__ sedge_of_slot, int32_t* __restrict__ sedge,
-                             int32_t* __restrict__ qflag, int kb) {
+                             int32_t* __restrict__ qflag, int32_t* __restrict__ quad_of_edge, int kb) {
   int64_t j = blockIdx.x * (int64_t)kBlk + threadIdx.x;
   int64_t S = totals[2];
   if (j == 0) totals[3] = urank[S];
@@ -355,6 +361,7 @@ __global__ void k_unique_map(const uint64_t* __restrict__ key, const int32_t* __
   sedge_of_slot[val[j]] = u;
   int is_q = 0;
   if (first) {
+    quad_of_edge[u] = -1;
     sedge[2 * u] = (int32_t)(key[j] >> kb);
     sedge[2 * u + 1] = (int32_t)(key[j] & ((uint64_t(1) << kb) - 1));
     int64_t run = 1;
@@ -370,7 +377,7 @@ __global__ void k_quads(const Tables* __restrict__ tb, const T* __restrict__ sdf
                         const int32_t* __restrict__ qrank, const int32_t* __restrict__ totals,
                         const uint8_t* __restrict__ case_s, const int32_t* __restrict__ vd_base,
                         const int32_t* __restrict__ sedge, const int32_t* __restrict__ sedge_of_slot,
-                        int32_t* __restrict__ quads) {
+                        int32_t* __restrict__ quads, int32_t* __restrict__ quad_of_edge) {
   int64_t j = blockIdx.x * (int64_t)kBlk + threadIdx.x;
   if (j >= totals[2] || !qflag[j]) return;
   int32_t q = qrank[j];
@@ -381,6 +388,7 @@ __global__ void k_quads(const Tables* __restrict__ tb, const T* __restrict__ sdf
     v[k] = vd_base[i] + tb->group_of_edge[case_s[i]][e];
   }
   int32_t u = sedge_of_slot[val[j]];
+  quad_of_edge[u] = q;
   bool pos = field(sdf, (int64_t)sedge[2 * u], iso) > T(0);
   int32_t* o = quads + 4 * (int64_t)q;
   if (pos) { o[0] = v[0]; o[1] = v[1]; o[2] = v[3]; o[3] = v[2]; }
@@ -478,10 +486,13 @@ template <typename T>
 __device__ __forceinline__ T norm3(const T* v) { return sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]); }
 
 // One thread per edge-group entry: p_alpha and the edge-direction normal
-// (sparse_diffdmc.py:263-287, 712-715).
+// (sparse_diffdmc.py:263-287, 712-715), and the entry's record for the
+// per-vertex passes (its plain crossing zc = interp(s0, s1, x0, x1) and its
+// beta weight), so that they read 4 contiguous values per entry instead of
+// gathering the corners again.
 template <typename T>
 __global__ void k_entry_fwd(Geom g, const int32_t* __restrict__ totals, Fields<T> f, T* __restrict__ p_alpha,
-                            T* __restrict__ n_alpha) {
+                            T* __restrict__ n_alpha, T* __restrict__ erec) {
   int64_t e = blockIdx.x * (int64_t)kBlk + threadIdx.x;
   if (e >= totals[1]) return;
   EdgeVals<T> v;
@@ -489,6 +500,10 @@ __global__ void k_entry_fwd(Geom g, const int32_t* __restrict__ totals, Fields<T
   T ue[3];
   interp(v.s0 * v.a0, v.s1 * v.a1, v.x0, v.x1, ue);
   for (int d = 0; d < 3; ++d) p_alpha[e * 3 + d] = ue[d];
+  T zc[3];
+  interp(v.s0, v.s1, v.x0, v.x1, zc);
+  for (int d = 0; d < 3; ++d) erec[e * 4 + d] = zc[d];
+  erec[e * 4 + 3] = v.beta;
   T ev[3] = {v.x1[0] - v.x0[0], v.x1[1] - v.x0[1], v.x1[2] - v.x0[2]};
   T el = norm3(ev) + T(1e-8);
   T gm = (v.s1 - v.s0) / el;
@@ -501,12 +516,13 @@ __global__ void k_entry_fwd(Geom g, const int32_t* __restrict__ totals, Fields<T
 }
 
 // One thread per dual vertex: the beta-weighted average of its alpha-weighted
-// crossings, summed in entry order, then L_dev over its entries
-// (sparse_diffdmc.py:255-261, 755-775).  Features, when given, are
-// interpolated with the same weights.
+// crossings (p_alpha, written by k_entry_fwd), summed in entry order, then
+// L_dev over its entries from their plain crossings (sparse_diffdmc.py:255-261,
+// 755-775).  Features, when given, are interpolated with the same weights.
 template <typename T>
-__global__ void k_vd_fwd(Geom g, const int32_t* __restrict__ totals, Fields<T> f, T* __restrict__ verts,
-                         T* __restrict__ L_dev, T* __restrict__ vfeat, T* __restrict__ aux) {
+__global__ void k_vd_fwd(Geom g, const int32_t* __restrict__ totals, Fields<T> f, const T* __restrict__ p_alpha,
+                         const T* __restrict__ erec, T* __restrict__ verts, T* __restrict__ L_dev,
+                         T* __restrict__ vfeat, T* __restrict__ aux) {
   int64_t v = blockIdx.x * (int64_t)kBlk + threadIdx.x;
   if (v >= totals[0]) return;
   const int32_t o0 = g.vd_off[v], o1 = g.vd_off[v + 1];
@@ -515,16 +531,16 @@ __global__ void k_vd_fwd(Geom g, const int32_t* __restrict__ totals, Fields<T> f
   if (vfeat)
     for (int c = 0; c < C; ++c) vfeat[v * C + c] = T(0);
   for (int32_t e = o0; e < o1; ++e) {
-    EdgeVals<T> ev;
-    edge_vals(g, g.ent_slot[e], f, true, ev);
-    const T w0 = ev.s0 * ev.a0, w1 = ev.s1 * ev.a1;
-    T ue[3];
-    interp(w0, w1, ev.x0, ev.x1, ue);
-    for (int d = 0; d < 3; ++d) num[d] = num[d] + ue[d] * ev.beta;
-    bs = bs + ev.beta;
-    if (vfeat)
+    const T beta = erec[(int64_t)e * 4 + 3];
+    for (int d = 0; d < 3; ++d) num[d] = num[d] + p_alpha[(int64_t)e * 3 + d] * beta;
+    bs = bs + beta;
+    if (vfeat) {
+      EdgeVals<T> ev;
+      edge_vals(g, g.ent_slot[e], f, true, ev);
+      const T w0 = ev.s0 * ev.a0, w1 = ev.s1 * ev.a1;
       for (int c = 0; c < C; ++c)
         vfeat[v * C + c] += interp1(w0, w1, f.feat[ev.c0 * C + c], f.feat[ev.c1 * C + c]) * ev.beta;
+    }
   }
   const T D = bs + T(1e-8);
   T vd[3];
@@ -536,10 +552,7 @@ __global__ void k_vd_fwd(Geom g, const int32_t* __restrict__ totals, Fields<T> f
     for (int c = 0; c < C; ++c) vfeat[v * C + c] = vfeat[v * C + c] / D;
   T sum = T(0);
   for (int32_t e = o0; e < o1; ++e) {
-    EdgeVals<T> ev;
-    edge_vals(g, g.ent_slot[e], f, false, ev);
-    T zc[3];
-    interp(ev.s0, ev.s1, ev.x0, ev.x1, zc);
+    const T* zc = erec + (int64_t)e * 4;
     T dd[3] = {zc[0] - vd[0], zc[1] - vd[1], zc[2] - vd[2]};
     T dist = norm3(dd);
     L_dev[e] = dist;
@@ -889,32 +902,17 @@ __global__ void k_vd_qef(Geom g, const int32_t* __restrict__ totals, Fields<T> f
 }
 
 // ------------------------------------------------------------------ backward
-// Adjoint of the forward above.  Nothing but the topology is kept from the
-// forward pass: the interpolation weights, crossings and dual vertices are
-// recomputed per dual vertex.  Gradients reach the corners and dual vertices
+// Adjoint of the forward above.  Kept from the forward pass: the topology,
+// each dual vertex's position, weight sum and mean crossing distance, and
+// each entry's plain crossing and beta weight; the interpolation weights are
+// recomputed per entry.  Gradients reach the corners and dual vertices
 // without atomics: each entry writes its two per-corner contributions, the
 // (corner, entry side) pairs are radix-sorted (stable, so each corner's
 // contributions stay in entry order) and every corner sums its run; alpha
-// sums per surface cube over its contiguous entries, and quad-centre
-// gradients reach their 4 dual vertices through a sort of (vertex, quad)
-// pairs the same way.  The sums are therefore deterministic.  Feature
-// gradients (optional, [M, C]) still use atomics.
-
-// g_vd = g_vertices[:Vd] (+ a quarter of each incident quad centre's gradient).
-template <typename T>
-__global__ void k_gvd_init(const T* __restrict__ gsrc, int64_t n, T* __restrict__ gdst) {
-  int64_t t = blockIdx.x * (int64_t)kBlk + threadIdx.x;
-  if (t < n) gdst[t] = gsrc ? gsrc[t] : T(0);
-}
-
-// (dual vertex, quad) pairs of the quads' corners, for the centre gradients.
-__global__ void k_quad_pairs(const int32_t* __restrict__ quads, int64_t n4, uint32_t* __restrict__ key,
-                             int32_t* __restrict__ val) {
-  int64_t t = blockIdx.x * (int64_t)kBlk + threadIdx.x;
-  if (t >= n4) return;
-  key[t] = (uint32_t)quads[t];
-  val[t] = (int32_t)(t >> 2);
-}
+// sums per surface cube over its contiguous entries, and each dual vertex
+// gathers the quad-centre gradients of its entries' quads.  The sums are
+// therefore deterministic.  Feature gradients (optional, [M, C]) still use
+// atomics.
 
 // Segmented sums over sorted (key, value) pairs, one pair per lane: after
 // seg_suffix, each lane holds the sum of its run's values from itself to the
@@ -940,32 +938,6 @@ __device__ __forceinline__ void run_bounds(const uint32_t* __restrict__ key, int
   const uint32_t last = __shfl(k, 63);
   const int64_t next = j - lane + 64;  // first pair of the next wavefront
   spill = head && last == k && next < n && key[next] == k;
-}
-
-// Each quad centre is the mean of its 4 dual vertices: the sorted (vertex,
-// quad) pairs' quarter centre gradients are summed per vertex run and added
-// to gdst[v] (which holds g_vertices[v]).
-template <typename T>
-__global__ __launch_bounds__(kBlk) void k_centers_bwd(const uint32_t* __restrict__ key,
-                                                      const int32_t* __restrict__ val, int64_t n, int64_t Vd, int C,
-                                                      const T* __restrict__ gsrc, T* __restrict__ gdst) {
-  const int64_t j = blockIdx.x * (int64_t)kBlk + threadIdx.x;
-  const int lane = threadIdx.x & 63;
-  if (j - lane >= n) return;  // whole wavefront past the end
-  const bool valid = j < n;
-  const uint32_t v = valid ? key[j] : 0xffffffffu;  // keys are dual vertex ids < 2^31
-  const int64_t q = valid ? val[j] : 0;
-  bool head, spill;
-  run_bounds(key, j, n, lane, v, valid, head, spill);
-  for (int c = 0; c < C; ++c) {
-    T acc = valid ? gsrc[(Vd + q) * C + c] / T(4) : T(0);
-    seg_suffix(v, lane, acc);
-    if (head) {
-      if (spill)
-        for (int64_t r = j - lane + 64; r < n && key[r] == v; ++r) acc += gsrc[(Vd + val[r]) * C + c] / T(4);
-      gdst[(int64_t)v * C + c] += acc;
-    }
-  }
 }
 
 // Sorted (corner, entry side) pairs: g_sdf[c] and g_deform[c] = the sums of
@@ -1053,41 +1025,88 @@ __device__ __forceinline__ void interp1_bwd(T w0, T w1, T x0, T x1, T g, T& gw0,
 template <typename T>
 __device__ __forceinline__ T sgn(T x) { return x > T(0) ? T(1) : (x < T(0) ? T(-1) : T(0)); }
 
-// Pass A, one thread per dual vertex: the L_dev terms that feed back into
-// the vertex, g_vd -= sum_e gdist_e (zc_e - vd) / dist_e with
-// gdist_e = gL_e sign(r_e) - (sum_k gL_k sign(r_k)) / n, r = dist - mean.
+// The quad (index into the quad centres) that entry e's edge gives, or -1:
+// entry -> (surface cube, edge) slot -> surface edge -> quad.
+__device__ __forceinline__ int32_t entry_quad(const int32_t* __restrict__ ent_slot,
+                                              const int32_t* __restrict__ sedge_of_slot,
+                                              const int32_t* __restrict__ quad_of_edge, int32_t e, int32_t Se,
+                                              int32_t Q) {
+  const int32_t u = sedge_of_slot[ent_slot[e]];
+  if (u < 0 || u >= Se) return -1;
+  const int32_t q = quad_of_edge[u];
+  return q < Q ? q : -1;
+}
+
+// One thread per dual vertex: its gradient g_vd = g_vertices[v] + a quarter
+// of the gradient of each quad centre it is a corner of (a vertex is a corner
+// of the quad of each of its entries' edges that 4 cubes share; summed in
+// entry order), then the L_dev terms that feed back into the vertex,
+// g_vd -= sum_e gdist_e (zc_e - vd) / dist_e with
+// gdist_e = gL_e sign(r_e) - (sum_k gL_k sign(r_k)) / n, r = dist - mean;
+// the plain crossings zc come from the forward's entry records.
 template <typename T>
-__global__ void k_vd_bwd_a(Geom g, const int32_t* __restrict__ totals, Fields<T> f, const T* __restrict__ aux,
-                           const T* __restrict__ gL, T* __restrict__ gvd, T* __restrict__ sgs) {
-  int64_t v = blockIdx.x * (int64_t)kBlk + threadIdx.x;
-  if (v >= totals[0]) return;
-  const int32_t o0 = g.vd_off[v], o1 = g.vd_off[v + 1];
-  const T vd[3] = {aux[v * 5], aux[v * 5 + 1], aux[v * 5 + 2]};
-  const T mean = aux[v * 5 + 4];
-  const T n = (T)(o1 - o0);
-  T sg_sum = T(0);
-  for (int32_t e = o0; e < o1; ++e) {
-    EdgeVals<T> ev;
-    edge_vals(g, g.ent_slot[e], f, false, ev);
-    T zc[3];
-    interp(ev.s0, ev.s1, ev.x0, ev.x1, zc);
-    T dd[3] = {zc[0] - vd[0], zc[1] - vd[1], zc[2] - vd[2]};
-    sg_sum += gL[e] * sgn(norm3(dd) - mean);
+__global__ void k_vd_bwd(const int32_t* __restrict__ vd_off, const int32_t* __restrict__ ent_slot,
+                         const int32_t* __restrict__ sedge_of_slot, const int32_t* __restrict__ quad_of_edge,
+                         const int32_t* __restrict__ totals, const T* __restrict__ erec, const T* __restrict__ aux,
+                         const T* __restrict__ g_verts, const T* __restrict__ gL, T* __restrict__ gvd,
+                         T* __restrict__ sgs) {
+  const int64_t v = blockIdx.x * (int64_t)kBlk + threadIdx.x;
+  const int32_t Vd = totals[0];
+  if (v >= Vd) return;
+  const int32_t o0 = vd_off[v], o1 = vd_off[v + 1];
+  T gv[3] = {T(0), T(0), T(0)};
+  if (g_verts) {
+    const int32_t Se = totals[3], Q = totals[4];
+    T acc[3] = {T(0), T(0), T(0)};
+    for (int32_t e = o0; e < o1; ++e) {
+      const int32_t q = entry_quad(ent_slot, sedge_of_slot, quad_of_edge, e, Se, Q);
+      if (q >= 0)
+        for (int d = 0; d < 3; ++d) acc[d] += g_verts[((int64_t)Vd + q) * 3 + d] / T(4);
+    }
+    for (int d = 0; d < 3; ++d) gv[d] = g_verts[v * 3 + d] + acc[d];
   }
-  T gv[3] = {gvd[v * 3], gvd[v * 3 + 1], gvd[v * 3 + 2]};
-  for (int32_t e = o0; e < o1; ++e) {
-    EdgeVals<T> ev;
-    edge_vals(g, g.ent_slot[e], f, false, ev);
-    T zc[3];
-    interp(ev.s0, ev.s1, ev.x0, ev.x1, zc);
-    T dd[3] = {zc[0] - vd[0], zc[1] - vd[1], zc[2] - vd[2]};
-    const T dist = norm3(dd);
-    const T gdist = gL[e] * sgn(dist - mean) - sg_sum / n;
-    if (dist > T(0) && gdist != T(0))
-      for (int d = 0; d < 3; ++d) gv[d] -= gdist * dd[d] / dist;
+  T sg_sum = T(0);
+  if (gL) {
+    const T vd[3] = {aux[v * 5], aux[v * 5 + 1], aux[v * 5 + 2]};
+    const T mean = aux[v * 5 + 4];
+    const T n = (T)(o1 - o0);
+    for (int32_t e = o0; e < o1; ++e) {
+      const T* zc = erec + (int64_t)e * 4;
+      T dd[3] = {zc[0] - vd[0], zc[1] - vd[1], zc[2] - vd[2]};
+      sg_sum += gL[e] * sgn(norm3(dd) - mean);
+    }
+    for (int32_t e = o0; e < o1; ++e) {
+      const T* zc = erec + (int64_t)e * 4;
+      T dd[3] = {zc[0] - vd[0], zc[1] - vd[1], zc[2] - vd[2]};
+      const T dist = norm3(dd);
+      const T gdist = gL[e] * sgn(dist - mean) - sg_sum / n;
+      if (dist > T(0) && gdist != T(0))
+        for (int d = 0; d < 3; ++d) gv[d] -= gdist * dd[d] / dist;
+    }
   }
   for (int d = 0; d < 3; ++d) gvd[v * 3 + d] = gv[d];
   sgs[v] = sg_sum;
+}
+
+// Feature gradient per dual vertex: g_vfeat[v] + a quarter of each incident
+// quad centre's, in entry order (as g_vd above).
+template <typename T>
+__global__ void k_vf_bwd(const int32_t* __restrict__ vd_off, const int32_t* __restrict__ ent_slot,
+                         const int32_t* __restrict__ sedge_of_slot, const int32_t* __restrict__ quad_of_edge,
+                         const int32_t* __restrict__ totals, const T* __restrict__ g_vfeat, int C,
+                         T* __restrict__ gvf) {
+  const int64_t v = blockIdx.x * (int64_t)kBlk + threadIdx.x;
+  const int32_t Vd = totals[0], Se = totals[3], Q = totals[4];
+  if (v >= Vd) return;
+  const int32_t o0 = vd_off[v], o1 = vd_off[v + 1];
+  for (int c = 0; c < C; ++c) {
+    T acc = T(0);
+    for (int32_t e = o0; e < o1; ++e) {
+      const int32_t q = entry_quad(ent_slot, sedge_of_slot, quad_of_edge, e, Se, Q);
+      if (q >= 0) acc += g_vfeat[((int64_t)Vd + q) * C + c] / T(4);
+    }
+    gvf[v * C + c] = g_vfeat[v * C + c] + acc;
+  }
 }
 
 // Pass B, one thread per edge-group entry: every gradient path through the
@@ -1347,14 +1366,14 @@ extern "C" int a3d_dmc_topology(const int32_t* vox, const void* sdf, int dtype, 
   tbytes = w.cub_bytes;
   A3D_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(w.cub, tbytes, w.uflag, w.urank, S_slots + 1, s));
   hipLaunchKernelGGL(k_unique_map, dim3(gH), dim3(kBlk), 0, s, w.skey1, w.sval1, w.urank, w.totals, w.sedge_of_slot,
-                     w.sedge, w.qflag, kb);
+                     w.sedge, w.qflag, w.quad_of_edge, kb);
   A3D_LAUNCH_CHECK();
   tbytes = w.cub_bytes;
   A3D_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(w.cub, tbytes, w.qflag, w.qrank, S_slots + 1, s));
   A3D_HIP_CHECK(hipMemcpyAsync(w.totals + 4, w.qrank + S_slots, 4, hipMemcpyDeviceToDevice, s));
 #define A3D_DMC_T3(T)                                                                                                  \
   hipLaunchKernelGGL(k_quads<T>, dim3(gH), dim3(kBlk), 0, s, tb, (const T*)sdf, (T)isovalue, w.sval1, w.qflag, w.qrank, \
-                     w.totals, w.case_s, w.vd_base, w.sedge, w.sedge_of_slot, w.quads)
+                     w.totals, w.case_s, w.vd_base, w.sedge, w.sedge_of_slot, w.quads, w.quad_of_edge)
   if (dtype == 0) A3D_DMC_T3(float); else A3D_DMC_T3(double);
 #undef A3D_DMC_T3
   A3D_LAUNCH_CHECK();
@@ -1402,10 +1421,11 @@ int dmc_forward_t(const DmcWs& w, const int64_t* sizes, const Geom& g, const Fie
                   void* p_alpha, void* n_alpha, void* vfeat, hipStream_t s) {
   const int64_t Vd = sizes[1], E = sizes[2], Q = sizes[4];
   if (Vd == 0) return A3D_OK;
-  hipLaunchKernelGGL(k_entry_fwd<T>, dim3(gridn(E)), dim3(kBlk), 0, s, g, w.totals, f, (T*)p_alpha, (T*)n_alpha);
+  hipLaunchKernelGGL(k_entry_fwd<T>, dim3(gridn(E)), dim3(kBlk), 0, s, g, w.totals, f, (T*)p_alpha, (T*)n_alpha,
+                     (T*)w.erec);
   A3D_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_vd_fwd<T>, dim3(gridn(Vd)), dim3(kBlk), 0, s, g, w.totals, f, (T*)verts, (T*)L_dev,
-                     (T*)vfeat, (T*)w.aux);
+  hipLaunchKernelGGL(k_vd_fwd<T>, dim3(gridn(Vd)), dim3(kBlk), 0, s, g, w.totals, f, (const T*)p_alpha,
+                     (const T*)w.erec, (T*)verts, (T*)L_dev, (T*)vfeat, (T*)w.aux);
   A3D_LAUNCH_CHECK();
   if (Q > 0) {
     hipLaunchKernelGGL(k_centers<T>, dim3(gridn(Q)), dim3(kBlk), 0, s, w.quads, Q, Vd, (T*)verts, (T*)vfeat, f.C);
@@ -1415,8 +1435,8 @@ int dmc_forward_t(const DmcWs& w, const int64_t* sizes, const Geom& g, const Fie
 }
 
 // Backward scratch, sized from the topology's sizes: per entry side 4
-// contribution values and 1 alpha value (T), the (corner | dual vertex, index)
-// sort pairs twice (radix sort ping-pong) and the sort's temporary storage.
+// contribution values and 1 alpha value (T), the (corner, entry side) sort
+// pairs twice (radix sort ping-pong) and the sort's temporary storage.
 struct BwdScratch {
   void* rec;
   void* ra;
@@ -1430,8 +1450,8 @@ struct BwdScratch {
 };
 
 BwdScratch carve_bwd(void* base, const int64_t* sizes, int tsize) {
-  const int64_t E = sizes[2], Q = sizes[4];
-  const int64_t P = std::max<int64_t>(std::max<int64_t>(2 * E, 4 * Q), 1);
+  const int64_t E = sizes[2];
+  const int64_t P = std::max<int64_t>(2 * E, 1);
   size_t t = 0, tb = 0;
   (void)hipcub::DeviceRadixSort::SortPairs(nullptr, t, (uint32_t*)nullptr, (uint32_t*)nullptr, (int32_t*)nullptr,
                                            (int32_t*)nullptr, (int)P, 0, 32);
@@ -1457,7 +1477,7 @@ int dmc_backward_t(const DmcWs& w, const BwdScratch& b, const int64_t* sizes, in
                    const Fields<T>& f, const void* vfeat, const void* g_verts, const void* g_L_dev,
                    const void* g_p_alpha, const void* g_n_alpha, const void* g_vfeat, void* gvf_scratch, void* g_sdf,
                    void* g_deform, void* g_alpha, void* g_beta, void* g_feat, hipStream_t s) {
-  const int64_t Vd = sizes[1], E = sizes[2], Q = sizes[4];
+  const int64_t Vd = sizes[1], E = sizes[2];
   const int C = f.C;
   struct Z { void* p; int64_t n; } zs[5] = {{g_sdf, M}, {g_deform, 3 * M}, {g_alpha, 8 * N}, {g_beta, 12 * N},
                                             {g_feat, (int64_t)C * M}};
@@ -1467,35 +1487,16 @@ int dmc_backward_t(const DmcWs& w, const BwdScratch& b, const int64_t* sizes, in
       A3D_LAUNCH_CHECK();
     }
   if (Vd == 0) return A3D_OK;
-  size_t tb = b.tmp_bytes;
-  if (Q > 0 && (g_verts || (g_vfeat && C > 0))) {  // (vertex, quad) pairs, sorted by vertex
-    hipLaunchKernelGGL(k_quad_pairs, dim3(gridn(4 * Q)), dim3(kBlk), 0, s, w.quads, 4 * Q, b.k0, b.v0);
-    A3D_LAUNCH_CHECK();
-    A3D_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(b.tmp, tb, b.k0, b.k1, b.v0, b.v1, (int)(4 * Q), 0,
-                                                     key_bits(Vd), s));
-  }
   T* gvd = (T*)w.gvd;
-  hipLaunchKernelGGL(k_gvd_init<T>, dim3(gridn(3 * Vd)), dim3(kBlk), 0, s, (const T*)g_verts, 3 * Vd, gvd);
+  hipLaunchKernelGGL(k_vd_bwd<T>, dim3(gridn(Vd)), dim3(kBlk), 0, s, w.vd_off, w.ent_slot, w.sedge_of_slot,
+                     w.quad_of_edge, w.totals, (const T*)w.erec, (const T*)w.aux, (const T*)g_verts,
+                     (const T*)g_L_dev, gvd, (T*)w.sgs);
   A3D_LAUNCH_CHECK();
-  if (Q > 0 && g_verts) {
-    hipLaunchKernelGGL(k_centers_bwd<T>, dim3(gridn(4 * Q)), dim3(kBlk), 0, s, b.k1, b.v1, 4 * Q, Vd, 3,
-                       (const T*)g_verts, gvd);
-    A3D_LAUNCH_CHECK();
-  }
   T* gvf = nullptr;
   if (g_vfeat && C > 0) {
     gvf = (T*)gvf_scratch;
-    hipLaunchKernelGGL(k_gvd_init<T>, dim3(gridn(C * Vd)), dim3(kBlk), 0, s, (const T*)g_vfeat, C * Vd, gvf);
-    A3D_LAUNCH_CHECK();
-    if (Q > 0) {
-      hipLaunchKernelGGL(k_centers_bwd<T>, dim3(gridn(4 * Q)), dim3(kBlk), 0, s, b.k1, b.v1, 4 * Q, Vd, C,
-                         (const T*)g_vfeat, gvf);
-      A3D_LAUNCH_CHECK();
-    }
-  }
-  if (g_L_dev) {
-    hipLaunchKernelGGL(k_vd_bwd_a<T>, dim3(gridn(Vd)), dim3(kBlk), 0, s, g, w.totals, f, (const T*)w.aux,
-                       (const T*)g_L_dev, gvd, (T*)w.sgs);
+    hipLaunchKernelGGL(k_vf_bwd<T>, dim3(gridn(Vd)), dim3(kBlk), 0, s, w.vd_off, w.ent_slot, w.sedge_of_slot,
+                       w.quad_of_edge, w.totals, (const T*)g_vfeat, C, gvf);
     A3D_LAUNCH_CHECK();
   }
   T* ra = (g_alpha && f.alpha) ? (T*)b.ra : nullptr;
@@ -1505,7 +1506,7 @@ int dmc_backward_t(const DmcWs& w, const BwdScratch& b, const int64_t* sizes, in
                      (T*)g_beta, (T*)g_feat);
   A3D_LAUNCH_CHECK();
   if (E > 0) {
-    tb = b.tmp_bytes;
+    size_t tb = b.tmp_bytes;
     A3D_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(b.tmp, tb, b.k0, b.k1, b.v0, b.v1, (int)(2 * E), 0,
                                                      key_bits(M), s));
     hipLaunchKernelGGL(k_corner_sum<T>, dim3(gridn(2 * E)), dim3(kBlk), 0, s, b.k1, b.v1, 2 * E,
