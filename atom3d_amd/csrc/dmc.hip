@@ -22,9 +22,9 @@
 //   - one thread per dual vertex sums its entries in table order (a fixed
 //     order, unlike the reference's atomic index_add_), then L_dev;
 //   - quad centres (mean of 4) appended after the dual vertices.
-// Backward (hand-written adjoint): recomputes the interpolation weights and
-// the dual-vertex sums instead of storing them, and scatters into sdf,
-// deform, alpha and beta gradients with atomics.
+// Backward (hand-written adjoint): recomputes the interpolation weights per
+// entry, reuses the dual vertices and crossings the forward stored, and sums
+// every gradient in a fixed order without atomics (see the backward section).
 #include <hipcub/hipcub.hpp>
 
 #include "a3d_common.hpp"
@@ -80,6 +80,8 @@ struct DmcWs {
   // per entry
   int32_t* ent_slot;  // [12N]
   int32_t* ent_vd;    // [12N]
+  int32_t* ent_of_slot;  // [12N] entry of each sign-change (surface cube, edge) slot
+  int32_t* epos;      // [12N] position of each entry's slot in the sorted slot list
   // sign-change slots
   int32_t* hot_cnt;   // [N]
   int32_t* hot_off;   // [N]
@@ -90,6 +92,7 @@ struct DmcWs {
   int32_t* uflag;     // [12N]
   int32_t* urank;     // [12N]
   int32_t* sedge_of_slot;  // [12N]
+  int32_t* ustart;    // [12N + 1] first sorted slot of each surface edge, then the slot count
   int32_t* sedge;     // [12N][2]
   int32_t* qflag;     // [12N]
   int32_t* qrank;     // [12N]
@@ -162,6 +165,8 @@ DmcWs carve(void* base, int64_t N) {
   w.vd_off = (int32_t*)take(4 * (4 * n + 1));
   w.ent_slot = (int32_t*)take(4 * 12 * n);
   w.ent_vd = (int32_t*)take(4 * 12 * n);
+  w.ent_of_slot = (int32_t*)take(4 * 12 * n);
+  w.epos = (int32_t*)take(4 * 12 * n);
   w.hot_cnt = (int32_t*)take(4 * (n + 1));
   w.hot_off = (int32_t*)take(4 * (n + 1));
   w.skey0 = (uint64_t*)take(8 * 12 * n);
@@ -171,6 +176,7 @@ DmcWs carve(void* base, int64_t N) {
   w.uflag = (int32_t*)take(4 * (12 * n + 1));
   w.urank = (int32_t*)take(4 * (12 * n + 1));
   w.sedge_of_slot = (int32_t*)take(4 * 12 * n);
+  w.ustart = (int32_t*)take(4 * (12 * n + 1));
   w.sedge = (int32_t*)take(8 * 12 * n);
   w.qflag = (int32_t*)take(4 * (12 * n + 1));
   w.qrank = (int32_t*)take(4 * (12 * n + 1));
@@ -281,7 +287,8 @@ __global__ void k_vd_base(const Tables* __restrict__ tb, const int64_t* __restri
 
 __global__ void k_entries(const Tables* __restrict__ tb, const int64_t* __restrict__ counters,
                           const uint8_t* __restrict__ case_s, const int32_t* __restrict__ vd_base,
-                          const int32_t* __restrict__ vd_off, int32_t* __restrict__ ent_slot, int32_t* __restrict__ ent_vd) {
+                          const int32_t* __restrict__ vd_off, int32_t* __restrict__ ent_slot, int32_t* __restrict__ ent_vd,
+                          int32_t* __restrict__ ent_of_slot) {
   int64_t i = blockIdx.x * (int64_t)kBlk + threadIdx.x;
   if (i >= counters[0]) return;
   uint8_t c = case_s[i];
@@ -290,8 +297,10 @@ __global__ void k_entries(const Tables* __restrict__ tb, const int64_t* __restri
     int32_t v = vd_base[i] + k;
     int32_t o = vd_off[v];
     for (int j = 0; j < tb->gsize[c][k]; ++j) {
-      ent_slot[o + j] = (int32_t)(i * 12 + tb->groups[c][k][j]);
+      const int32_t slot = (int32_t)(i * 12 + tb->groups[c][k][j]);
+      ent_slot[o + j] = slot;
       ent_vd[o + j] = v;
+      ent_of_slot[slot] = o + j;
     }
   }
 }
@@ -345,23 +354,32 @@ __global__ void k_unique_flags(const uint64_t* __restrict__ key, const int32_t* 
   uflag[j] = (j < S && (j == 0 || key[j] != key[j - 1])) ? 1 : 0;
 }
 
-// Rank of each sorted slot's edge; unique edge endpoints; run length == 4
-// flag; quad_of_edge cleared to -1 (k_quads sets the edges that get a quad).
+// Rank of each sorted slot's edge; unique edge endpoints and first sorted
+// slot; run length == 4 flag; quad_of_edge cleared to -1 (k_quads sets the
+// edges that get a quad); each entry's sorted position.
 __global__ void k_unique_map(const uint64_t* __restrict__ key, const int32_t* __restrict__ val,
                              const int32_t* __restrict__ urank, int32_t* __restrict__ totals,
                              int32_t* __restrict__ sedge_of_slot, int32_t* __restrict__ sedge,
-                             int32_t* __restrict__ qflag, int32_t* __restrict__ quad_of_edge, int kb) {
+                             int32_t* __restrict__ qflag, int32_t* __restrict__ quad_of_edge,
+                             int32_t* __restrict__ ustart, const int32_t* __restrict__ ent_of_slot,
+                             int32_t* __restrict__ epos, int kb) {
   int64_t j = blockIdx.x * (int64_t)kBlk + threadIdx.x;
   int64_t S = totals[2];
   if (j == 0) totals[3] = urank[S];
   if (j > S) return;
-  if (j == S) { qflag[j] = 0; return; }
+  if (j == S) {
+    qflag[j] = 0;
+    ustart[urank[S]] = (int32_t)S;
+    return;
+  }
   bool first = (j == 0 || key[j] != key[j - 1]);
   int32_t u = urank[j] + (first ? 0 : -1);  // urank is exclusive
   sedge_of_slot[val[j]] = u;
+  epos[ent_of_slot[val[j]]] = (int32_t)j;
   int is_q = 0;
   if (first) {
     quad_of_edge[u] = -1;
+    ustart[u] = (int32_t)j;
     sedge[2 * u] = (int32_t)(key[j] >> kb);
     sedge[2 * u + 1] = (int32_t)(key[j] & ((uint64_t(1) << kb) - 1));
     int64_t run = 1;
@@ -906,98 +924,107 @@ __global__ void k_vd_qef(Geom g, const int32_t* __restrict__ totals, Fields<T> f
 // each dual vertex's position, weight sum and mean crossing distance, and
 // each entry's plain crossing and beta weight; the interpolation weights are
 // recomputed per entry.  Gradients reach the corners and dual vertices
-// without atomics: each entry writes its two per-corner contributions, the
-// (corner, entry side) pairs are radix-sorted (stable, so each corner's
-// contributions stay in entry order) and every corner sums its run; alpha
-// sums per surface cube over its contiguous entries, and each dual vertex
-// gathers the quad-centre gradients of its entries' quads.  The sums are
-// therefore deterministic.  Feature gradients (optional, [M, C]) still use
-// atomics.
+// without atomics: each entry writes its two per-corner contributions; the
+// surface edges (sorted by their corner pair) sum their slots' contributions,
+// each corner takes the first-corner sums of its run of edges and then the
+// second-corner sums of the edges that end at it (found by one stable sort of
+// the edges by second corner); alpha sums per surface cube over its
+// contiguous entries, and each dual vertex gathers the quad-centre gradients
+// of its entries' quads.  Every sum runs in a fixed order, so the gradients
+// are deterministic.  Feature gradients (optional, [M, C]) still use atomics.
 
-// Segmented sums over sorted (key, value) pairs, one pair per lane: after
-// seg_suffix, each lane holds the sum of its run's values from itself to the
-// run's last lane in this wavefront (runs are contiguous in sorted order, so
-// equal keys at distance `off` mean the whole span is one run); the run's
-// first lane then holds the in-wave total, summed in a fixed tree order.
+// Corner sums, pass 1.  The surface edges u are sorted by (first corner,
+// second corner) and each edge's slots are contiguous in the sorted slot list
+// (ustart), so the edges sharing a first corner c0 form a run.  One thread
+// per run head sums, edge by edge and slot by slot, the first-corner
+// contributions of the entries (rec, in sorted slot order: per slot, 4 values
+// for the first corner then 4 for the second, sdf then xyz) and writes
+// g_sdf[c0], g_deform[c0];
+// every edge's second-corner sum goes to e1[u] for pass 2, and every edge
+// emits the (second corner, edge) pair that pass 2 sorts.
 template <typename T>
-__device__ __forceinline__ void seg_suffix(uint32_t key, int lane, T& v) {
-  for (int off = 1; off < 64; off <<= 1) {
-    const uint32_t ko = __shfl_down(key, off);
-    const T vo = __shfl_down(v, off);
-    if (lane + off < 64 && ko == key) v += vo;
-  }
-}
-
-// Whether sorted pair j (lane `lane`, key k, valid) starts a run, and whether
-// that run goes on past this wavefront (then the rest is added serially).
-__device__ __forceinline__ void run_bounds(const uint32_t* __restrict__ key, int64_t j, int64_t n, int lane,
-                                           uint32_t k, bool valid, bool& head, bool& spill) {
-  uint32_t prev = __shfl_up(k, 1);
-  if (lane == 0) prev = (valid && j > 0) ? key[j - 1] : ~k;
-  head = valid && (j == 0 || prev != k);
-  const uint32_t last = __shfl(k, 63);
-  const int64_t next = j - lane + 64;  // first pair of the next wavefront
-  spill = head && last == k && next < n && key[next] == k;
-}
-
-// Sorted (corner, entry side) pairs: g_sdf[c] and g_deform[c] = the sums of
-// each corner run's contributions (rec: 4 values per entry side, sdf then
-// xyz).  Corners with no pair keep the zero written before.
-template <typename T>
-__global__ __launch_bounds__(kBlk) void k_corner_sum(const uint32_t* __restrict__ key,
-                                                     const int32_t* __restrict__ val, int64_t n,
-                                                     const T* __restrict__ rec, T* __restrict__ g_sdf,
-                                                     T* __restrict__ g_deform) {
-  const int64_t j = blockIdx.x * (int64_t)kBlk + threadIdx.x;
-  const int lane = threadIdx.x & 63;
-  if (j - lane >= n) return;
-  const bool valid = j < n;
-  const uint32_t c = valid ? key[j] : 0xffffffffu;  // keys are corner ids < 2^31
+__global__ void k_corner_c0(const int32_t* __restrict__ sedge, const int32_t* __restrict__ ustart,
+                            const int32_t* __restrict__ totals, const T* __restrict__ rec, T* __restrict__ e1,
+                            uint32_t* __restrict__ c1key, int32_t* __restrict__ c1val, T* __restrict__ g_sdf,
+                            T* __restrict__ g_deform) {
+  const int64_t u = blockIdx.x * (int64_t)kBlk + threadIdx.x;
+  const int32_t U = totals[3];
+  if (u >= U) return;
+  c1key[u] = (uint32_t)sedge[2 * u + 1];
+  c1val[u] = (int32_t)u;
+  const int32_t c0 = sedge[2 * u];
+  if (u > 0 && sedge[2 * (u - 1)] == c0) return;  // not the first edge of its run
   T a = T(0), x = T(0), y = T(0), z = T(0);
-  if (valid) {
-    const T* r = rec + (int64_t)val[j] * 4;
-    a = r[0];
-    x = r[1];
-    y = r[2];
-    z = r[3];
-  }
-  bool head, spill;
-  run_bounds(key, j, n, lane, c, valid, head, spill);
-  seg_suffix(c, lane, a);
-  seg_suffix(c, lane, x);
-  seg_suffix(c, lane, y);
-  seg_suffix(c, lane, z);
-  if (!head) return;
-  if (spill)
-    for (int64_t q = j - lane + 64; q < n && key[q] == c; ++q) {
-      const T* r = rec + (int64_t)val[q] * 4;
+  for (int64_t w = u; w < U && sedge[2 * w] == c0; ++w) {
+    T b0 = T(0), b1 = T(0), b2 = T(0), b3 = T(0);
+    for (int32_t p = ustart[w]; p < ustart[w + 1]; ++p) {
+      const T* r = rec + (int64_t)p * 8;
       a += r[0];
       x += r[1];
       y += r[2];
       z += r[3];
+      b0 += r[4];
+      b1 += r[5];
+      b2 += r[6];
+      b3 += r[7];
     }
-  g_sdf[c] = a;
+    T* o = e1 + w * 4;
+    o[0] = b0;
+    o[1] = b1;
+    o[2] = b2;
+    o[3] = b3;
+  }
+  g_sdf[c0] = a;
   if (g_deform) {
-    g_deform[(int64_t)c * 3] = x;
-    g_deform[(int64_t)c * 3 + 1] = y;
-    g_deform[(int64_t)c * 3 + 2] = z;
+    g_deform[(int64_t)c0 * 3] = x;
+    g_deform[(int64_t)c0 * 3 + 1] = y;
+    g_deform[(int64_t)c0 * 3 + 2] = z;
   }
 }
 
-// One thread per surface cube (the first of its entries, which are
-// contiguous): g_alpha of its 8 corners = the sums of its entries' alpha
-// contributions (ra: 2 per entry, at the edge's two corners), in entry order.
+// Corner sums, pass 2: the (second corner, edge) pairs sorted by corner
+// (stable: edges in order within a run); each run head adds its edges'
+// second-corner sums to the corner (after pass 1, one writer per corner).
 template <typename T>
-__global__ void k_alpha_sum(const int32_t* __restrict__ ent_slot, int64_t E, const T* __restrict__ ra,
-                            const int32_t* __restrict__ surf_cube, T* __restrict__ g_alpha) {
-  int64_t e = blockIdx.x * (int64_t)kBlk + threadIdx.x;
-  if (e >= E) return;
-  const int32_t i = ent_slot[e] / 12;
-  if (e > 0 && ent_slot[e - 1] / 12 == i) return;
+__global__ void k_corner_c1(const uint32_t* __restrict__ key, const int32_t* __restrict__ val,
+                            const int32_t* __restrict__ totals, const T* __restrict__ e1, T* __restrict__ g_sdf,
+                            T* __restrict__ g_deform) {
+  const int64_t j = blockIdx.x * (int64_t)kBlk + threadIdx.x;
+  const int32_t U = totals[3];
+  if (j >= U) return;
+  const uint32_t c = key[j];
+  if (j > 0 && key[j - 1] == c) return;
+  T a = T(0), x = T(0), y = T(0), z = T(0);
+  for (int64_t r = j; r < U && key[r] == c; ++r) {
+    const T* b = e1 + (int64_t)val[r] * 4;
+    a += b[0];
+    x += b[1];
+    y += b[2];
+    z += b[3];
+  }
+  g_sdf[c] += a;
+  if (g_deform) {
+    g_deform[(int64_t)c * 3] += x;
+    g_deform[(int64_t)c * 3 + 1] += y;
+    g_deform[(int64_t)c * 3 + 2] += z;
+  }
+}
+
+// One thread per surface cube i: g_alpha of its 8 corners = the sums of its
+// entries' alpha contributions (ra: 2 per entry, at the edge's two corners),
+// in entry order.  A cube's dual vertices are consecutive, so its entries are
+// contiguous, starting at its first vertex's first entry.
+template <typename T>
+__global__ void k_alpha_sum(const int64_t* __restrict__ counters, const int32_t* __restrict__ vd_base,
+                            const int32_t* __restrict__ vd_off, const int32_t* __restrict__ ent_slot, int64_t E,
+                            const T* __restrict__ ra, const int32_t* __restrict__ surf_cube,
+                            T* __restrict__ g_alpha) {
+  const int64_t i = blockIdx.x * (int64_t)kBlk + threadIdx.x;
+  if (i >= counters[0]) return;
   T acc[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) acc[k] = T(0);
-  for (int64_t q = e; q < E && ent_slot[q] / 12 == i; ++q) {
+  for (int64_t q = vd_off[vd_base[i]]; q < E && ent_slot[q] / 12 == i; ++q) {
     const int ed = ent_slot[q] % 12;
     const int k0 = kE0[ed], k1 = kE1[ed];
     const T r0 = ra[2 * q], r1 = ra[2 * q + 1];
@@ -1118,9 +1145,9 @@ template <typename T>
 __global__ void k_entry_bwd(Geom g, const int32_t* __restrict__ totals, Fields<T> f, const int32_t* __restrict__ ent_vd,
                             const T* __restrict__ aux, const T* __restrict__ gvd, const T* __restrict__ sgs,
                             const T* __restrict__ vfeat, const T* __restrict__ gvf, const T* __restrict__ gL,
-                            const T* __restrict__ gP, const T* __restrict__ gN, T* __restrict__ rec,
-                            uint32_t* __restrict__ ckey, int32_t* __restrict__ cval, T* __restrict__ ra,
-                            bool g_deform, T* __restrict__ g_beta, T* __restrict__ g_feat) {
+                            const T* __restrict__ gP, const T* __restrict__ gN, const int32_t* __restrict__ epos,
+                            T* __restrict__ rec, T* __restrict__ ra, bool g_deform, T* __restrict__ g_beta,
+                            T* __restrict__ g_feat) {
   int64_t e = blockIdx.x * (int64_t)kBlk + threadIdx.x;
   if (e >= totals[1]) return;
   const int32_t v = ent_vd[e];
@@ -1197,18 +1224,15 @@ __global__ void k_entry_bwd(Geom g, const int32_t* __restrict__ totals, Fields<T
       gx0[d] -= gev;
     }
   }
-  // per entry side: the corner's contributions (summed per corner after the sort)
-  T* r = rec + e * 8;
+  // per entry side: the corner's contributions, stored at the entry's sorted
+  // slot position (summed per corner by k_corner_c0 / k_corner_c1)
+  T* r = rec + (int64_t)epos[e] * 8;
   r[0] = gw0 * ev.a0 + gs0;
   r[4] = gw1 * ev.a1 + gs1;
   for (int d = 0; d < 3; ++d) {
     r[1 + d] = g_deform ? gx0[d] : T(0);
     r[5 + d] = g_deform ? gx1[d] : T(0);
   }
-  ckey[2 * e] = (uint32_t)ev.c0;
-  ckey[2 * e + 1] = (uint32_t)ev.c1;
-  cval[2 * e] = (int32_t)(2 * e);
-  cval[2 * e + 1] = (int32_t)(2 * e + 1);
   if (ra) {
     const int k0 = kE0[ev.e], k1 = kE1[ev.e];
     const T t0 = tanh(f.alpha[ev.cube * 8 + k0]), t1 = tanh(f.alpha[ev.cube * 8 + k1]);
@@ -1338,7 +1362,7 @@ extern "C" int a3d_dmc_topology(const int32_t* vox, const void* sdf, int dtype, 
   // totals[1] = entries
   A3D_HIP_CHECK(hipMemcpyAsync(w.totals + 1, w.vd_off + Vd, 4, hipMemcpyDeviceToDevice, s));
   hipLaunchKernelGGL(k_entries, dim3(gS), dim3(kBlk), 0, s, tb, w.counters, w.case_s, w.vd_base, w.vd_off, w.ent_slot,
-                     w.ent_vd);
+                     w.ent_vd, w.ent_of_slot);
   A3D_LAUNCH_CHECK();
 #define A3D_DMC_T2(T)                                                                                               \
   do {                                                                                                              \
@@ -1366,7 +1390,7 @@ extern "C" int a3d_dmc_topology(const int32_t* vox, const void* sdf, int dtype, 
   tbytes = w.cub_bytes;
   A3D_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(w.cub, tbytes, w.uflag, w.urank, S_slots + 1, s));
   hipLaunchKernelGGL(k_unique_map, dim3(gH), dim3(kBlk), 0, s, w.skey1, w.sval1, w.urank, w.totals, w.sedge_of_slot,
-                     w.sedge, w.qflag, w.quad_of_edge, kb);
+                     w.sedge, w.qflag, w.quad_of_edge, w.ustart, w.ent_of_slot, w.epos, kb);
   A3D_LAUNCH_CHECK();
   tbytes = w.cub_bytes;
   A3D_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(w.cub, tbytes, w.qflag, w.qrank, S_slots + 1, s));
@@ -1435,11 +1459,13 @@ int dmc_forward_t(const DmcWs& w, const int64_t* sizes, const Geom& g, const Fie
 }
 
 // Backward scratch, sized from the topology's sizes: per entry side 4
-// contribution values and 1 alpha value (T), the (corner, entry side) sort
-// pairs twice (radix sort ping-pong) and the sort's temporary storage.
+// contribution values and 1 alpha value (T), per surface edge its 4
+// second-corner sums (T), the (second corner, edge) sort pairs twice (radix
+// sort ping-pong) and the sort's temporary storage.
 struct BwdScratch {
   void* rec;
   void* ra;
+  void* e1;
   uint32_t* k0;
   uint32_t* k1;
   int32_t* v0;
@@ -1451,7 +1477,7 @@ struct BwdScratch {
 
 BwdScratch carve_bwd(void* base, const int64_t* sizes, int tsize) {
   const int64_t E = sizes[2];
-  const int64_t P = std::max<int64_t>(2 * E, 1);
+  const int64_t P = std::max<int64_t>(sizes[3], 1);
   size_t t = 0, tb = 0;
   (void)hipcub::DeviceRadixSort::SortPairs(nullptr, t, (uint32_t*)nullptr, (uint32_t*)nullptr, (int32_t*)nullptr,
                                            (int32_t*)nullptr, (int)P, 0, 32);
@@ -1462,6 +1488,7 @@ BwdScratch carve_bwd(void* base, const int64_t* sizes, int tsize) {
   auto take = [&](int64_t bytes) { void* q = p ? p + o : nullptr; o += al256(bytes > 0 ? bytes : 1); return q; };
   b.rec = take(2 * E * 4 * tsize);
   b.ra = take(2 * E * tsize);
+  b.e1 = take(4 * P * tsize);
   b.k0 = (uint32_t*)take(4 * P);
   b.k1 = (uint32_t*)take(4 * P);
   b.v0 = (int32_t*)take(4 * P);
@@ -1502,19 +1529,22 @@ int dmc_backward_t(const DmcWs& w, const BwdScratch& b, const int64_t* sizes, in
   T* ra = (g_alpha && f.alpha) ? (T*)b.ra : nullptr;
   hipLaunchKernelGGL(k_entry_bwd<T>, dim3(gridn(E)), dim3(kBlk), 0, s, g, w.totals, f, w.ent_vd, (const T*)w.aux,
                      (const T*)gvd, (const T*)w.sgs, (const T*)vfeat, (const T*)gvf, (const T*)g_L_dev,
-                     (const T*)g_p_alpha, (const T*)g_n_alpha, (T*)b.rec, b.k0, b.v0, ra, g_deform != nullptr,
+                     (const T*)g_p_alpha, (const T*)g_n_alpha, w.epos, (T*)b.rec, ra, g_deform != nullptr,
                      (T*)g_beta, (T*)g_feat);
   A3D_LAUNCH_CHECK();
-  if (E > 0) {
+  const int64_t U = sizes[3];
+  if (E > 0 && U > 0) {
+    hipLaunchKernelGGL(k_corner_c0<T>, dim3(gridn(U)), dim3(kBlk), 0, s, w.sedge, w.ustart, w.totals,
+                       (const T*)b.rec, (T*)b.e1, b.k0, b.v0, (T*)g_sdf, (T*)g_deform);
+    A3D_LAUNCH_CHECK();
     size_t tb = b.tmp_bytes;
-    A3D_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(b.tmp, tb, b.k0, b.k1, b.v0, b.v1, (int)(2 * E), 0,
-                                                     key_bits(M), s));
-    hipLaunchKernelGGL(k_corner_sum<T>, dim3(gridn(2 * E)), dim3(kBlk), 0, s, b.k1, b.v1, 2 * E,
-                       (const T*)b.rec, (T*)g_sdf, (T*)g_deform);
+    A3D_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(b.tmp, tb, b.k0, b.k1, b.v0, b.v1, (int)U, 0, key_bits(M), s));
+    hipLaunchKernelGGL(k_corner_c1<T>, dim3(gridn(U)), dim3(kBlk), 0, s, b.k1, b.v1, w.totals, (const T*)b.e1,
+                       (T*)g_sdf, (T*)g_deform);
     A3D_LAUNCH_CHECK();
     if (ra) {
-      hipLaunchKernelGGL(k_alpha_sum<T>, dim3(gridn(E)), dim3(kBlk), 0, s, w.ent_slot, E, (const T*)ra,
-                         w.surf_cube, (T*)g_alpha);
+      hipLaunchKernelGGL(k_alpha_sum<T>, dim3(gridn(sizes[0])), dim3(kBlk), 0, s, w.counters, w.vd_base, w.vd_off,
+                         w.ent_slot, E, (const T*)ra, w.surf_cube, (T*)g_alpha);
       A3D_LAUNCH_CHECK();
     }
   }
@@ -1660,7 +1690,7 @@ extern "C" int a3d_dmc_forward_qef(void* ws, int64_t N, const int64_t* sizes, co
 extern "C" int a3d_dmc_bwd_scratch_bytes(const int64_t* sizes, int dtype, int64_t* bytes) {
   if (!sizes || !bytes) return fail(A3D_ERR_ARG, "a3d_dmc_bwd_scratch_bytes: null argument");
   if (dtype != 0 && dtype != 1) return fail(A3D_ERR_ARG, "a3d_dmc_bwd_scratch_bytes: dtype %d not in {0, 1}", dtype);
-  if (sizes[2] < 0 || sizes[4] < 0 || 2 * sizes[2] > INT32_MAX || 4 * sizes[4] > INT32_MAX)
+  if (sizes[2] < 0 || sizes[3] < 0 || sizes[4] < 0 || 2 * sizes[2] > INT32_MAX || 4 * sizes[4] > INT32_MAX)
     return fail(A3D_ERR_ARG, "a3d_dmc_bwd_scratch_bytes: sizes out of range");
   *bytes = carve_bwd(nullptr, sizes, dtype ? 8 : 4).total;
   return A3D_OK;
