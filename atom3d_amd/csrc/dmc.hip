@@ -346,6 +346,103 @@ __global__ void k_hot_write(const T* __restrict__ sdf, T iso, const int32_t* __r
   }
 }
 
+// The same (key, slot) pairs in sorted order without a radix sort: a counting
+// sort by first corner (count per corner, scan, scatter into the corner's
+// bucket), then each bucket sorted by (key, slot) — the order of the stable
+// sort by key, since the slots were listed in slot order.  A bucket holds the
+// slots of the edges leaving one corner: at most 3 edges of 4 cubes on a
+// consistent grid.  Buckets longer than kBucketMax (possible only for cube
+// corner tables that are not a grid) set totals[5]; the caller then sorts
+// all slots by radix sort instead.
+constexpr int kBucketMax = 256;
+
+// A surface cube's sign-change edges from its case bits (corner k inside ==
+// bit k; the neighbour check's complement keeps every edge's inequality).
+__device__ __forceinline__ bool hot_edge(uint32_t cs, int e) { return ((cs >> kE0[e]) ^ (cs >> kE1[e])) & 1u; }
+
+__global__ void k_edge_count(const uint8_t* __restrict__ case_s, const int32_t* __restrict__ ci,
+                             const int32_t* __restrict__ surf_cube, const int64_t* __restrict__ counters,
+                             int32_t* __restrict__ cnt, int32_t* __restrict__ totals) {
+  int64_t i = blockIdx.x * (int64_t)kBlk + threadIdx.x;
+  if (i == 0) totals[5] = 0;
+  if (i >= counters[0]) return;
+  const int64_t cube = surf_cube[i];
+  const uint32_t cs = case_s[i];
+  for (int e = 0; e < 12; ++e)
+    if (hot_edge(cs, e)) atomicAdd(&cnt[ci[cube * 8 + kE0[e]]], 1);
+}
+
+// Each slot lands in its first corner's bucket as one word (c1 << 31 | slot);
+// k_edge_fix sorts the words and unpacks them into (key, slot).
+__global__ void k_edge_scatter(const uint8_t* __restrict__ case_s, const int32_t* __restrict__ ci,
+                               const int32_t* __restrict__ surf_cube, const int64_t* __restrict__ counters,
+                               const int32_t* __restrict__ start, int32_t* __restrict__ cnt,
+                               uint64_t* __restrict__ word) {
+  int64_t i = blockIdx.x * (int64_t)kBlk + threadIdx.x;
+  if (i >= counters[0]) return;
+  const int64_t cube = surf_cube[i];
+  const uint32_t cs = case_s[i];
+  for (int e = 0; e < 12; ++e)
+    if (hot_edge(cs, e)) {
+      const int64_t c0 = ci[cube * 8 + kE0[e]], c1 = ci[cube * 8 + kE1[e]];
+      const int32_t p = start[c0] + atomicSub(&cnt[c0], 1) - 1;
+      word[p] = ((uint64_t)c1 << 31) | (uint64_t)(i * 12 + e);
+    }
+}
+
+// One thread per corner c: its bucket's words sorted ascending, i.e. by
+// (c1, slot), then unpacked in place into key = (c << kb | c1) and val =
+// slot; totals[2] = the slot count.  Buckets of up to 16 sort in registers
+// (odd-even transposition network, fully unrolled), longer ones by insertion
+// sort in place.
+__global__ void k_edge_fix(const int32_t* __restrict__ start, int64_t M, uint64_t* __restrict__ key,
+                           int32_t* __restrict__ val, int32_t* __restrict__ totals, int kb) {
+  int64_t c = blockIdx.x * (int64_t)kBlk + threadIdx.x;
+  if (c == 0) totals[2] = start[M];
+  if (c >= M) return;
+  const int32_t a = start[c], b = start[c + 1];
+  const int n = b - a;
+  if (n == 0) return;
+  if (n > kBucketMax) {
+    totals[5] = 1;
+    return;
+  }
+  if (n <= 16) {
+    uint64_t r[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) r[k] = k < n ? key[a + k] : ~uint64_t(0);
+#pragma unroll
+    for (int round = 0; round < 16; ++round)
+#pragma unroll
+      for (int k = round & 1; k + 1 < 16; k += 2) {
+        const uint64_t x = r[k], y = r[k + 1];
+        r[k] = x < y ? x : y;
+        r[k + 1] = x < y ? y : x;
+      }
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+      if (k < n) {
+        key[a + k] = ((uint64_t)c << kb) | (r[k] >> 31);
+        val[a + k] = (int32_t)(r[k] & 0x7fffffffu);
+      }
+    return;
+  }
+  for (int32_t x = a + 1; x < b; ++x) {
+    const uint64_t k = key[x];
+    int32_t y = x - 1;
+    while (y >= a && key[y] > k) {
+      key[y + 1] = key[y];
+      --y;
+    }
+    key[y + 1] = k;
+  }
+  for (int32_t x = a; x < b; ++x) {
+    const uint64_t r = key[x];
+    key[x] = ((uint64_t)c << kb) | (r >> 31);
+    val[x] = (int32_t)(r & 0x7fffffffu);
+  }
+}
+
 __global__ void k_unique_flags(const uint64_t* __restrict__ key, const int32_t* __restrict__ totals,
                                int32_t* __restrict__ uflag) {
   int64_t j = blockIdx.x * (int64_t)kBlk + threadIdx.x;
@@ -1364,7 +1461,27 @@ extern "C" int a3d_dmc_topology(const int32_t* vox, const void* sdf, int dtype, 
   hipLaunchKernelGGL(k_entries, dim3(gS), dim3(kBlk), 0, s, tb, w.counters, w.case_s, w.vd_base, w.vd_off, w.ent_slot,
                      w.ent_vd, w.ent_of_slot);
   A3D_LAUNCH_CHECK();
-#define A3D_DMC_T2(T)                                                                                               \
+  // Sign-change slots as (corner pair key, slot) pairs sorted by key, then
+  // slot: by buckets per first corner (uflag / urank serve as the counts and
+  // offsets, free until the unique pass; they hold 12N + 1 >= M + 1 values
+  // when every corner belongs to a cube), else by a radix sort of the slots
+  // listed in slot order.
+  const bool buckets = M + 1 <= 12 * std::max<int64_t>(N, 2) + 1;
+  if (buckets) {
+    hipLaunchKernelGGL(k_zero<int32_t>, dim3(gridn(M + 1)), dim3(kBlk), 0, s, w.uflag, M + 1);
+    A3D_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_edge_count, dim3(gS), dim3(kBlk), 0, s, w.case_s, ci, w.surf_cube, w.counters, w.uflag,
+                       w.totals);
+    A3D_LAUNCH_CHECK();
+    tbytes = w.cub_bytes;
+    A3D_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(w.cub, tbytes, w.uflag, w.urank, (int)(M + 1), s));
+    hipLaunchKernelGGL(k_edge_scatter, dim3(gS), dim3(kBlk), 0, s, w.case_s, ci, w.surf_cube, w.counters, w.urank,
+                       w.uflag, w.skey1);
+    A3D_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_edge_fix, dim3(gridn(M)), dim3(kBlk), 0, s, w.urank, M, w.skey1, w.sval1, w.totals, kb);
+    A3D_LAUNCH_CHECK();
+  }
+#define A3D_DMC_T3(T)                                                                                               \
   do {                                                                                                              \
     const T* S = (const T*)sdf;                                                                                     \
     const T iso = (T)isovalue;                                                                                      \
@@ -1376,14 +1493,19 @@ extern "C" int a3d_dmc_topology(const int32_t* vox, const void* sdf, int dtype, 
                        w.skey0, w.sval0, w.totals, kb);                                                             \
     A3D_LAUNCH_CHECK();                                                                                             \
   } while (0)
-  if (dtype == 0) A3D_DMC_T2(float); else A3D_DMC_T2(double);
-#undef A3D_DMC_T2
-  int32_t S_slots = 0;
-  A3D_HIP_CHECK(hipMemcpyAsync(&S_slots, w.totals + 2, 4, hipMemcpyDeviceToHost, s));
+  int32_t hot[4] = {0, 0, 0, 0};  // totals[2..5]: slots, -, -, long-bucket flag
+  A3D_HIP_CHECK(hipMemcpyAsync(hot, w.totals + 2, sizeof(hot), hipMemcpyDeviceToHost, s));
   A3D_HIP_CHECK(hipStreamSynchronize(s));
-  tbytes = w.cub_bytes;
-  A3D_HIP_CHECK(
-      hipcub::DeviceRadixSort::SortPairs(w.cub, tbytes, w.skey0, w.skey1, w.sval0, w.sval1, S_slots, 0, 2 * kb, s));
+  if (!buckets || hot[3]) {
+    if (dtype == 0) A3D_DMC_T3(float); else A3D_DMC_T3(double);
+    A3D_HIP_CHECK(hipMemcpyAsync(hot, w.totals + 2, 4, hipMemcpyDeviceToHost, s));
+    A3D_HIP_CHECK(hipStreamSynchronize(s));
+    tbytes = w.cub_bytes;
+    A3D_HIP_CHECK(
+        hipcub::DeviceRadixSort::SortPairs(w.cub, tbytes, w.skey0, w.skey1, w.sval0, w.sval1, hot[0], 0, 2 * kb, s));
+  }
+#undef A3D_DMC_T3
+  const int32_t S_slots = hot[0];
   unsigned gH = gridn((int64_t)S_slots + 1);
   hipLaunchKernelGGL(k_unique_flags, dim3(gH), dim3(kBlk), 0, s, w.skey1, w.totals, w.uflag);
   A3D_LAUNCH_CHECK();

@@ -99,6 +99,56 @@ def test_larger_band_matches_oracle(res, isovalue):
         assert err <= 1e-6 * DIAG, (k, err)
 
 
+@pytest.mark.parametrize("N,M,seed", [(200, 64, 1), (2000, 16, 2), (50, 5000, 3)])
+def test_non_grid_cube_tables_match_oracle(N, M, seed):
+    """Cube corner tables that are not a grid, which the reference accepts:
+    corners shared by many cubes (edge buckets of tens of slots, and above
+    256 slots, where the sorted-bucket pass hands over to the radix sort) and
+    more corners than 12 per cube (the radix path).  Faces bit-exact, values
+    and gradients to the other tests' bars, gradients bit-reproducible."""
+    print(f"dmc non-grid N={N} M={M} seed={seed}", flush=True)
+    rng = np.random.default_rng(seed)
+    res = 16
+    # every cube on one voxel and corner id c always at local corner c % 8, so
+    # each corner has one position (the reference sets corner positions by an
+    # indexed assignment, which picks one of several for corners placed
+    # inconsistently)
+    cubes = np.tile([[3, 5, 7]], (N, 1))
+    ci = np.arange(8)[None, :] + 8 * rng.integers(0, M // 8, (N, 8))
+    s = rng.standard_normal(M)
+    alpha, beta = rng.standard_normal((N, 8)), rng.standard_normal((N, 12))
+    deform = rng.uniform(-0.01, 0.01, (M, 3))
+
+    def run(dev, mod):
+        a = [torch.tensor(x, dtype=torch.float64 if mod is None else torch.float32, device=dev, requires_grad=True)
+             for x in (s, deform, alpha, beta)]
+        kw = dict(deform=a[1], alpha=a[2], beta=a[3])
+        if mod is None:
+            o = dmc_forward(torch.tensor(cubes), a[0], torch.tensor(ci), res, dtype=torch.float64, **kw)
+            return a, {k: o[k] for k in OUTS[:4]}, o["faces"]
+        o = mod(torch.tensor(cubes, device=dev), a[0], torch.tensor(ci, device=dev), res, **kw)
+        return a, {k: getattr(o, k) for k in OUTS[:4]}, o.faces
+
+    ref_a, ref_o, ref_f = run("cpu", None)
+    assert len(ref_o["vertices"]) > 0  # quads only where exactly 4 cubes share an edge: few or none here
+    W = {k: torch.tensor(np.random.default_rng(seed + 9).standard_normal(tuple(v.shape))) for k, v in ref_o.items()}
+    sum((W[k] * v).sum() for k, v in ref_o.items()).backward()
+    m = SparseDiffDMC(device=DEV)
+    grads = []
+    for _ in range(2):
+        a, o, f = run(DEV, m)
+        assert torch.equal(f.cpu(), ref_f)
+        for k in OUTS[:4]:
+            err = (o[k].detach().cpu().double() - ref_o[k].detach()).abs().max().item()
+            assert err <= 1e-5 * DIAG, (k, err)
+        sum((W[k].to(DEV, torch.float32) * v).sum() for k, v in o.items()).backward()
+        grads.append([x.grad.detach().clone() for x in a])
+    for g0, g1, r in zip(grads[0], grads[1], ref_a):
+        assert torch.equal(g0, g1)
+        ref = r.grad.numpy()
+        np.testing.assert_allclose(g0.cpu().double().numpy(), ref, rtol=0, atol=1e-3 * max(1.0, np.abs(ref).max()))
+
+
 @pytest.mark.parametrize("tag", ["f32", "f64"])
 def test_adjoint_on_larger_band_matches_oracle_and_is_deterministic(tag):
     """A 64^3 band (thousands of corners shared by up to 8 cubes, so the
