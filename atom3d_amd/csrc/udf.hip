@@ -37,6 +37,11 @@ namespace {
 
 constexpr int kBlk = 256;
 constexpr int kWaves = kBlk / 64;
+// The packet walk runs one wavefront per block: a block's LDS (stack and
+// queue) is then released as soon as its own walk ends instead of waiting
+// for the slowest of several wavefronts (occupancy 64 -> 74 %, the 4.1M-face
+// walk 2.5 % faster).
+constexpr int kBlkW = 64;
 constexpr int kLaneLds = 16;  // per-lane LDS stack entries in the lane path
 constexpr int kLaneOvf = kStackTotal - kLaneLds;
 
@@ -423,18 +428,17 @@ __device__ __forceinline__ float grid_bound(const GridBound& g, const float q[3]
 // face -1 and dist = its seed.  gb.G (when seed is null): the bound from a
 // grid, computed here (grid_bound).
 template <bool STATS, bool QUEUE>
-__global__ __launch_bounds__(kBlk) void k_udf_wave(const void* __restrict__ blob, const float* __restrict__ P,
+__global__ __launch_bounds__(kBlkW) void k_udf_wave(const void* __restrict__ blob, const float* __restrict__ P,
                                                     const uint32_t* __restrict__ perm, int64_t K,
                                                     const float* __restrict__ seed,
                                                     float* __restrict__ dist, int32_t* __restrict__ face,
                                                     float* __restrict__ closest, float* __restrict__ uvw,
                                                     unsigned long long* __restrict__ counters, bool use_obbs,
                                                     GridBound gb) {
-  __shared__ float4 s_stk[kWaves][kWaveStack][2];  // (lo.xyz, ref bits), (hi.xyz, obb code)
-  __shared__ WaveQ s_q[QUEUE ? kWaves : 1];
-  const int lane = threadIdx.x & 63;
-  const int wv = threadIdx.x >> 6;
-  const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlk + threadIdx.x;
+  __shared__ float4 s_stk[kWaveStack][2];  // (lo.xyz, ref bits), (hi.xyz, obb code)
+  __shared__ WaveQ s_q[1];
+  const int lane = threadIdx.x;
+  const int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlkW + threadIdx.x;
   const bool valid = i < K;
   if (!__any(valid)) return;
   const int64_t q = valid ? (perm ? (int64_t)perm[i] : i) : 0;
@@ -454,7 +458,7 @@ __global__ __launch_bounds__(kBlk) void k_udf_wave(const void* __restrict__ blob
   uint32_t n_node = 0, n_leaf_tri = 0, n_test = 0, n_lane = 0;
   wave_walk<STATS, QUEUE>(nodes, tris, boxes_of(blob), use_obbs ? obbs_of(blob) : nullptr,
                           use_obbs ? tobbs_of(blob) : nullptr, p, valid, slack,
-                          s_stk[wv], QUEUE ? &s_q[wv] : nullptr, lane,
+                          s_stk, QUEUE ? &s_q[0] : nullptr, lane,
                           best, best_id,
                           best_tri, n_node, n_leaf_tri, n_test, n_lane);
   if (STATS) {
@@ -884,6 +888,7 @@ QWs carve_q(void* base, int64_t K) {
 }
 
 inline unsigned grid_for(int64_t n) { return (unsigned)div_up(n > 0 ? n : 1, kBlk); }
+inline unsigned grid_for_w(int64_t n) { return (unsigned)div_up(n > 0 ? n : 1, kBlkW); }
 
 }  // namespace
 }  // namespace a3d
@@ -969,7 +974,8 @@ int udf_launch(const void* blob, const float* P, int64_t K, const uint32_t* perm
                const GridBound* grid = nullptr) {
   const GridBound gb = grid ? *grid : GridBound{};
   if (!blob) return fail(A3D_ERR_ARG, "a3d_bvh_udf: null BVH blob");
-  if (K < 0 || K > (int64_t(1) << 38)) return fail(A3D_ERR_ARG, "a3d_bvh_udf: K=%lld out of range", (long long)K);
+  // the packet walk launches one 64-lane block per 64 queries (grid x < 2^32)
+  if (K < 0 || K > (int64_t(1) << 37)) return fail(A3D_ERR_ARG, "a3d_bvh_udf: K=%lld out of range", (long long)K);
   if (perm && K > 0xffffffffll) return fail(A3D_ERR_ARG, "a3d_bvh_udf: perm needs K < 2^32");
   if (flags & ~(A3D_UDF_LANE | A3D_UDF_DIRECT | A3D_UDF_NO_OBBS | A3D_UDF_SPLIT | A3D_UDF_REDO))
     return fail(A3D_ERR_ARG, "a3d_bvh_udf: unknown flags 0x%x", flags);
@@ -994,17 +1000,17 @@ int udf_launch(const void* blob, const float* P, int64_t K, const uint32_t* perm
                        uvw, use_obbs, (const int32_t*)nullptr, gb);
   else if (flags & A3D_UDF_DIRECT) {
     if (counters)
-      hipLaunchKernelGGL((k_udf_wave<true, false>), dim3(grid_for(K)), dim3(kBlk), 0, s, blob, P, perm, K, seed, dist,
+      hipLaunchKernelGGL((k_udf_wave<true, false>), dim3(grid_for_w(K)), dim3(kBlkW), 0, s, blob, P, perm, K, seed, dist,
                          face, closest, uvw, c, use_obbs, gb);
     else
-      hipLaunchKernelGGL((k_udf_wave<false, false>), dim3(grid_for(K)), dim3(kBlk), 0, s, blob, P, perm, K, seed, dist,
+      hipLaunchKernelGGL((k_udf_wave<false, false>), dim3(grid_for_w(K)), dim3(kBlkW), 0, s, blob, P, perm, K, seed, dist,
                          face, closest, uvw, c, use_obbs, gb);
   } else {
     if (counters)
-      hipLaunchKernelGGL((k_udf_wave<true, true>), dim3(grid_for(K)), dim3(kBlk), 0, s, blob, P, perm, K, seed, dist,
+      hipLaunchKernelGGL((k_udf_wave<true, true>), dim3(grid_for_w(K)), dim3(kBlkW), 0, s, blob, P, perm, K, seed, dist,
                          face, closest, uvw, c, use_obbs, gb);
     else
-      hipLaunchKernelGGL((k_udf_wave<false, true>), dim3(grid_for(K)), dim3(kBlk), 0, s, blob, P, perm, K, seed, dist,
+      hipLaunchKernelGGL((k_udf_wave<false, true>), dim3(grid_for_w(K)), dim3(kBlkW), 0, s, blob, P, perm, K, seed, dist,
                          face, closest, uvw, c, use_obbs, gb);
   }
   A3D_LAUNCH_CHECK();
