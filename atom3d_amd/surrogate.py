@@ -166,10 +166,12 @@ class UDFSurrogate:
                       wsb.value, ctypes.byref(n), self._s())
         return dist, face, closest, uvw, n.value
 
-    def evaluate(self, P: torch.Tensor, band: float = 0.01) -> dict:
-        """Errors of the routed answers against the exact BVH on P."""
+    def evaluate(self, P: torch.Tensor, band: float = 0.01, ref: torch.Tensor | None = None) -> dict:
+        """Errors of the routed answers against the exact BVH on P (`ref`: the
+        exact distances, when the caller has them already)."""
         d, face, _, _, n_exact = self.udf(P, band)
-        ref = self.bvh.udf(P)[0]
+        if ref is None:
+            ref = self.bvh.udf(P)[0]
         err = (d - ref).abs()
         far = face < 0
         out = {"n": int(P.shape[0]), "n_exact": n_exact, "frac_exact": n_exact / max(int(P.shape[0]), 1),

@@ -619,7 +619,10 @@ def glue_leg(vox, lists, oc, a):
     return out
 
 
-def measure_surrogate(mesh, V, P, a, fp64_check: bool):
+SWEEP_BANDS = (0.002, 0.005, 0.01, 0.02, 0.05, 0.1, 0.2, 0.5, 4.0)
+
+
+def measure_surrogate(mesh, V, P, a, fp64_check: bool, sweep: bool = False):
     """Train the surrogate on the mesh (2^24 BVH-labelled samples, half
     uniform in [-1,1]^3, half within 0.02 of the surface; 2000 Adam steps of
     2^18) and time the router on P: Morton ordering + network on every query
@@ -672,7 +675,17 @@ def measure_surrogate(mesh, V, P, a, fp64_check: bool):
         d64, _ = oracle.fp64_udf(V.numpy(), F.numpy(), sub)
         dr = sg.udf(P[:256], band=band)[0].cpu().numpy().astype(np.float64)
         fp64 = {"n": 256, "max_err": float(np.abs(dr - d64).max()), "mean_err": float(np.abs(dr - d64).mean())}
+    rows = []
+    if sweep:  # the safety band against speed and error (4.0 > the box diagonal: every query exact)
+        ref = mesh._bvh.udf(P)[0]
+        for b in SWEEP_BANDS:
+            bms = timed(lambda: sg.udf(P, band=b), max(3, a.steps // 4))
+            e = sg.evaluate(P, band=b, ref=ref)
+            rows.append({"band": b, "ms": bms, "queries_per_s": K / (bms * 1e-3), "frac_exact": e["frac_exact"],
+                         "max_err": e["max_err"], "mean_err": e["mean_err"], "missed_band": e["missed_band"],
+                         "max_err_over_diag": e["max_err"] / float((V.max(0)[0] - V.min(0)[0]).norm())})
     return {
+        "band_sweep": rows,
         "queries_per_s": K / (ms * 1e-3), "ms_per_step": ms, "band": band,
         "train": {"samples": 1 << 24, "steps": hist["steps"], "batch": hist["batch"], "train_s": hist["train_s"],
                   "losses": hist["losses"]},
@@ -697,7 +710,7 @@ def bench_surrogate(a):
     V, F = datasets.sphere_union()
     mesh = MeshBVH(V.to(dev), F.to(dev))
     P = datasets.uniform_points(a.queries, seed=7).to(dev)
-    r = measure_surrogate(mesh, V, P, a, fp64_check=not a.no_cpu)
+    r = measure_surrogate(mesh, V, P, a, fp64_check=not a.no_cpu, sweep=True)
     for _ in range(a.warmup):
         mesh._bvh.udf(P)
     torch.cuda.synchronize()
@@ -719,6 +732,7 @@ def bench_surrogate(a):
         "speedup_vs_plain_bvh": plain_ms / r["ms_per_step"],
         **{k: r[k] for k in ("train", "errors_vs_bvh", "errors_vs_fp64", "bbox_diag", "meets_max_err_1e-5_diag",
                              "roofline")},
+        "band_sweep": [dict(row, speedup_vs_plain_bvh=plain_ms / row["ms"]) for row in r["band_sweep"]],
         "cpu_baseline": None,
     }
     print(json.dumps(res), flush=True)
