@@ -45,6 +45,38 @@ def test_batched_equals_per_mesh(small_batch):
     assert torch.equal(got[3][has], ref[3][has])
 
 
+def test_short_bounds_are_walked_again(small_batch):
+    """Bound records whose grid distances are halved and that carry no
+    closest-face bound fall short of many queries' distances: the bounded walk
+    reports those as unfound and the redo pass walks them again, so the
+    results still equal the per-mesh walk bit for bit."""
+    import numpy as np
+
+    from atom3d_amd.batch import _BOUND
+
+    meshes, P, counts, bu = small_batch
+    assert bu.bounds is not None
+    rec = bu.bounds.cpu().numpy().view(_BOUND).copy()
+    halves = []
+    for m, b in enumerate(bu.bvhs):
+        h = (b.seed_grid(bu.BOUND_GRID_RES)[0] * 0.5).contiguous()
+        halves.append(h)
+        rec[m]["G"] = h.data_ptr()
+        rec[m]["GF"] = 0
+    saved = bu.bounds
+    bu.bounds = torch.from_numpy(rec.view(np.uint8).copy()).to(bu.device)
+    try:
+        plan = bu.plan(P, counts)
+        got = [t.clone() for t in bu.run(plan, {"waves_per_block": 4, "waves_per_cu": 16, "order": "cost"})]
+        torch.cuda.synchronize()
+    finally:
+        bu.bounds = saved
+    ref = _per_mesh(bu, P, counts)
+    assert _same(got[:3], ref[:3])
+    has = ref[1] >= 0
+    assert torch.equal(got[3][has], ref[3][has])
+
+
 def test_graph_replays_match_eager(small_batch):
     meshes, P, counts, bu = small_batch
     plan = bu.plan(P, counts)
