@@ -37,6 +37,11 @@ constexpr int kBlk = 256;
 // Local edge e joins corners (kE0[e], kE1[e]) (sparse_diffdmc.py:73-77).
 __constant__ int8_t kE0[12] = {0, 1, 4, 0, 2, 3, 6, 2, 2, 3, 7, 6};
 __constant__ int8_t kE1[12] = {1, 5, 5, 4, 3, 7, 7, 6, 0, 1, 5, 4};
+// The same as compile-time constants (for fully unrolled loops), and the
+// edges leaving each local corner as 12-bit masks (kE0h[e] == j).
+constexpr int kE0h[12] = {0, 1, 4, 0, 2, 3, 6, 2, 2, 3, 7, 6};
+constexpr int kE1h[12] = {1, 5, 5, 4, 3, 7, 7, 6, 0, 1, 5, 4};
+constexpr uint32_t kEdgesFrom[8] = {0x009, 0x002, 0x190, 0x220, 0x004, 0x000, 0x840, 0x400};
 
 struct I4 {
   int32_t v[4];
@@ -357,8 +362,42 @@ __global__ void k_hot_write(const T* __restrict__ sdf, T iso, const int32_t* __r
 constexpr int kBucketMax = 256;
 
 // A surface cube's sign-change edges from its case bits (corner k inside ==
-// bit k; the neighbour check's complement keeps every edge's inequality).
-__device__ __forceinline__ bool hot_edge(uint32_t cs, int e) { return ((cs >> kE0[e]) ^ (cs >> kE1[e])) & 1u; }
+// bit k; the neighbour check's complement keeps every edge's inequality), as
+// a 12-bit mask.
+__device__ __forceinline__ uint32_t hot_mask(uint32_t cs) {
+  uint32_t m = 0;
+#pragma unroll
+  for (int e = 0; e < 12; ++e) m |= (((cs >> kE0h[e]) ^ (cs >> kE1h[e])) & 1u) << e;
+  return m;
+}
+
+// Adds a cube's edges to their first corners' bucket counts, one atomic per
+// first corner (7 of a cube's 8 corners start 1-3 of its edges): `mask`
+// selects the edges.
+__device__ __forceinline__ void bucket_count(const int32_t* __restrict__ row, uint32_t mask, int32_t* __restrict__ cnt) {
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const uint32_t m = mask & kEdgesFrom[j];
+    if (m) atomicAdd(&cnt[row[j]], __popc(m));
+  }
+}
+
+// Places a cube's selected edges in their first corners' buckets as words
+// (corner1 << 31 | slot0 + edge), one atomic per first corner.
+__device__ __forceinline__ void bucket_scatter(const int32_t* __restrict__ row, uint32_t mask, int64_t slot0,
+                                               const int32_t* __restrict__ start, int32_t* __restrict__ cnt,
+                                               uint64_t* __restrict__ word) {
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const uint32_t m = mask & kEdgesFrom[j];
+    if (!m) continue;
+    const int32_t c0 = row[j], k = __popc(m);
+    int32_t p = start[c0] + atomicSub(&cnt[c0], k) - k;
+#pragma unroll
+    for (int e = 0; e < 12; ++e)
+      if ((m >> e) & 1u) word[p++] = ((uint64_t)(uint32_t)row[kE1h[e]] << 31) | (uint64_t)(slot0 + e);
+  }
+}
 
 __global__ void k_edge_count(const uint8_t* __restrict__ case_s, const int32_t* __restrict__ ci,
                              const int32_t* __restrict__ surf_cube, const int64_t* __restrict__ counters,
@@ -366,10 +405,7 @@ __global__ void k_edge_count(const uint8_t* __restrict__ case_s, const int32_t* 
   int64_t i = blockIdx.x * (int64_t)kBlk + threadIdx.x;
   if (i == 0) totals[5] = 0;
   if (i >= counters[0]) return;
-  const int64_t cube = surf_cube[i];
-  const uint32_t cs = case_s[i];
-  for (int e = 0; e < 12; ++e)
-    if (hot_edge(cs, e)) atomicAdd(&cnt[ci[cube * 8 + kE0[e]]], 1);
+  bucket_count(ci + (int64_t)surf_cube[i] * 8, hot_mask(case_s[i]), cnt);
 }
 
 // Each slot lands in its first corner's bucket as one word (c1 << 31 | slot);
@@ -380,14 +416,7 @@ __global__ void k_edge_scatter(const uint8_t* __restrict__ case_s, const int32_t
                                uint64_t* __restrict__ word) {
   int64_t i = blockIdx.x * (int64_t)kBlk + threadIdx.x;
   if (i >= counters[0]) return;
-  const int64_t cube = surf_cube[i];
-  const uint32_t cs = case_s[i];
-  for (int e = 0; e < 12; ++e)
-    if (hot_edge(cs, e)) {
-      const int64_t c0 = ci[cube * 8 + kE0[e]], c1 = ci[cube * 8 + kE1[e]];
-      const int32_t p = start[c0] + atomicSub(&cnt[c0], 1) - 1;
-      word[p] = ((uint64_t)c1 << 31) | (uint64_t)(i * 12 + e);
-    }
+  bucket_scatter(ci + (int64_t)surf_cube[i] * 8, hot_mask(case_s[i]), i * 12, start, cnt, word);
 }
 
 // One thread per corner c: its bucket's words sorted ascending, i.e. by
@@ -794,6 +823,183 @@ __device__ __forceinline__ void cg_edge(const uint64_t* __restrict__ ekey, const
   gp = (s1 - s0) / el;
 }
 
+// Unit direction and slope of the edge (c0, c1) of slot `slot` (as cg_edge).
+template <typename T>
+__device__ __forceinline__ void cg_edge_at(int64_t c0, int64_t c1, int64_t slot, const int32_t* __restrict__ vox,
+                                           const Fields<T>& f, T* d, T& gp) {
+  const int64_t cube = slot / 12;
+  const int e = (int)(slot % 12);
+  T x0[3], x1[3];
+  corner_world(vox, cube, kE0[e], c0, f, x0);
+  corner_world(vox, cube, kE1[e], c1, f, x1);
+  const T s0 = f.sdf[c0] - f.iso, s1 = f.sdf[c1] - f.iso;
+  T ev[3] = {x1[0] - x0[0], x1[1] - x0[1], x1[2] - x0[2]};
+  T el = norm3(ev);
+  el = el > T(1e-8) ? el : T(1e-8);
+  for (int q = 0; q < 3; ++q) d[q] = ev[q] / el;
+  gp = (s1 - s0) / el;
+}
+
+// Adds one edge (direction d, slope gp) to a corner's normal equations a[9]
+// (6 of DtD, 3 of Dtb): +slope*dir on the first-corner side, -slope*dir on
+// the second (sparse_diffdmc.py:488-502).
+template <typename T>
+__device__ __forceinline__ void cg_accum(T* a, const T* d, T gp, bool second) {
+  a[0] = a[0] + d[0] * d[0];
+  a[1] = a[1] + d[0] * d[1];
+  a[2] = a[2] + d[0] * d[2];
+  a[3] = a[3] + d[1] * d[1];
+  a[4] = a[4] + d[1] * d[2];
+  a[5] = a[5] + d[2] * d[2];
+  for (int q = 0; q < 3; ++q) a[6 + q] = a[6 + q] + (second ? -(gp * d[q]) : gp * d[q]);
+}
+
+// (DtD + 1e-4 I) g = Dtb solved in FP64 by the adjugate (the matrix is
+// symmetric positive definite), rounded to T; NaN rows become 0 (:504-513).
+template <typename T>
+__device__ __forceinline__ void cg_store(const T* a, int64_t m, T* __restrict__ cg) {
+  const T eps = T(1e-4);
+  const double xx = (double)(a[0] + eps), xy = (double)a[1], xz = (double)a[2];
+  const double yy = (double)(a[3] + eps), yz = (double)a[4], zz = (double)(a[5] + eps);
+  const double bx = (double)a[6], by = (double)a[7], bz = (double)a[8];
+  const double c00 = yy * zz - yz * yz, c01 = xz * yz - xy * zz, c02 = xy * yz - xz * yy;
+  const double c11 = xx * zz - xz * xz, c12 = xy * xz - xx * yz, c22 = xx * yy - xy * xy;
+  const double det = xx * c00 + xy * c01 + xz * c02;
+  T g[3] = {(T)((c00 * bx + c01 * by + c02 * bz) / det), (T)((c01 * bx + c11 * by + c12 * bz) / det),
+            (T)((c02 * bx + c12 * by + c22 * bz) / det)};
+  const bool bad = isnan(g[0]) || isnan(g[1]) || isnan(g[2]);
+  for (int d = 0; d < 3; ++d) cg[m * 3 + d] = bad ? T(0) : g[d];
+}
+
+// Corner gradients by buckets (the same edges, order and sums as the sorted
+// path below, without a radix sort): every (cube, edge) slot goes to its
+// first corner's bucket as the word (corner1 << 31 | slot); a bucket sorted
+// by word lists the corner's edges as first corner in edge order, each
+// edge's first word holding its lowest slot (the one the sorted path keeps).
+// Then each distinct edge goes to its second corner's bucket as (corner0 <<
+// 31 | that slot), which sorted lists the corner's edges as second corner in
+// edge order.  Buckets above kBucketMax set totals[5] (the caller then takes
+// the sorted path).
+__global__ void k_cgb_count(const int32_t* __restrict__ ci, int64_t N, int32_t* __restrict__ cnt,
+                            int32_t* __restrict__ totals) {
+  const int64_t n = blockIdx.x * (int64_t)kBlk + threadIdx.x;
+  if (n == 0) totals[5] = 0;
+  if (n >= N) return;
+  bucket_count(ci + n * 8, 0xfffu, cnt);
+}
+
+__global__ void k_cgb_scatter(const int32_t* __restrict__ ci, int64_t N, const int32_t* __restrict__ start,
+                              int32_t* __restrict__ cnt, uint64_t* __restrict__ word) {
+  const int64_t n = blockIdx.x * (int64_t)kBlk + threadIdx.x;
+  if (n >= N) return;
+  bucket_scatter(ci + n * 8, 0xfffu, n * 12, start, cnt, word);
+}
+
+// Sorts words[a, b) ascending: up to 16 in registers (odd-even transposition
+// network, fully unrolled), up to kBucketMax by insertion in place; false for
+// longer buckets (left unsorted).
+__device__ __forceinline__ bool sort_bucket(uint64_t* __restrict__ w, int32_t a, int32_t b) {
+  const int n = b - a;
+  if (n < 2) return true;
+  if (n > kBucketMax) return false;
+  if (n <= 16) {
+    uint64_t r[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) r[k] = k < n ? w[a + k] : ~uint64_t(0);
+#pragma unroll
+    for (int round = 0; round < 16; ++round)
+#pragma unroll
+      for (int k = round & 1; k + 1 < 16; k += 2) {
+        const uint64_t x = r[k], y = r[k + 1];
+        r[k] = x < y ? x : y;
+        r[k + 1] = x < y ? y : x;
+      }
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+      if (k < n) w[a + k] = r[k];
+    return true;
+  }
+  for (int32_t x = a + 1; x < b; ++x) {
+    const uint64_t k = w[x];
+    int32_t y = x - 1;
+    while (y >= a && w[y] > k) {
+      w[y + 1] = w[y];
+      --y;
+    }
+    w[y + 1] = k;
+  }
+  return true;
+}
+
+// One thread per corner c: its first-corner bucket sorted, then one count per
+// distinct edge at the edge's second corner (cnt is all zero again after
+// k_cgb_scatter).
+__global__ void k_cgb_fix(const int32_t* __restrict__ start, int64_t M, uint64_t* __restrict__ word,
+                          int32_t* __restrict__ cnt, int32_t* __restrict__ totals) {
+  const int64_t c = blockIdx.x * (int64_t)kBlk + threadIdx.x;
+  if (c >= M) return;
+  const int32_t a = start[c], b = start[c + 1];
+  if (!sort_bucket(word, a, b)) {
+    totals[5] = 1;
+    return;
+  }
+  uint64_t prev = ~uint64_t(0);
+  for (int32_t x = a; x < b; ++x) {
+    const uint64_t c1 = word[x] >> 31;
+    if (c1 != prev) atomicAdd(&cnt[c1], 1);
+    prev = c1;
+  }
+}
+
+__global__ void k_cgb_scatter2(const int32_t* __restrict__ start, int64_t M, const uint64_t* __restrict__ word,
+                               const int32_t* __restrict__ start2, int32_t* __restrict__ cnt,
+                               uint64_t* __restrict__ word2, int32_t* __restrict__ totals) {
+  const int64_t c = blockIdx.x * (int64_t)kBlk + threadIdx.x;
+  if (c >= M) return;
+  if (start2[c + 1] - start2[c] > kBucketMax) totals[5] = 1;
+  const int32_t a = start[c], b = start[c + 1];
+  uint64_t prev = ~uint64_t(0);
+  for (int32_t x = a; x < b; ++x) {
+    const uint64_t w = word[x], c1 = w >> 31;
+    if (c1 != prev) {
+      const int32_t p = start2[c1] + atomicSub(&cnt[c1], 1) - 1;
+      word2[p] = ((uint64_t)c << 31) | (w & 0x7fffffffu);
+    }
+    prev = c1;
+  }
+}
+
+// One thread per corner: as k_cg_solve below, its edges from the two
+// buckets (second-corner bucket sorted here first).
+template <typename T>
+__global__ void k_cgb_solve(const int32_t* __restrict__ start, const uint64_t* __restrict__ word,
+                            const int32_t* __restrict__ start2, uint64_t* __restrict__ word2, int64_t M,
+                            const int32_t* __restrict__ vox, Fields<T> f, T* __restrict__ cg) {
+  int64_t m = blockIdx.x * (int64_t)kBlk + threadIdx.x;
+  if (m >= M) return;
+  const int32_t a2 = start2[m], b2 = start2[m + 1];
+  sort_bucket(word2, a2, b2);  // its length was checked by k_cgb_scatter2
+  T a[9] = {T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
+  for (int side = 0; side < 2; ++side) {
+    const int32_t lo = side ? a2 : start[m], hi = side ? b2 : start[m + 1];
+    const uint64_t* w = side ? word2 : word;
+    uint64_t prev = ~uint64_t(0);
+    for (int32_t p = lo; p < hi; ++p) {
+      const uint64_t o = w[p] >> 31;  // the other corner
+      if (o == prev) continue;        // the same edge again (first-corner bucket only)
+      prev = o;
+      T d[3], gp;
+      const int64_t slot = (int64_t)(w[p] & 0x7fffffffu);
+      if (side)
+        cg_edge_at((int64_t)o, m, slot, vox, f, d, gp);
+      else
+        cg_edge_at(m, (int64_t)o, slot, vox, f, d, gp);
+      cg_accum(a, d, gp, side != 0);
+    }
+  }
+  cg_store(a, m, cg);
+}
+
 // One thread per corner: the normal matrix and right-hand side summed over
 // its edges in the reference's CPU order (all edges where it is the first
 // corner, in edge order, then all where it is the second), with +slope*dir
@@ -814,26 +1020,10 @@ __global__ void k_cg_solve(const uint64_t* __restrict__ ekey, const int32_t* __r
     for (int32_t p = lo; p < hi; ++p) {
       T d[3], gp;
       cg_edge(ekey, eslot, side ? v1s[p] : p, vox, f, d, gp);
-      a[0] = a[0] + d[0] * d[0];
-      a[1] = a[1] + d[0] * d[1];
-      a[2] = a[2] + d[0] * d[2];
-      a[3] = a[3] + d[1] * d[1];
-      a[4] = a[4] + d[1] * d[2];
-      a[5] = a[5] + d[2] * d[2];
-      for (int q = 0; q < 3; ++q) a[6 + q] = a[6 + q] + (side ? -(gp * d[q]) : gp * d[q]);
+      cg_accum(a, d, gp, side != 0);
     }
   }
-  const T eps = T(1e-4);
-  const double xx = (double)(a[0] + eps), xy = (double)a[1], xz = (double)a[2];
-  const double yy = (double)(a[3] + eps), yz = (double)a[4], zz = (double)(a[5] + eps);
-  const double bx = (double)a[6], by = (double)a[7], bz = (double)a[8];
-  const double c00 = yy * zz - yz * yz, c01 = xz * yz - xy * zz, c02 = xy * yz - xz * yy;
-  const double c11 = xx * zz - xz * xz, c12 = xy * xz - xx * yz, c22 = xx * yy - xy * xy;
-  const double det = xx * c00 + xy * c01 + xz * c02;
-  T g[3] = {(T)((c00 * bx + c01 * by + c02 * bz) / det), (T)((c01 * bx + c11 * by + c12 * bz) / det),
-            (T)((c02 * bx + c12 * by + c22 * bz) / det)};
-  const bool bad = isnan(g[0]) || isnan(g[1]) || isnan(g[2]);
-  for (int d = 0; d < 3; ++d) cg[m * 3 + d] = bad ? T(0) : g[d];
+  cg_store(a, m, cg);
 }
 
 // One thread per entry: p_alpha as in training mode, and the crossing normal
@@ -1701,6 +1891,36 @@ namespace {
 template <typename T>
 int dmc_corner_grads_t(const DmcWs& w, int64_t N, int64_t M, const int32_t* vox, const int32_t* ci,
                        const Fields<T>& f, int32_t* ranges, T* cg, hipStream_t s) {
+  // Bucket path (the workspace's unique-pass buffers hold the per-corner
+  // counts and offsets when M + 1 <= 12N + 1, i.e. every corner in a cube);
+  // the sorted path below when a bucket is too long or the corners too many.
+  if (N > 0 && M + 1 <= 12 * std::max<int64_t>(N, 2) + 1) {
+    int32_t *cnt = w.uflag, *st1 = w.urank, *st2 = w.qrank;
+    uint64_t *w1 = w.skey1, *w2 = w.skey0;
+    const unsigned gN = gridn(N), gM = gridn(M);
+    hipLaunchKernelGGL(k_zero<int32_t>, dim3(gridn(M + 1)), dim3(kBlk), 0, s, cnt, M + 1);
+    A3D_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_cgb_count, dim3(gN), dim3(kBlk), 0, s, ci, N, cnt, w.totals);
+    A3D_LAUNCH_CHECK();
+    size_t tb = w.cub_bytes;
+    A3D_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(w.cub, tb, cnt, st1, (int)(M + 1), s));
+    hipLaunchKernelGGL(k_cgb_scatter, dim3(gN), dim3(kBlk), 0, s, ci, N, st1, cnt, w1);
+    A3D_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_cgb_fix, dim3(gM), dim3(kBlk), 0, s, st1, M, w1, cnt, w.totals);
+    A3D_LAUNCH_CHECK();
+    tb = w.cub_bytes;
+    A3D_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(w.cub, tb, cnt, st2, (int)(M + 1), s));
+    hipLaunchKernelGGL(k_cgb_scatter2, dim3(gM), dim3(kBlk), 0, s, st1, M, w1, st2, cnt, w2, w.totals);
+    A3D_LAUNCH_CHECK();
+    int32_t long_bucket = 0;
+    A3D_HIP_CHECK(hipMemcpyAsync(&long_bucket, w.totals + 5, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    A3D_HIP_CHECK(hipStreamSynchronize(s));
+    if (!long_bucket) {
+      hipLaunchKernelGGL(k_cgb_solve<T>, dim3(gM), dim3(kBlk), 0, s, st1, w1, st2, w2, M, vox, f, cg);
+      A3D_LAUNCH_CHECK();
+      return A3D_OK;
+    }
+  }
   int32_t *s0 = ranges, *e0 = ranges + M, *s1 = ranges + 2 * M, *e1 = ranges + 3 * M;
   hipLaunchKernelGGL(k_zero<int32_t>, dim3(gridn(4 * M)), dim3(kBlk), 0, s, ranges, 4 * M);
   A3D_LAUNCH_CHECK();

@@ -378,7 +378,8 @@ int a3d_flood_dam_mask(const uint32_t* state, int D, int H, int W, const int64_t
  *   the unique edges of all N cubes, cgrad [M,3], each corner's sums taken
  *   in the reference's CPU order (no atomics).  scratch: caller memory of
  *   4*M int32.  Uses the workspace's sort buffers but none of the topology
- *   that a3d_dmc_topology leaves there; synchronises the stream once.
+ *   that a3d_dmc_topology leaves there; synchronises the stream once (twice
+ *   for corner tables that are not a grid).
  * a3d_dmc_forward_qef: replaces the qef branch of _compute_vd and
  *   _solve_qef_scatter (sparse_diffdmc.py:617-655, 724-747, 289-389): the
  *   outputs of a3d_dmc_forward with dual vertices placed by the reference's

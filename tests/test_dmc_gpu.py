@@ -323,6 +323,56 @@ def _world(cubes, ci, M, res, deform, dt):
     return torch.tensor(pos, dtype=dt) * (2.0 / res) - 1.0 + torch.tensor(deform, dtype=dt)
 
 
+@pytest.mark.parametrize("N,M,seed", [(200, 64, 1), (2000, 16, 2)])
+def test_corner_gradients_non_grid_tables(N, M, seed):
+    """Corner gradients on corner tables that are not a grid (all cubes on one
+    voxel, corner id c always at local corner c % 8): first-corner buckets of
+    ~75 slots (sorted in place), and of ~3,000 (beyond 256: the sorted path),
+    against the oracle in float64."""
+    from oracle.dmc import estimate_corner_gradients
+
+    from atom3d_amd.mesh_extractor.sparse_diffdmc import DMCTopology
+
+    print(f"dmc corner gradients non-grid N={N} M={M}", flush=True)
+    rng = np.random.default_rng(seed)
+    res = 16
+    cubes = np.tile([[3, 5, 7]], (N, 1))
+    ci = np.arange(8)[None, :] + 8 * rng.integers(0, M // 8, (N, 8))
+    s = rng.standard_normal(M)
+    deform = rng.uniform(-0.01, 0.01, (M, 3))
+    dt = torch.float64
+    sd = torch.tensor(s, dtype=dt, device=DEV)
+    topo = DMCTopology(torch.tensor(cubes, device=DEV), sd, torch.tensor(ci, device=DEV), res, 0.0, 0.99)
+    cg = topo.corner_gradients(sd, torch.tensor(deform, dtype=dt, device=DEV)).cpu()
+    cref = estimate_corner_gradients(_world(cubes, ci, M, res, deform, dt), torch.tensor(s, dtype=dt),
+                                     torch.tensor(ci), dt)
+    assert (cg - cref).abs().max().item() <= 1e-10 * max(1.0, cref.abs().max().item())
+
+
+@pytest.mark.parametrize("tag", ["f32", "f64"])
+def test_corner_gradients_bucket_path_equals_sorted_path(tag):
+    """The bucket path (every corner in some cube) and the sorted path (taken
+    when unreferenced corners make M larger than 12 N) give the same bits on
+    the referenced corners, and zero on the others."""
+    from atom3d_amd.mesh_extractor.sparse_diffdmc import DMCTopology
+
+    print(f"dmc corner gradients bucket vs sorted {tag}", flush=True)
+    dt = torch.float64 if tag == "f64" else torch.float32
+    res = 32
+    cubes, s, ci = grid_band(lambda P: shapes_sdf(P, sdf_shapes(6, 3, seed=5)), res, 2 * 2.0 / res)
+    N, M = len(cubes), len(s)
+    deform = np.random.default_rng(9).uniform(-0.2, 0.2, (M, 3)) * (2.0 / res)
+    pad = 12 * N + 8 - M
+    outs = []
+    for extra in (0, pad):
+        sd = torch.tensor(np.concatenate([s, np.ones(extra)]), dtype=dt, device=DEV)
+        df = torch.tensor(np.concatenate([deform, np.zeros((extra, 3))]), dtype=dt, device=DEV)
+        topo = DMCTopology(torch.tensor(cubes, device=DEV), sd, torch.tensor(ci, device=DEV), res, 0.0, 0.99)
+        outs.append(topo.corner_gradients(sd, df).cpu())
+    assert torch.equal(outs[0], outs[1][:M])
+    assert bool((outs[1][M:] == 0).all())
+
+
 def test_qef_with_exact_gradients_and_training_split():
     """corner_gradients from the analytic SDF (as a BVH would give them) and
     inference_mode="qef" in training mode (4-triangle split around appended
