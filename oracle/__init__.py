@@ -14,7 +14,8 @@ import subprocess
 import numpy as np
 
 _DIR = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_DIR, "liba3d_oracle.so")
+# A3D_ORACLE_LIB: another build of the same sources (the sanitized one)
+LIB_PATH = os.environ.get("A3D_ORACLE_LIB") or os.path.join(_DIR, "liba3d_oracle.so")
 _lib = None
 
 _f = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
