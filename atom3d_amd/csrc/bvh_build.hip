@@ -17,7 +17,7 @@
 // children are read only in a later launch.  No box crosses workgroups
 // inside one launch, so no device-scope fence (an L2 write-back on a
 // multi-XCD part) is needed.
-#include <hipcub/hipcub.hpp>
+#include "a3d_prim.hpp"
 
 #include <cstddef>
 #include <stdarg.h>
@@ -100,9 +100,9 @@ inline int64_t al(int64_t x) { return (x + 255) & ~int64_t(255); }
 size_t cub_bytes_for(int64_t n) {
   size_t sort_b = 0, scan_b = 0;
   int nn = (int)(n > 1 ? n : 2);
-  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, sort_b, (uint64_t*)nullptr, (uint64_t*)nullptr,
+  (void)rocprim::radix_sort_pairs(nullptr, sort_b, (uint64_t*)nullptr, (uint64_t*)nullptr,
                                      (uint32_t*)nullptr, (uint32_t*)nullptr, nn, 0, 63);
-  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, scan_b, (int32_t*)nullptr, (int32_t*)nullptr, nn);
+  (void)a3d::exclusive_sum(nullptr, scan_b, (int32_t*)nullptr, (int32_t*)nullptr, nn);
   return sort_b > scan_b ? sort_b : scan_b;
 }
 
@@ -1059,7 +1059,7 @@ extern "C" int a3d_bvh_build_ex(const float* V, int64_t nV, const int32_t* F, in
   hipLaunchKernelGGL(k_morton, dim3(gn), dim3(kBlk), 0, s, V, F, n, w.red, w.keys_in, w.vals_in);
   A3D_LAUNCH_CHECK();
   size_t cb = w.cub_bytes;
-  A3D_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(w.cub, cb, w.keys_in, w.keys_out, w.vals_in, w.vals_out, (int)n, 0, 63, s));
+  A3D_HIP_CHECK(rocprim::radix_sort_pairs(w.cub, cb, w.keys_in, w.keys_out, w.vals_in, w.vals_out, (int)n, 0, 63, s));
   hipLaunchKernelGGL(k_karras, dim3(gm), dim3(kBlk), 0, s, w.keys_out, n, w.kl, w.kr, w.first, w.size, w.par_int, w.par_leaf, w.flag);
   A3D_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_leaf_boxes, dim3(gn), dim3(kBlk), 0, s, V, F, w.vals_out, n, w.leaf_box);
@@ -1108,7 +1108,7 @@ extern "C" int a3d_bvh_build_ex(const float* V, int64_t nV, const int32_t* F, in
     }
   }
   cb = w.cub_bytes;
-  A3D_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(w.cub, cb, w.kept, w.remap, (int)m, s));
+  A3D_HIP_CHECK(a3d::exclusive_sum(w.cub, cb, w.kept, w.remap, (int)m, s));
   hipLaunchKernelGGL(k_emit_nodes, dim3(gm), dim3(kBlk), 0, s, m, w.kept, w.remap, w.kl, w.kr, w.size, w.first, w.leaf_box, w.node_box, nodes);
   A3D_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_emit_tris, dim3(gn), dim3(kBlk), 0, s, V, F, w.vals_out, n, tris, boxes);

@@ -18,7 +18,7 @@
 // triangle distances relative to the coordinates' magnitude and (b) the
 // plane-branch error of thin triangles (folded into the leaf boxes at build
 // time), so a face the reference formula would rank first is never skipped.
-#include <hipcub/hipcub.hpp>
+#include "a3d_prim.hpp"
 
 #include "a3d_common.hpp"
 
@@ -314,14 +314,14 @@ int check_blob(const void* blob, const char* who) {
 
 size_t octree_cub_bytes(int64_t n) {
   size_t b = 0;
-  (void)hipcub::DeviceSelect::Flagged(nullptr, b, (uint64_t*)nullptr, (uint8_t*)nullptr, (uint64_t*)nullptr,
+  (void)rocprim::select(nullptr, b, (uint64_t*)nullptr, (uint8_t*)nullptr, (uint64_t*)nullptr,
                                 (int64_t*)nullptr, (int)(n > 1 ? n : 2));
   return b;
 }
 
 size_t pairs_cub_bytes(int64_t n) {
   size_t b = 0;
-  (void)hipcub::DeviceRadixSort::SortKeys(nullptr, b, (uint64_t*)nullptr, (uint64_t*)nullptr, (int)(n > 1 ? n : 2));
+  (void)rocprim::radix_sort_keys(nullptr, b, (uint64_t*)nullptr, (uint64_t*)nullptr, (int)(n > 1 ? n : 2));
   return b;
 }
 
@@ -404,7 +404,7 @@ extern "C" int a3d_pairs_sort(int32_t* pair_a, int32_t* pair_f, int64_t n, void*
   size_t tb = pairs_cub_bytes(n);
   hipLaunchKernelGGL(k_pairs_pack, dim3(grid_for(n)), dim3(kBlk), 0, s, pair_a, pair_f, n, k0);
   A3D_LAUNCH_CHECK();
-  A3D_HIP_CHECK(hipcub::DeviceRadixSort::SortKeys(tmp, tb, k0, k1, (int)n, 0, 64, s));
+  A3D_HIP_CHECK(rocprim::radix_sort_keys(tmp, tb, k0, k1, (int)n, 0, 64, s));
   hipLaunchKernelGGL(k_pairs_unpack, dim3(grid_for(n)), dim3(kBlk), 0, s, k1, n, pair_a, pair_f);
   A3D_LAUNCH_CHECK();
   return A3D_OK;
@@ -451,7 +451,7 @@ extern "C" int a3d_octree_level(const void* blob, const float* bounds_host, int 
   hipLaunchKernelGGL(k_octree_test, dim3(grid_for(n_cand)), dim3(kBlk), 0, s, blob, g, parents, n_cand, keys, flags);
   A3D_LAUNCH_CHECK();
   size_t tb = octree_cub_bytes(n_cand);
-  A3D_HIP_CHECK(hipcub::DeviceSelect::Flagged(tmp, tb, keys, flags, kept, n_kept, (int)n_cand, s));
+  A3D_HIP_CHECK(rocprim::select(tmp, tb, keys, flags, kept, n_kept, (int)n_cand, s));
   return A3D_OK;
 }
 
@@ -487,8 +487,8 @@ extern "C" int a3d_point_dist_backward(const float* P, const float* C, const flo
 namespace {
 size_t vox_cub_bytes(int64_t n) {
   size_t t = 0;
-  hipcub::CountingInputIterator<int32_t> it(0);
-  (void)hipcub::DeviceSelect::Flagged(nullptr, t, it, (uint8_t*)nullptr, (int32_t*)nullptr, (int64_t*)nullptr,
+  rocprim::counting_iterator<int32_t> it(0);
+  (void)rocprim::select(nullptr, t, it, (uint8_t*)nullptr, (int32_t*)nullptr, (int64_t*)nullptr,
                                       (int)(n > 0 ? n : 1));
   return t;
 }
@@ -537,8 +537,8 @@ extern "C" int a3d_vox_cells(const void* blob, const float* b0, const float* cel
   hipLaunchKernelGGL(k_vox_test, dim3(grid_for(n)), dim3(kBlk), 0, s, blob, g, n, flags);
   A3D_LAUNCH_CHECK();
   size_t tb = vox_cub_bytes(n);
-  hipcub::CountingInputIterator<int32_t> it(0);
-  A3D_HIP_CHECK(hipcub::DeviceSelect::Flagged(tmp, tb, it, flags, idx, cnt, (int)n, s));
+  rocprim::counting_iterator<int32_t> it(0);
+  A3D_HIP_CHECK(rocprim::select(tmp, tb, it, flags, idx, cnt, (int)n, s));
   A3D_HIP_CHECK(hipMemcpyAsync(n_hit, cnt, sizeof(int64_t), hipMemcpyDeviceToHost, s));
   A3D_HIP_CHECK(hipStreamSynchronize(s));
   return A3D_OK;

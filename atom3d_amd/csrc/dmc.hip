@@ -25,7 +25,7 @@
 // Backward (hand-written adjoint): recomputes the interpolation weights per
 // entry, reuses the dual vertices and crossings the forward stored, and sums
 // every gradient in a fixed order without atomics (see the backward section).
-#include <hipcub/hipcub.hpp>
+#include "a3d_prim.hpp"
 
 #include "a3d_common.hpp"
 
@@ -127,22 +127,22 @@ inline int key_bits(int64_t n) {
 size_t cub_need(int64_t N) {
   int n = (int)(N > 1 ? N : 2), n12 = (int)(12 * (N > 1 ? N : 2)), n4 = (int)(4 * (N > 1 ? N : 2));
   size_t b = 0, t = 0;
-  (void)hipcub::DeviceSelect::Flagged(nullptr, t, (int32_t*)nullptr, (uint8_t*)nullptr, (int32_t*)nullptr, (int64_t*)nullptr, n);
+  (void)rocprim::select(nullptr, t, (int32_t*)nullptr, (uint8_t*)nullptr, (int32_t*)nullptr, (int64_t*)nullptr, n);
   b = t > b ? t : b;
-  (void)hipcub::DeviceSelect::Flagged(nullptr, t, (uint64_t*)nullptr, (uint8_t*)nullptr, (uint64_t*)nullptr, (int64_t*)nullptr, n);
+  (void)rocprim::select(nullptr, t, (uint64_t*)nullptr, (uint8_t*)nullptr, (uint64_t*)nullptr, (int64_t*)nullptr, n);
   b = t > b ? t : b;
-  (void)hipcub::DeviceRadixSort::SortKeys(nullptr, t, (uint64_t*)nullptr, (uint64_t*)nullptr, n, 0, 64);
+  (void)rocprim::radix_sort_keys(nullptr, t, (uint64_t*)nullptr, (uint64_t*)nullptr, n, 0, 64);
   b = t > b ? t : b;
-  (void)hipcub::DeviceScan::ExclusiveScan(nullptr, t, (I4*)nullptr, (I4*)nullptr, I4Sum(), I4{{0, 0, 0, 0}}, n);
+  (void)rocprim::exclusive_scan(nullptr, t, (I4*)nullptr, (I4*)nullptr, I4{{0, 0, 0, 0}}, (size_t)n, I4Sum());
   b = t > b ? t : b;
-  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, t, (int32_t*)nullptr, (int32_t*)nullptr, n12 + 1);
+  (void)a3d::exclusive_sum(nullptr, t, (int32_t*)nullptr, (int32_t*)nullptr, n12 + 1);
   b = t > b ? t : b;
-  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, t, (int32_t*)nullptr, (int32_t*)nullptr, n4 + 1);
+  (void)a3d::exclusive_sum(nullptr, t, (int32_t*)nullptr, (int32_t*)nullptr, n4 + 1);
   b = t > b ? t : b;
-  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, t, (uint64_t*)nullptr, (uint64_t*)nullptr, (int32_t*)nullptr,
+  (void)rocprim::radix_sort_pairs(nullptr, t, (uint64_t*)nullptr, (uint64_t*)nullptr, (int32_t*)nullptr,
                                            (int32_t*)nullptr, n12, 0, 64);
   b = t > b ? t : b;
-  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, t, (uint32_t*)nullptr, (uint32_t*)nullptr, (int32_t*)nullptr,
+  (void)rocprim::radix_sort_pairs(nullptr, t, (uint32_t*)nullptr, (uint32_t*)nullptr, (int32_t*)nullptr,
                                            (int32_t*)nullptr, n12, 0, 32);
   b = t > b ? t : b;
   return b;
@@ -1608,12 +1608,12 @@ extern "C" int a3d_dmc_topology(const int32_t* vox, const void* sdf, int dtype, 
     hipLaunchKernelGGL(k_iota, dim3(gN), dim3(kBlk), 0, s, w.sval0, N);                                           \
     A3D_LAUNCH_CHECK();                                                                                           \
     tbytes = w.cub_bytes;                                                                                         \
-    A3D_HIP_CHECK(hipcub::DeviceSelect::Flagged(w.cub, tbytes, w.sval0, w.surf, w.surf_cube, w.counters, (int)N, s)); \
+    A3D_HIP_CHECK(rocprim::select(w.cub, tbytes, w.sval0, w.surf, w.surf_cube, w.counters, (int)N, s)); \
     hipLaunchKernelGGL(k_prob, dim3(gN), dim3(kBlk), 0, s, tb, w.surf_cube, w.counters, w.case0, vox, R, N, w.prob, \
                        w.pkey0, w.case_s);                                                                        \
     A3D_LAUNCH_CHECK();                                                                                           \
     tbytes = w.cub_bytes;                                                                                         \
-    A3D_HIP_CHECK(hipcub::DeviceSelect::Flagged(w.cub, tbytes, w.pkey0, w.prob, w.pkey1, w.counters + 1, (int)N, s)); \
+    A3D_HIP_CHECK(rocprim::select(w.cub, tbytes, w.pkey0, w.prob, w.pkey1, w.counters + 1, (int)N, s)); \
   } while (0)
   if (dtype == 0) A3D_DMC_T(float); else A3D_DMC_T(double);
 #undef A3D_DMC_T
@@ -1626,7 +1626,7 @@ extern "C" int a3d_dmc_topology(const int32_t* vox, const void* sdf, int dtype, 
   unsigned gS = gridn(Ns + 1);
   if (Np > 0) {
     tbytes = w.cub_bytes;
-    A3D_HIP_CHECK(hipcub::DeviceRadixSort::SortKeys(w.cub, tbytes, w.pkey1, w.pkey0, (int)Np, 0, vb, s));
+    A3D_HIP_CHECK(rocprim::radix_sort_keys(w.cub, tbytes, w.pkey1, w.pkey0, (int)Np, 0, vb, s));
     hipLaunchKernelGGL(k_check, dim3(gS), dim3(kBlk), 0, s, tb, w.surf_cube, w.counters, w.case0, vox, R, w.pkey0,
                        w.case_s);
     A3D_LAUNCH_CHECK();
@@ -1634,7 +1634,7 @@ extern "C" int a3d_dmc_topology(const int32_t* vox, const void* sdf, int dtype, 
   hipLaunchKernelGGL(k_nv, dim3(gS), dim3(kBlk), 0, s, tb, w.counters, w.case_s, w.nv_in);
   A3D_LAUNCH_CHECK();
   tbytes = w.cub_bytes;
-  A3D_HIP_CHECK(hipcub::DeviceScan::ExclusiveScan(w.cub, tbytes, w.nv_in, w.nv_ex, I4Sum(), I4{{0, 0, 0, 0}}, (int)Ns, s));
+  A3D_HIP_CHECK(rocprim::exclusive_scan(w.cub, tbytes, w.nv_in, w.nv_ex, I4{{0, 0, 0, 0}}, (size_t)Ns, I4Sum(), s));
   hipLaunchKernelGGL(k_vd_base, dim3(gS), dim3(kBlk), 0, s, tb, w.counters, w.case_s, w.nv_in, w.nv_ex, w.vd_base,
                      w.vsize, w.totals);
   A3D_LAUNCH_CHECK();
@@ -1645,7 +1645,7 @@ extern "C" int a3d_dmc_topology(const int32_t* vox, const void* sdf, int dtype, 
   hipLaunchKernelGGL(k_zero<int32_t>, dim3(1), dim3(kBlk), 0, s, w.vsize + Vd, (int64_t)1);
   A3D_LAUNCH_CHECK();
   tbytes = w.cub_bytes;
-  A3D_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(w.cub, tbytes, w.vsize, w.vd_off, Vd + 1, s));
+  A3D_HIP_CHECK(a3d::exclusive_sum(w.cub, tbytes, w.vsize, w.vd_off, Vd + 1, s));
   // totals[1] = entries
   A3D_HIP_CHECK(hipMemcpyAsync(w.totals + 1, w.vd_off + Vd, 4, hipMemcpyDeviceToDevice, s));
   hipLaunchKernelGGL(k_entries, dim3(gS), dim3(kBlk), 0, s, tb, w.counters, w.case_s, w.vd_base, w.vd_off, w.ent_slot,
@@ -1664,7 +1664,7 @@ extern "C" int a3d_dmc_topology(const int32_t* vox, const void* sdf, int dtype, 
                        w.totals);
     A3D_LAUNCH_CHECK();
     tbytes = w.cub_bytes;
-    A3D_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(w.cub, tbytes, w.uflag, w.urank, (int)(M + 1), s));
+    A3D_HIP_CHECK(a3d::exclusive_sum(w.cub, tbytes, w.uflag, w.urank, (int)(M + 1), s));
     hipLaunchKernelGGL(k_edge_scatter, dim3(gS), dim3(kBlk), 0, s, w.case_s, ci, w.surf_cube, w.counters, w.urank,
                        w.uflag, w.skey1);
     A3D_LAUNCH_CHECK();
@@ -1678,7 +1678,7 @@ extern "C" int a3d_dmc_topology(const int32_t* vox, const void* sdf, int dtype, 
     hipLaunchKernelGGL(k_hot_count<T>, dim3(gS), dim3(kBlk), 0, s, S, iso, ci, w.surf_cube, w.counters, w.hot_cnt); \
     A3D_LAUNCH_CHECK();                                                                                             \
     tbytes = w.cub_bytes;                                                                                           \
-    A3D_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(w.cub, tbytes, w.hot_cnt, w.hot_off, (int)(Ns + 1), s));         \
+    A3D_HIP_CHECK(a3d::exclusive_sum(w.cub, tbytes, w.hot_cnt, w.hot_off, (int)(Ns + 1), s));         \
     hipLaunchKernelGGL(k_hot_write<T>, dim3(gS), dim3(kBlk), 0, s, S, iso, ci, w.surf_cube, w.counters, w.hot_off,  \
                        w.skey0, w.sval0, w.totals, kb);                                                             \
     A3D_LAUNCH_CHECK();                                                                                             \
@@ -1692,7 +1692,7 @@ extern "C" int a3d_dmc_topology(const int32_t* vox, const void* sdf, int dtype, 
     A3D_HIP_CHECK(hipStreamSynchronize(s));
     tbytes = w.cub_bytes;
     A3D_HIP_CHECK(
-        hipcub::DeviceRadixSort::SortPairs(w.cub, tbytes, w.skey0, w.skey1, w.sval0, w.sval1, hot[0], 0, 2 * kb, s));
+        rocprim::radix_sort_pairs(w.cub, tbytes, w.skey0, w.skey1, w.sval0, w.sval1, hot[0], 0, 2 * kb, s));
   }
 #undef A3D_DMC_T3
   const int32_t S_slots = hot[0];
@@ -1700,12 +1700,12 @@ extern "C" int a3d_dmc_topology(const int32_t* vox, const void* sdf, int dtype, 
   hipLaunchKernelGGL(k_unique_flags, dim3(gH), dim3(kBlk), 0, s, w.skey1, w.totals, w.uflag);
   A3D_LAUNCH_CHECK();
   tbytes = w.cub_bytes;
-  A3D_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(w.cub, tbytes, w.uflag, w.urank, S_slots + 1, s));
+  A3D_HIP_CHECK(a3d::exclusive_sum(w.cub, tbytes, w.uflag, w.urank, S_slots + 1, s));
   hipLaunchKernelGGL(k_unique_map, dim3(gH), dim3(kBlk), 0, s, w.skey1, w.sval1, w.urank, w.totals, w.sedge_of_slot,
                      w.sedge, w.qflag, w.quad_of_edge, w.ustart, w.ent_of_slot, w.epos, kb);
   A3D_LAUNCH_CHECK();
   tbytes = w.cub_bytes;
-  A3D_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(w.cub, tbytes, w.qflag, w.qrank, S_slots + 1, s));
+  A3D_HIP_CHECK(a3d::exclusive_sum(w.cub, tbytes, w.qflag, w.qrank, S_slots + 1, s));
   A3D_HIP_CHECK(hipMemcpyAsync(w.totals + 4, w.qrank + S_slots, 4, hipMemcpyDeviceToDevice, s));
 #define A3D_DMC_T3(T)                                                                                                  \
   hipLaunchKernelGGL(k_quads<T>, dim3(gH), dim3(kBlk), 0, s, tb, (const T*)sdf, (T)isovalue, w.sval1, w.qflag, w.qrank, \
@@ -1791,7 +1791,7 @@ BwdScratch carve_bwd(void* base, const int64_t* sizes, int tsize) {
   const int64_t E = sizes[2];
   const int64_t P = std::max<int64_t>(sizes[3], 1);
   size_t t = 0, tb = 0;
-  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, t, (uint32_t*)nullptr, (uint32_t*)nullptr, (int32_t*)nullptr,
+  (void)rocprim::radix_sort_pairs(nullptr, t, (uint32_t*)nullptr, (uint32_t*)nullptr, (int32_t*)nullptr,
                                            (int32_t*)nullptr, (int)P, 0, 32);
   tb = t;
   BwdScratch b{};
@@ -1850,7 +1850,7 @@ int dmc_backward_t(const DmcWs& w, const BwdScratch& b, const int64_t* sizes, in
                        (const T*)b.rec, (T*)b.e1, b.k0, b.v0, (T*)g_sdf, (T*)g_deform);
     A3D_LAUNCH_CHECK();
     size_t tb = b.tmp_bytes;
-    A3D_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(b.tmp, tb, b.k0, b.k1, b.v0, b.v1, (int)U, 0, key_bits(M), s));
+    A3D_HIP_CHECK(rocprim::radix_sort_pairs(b.tmp, tb, b.k0, b.k1, b.v0, b.v1, (int)U, 0, key_bits(M), s));
     hipLaunchKernelGGL(k_corner_c1<T>, dim3(gridn(U)), dim3(kBlk), 0, s, b.k1, b.v1, w.totals, (const T*)b.e1,
                        (T*)g_sdf, (T*)g_deform);
     A3D_LAUNCH_CHECK();
@@ -1903,13 +1903,13 @@ int dmc_corner_grads_t(const DmcWs& w, int64_t N, int64_t M, const int32_t* vox,
     hipLaunchKernelGGL(k_cgb_count, dim3(gN), dim3(kBlk), 0, s, ci, N, cnt, w.totals);
     A3D_LAUNCH_CHECK();
     size_t tb = w.cub_bytes;
-    A3D_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(w.cub, tb, cnt, st1, (int)(M + 1), s));
+    A3D_HIP_CHECK(a3d::exclusive_sum(w.cub, tb, cnt, st1, (int)(M + 1), s));
     hipLaunchKernelGGL(k_cgb_scatter, dim3(gN), dim3(kBlk), 0, s, ci, N, st1, cnt, w1);
     A3D_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_cgb_fix, dim3(gM), dim3(kBlk), 0, s, st1, M, w1, cnt, w.totals);
     A3D_LAUNCH_CHECK();
     tb = w.cub_bytes;
-    A3D_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(w.cub, tb, cnt, st2, (int)(M + 1), s));
+    A3D_HIP_CHECK(a3d::exclusive_sum(w.cub, tb, cnt, st2, (int)(M + 1), s));
     hipLaunchKernelGGL(k_cgb_scatter2, dim3(gM), dim3(kBlk), 0, s, st1, M, w1, st2, cnt, w2, w.totals);
     A3D_LAUNCH_CHECK();
     int32_t long_bucket = 0;
@@ -1933,14 +1933,14 @@ int dmc_corner_grads_t(const DmcWs& w, int64_t N, int64_t M, const int32_t* vox,
     A3D_LAUNCH_CHECK();
     const int end_bit = 2 * kb;
     size_t tbytes = w.cub_bytes;
-    A3D_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(w.cub, tbytes, w.skey0, w.skey1, w.sval0, w.sval1, (int)n12, 0,
+    A3D_HIP_CHECK(rocprim::radix_sort_pairs(w.cub, tbytes, w.skey0, w.skey1, w.sval0, w.sval1, (int)n12, 0,
                                                      end_bit, s));
     hipLaunchKernelGGL(k_cg_heads, dim3(gridn(n12 + 1)), dim3(kBlk), 0, s, w.skey1, n12, w.uflag);
     A3D_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_zero<int32_t>, dim3(1), dim3(kBlk), 0, s, w.uflag + n12, (int64_t)1);
     A3D_LAUNCH_CHECK();
     tbytes = w.cub_bytes;
-    A3D_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(w.cub, tbytes, w.uflag, w.urank, (int)(n12 + 1), s));
+    A3D_HIP_CHECK(a3d::exclusive_sum(w.cub, tbytes, w.uflag, w.urank, (int)(n12 + 1), s));
     uint32_t* k1 = (uint32_t*)w.qflag;
     uint32_t* k1s = (uint32_t*)w.skey1;
     hipLaunchKernelGGL(k_cg_compact, dim3(gridn(n12)), dim3(kBlk), 0, s, w.skey1, w.sval1, w.uflag, w.urank, n12,
@@ -1952,7 +1952,7 @@ int dmc_corner_grads_t(const DmcWs& w, int64_t N, int64_t M, const int32_t* vox,
     int eb = 1;
     while (eb < 32 && ((uint64_t)(M - 1) >> eb) != 0) ++eb;
     tbytes = w.cub_bytes;
-    A3D_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(w.cub, tbytes, k1, k1s, w.qrank, w.sval1, (int)U, 0, eb, s));
+    A3D_HIP_CHECK(rocprim::radix_sort_pairs(w.cub, tbytes, k1, k1s, w.qrank, w.sval1, (int)U, 0, eb, s));
     hipLaunchKernelGGL(k_cg_ranges, dim3(gridn(U)), dim3(kBlk), 0, s, (const uint64_t*)w.skey0, (const uint32_t*)k1s,
                        w.urank + n12, s0, e0, s1, e1);
     A3D_LAUNCH_CHECK();

@@ -16,7 +16,7 @@
 // in range) with head flags and a scan, the cube -> vertex ranks come from the
 // same scan, neighbours from binary search (method 1).  The collision set of
 // the wet masks is a bitmap of resolution^3 bits.
-#include <hipcub/hipcub.hpp>
+#include "a3d_prim.hpp"
 
 #include "a3d_common.hpp"
 
@@ -57,10 +57,10 @@ struct CellWs {
 template <typename K>
 size_t cell_cub_need(int64_t n8) {
   size_t b = 0, t = 0;
-  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, t, (K*)nullptr, (K*)nullptr, (int32_t*)nullptr,
+  (void)rocprim::radix_sort_pairs(nullptr, t, (K*)nullptr, (K*)nullptr, (int32_t*)nullptr,
                                            (int32_t*)nullptr, (int)n8, 0, (int)(8 * sizeof(K)));
   b = t > b ? t : b;
-  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, t, (int32_t*)nullptr, (int32_t*)nullptr, (int)(n8 + 1));
+  (void)a3d::exclusive_sum(nullptr, t, (int32_t*)nullptr, (int32_t*)nullptr, (int)(n8 + 1));
   b = t > b ? t : b;
   return b;
 }
@@ -403,11 +403,11 @@ int unique_keys(const CellWs& w, const int32_t* ijk, int64_t n, int corners, int
   A3D_LAUNCH_CHECK();
   const int eb = bits_for((uint64_t)(P * P * P - 1));
   size_t tb = w.cub_bytes;
-  A3D_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(w.cub, tb, kin, kout, w.vin, w.vout, (int)m, 0, eb, s));
+  A3D_HIP_CHECK(rocprim::radix_sort_pairs(w.cub, tb, kin, kout, w.vin, w.vout, (int)m, 0, eb, s));
   hipLaunchKernelGGL(k_heads<K>, dim3(gridn(m + 1)), dim3(kBlk), 0, s, kout, m, w.flag);
   A3D_LAUNCH_CHECK();
   tb = w.cub_bytes;
-  A3D_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(w.cub, tb, w.flag, w.rank, (int)(m + 1), s));
+  A3D_HIP_CHECK(a3d::exclusive_sum(w.cub, tb, w.flag, w.rank, (int)(m + 1), s));
   // unique keys go to kin (free after the sort)
   hipLaunchKernelGGL(k_rank_scatter<K>, dim3(gridn(m)), dim3(kBlk), 0, s, kout, w.vout, w.flag, w.rank, m, kin,
                      slot_rank, w.count);
@@ -460,8 +460,8 @@ BmWs carve_bm(void* base, const BmGeom& g) {
   w.vrank = (int32_t*)take(4 * (vw + 1));
   w.count = (int64_t*)take(16);
   size_t t = 0;
-  hipcub::TransformInputIterator<int32_t, Popc, const uint32_t*> it(nullptr, Popc());
-  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, t, it, (int32_t*)nullptr, (int)((cw > vw ? cw : vw) + 1));
+  rocprim::transform_iterator<const uint32_t*, Popc, int32_t> it(nullptr, Popc());
+  (void)a3d::exclusive_sum(nullptr, t, it, (int32_t*)nullptr, (int)((cw > vw ? cw : vw) + 1));
   w.cub_bytes = t;
   w.cub = take((int64_t)t);
   w.total = o;
@@ -471,9 +471,9 @@ BmWs carve_bm(void* base, const BmGeom& g) {
 // Ranks of `words` bitmap words (the scan runs over words + 1 items; the
 // extra item reads the zero word past the end, so rank[words] = total).
 int bm_scan(const BmWs& w, const uint32_t* bits, int32_t* rank, int64_t words, hipStream_t s) {
-  hipcub::TransformInputIterator<int32_t, Popc, const uint32_t*> it(bits, Popc());
+  rocprim::transform_iterator<const uint32_t*, Popc, int32_t> it(bits, Popc());
   size_t tb = w.cub_bytes;
-  A3D_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(w.cub, tb, it, rank, (int)(words + 1), s));
+  A3D_HIP_CHECK(a3d::exclusive_sum(w.cub, tb, it, rank, (int)(words + 1), s));
   return A3D_OK;
 }
 

@@ -23,7 +23,7 @@
 // Classification, the k-ring neighbourhood (26-dilations of the dam plane)
 // and the lexicographic coordinate extraction (popcount, prefix sum, write)
 // run on the same planes.
-#include <hipcub/hipcub.hpp>
+#include "a3d_prim.hpp"
 
 #include "a3d_common.hpp"
 
@@ -391,7 +391,7 @@ FWs carve_f(void* base, const Geo& g) {
   w.cnt = (uint32_t*)take(4 * nr);
   w.off = (uint32_t*)take(4 * nr);
   size_t tb = 0;
-  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, tb, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)nr);
+  (void)a3d::exclusive_sum(nullptr, tb, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)nr);
   w.cub_bytes = tb;
   w.cub = take((int64_t)tb);
   w.total = o;
@@ -587,7 +587,7 @@ extern "C" int a3d_flood_extract(const uint32_t* state, int D, int H, int W, int
   hipLaunchKernelGGL(k_ff_count, dim3(gridn(rows)), dim3(kBlk), 0, s, g, pl, w.cnt);
   A3D_LAUNCH_CHECK();
   size_t tb = w.cub_bytes;
-  A3D_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(w.cub, tb, w.cnt, w.off, (int)rows, s));
+  A3D_HIP_CHECK(a3d::exclusive_sum(w.cub, tb, w.cnt, w.off, (int)rows, s));
   hipLaunchKernelGGL(k_ff_write, dim3(gridn(rows)), dim3(kBlk), 0, s, g, pl, w.off, lo_host[0], lo_host[1], lo_host[2],
                      out_ijk);
   A3D_LAUNCH_CHECK();

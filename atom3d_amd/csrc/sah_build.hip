@@ -39,7 +39,7 @@
 // Tri48 triangles in the reference's triangle order), without the UDF
 // walk's padded and oriented boxes (boxes_off = obbs_off = tobbs_off = 0);
 // header.pad[0] = kBlobSah.
-#include <hipcub/hipcub.hpp>
+#include "a3d_prim.hpp"
 
 #include <cstddef>
 
@@ -1019,7 +1019,7 @@ struct SahWs {
 
 size_t sah_cub_bytes(int64_t n) {
   size_t b = 0;
-  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, b, (int32_t*)nullptr, (int32_t*)nullptr, (int)(n > 1 ? n : 2));
+  (void)a3d::exclusive_sum(nullptr, b, (int32_t*)nullptr, (int32_t*)nullptr, (int)(n > 1 ? n : 2));
   return b;
 }
 
@@ -1151,7 +1151,7 @@ extern "C" int a3d_sah_build(const float* V, int64_t nV, const int32_t* F, int64
                            w.axis, w.split, w.mode, leaf_max, w.chunk_true);
         A3D_LAUNCH_CHECK();
         size_t cb = w.cub_bytes;
-        A3D_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(w.cub, cb, w.chunk_true, w.chunk_pre, (int)(nc + 1), s));
+        A3D_HIP_CHECK(a3d::exclusive_sum(w.cub, cb, w.chunk_true, w.chunk_pre, (int)(nc + 1), s));
         hipLaunchKernelGGL(k_sah_rank, dim3((unsigned)nc), dim3(kBlk), 0, s, chunks, lv, w.perm, w.tri, w.cz, w.axis,
                            w.split, w.mode, w.chunk_pre, leaf_max, w.pos_lf, w.pos_rt, w.swaps);
         A3D_LAUNCH_CHECK();
@@ -1166,11 +1166,11 @@ extern "C" int a3d_sah_build(const float* V, int64_t nV, const int32_t* F, int64
         A3D_LAUNCH_CHECK();
         const int64_t nch = 2 * nb;
         cb = w.cub_bytes;
-        A3D_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(w.cub, cb, w.f_big, w.o_big, (int)nch, s));
+        A3D_HIP_CHECK(a3d::exclusive_sum(w.cub, cb, w.f_big, w.o_big, (int)nch, s));
         cb = w.cub_bytes;
-        A3D_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(w.cub, cb, w.f_small, w.o_small, (int)nch, s));
+        A3D_HIP_CHECK(a3d::exclusive_sum(w.cub, cb, w.f_small, w.o_small, (int)nch, s));
         cb = w.cub_bytes;
-        A3D_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(w.cub, cb, w.f_chunk, w.o_chunk, (int)nch, s));
+        A3D_HIP_CHECK(a3d::exclusive_sum(w.cub, cb, w.f_chunk, w.o_chunk, (int)nch, s));
         if (ns + nch > w.ns_max) return fail(A3D_ERR_INTERNAL, "a3d_sah_build: small-subtree list overflow");
         hipLaunchKernelGGL(k_sah_emit, dim3(gr(nch)), dim3(kBlk), 0, s, nch, w.child, w.f_big, w.f_small, w.f_chunk,
                            w.o_big, w.o_small, w.o_chunk, (int32_t)(rec_base + nb), (int32_t)ns, w.nb_max, w.nc_max,
@@ -1196,7 +1196,7 @@ extern "C" int a3d_sah_build(const float* V, int64_t nV, const int32_t* F, int64
       hipLaunchKernelGGL(k_sah_small_counts, dim3(gr(ns)), dim3(kBlk), 0, s, ns, w.small, w.small_cnt);
       A3D_LAUNCH_CHECK();
       size_t cb = w.cub_bytes;
-      A3D_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(w.cub, cb, w.small_cnt, w.small_off, (int)ns, s));
+      A3D_HIP_CHECK(a3d::exclusive_sum(w.cub, cb, w.small_cnt, w.small_off, (int)ns, s));
       hipLaunchKernelGGL(k_sah_small, dim3(gr(ns, 64)), dim3(64), 0, s, ns, w.small, w.small_off, rec_base, leaf_max,
                          w.perm, w.tri, w.cz, nodes, w.ctr);
       A3D_LAUNCH_CHECK();

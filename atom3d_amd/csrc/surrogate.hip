@@ -18,7 +18,7 @@
 // 4s + 2g + 1 (s = 0..3), which is the layer-1 B operand, and receives in the
 // backward pass (dX^T = W1^T dH1) the gradient of exactly those features.
 #include <hip/hip_fp16.h>
-#include <hipcub/hipcub.hpp>
+#include "a3d_prim.hpp"
 
 #include <cmath>
 #include <cstring>
@@ -794,7 +794,7 @@ extern "C" int a3d_sg_infer(const void* params, const float* P, int64_t K, const
 extern "C" int a3d_sg_route_ws_bytes(int64_t K, int64_t* bytes) {
   if (K < 0 || !bytes) return fail(A3D_ERR_ARG, "a3d_sg_route_ws_bytes: bad argument");
   size_t t = 0;
-  (void)hipcub::DeviceSelect::Flagged(nullptr, t, (uint32_t*)nullptr, (uint8_t*)nullptr, (uint32_t*)nullptr,
+  (void)rocprim::select(nullptr, t, (uint32_t*)nullptr, (uint8_t*)nullptr, (uint32_t*)nullptr,
                                       (int64_t*)nullptr, (int)(K > 1 ? K : 2));
   *bytes = al256(K > 0 ? K : 1) + al256(4 * (K > 0 ? K : 1)) + al256(4 * (K > 0 ? K : 1)) + 256 + al256((int64_t)t);
   return A3D_OK;
@@ -843,7 +843,7 @@ extern "C" int a3d_sg_udf(const void* params, const void* blob, const float* P, 
   hipLaunchKernelGGL(k_sg_infer, dim3((unsigned)blocks), dim3(kBlkS), 0, s, params, P, order, K, dist, band, near,
                      face, closest, uvw);
   A3D_LAUNCH_CHECK();
-  A3D_HIP_CHECK(hipcub::DeviceSelect::Flagged(w, tb, order, near, list, cnt, (int)K, s));
+  A3D_HIP_CHECK(rocprim::select(w, tb, order, near, list, cnt, (int)K, s));
   int64_t n_near = 0;
   A3D_HIP_CHECK(hipMemcpyAsync(&n_near, cnt, 8, hipMemcpyDeviceToHost, s));
   A3D_HIP_CHECK(hipStreamSynchronize(s));

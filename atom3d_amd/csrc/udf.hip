@@ -26,7 +26,7 @@
 // time), so no face the formula ranks first can be skipped; the minimum over
 // the tested faces is therefore the minimum over all faces, whatever order or
 // extra nodes a wave visits.
-#include <hipcub/hipcub.hpp>
+#include "a3d_prim.hpp"
 
 #include <mutex>
 
@@ -879,7 +879,7 @@ QWs carve_q(void* base, int64_t K) {
   w.k1 = (uint32_t*)take(4 * n);
   w.v0 = (uint32_t*)take(4 * n);
   size_t tb = 0;
-  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, tb, (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
+  (void)rocprim::radix_sort_pairs(nullptr, tb, (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
                                            (uint32_t*)nullptr, (int)n, 0, 30);
   w.cub_bytes = tb;
   w.cub = take((int64_t)tb);
@@ -919,7 +919,7 @@ extern "C" int a3d_query_order(const float* P, int64_t K, uint32_t* perm, void* 
   hipLaunchKernelGGL(k_qmorton, dim3(g), dim3(kBlk), 0, s, P, K, w.red, w.k0, w.v0);
   A3D_LAUNCH_CHECK();
   size_t tb = w.cub_bytes;
-  A3D_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(w.cub, tb, w.k0, w.k1, w.v0, perm, (int)K, 0, 30, s));
+  A3D_HIP_CHECK(rocprim::radix_sort_pairs(w.cub, tb, w.k0, w.k1, w.v0, perm, (int)K, 0, 30, s));
   return A3D_OK;
 }
 
@@ -1225,7 +1225,7 @@ __global__ void k_udf_batch(const void* const* __restrict__ blobs, const float* 
 
 size_t sort_bytes(int64_t K) {
   size_t t = 0;
-  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, t, (uint64_t*)nullptr, (uint64_t*)nullptr, (uint32_t*)nullptr,
+  (void)rocprim::radix_sort_pairs(nullptr, t, (uint64_t*)nullptr, (uint64_t*)nullptr, (uint32_t*)nullptr,
                                            (uint32_t*)nullptr, (int)(K > 1 ? K : 2), 0, 40);
   return t;
 }
@@ -1263,7 +1263,7 @@ extern "C" int a3d_batch_order(const void* const* blobs, int n_meshes, const flo
   A3D_LAUNCH_CHECK();
   int bits = 30;
   while ((1ll << (bits - 30)) < n_meshes) ++bits;
-  A3D_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, k0, k1, v0, perm, (int)K, 0, bits, s));
+  A3D_HIP_CHECK(rocprim::radix_sort_pairs(tmp, tb, k0, k1, v0, perm, (int)K, 0, bits, s));
   return A3D_OK;
 }
 
