@@ -434,7 +434,7 @@ __global__ __launch_bounds__(kBlkW) void k_udf_wave(const void* __restrict__ blo
                                                     float* __restrict__ dist, int32_t* __restrict__ face,
                                                     float* __restrict__ closest, float* __restrict__ uvw,
                                                     unsigned long long* __restrict__ counters, bool use_obbs,
-                                                    GridBound gb) {
+                                                    GridBound gb, uint32_t* __restrict__ work) {
   __shared__ float4 s_stk[kWaveStack][2];  // (lo.xyz, ref bits), (hi.xyz, obb code)
   __shared__ WaveQ s_q[1];
   const int lane = threadIdx.x;
@@ -462,9 +462,10 @@ __global__ __launch_bounds__(kBlkW) void k_udf_wave(const void* __restrict__ blo
                           best, best_id,
                           best_tri, n_node, n_leaf_tri, n_test, n_lane);
   if (STATS) {
+    if (work && valid) work[q] = n_lane;
     if (!valid) n_lane = 0;
     for (int o = 32; o > 0; o >>= 1) n_lane += __shfl_xor(n_lane, o);
-    if (lane == 0) {
+    if (counters && lane == 0) {
       atomicAdd(&counters[0], 1ull);
       atomicAdd(&counters[1], (unsigned long long)n_node);
       atomicAdd(&counters[2], (unsigned long long)n_leaf_tri);
@@ -856,11 +857,58 @@ __global__ __launch_bounds__(kBlk) void k_qmorton(const float* __restrict__ P, i
   vals[i] = (uint32_t)i;
 }
 
+// Cost class of each full 64-query packet of the Morton order pm (one
+// wavefront per packet): the largest work of the grid points around the
+// packet's queries, each query reading the 8 corners of its grid cell (the
+// cell grid_bound reads, clamped the same way).  key = 32 - floor(log2(max +
+// 1)), so a stable ascending sort puts the heaviest class first and keeps the
+// Morton order inside each class.
+__global__ __launch_bounds__(kBlk) void k_qpacket(const float* __restrict__ P, const uint32_t* __restrict__ pm,
+                                                  int64_t nfull, const uint32_t* __restrict__ work, int R, float lo0,
+                                                  float lo1, float lo2, float h, uint32_t* __restrict__ pkey,
+                                                  uint32_t* __restrict__ pval) {
+  const int64_t i = blockIdx.x * (int64_t)kBlk + threadIdx.x;
+  if (i >= nfull) return;  // nfull is a multiple of 64: whole wavefronts leave together
+  const int64_t q = pm[i];
+  const float lo[3] = {lo0, lo1, lo2};
+  int c0[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const float f = floorf((P[q * 3 + d] - lo[d]) / h);
+    c0[d] = f >= 0.0f ? (f < (float)(R - 1) ? (int)f : R - 1) : 0;  // NaN -> 0
+  }
+  uint32_t w = 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int64_t gi = ((int64_t)(c0[0] + (k & 1)) * (R + 1) + c0[1] + ((k >> 1) & 1)) * (R + 1) + c0[2] + (k >> 2);
+    w = max(w, work[gi]);
+  }
+  for (int o = 32; o > 0; o >>= 1) w = max(w, (uint32_t)__shfl_xor((int)w, o));
+  if ((threadIdx.x & 63) == 0) {
+    const int64_t j = i >> 6;
+    pkey[j] = (uint32_t)(32 - (63 - __builtin_clzll((unsigned long long)w + 1ull)));
+    pval[j] = (uint32_t)j;
+  }
+}
+
+// perm = the packets of pm in the order porder, the trailing partial packet last.
+__global__ __launch_bounds__(kBlk) void k_qgather(const uint32_t* __restrict__ pm, const uint32_t* __restrict__ porder,
+                                                  int64_t K, int64_t nfull, uint32_t* __restrict__ perm) {
+  const int64_t i = blockIdx.x * (int64_t)kBlk + threadIdx.x;
+  if (i >= K) return;
+  perm[i] = i < nfull ? pm[(int64_t)porder[i >> 6] * 64 + (i & 63)] : pm[i];
+}
+
 struct QWs {
   uint32_t* red;
   uint32_t* k0;
   uint32_t* k1;
   uint32_t* v0;
+  uint32_t* pm;  // packet order only: the Morton order, then per packet key/value in and out
+  uint32_t* pk0;
+  uint32_t* pk1;
+  uint32_t* pv0;
+  uint32_t* pv1;
   void* cub;
   size_t cub_bytes;
   int64_t total;
@@ -868,8 +916,8 @@ struct QWs {
 
 inline int64_t al256(int64_t x) { return (x + 255) & ~int64_t(255); }
 
-QWs carve_q(void* base, int64_t K) {
-  QWs w;
+QWs carve_q(void* base, int64_t K, bool packets = false) {
+  QWs w{};
   int64_t n = K > 1 ? K : 2;
   char* b = (char*)base;
   int64_t off = 0;
@@ -881,10 +929,36 @@ QWs carve_q(void* base, int64_t K) {
   size_t tb = 0;
   (void)rocprim::radix_sort_pairs(nullptr, tb, (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
                                            (uint32_t*)nullptr, (int)n, 0, 30);
+  if (packets) {
+    const int64_t np = n / 64 > 1 ? n / 64 : 2;
+    w.pm = (uint32_t*)take(4 * n);
+    w.pk0 = (uint32_t*)take(4 * np);
+    w.pk1 = (uint32_t*)take(4 * np);
+    w.pv0 = (uint32_t*)take(4 * np);
+    w.pv1 = (uint32_t*)take(4 * np);
+    size_t tp = 0;
+    (void)rocprim::radix_sort_pairs(nullptr, tp, (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                    (uint32_t*)nullptr, (int)np, 0, 6);
+    tb = tb > tp ? tb : tp;
+  }
   w.cub_bytes = tb;
   w.cub = take((int64_t)tb);
   w.total = off;
   return w;
+}
+
+// The Morton order of P into out (uint32[K], K >= 1) from workspace w.
+int morton_sort(const QWs& w, const float* P, int64_t K, uint32_t* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_qinit, dim3(1), dim3(64), 0, s, w.red);
+  A3D_LAUNCH_CHECK();
+  const unsigned g = (unsigned)div_up(K, kBlk);
+  hipLaunchKernelGGL(k_qbounds, dim3(g < 256 ? g : 256), dim3(kBlk), 0, s, P, K, w.red);
+  A3D_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_qmorton, dim3(g), dim3(kBlk), 0, s, P, K, w.red, w.k0, w.v0);
+  A3D_LAUNCH_CHECK();
+  size_t tb = w.cub_bytes;
+  A3D_HIP_CHECK(rocprim::radix_sort_pairs(w.cub, tb, w.k0, w.k1, w.v0, out, (int)K, 0, 30, s));
+  return A3D_OK;
 }
 
 inline unsigned grid_for(int64_t n) { return (unsigned)div_up(n > 0 ? n : 1, kBlk); }
@@ -909,17 +983,45 @@ extern "C" int a3d_query_order(const float* P, int64_t K, uint32_t* perm, void* 
   if (!P || !perm || !ws) return fail(A3D_ERR_ARG, "a3d_query_order: null argument");
   if (ws_bytes < need) return fail(A3D_ERR_CAPACITY, "a3d_query_order: workspace %lld < %lld bytes", (long long)ws_bytes, (long long)need);
   if ((uintptr_t)ws & 255) return fail(A3D_ERR_ARG, "a3d_query_order: workspace must be 256-byte aligned");
+  return morton_sort(carve_q(ws, K), P, K, perm, (hipStream_t)stream);
+}
+
+extern "C" int a3d_query_order_work_ws_bytes(int64_t K, int64_t* bytes) {
+  if (K < 0 || K > 0x7fffffff || !bytes)
+    return fail(A3D_ERR_ARG, "a3d_query_order_work_ws_bytes: K=%lld not in [0, 2^31)", (long long)K);
+  *bytes = carve_q(nullptr, K, true).total;
+  return A3D_OK;
+}
+
+// The Morton order of the queries with its 64-query packets regrouped by
+// estimated cost, heaviest first (longest-processing-time first: a packet
+// that walks far longer than the rest, started late, would end the walk
+// late), the Morton order kept within each cost class.  work: uint32 per
+// grid point (a3d_bvh_udf_work's output for the points of a3d_grid_points),
+// on the grid lo3 + (i, j, k) h, i, j, k in [0, R].
+extern "C" int a3d_query_order_work(const float* P, int64_t K, const uint32_t* work, int R, const float* lo3, float h,
+                                    uint32_t* perm, void* ws, int64_t ws_bytes, a3d_stream_t stream) {
+  int64_t need = 0;
+  if (int rc = a3d_query_order_work_ws_bytes(K, &need)) return rc;
+  if (R < 1 || R > 2048 || !(h > 0.0f) || !lo3)
+    return fail(A3D_ERR_ARG, "a3d_query_order_work: bad grid (R=%d, h=%g)", R, (double)h);
+  if (K == 0) return A3D_OK;
+  if (!P || !perm || !ws || !work) return fail(A3D_ERR_ARG, "a3d_query_order_work: null argument");
+  if (ws_bytes < need)
+    return fail(A3D_ERR_CAPACITY, "a3d_query_order_work: workspace %lld < %lld bytes", (long long)ws_bytes, (long long)need);
+  if ((uintptr_t)ws & 255) return fail(A3D_ERR_ARG, "a3d_query_order_work: workspace must be 256-byte aligned");
   hipStream_t s = (hipStream_t)stream;
-  QWs w = carve_q(ws, K);
-  hipLaunchKernelGGL(k_qinit, dim3(1), dim3(64), 0, s, w.red);
-  A3D_LAUNCH_CHECK();
-  unsigned g = grid_for(K);
-  hipLaunchKernelGGL(k_qbounds, dim3(g < 256 ? g : 256), dim3(kBlk), 0, s, P, K, w.red);
-  A3D_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_qmorton, dim3(g), dim3(kBlk), 0, s, P, K, w.red, w.k0, w.v0);
+  const QWs w = carve_q(ws, K, true);
+  const int64_t np = K / 64, nfull = np * 64;
+  if (np < 2) return morton_sort(w, P, K, perm, s);
+  if (int rc = morton_sort(w, P, K, w.pm, s)) return rc;
+  hipLaunchKernelGGL(k_qpacket, dim3(grid_for(nfull)), dim3(kBlk), 0, s, P, w.pm, nfull, work, R, lo3[0], lo3[1],
+                     lo3[2], h, w.pk0, w.pv0);
   A3D_LAUNCH_CHECK();
   size_t tb = w.cub_bytes;
-  A3D_HIP_CHECK(rocprim::radix_sort_pairs(w.cub, tb, w.k0, w.k1, w.v0, perm, (int)K, 0, 30, s));
+  A3D_HIP_CHECK(rocprim::radix_sort_pairs(w.cub, tb, w.pk0, w.pk1, w.pv0, w.pv1, (int)np, 0, 6, s));
+  hipLaunchKernelGGL(k_qgather, dim3(grid_for(K)), dim3(kBlk), 0, s, w.pm, w.pv1, K, nfull, perm);
+  A3D_LAUNCH_CHECK();
   return A3D_OK;
 }
 
@@ -971,7 +1073,7 @@ extern "C" int a3d_query_order_noalloc(const float* P, int64_t K, uint32_t* perm
 namespace {
 int udf_launch(const void* blob, const float* P, int64_t K, const uint32_t* perm, const float* seed, float* dist,
                int32_t* face, float* closest, float* uvw, int flags, int64_t* counters, a3d_stream_t stream,
-               const GridBound* grid = nullptr) {
+               const GridBound* grid = nullptr, uint32_t* work = nullptr) {
   const GridBound gb = grid ? *grid : GridBound{};
   if (!blob) return fail(A3D_ERR_ARG, "a3d_bvh_udf: null BVH blob");
   // the packet walk launches one 64-lane block per 64 queries (grid x < 2^32)
@@ -982,8 +1084,8 @@ int udf_launch(const void* blob, const float* P, int64_t K, const uint32_t* perm
   if ((flags & A3D_UDF_REDO) && !seed && !grid) return fail(A3D_ERR_ARG, "a3d_bvh_udf: A3D_UDF_REDO needs seeds");
   if (grid && (flags & A3D_UDF_SPLIT)) return fail(A3D_ERR_ARG, "a3d_bvh_udf_grid: no grid bounds in the split walk");
   const bool use_obbs = !(flags & A3D_UDF_NO_OBBS);
-  if (counters && (flags & (A3D_UDF_LANE | A3D_UDF_SPLIT)))
-    return fail(A3D_ERR_ARG, "a3d_bvh_udf: counters need the wave walk");
+  if ((counters || work) && (flags & (A3D_UDF_LANE | A3D_UDF_SPLIT)))
+    return fail(A3D_ERR_ARG, "a3d_bvh_udf: counters and work counts need the wave walk");
   if ((flags & A3D_UDF_SPLIT) && (seed || (flags & (A3D_UDF_LANE | A3D_UDF_DIRECT))))
     return fail(A3D_ERR_ARG, "a3d_bvh_udf: A3D_UDF_SPLIT takes no seed and no other walk flag");
   if ((flags & A3D_UDF_SPLIT) && K > (int64_t(1) << 36))
@@ -999,19 +1101,19 @@ int udf_launch(const void* blob, const float* P, int64_t K, const uint32_t* perm
     hipLaunchKernelGGL(k_udf_lane, dim3(grid_for(K)), dim3(kBlk), 0, s, blob, P, perm, K, seed, dist, face, closest,
                        uvw, use_obbs, (const int32_t*)nullptr, gb);
   else if (flags & A3D_UDF_DIRECT) {
-    if (counters)
+    if (counters || work)
       hipLaunchKernelGGL((k_udf_wave<true, false>), dim3(grid_for_w(K)), dim3(kBlkW), 0, s, blob, P, perm, K, seed, dist,
-                         face, closest, uvw, c, use_obbs, gb);
+                         face, closest, uvw, c, use_obbs, gb, work);
     else
       hipLaunchKernelGGL((k_udf_wave<false, false>), dim3(grid_for_w(K)), dim3(kBlkW), 0, s, blob, P, perm, K, seed, dist,
-                         face, closest, uvw, c, use_obbs, gb);
+                         face, closest, uvw, c, use_obbs, gb, (uint32_t*)nullptr);
   } else {
-    if (counters)
+    if (counters || work)
       hipLaunchKernelGGL((k_udf_wave<true, true>), dim3(grid_for_w(K)), dim3(kBlkW), 0, s, blob, P, perm, K, seed, dist,
-                         face, closest, uvw, c, use_obbs, gb);
+                         face, closest, uvw, c, use_obbs, gb, work);
     else
       hipLaunchKernelGGL((k_udf_wave<false, true>), dim3(grid_for_w(K)), dim3(kBlkW), 0, s, blob, P, perm, K, seed, dist,
-                         face, closest, uvw, c, use_obbs, gb);
+                         face, closest, uvw, c, use_obbs, gb, (uint32_t*)nullptr);
   }
   A3D_LAUNCH_CHECK();
   if (flags & A3D_UDF_REDO) {  // queries whose seed was below their distance: walk them again without one
@@ -1106,6 +1208,15 @@ extern "C" int a3d_bvh_udf(const void* blob, const float* P, int64_t K, const ui
                            int32_t* face, float* closest, float* uvw, int flags, int64_t* counters,
                            a3d_stream_t stream) {
   return udf_launch(blob, P, K, perm, nullptr, dist, face, closest, uvw, flags, counters, stream);
+}
+
+// a3d_bvh_udf (wave walk) that also writes each query's work: the number of
+// leaf triangles whose bound admitted the query during its packet's walk.
+extern "C" int a3d_bvh_udf_work(const void* blob, const float* P, int64_t K, const uint32_t* perm, float* dist,
+                                int32_t* face, float* closest, float* uvw, int flags, uint32_t* work,
+                                a3d_stream_t stream) {
+  if (!work && K > 0) return fail(A3D_ERR_ARG, "a3d_bvh_udf_work: null work output");
+  return udf_launch(blob, P, K, perm, nullptr, dist, face, closest, uvw, flags, nullptr, stream, nullptr, work);
 }
 
 extern "C" int a3d_bvh_udf_seeded(const void* blob, const float* P, int64_t K, const uint32_t* perm,

@@ -49,7 +49,10 @@ SIGNATURES: dict[str, list] = {
     "a3d_query_order_ws_bytes": [_i64, _pi64],
     "a3d_query_order": [_vp, _i64, _vp, _vp, _i64, _vp],
     "a3d_query_order_noalloc": [_vp, _i64, _vp, _vp],
+    "a3d_query_order_work_ws_bytes": [_i64, _pi64],
+    "a3d_query_order_work": [_vp, _i64, _vp, _i32, _pf32, _f32, _vp, _vp, _i64, _vp],
     "a3d_bvh_udf": [_vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp],
+    "a3d_bvh_udf_work": [_vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp],
     "a3d_bvh_udf_seeded": [_vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp],
     "a3d_grid_points": [ctypes.POINTER(ctypes.c_float), _f32, _i32, _vp, _vp, _vp],
     "a3d_udf_grid_seed": [_vp, _vp, _vp, _vp, _i32, _vp, _f32, _vp, _vp, _i64, _f32, _vp, _vp],

@@ -240,18 +240,27 @@ class BVHAccelerator:
     SPLIT_MAX_QUERIES = 1 << 15
     LANE_MAX_QUERIES = 1 << 17
 
-    def query_order(self, points: torch.Tensor) -> torch.Tensor:
+    def query_order(self, points: torch.Tensor, by_cost: bool = False) -> torch.Tensor:
         """Morton order of the query points (uint32 permutation stored in an
-        int32 tensor)."""
+        int32 tensor).  ``by_cost`` (with the seed grid built): the Morton
+        order's 64-query packets regrouped by the grid's per-point walk work,
+        heaviest class first (a3d_query_order_work), the order for the
+        grid-bounded walk."""
         p = self._points(points)
         K = p.shape[0]
+        sg = self.__dict__.get("_seed_grids", {}).get(self.SEED_GRID_RES) if by_cost else None
+        work = self.__dict__.get("_seed_work", {}).get(self.SEED_GRID_RES) if sg else None
+        name = "a3d_query_order" if work is None else "a3d_query_order_work"
         wsb = ctypes.c_int64()
-        _lib.call("a3d_query_order_ws_bytes", K, ctypes.byref(wsb))
+        _lib.call(name + "_ws_bytes", K, ctypes.byref(wsb))
         with torch.cuda.device(self.device):
             perm = torch.empty(K, dtype=torch.int32, device=self.device)
             ws = torch.empty(max(wsb.value, 1), dtype=torch.uint8, device=self.device)
-            _lib.call("a3d_query_order", _lib.ptr(p), K, _lib.ptr(perm), _lib.ptr(ws), wsb.value,
-                      _lib.stream(self.device))
+            if work is None:
+                _lib.call(name, _lib.ptr(p), K, _lib.ptr(perm), _lib.ptr(ws), wsb.value, _lib.stream(self.device))
+            else:
+                _lib.call(name, _lib.ptr(p), K, _lib.ptr(work), self.SEED_GRID_RES, (ctypes.c_float * 3)(*sg[2]),
+                          sg[3], _lib.ptr(perm), _lib.ptr(ws), wsb.value, _lib.stream(self.device))
         return perm
 
     def seed_grid(self, res: int | None = None):
@@ -262,8 +271,9 @@ class BVHAccelerator:
         a 5 % margin, built once per mesh and resolution: the points are
         generated on the device in a walk order of 4x4x4 bricks along a
         Morton curve (a3d_grid_points), then walked once by the plain wave
-        walk.  The source of the grid-bounded walk's per-query distance
-        bounds."""
+        walk, which also records each point's walk work (a3d_bvh_udf_work,
+        kept for query_order's cost order).  The source of the grid-bounded
+        walk's per-query distance bounds."""
         R = int(res or self.SEED_GRID_RES)
         side = R + 1
         if side < 4 or side & (side - 1):
@@ -297,9 +307,11 @@ class BVHAccelerator:
                 d = torch.empty(n, dtype=torch.float32, device=self.device)
                 f = torch.empty(n, dtype=torch.int32, device=self.device)
                 c = torch.empty(n, 3, dtype=torch.float32, device=self.device)
-                _lib.call("a3d_bvh_udf", _lib.ptr(self.blob), _lib.ptr(pts), n, _lib.ptr(perm), _lib.ptr(d),
-                          _lib.ptr(f), _lib.ptr(c), None, 0, None, st)
+                work = torch.empty(n, dtype=torch.int32, device=self.device)
+                _lib.call("a3d_bvh_udf_work", _lib.ptr(self.blob), _lib.ptr(pts), n, _lib.ptr(perm), _lib.ptr(d),
+                          _lib.ptr(f), _lib.ptr(c), None, 0, _lib.ptr(work), st)
                 del pts, perm, c
+            self.__dict__.setdefault("_seed_work", {})[R] = work
             grids[R] = (d, f, lo, h)
         return grids[R] or None
 
@@ -352,10 +364,12 @@ class BVHAccelerator:
             order = self.ORDER_MIN_QUERIES <= K < 2 ** 31
         if seed is None:
             seed = flags in (0, self.UDF_LANE) and counters is None and self._grid_pays(K)
-        perm = self.query_order(p) if order and K > 0 else None
+        with torch.cuda.device(self.device):
+            sg = self.seed_grid() if seed and K > 0 else None
+        # the grid-bounded packet walk takes its packets heaviest first
+        perm = self.query_order(p, by_cost=sg is not None and not flags & (self.UDF_LANE | self.UDF_SPLIT)) if order and K > 0 else None
         with torch.cuda.device(self.device):
             st = _lib.stream(self.device)
-            sg = self.seed_grid() if seed and K > 0 else None
             if sg is not None:
                 grid, gface, lo, h = sg
                 _lib.call("a3d_bvh_udf_grid", _lib.ptr(self.blob), _lib.ptr(p), K, _lib.ptr(perm), _lib.ptr(grid),

@@ -189,12 +189,11 @@ def main():
     uvw = torch.empty(K, 3, device=dev)
     perm = torch.empty(K, dtype=torch.int32, device=dev)
     wsb = ctypes.c_int64()
-    _lib.call("a3d_query_order_ws_bytes", K, ctypes.byref(wsb))
+    _lib.call("a3d_query_order_work_ws_bytes", K, ctypes.byref(wsb))
     ws = torch.empty(wsb.value, dtype=torch.uint8, device=dev)
     stream = torch.cuda.current_stream(dev)
     sp = stream.cuda_stream
     lib = _lib.lib()
-    order_args = (_lib.ptr(P), K, _lib.ptr(perm), _lib.ptr(ws), wsb.value, sp)
     # the distance grid that bounds each walk: built once per mesh, like the
     # tree (what MeshBVH.udf does once a mesh has answered a few grids' worth
     # of queries); timed twice: the process's first build and a rebuild
@@ -208,16 +207,20 @@ def main():
     grid, gface, glo, gh = bvh.seed_grid()
     torch.cuda.synchronize()
     grid_s = time.perf_counter() - tg
+    work = bvh.__dict__["_seed_work"][bvh.SEED_GRID_RES]
+    order_args = (_lib.ptr(P), K, _lib.ptr(work), bvh.SEED_GRID_RES, (ctypes.c_float * 3)(*glo), ctypes.c_float(gh),
+                  _lib.ptr(perm), _lib.ptr(ws), wsb.value, sp)
     udf_args = (_lib.ptr(bvh.blob), _lib.ptr(P), K, _lib.ptr(perm), _lib.ptr(grid), _lib.ptr(gface),
                 _lib.ptr(bvh.vertices), _lib.ptr(bvh.faces), bvh.SEED_GRID_RES, (ctypes.c_float * 3)(*glo),
                 ctypes.c_float(gh), ctypes.c_float(gh / 1024), _lib.ptr(dist), _lib.ptr(face), _lib.ptr(closest),
                 _lib.ptr(uvw), 0, sp)
 
-    # one step = Morton ordering of the queries + the wave-coherent walk, each
-    # query's walk bounded from its start by the grid (plus the walk of any
-    # query whose bound fell short)
+    # one step = ordering of the queries (Morton packets, heaviest cost class
+    # of the grid's walk work first) + the wave-coherent walk, each query's
+    # walk bounded from its start by the grid (plus the walk of any query
+    # whose bound fell short)
     def step(ev=None):
-        _lib.check(lib.a3d_query_order(*order_args), "a3d_query_order")
+        _lib.check(lib.a3d_query_order_work(*order_args), "a3d_query_order_work")
         if ev is not None:
             ev[0].record(stream)
         _lib.check(lib.a3d_bvh_udf_grid(*udf_args), "a3d_bvh_udf_grid")

@@ -104,6 +104,21 @@ int a3d_bvh_info(const void* blob, int64_t* out4, a3d_stream_t stream);
 int a3d_query_order_ws_bytes(int64_t K, int64_t* bytes);
 int a3d_query_order(const float* P, int64_t K, uint32_t* perm, void* ws, int64_t ws_bytes,
                     a3d_stream_t stream);
+/* a3d_query_order_work: a3d_query_order's Morton order with its 64-query
+ * packets (perm[64 j .. 64 j + 63], j < K / 64) regrouped by estimated cost,
+ * heaviest first, the Morton order kept within each cost class; the
+ * trailing partial packet stays last.  A packet's class is floor(log2(m +
+ * 1)), m the largest work[] over the 8 grid points around each of its
+ * queries (the corners of the query's cell of the grid lo3 + (i, j, k) h, i,
+ * j, k in [0, R], clamped into it as a3d_udf_grid_seed does); work uint32
+ * per grid point, a3d_bvh_udf_work's output for a3d_grid_points's points.
+ * A packet far costlier than the rest would end the walk late if started
+ * late: the cost order starts those first.  Workspace of
+ * a3d_query_order_work_ws_bytes(K) bytes, 256-byte aligned; K < 2^31.  No
+ * counterpart in the reference. */
+int a3d_query_order_work_ws_bytes(int64_t K, int64_t* bytes);
+int a3d_query_order_work(const float* P, int64_t K, const uint32_t* work, int R, const float* lo3, float h,
+                         uint32_t* perm, void* ws, int64_t ws_bytes, a3d_stream_t stream);
 /* a3d_query_order_noalloc: a3d_query_order (same perm, bit for bit) that
  * allocates nothing and takes no workspace: the workspace is a file-scope
  * device array of 48 MiB that each GPU holds until the process ends.  For
@@ -117,6 +132,13 @@ int a3d_query_order_noalloc(const float* P, int64_t K, uint32_t* perm, a3d_strea
 int a3d_bvh_udf(const void* blob, const float* P, int64_t K, const uint32_t* perm, float* dist,
                 int32_t* face, float* closest, float* uvw, int flags, int64_t* counters,
                 a3d_stream_t stream);
+/* a3d_bvh_udf_work: a3d_bvh_udf (wave walk; flags 0, A3D_UDF_DIRECT or
+ * A3D_UDF_NO_OBBS) that also writes work[q] (uint32): the number of leaf
+ * triangles whose bound admitted query q during its packet's walk, the
+ * per-query cost figure behind a3d_query_order_work.  Results equal
+ * a3d_bvh_udf's.  No counterpart in the reference. */
+int a3d_bvh_udf_work(const void* blob, const float* P, int64_t K, const uint32_t* perm, float* dist,
+                     int32_t* face, float* closest, float* uvw, int flags, uint32_t* work, a3d_stream_t stream);
 /* a3d_bvh_udf_seeded: a3d_bvh_udf (wave walk) where seed[q] (float, indexed
  * like the outputs) is an upper bound on query q's distance: the walk starts
  * with it as its best distance and never enters a box farther away.  Results

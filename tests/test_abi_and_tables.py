@@ -158,6 +158,23 @@ def test_new_entry_points_validate_arguments():
         _lib.call("a3d_ijk_bounds", None, 5, None, None)
     with pytest.raises(RuntimeError, match="null seed"):
         _lib.call("a3d_bvh_udf_seeded", 1, None, 5, None, None, None, None, None, None, 0, None)
+    with pytest.raises(RuntimeError, match="null work"):
+        _lib.call("a3d_bvh_udf_work", 1, 1, 5, None, 1, 1, 1, None, 0, None, None)
+    with pytest.raises(RuntimeError, match="need the wave walk"):
+        _lib.call("a3d_bvh_udf_work", 1, 1, 5, None, 1, 1, 1, None, 1, 1, None)
+    with pytest.raises(RuntimeError, match="not in"):
+        _lib.call("a3d_query_order_work_ws_bytes", -1, ctypes.byref(n))
+    _lib.call("a3d_query_order_work_ws_bytes", 1 << 20, ctypes.byref(n))
+    plain = ctypes.c_int64()
+    _lib.call("a3d_query_order_ws_bytes", 1 << 20, ctypes.byref(plain))
+    assert n.value >= plain.value + 4 * (1 << 20) + 16 * (1 << 14)  # + the Morton order and 4 packet arrays
+    lo3 = (ctypes.c_float * 3)()
+    with pytest.raises(RuntimeError, match="bad grid"):
+        _lib.call("a3d_query_order_work", 1, 5, 1, 0, lo3, 0.1, 1, 1, 1 << 20, None)
+    with pytest.raises(RuntimeError, match="null argument"):
+        _lib.call("a3d_query_order_work", 1, 5, None, 8, lo3, 0.1, 1, 1, 1 << 20, None)
+    with pytest.raises(RuntimeError, match="workspace"):
+        _lib.call("a3d_query_order_work", 1, 5, 1, 8, lo3, 0.1, 1, 256, 1, None)
     pb, sb = ctypes.c_int64(), ctypes.c_int64()
     _lib.call("a3d_sg_sizes", ctypes.byref(pb), ctypes.byref(sb))
     assert pb.value == 512 + 16 * 2 ** 19 * 4 + 64 * 32 * 2 + 64 * 64 * 2 + 64 * 4

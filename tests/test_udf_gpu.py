@@ -151,6 +151,67 @@ def test_grid_seeded_walk_equals_plain_walk():
         assert torch.equal(x, y)
 
 
+def _cost_order_expected(b, P, morton):
+    """a3d_query_order_work restated on the CPU: each full 64-query packet of
+    the Morton order gets class floor(log2(m + 1)), m the largest grid work
+    over the 8 corners of its queries' cells (float32 (p - lo) / h, floored,
+    clamped to [0, R - 1]); packets sorted stably by class, heaviest first;
+    the partial packet last."""
+    R = b.SEED_GRID_RES
+    _, _, lo, h = b.seed_grid()
+    work = b._seed_work[R].cpu().numpy().astype(np.int64)
+    p = P.cpu().numpy()
+    f = np.floor((p - np.asarray(lo, np.float32)) / np.float32(h))
+    c0 = np.where(f >= 0, np.where(f < R - 1, f, R - 1), 0).astype(np.int64)
+    w = np.zeros(len(p), np.int64)
+    for k in range(8):
+        c = c0 + np.array([k & 1, (k >> 1) & 1, k >> 2])
+        w = np.maximum(w, work[(c[:, 0] * (R + 1) + c[:, 1]) * (R + 1) + c[:, 2]])
+    mo = morton.cpu().numpy().astype(np.int64)
+    n_full = len(p) // 64 * 64
+    m = w[mo[:n_full]].reshape(-1, 64).max(1)
+    cls = sum((m + 1 >= (1 << t)).astype(np.int64) for t in range(1, 34))  # floor(log2(m + 1))
+    order = np.argsort(32 - cls, kind="stable")
+    return np.concatenate([mo[:n_full].reshape(-1, 64)[order].reshape(-1), mo[n_full:]])
+
+
+def test_cost_order_restated_and_walk_unchanged():
+    """a3d_query_order_work: the Morton order's packets regrouped by the seed
+    grid's walk work (a3d_bvh_udf_work) equal the CPU restatement bit for bit
+    (with a partial trailing packet), a3d_bvh_udf_work's results equal
+    a3d_bvh_udf's, and the grid-bounded walk over the cost order equals the
+    plain walk."""
+    V, F = datasets.noisy_icosphere(subdivisions=4, seed=3)
+    b = BVHAccelerator(V.to(DEV), F.to(DEV))
+    P = datasets.uniform_points((1 << 18) + 37, seed=21)
+    P[:5000] *= 0.3  # deep inside: the costly queries, first in input order
+    P[-3:] = torch.tensor([[4.0, 4.0, 4.0], [-5.0, 0.0, 0.0], [0.0, 0.0, 0.0]])  # outside the grid, the centre
+    Pd = P.to(DEV)
+    grid, gface, lo, h = b.seed_grid()
+    work = b._seed_work[b.SEED_GRID_RES]
+    assert work.shape == grid.shape and int(work.min()) >= 0 and int(work.max()) > 4 * float(work.float().median())
+    morton = b.query_order(Pd)
+    got = b.query_order(Pd, by_cost=True)
+    assert np.array_equal(got.cpu().numpy().astype(np.int64), _cost_order_expected(b, P, morton))
+    assert not torch.equal(got, morton)
+    # fewer than two full packets: the Morton order itself
+    for K in (1, 64, 127):
+        assert torch.equal(b.query_order(Pd[:K], by_cost=True), b.query_order(Pd[:K]))
+    # the work walk answers like the plain walk
+    from atom3d_amd.kernels import _lib
+
+    plain = b.udf(Pd, seed=False)
+    out = [torch.empty_like(t) for t in plain]
+    wk = torch.empty(len(P), dtype=torch.int32, device=DEV)
+    _lib.call("a3d_bvh_udf_work", _lib.ptr(b.blob), _lib.ptr(Pd), len(P), _lib.ptr(morton), *[_lib.ptr(t) for t in out],
+              0, _lib.ptr(wk), _lib.stream(torch.device(DEV)))
+    for x, y in zip(plain, out):
+        assert torch.equal(x, y)
+    assert int(wk.min()) >= 0 and int(wk.max()) > 0
+    for x, y in zip(plain, b.udf(Pd, seed=True)):
+        assert torch.equal(x, y)
+
+
 def test_grid_path_declines_non_finite_meshes():
     """Vertex data with a non-finite coordinate gets no bound grid and udf()
     walks plainly (the tree itself is built from valid data here)."""
