@@ -56,7 +56,8 @@ def parse():
     ap.add_argument("--conn", type=int, default=26, help="flood connectivity (6, 18 or 26)")
     ap.add_argument("--dmc-res", type=int, default=1024)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
-                    help="run one plain device copy of this many MiB before timing (counter calibration)")
+                    help="before timing, run one plain device copy of this many MiB and the VALU calibration kernel "
+                         "(counter calibration)")
     return ap.parse_args()
 
 
@@ -91,12 +92,16 @@ def roofline(kernel, kernel_ms, profile, algorithmic=None):
     if traffic:
         r["achieved"] = traffic / sec / 1e9
         r["frac"] = r["achieved"] / HBM_PEAK_GBS
-    if "valu_busy_percent" in f:
-        # rocprofv3's VALUBusy: SQ_ACTIVE_INST_VALU / (CUs x GRBM_GUI_ACTIVE), the share of the kernel's time
-        # the CUs' vector ALUs are issuing: the bound of a kernel that moves little memory
-        r["valu_busy_frac"] = f["valu_busy_percent"] / 100.0
+    if "valu_issue_per_cu_cycle" in f and "valu_issue_calib_per_cu_cycle" in f:
+        # VALU issue: the kernel's SQ_INSTS_VALU per CU per cycle (GRBM_GUI_ACTIVE) against the rate the
+        # calibration kernel (dependency-free FP32 fused multiply-adds) reaches with the same counters; a
+        # kernel at or above it issues VALU instructions as fast as plain FP32 work does (its mix holds
+        # instructions cheaper than an FMA), the bound of a kernel that moves little memory
+        r["valu_issue"] = {"achieved": f["valu_issue_per_cu_cycle"], "calib": f["valu_issue_calib_per_cu_cycle"],
+                           "unit": "wave-instructions per CU per cycle",
+                           "ratio": f["valu_issue_per_cu_cycle"] / f["valu_issue_calib_per_cu_cycle"]}
     for k in ("valu_insts_per_wave", "salu_insts_per_wave", "l2_hit_rate", "occupancy_percent",
-              "valu_utilization_percent", "kernel_ms_profiled"):
+              "valu_utilization_percent", "valu_busy_percent", "kernel_ms_profiled"):
         if k in f:
             r[k] = f[k]
     if algorithmic:
@@ -133,14 +138,23 @@ def main():
               "multi-GPU query sharding is not built.", file=sys.stderr)
         sys.exit(2)
 
-    if a.copy_probe_mb:  # counter calibration: one elementwise kernel reading and writing copy_probe_mb MiB
+    if a.copy_probe_mb:
+        # counter calibration: one elementwise kernel reading and writing
+        # copy_probe_mb MiB (the HBM byte counters' scale), and the VALU
+        # calibration kernel (the issue rate of plain FP32 VALU instructions)
         import torch
+
+        from atom3d_amd.kernels import _lib
 
         src = torch.ones(a.copy_probe_mb << 18, device="cuda:0")
         dst = torch.empty_like(src)
         torch.mul(src, 2.0, out=dst)
         torch.cuda.synchronize()
         del src, dst
+        out = torch.empty(256 * 8192, device="cuda:0")
+        _lib.call("a3d_calib_valu", _lib.ptr(out), out.numel(), 4096, _lib.stream(out.device))
+        torch.cuda.synchronize()
+        del out
     if a.workload == "dmc":
         return bench_dmc(a)
     if a.workload in ("octree", "flood"):

@@ -599,6 +599,14 @@ int a3d_graph_end(a3d_stream_t stream, int64_t* handle);
 int a3d_graph_launch(int64_t handle, a3d_stream_t stream);
 int a3d_graph_destroy(int64_t handle);
 
+/* a3d_calib_valu: counter calibration, no counterpart in the reference.
+ * Launches n threads (n a positive multiple of 256; out: device float [n])
+ * that each run `iters` iterations of 8 independent FP32 fused
+ * multiply-adds and write one sum; with the VALU and cycle counters this
+ * gives the chip's issue rate of plain VALU instructions, the peak the
+ * walks' VALU rates are compared with. */
+int a3d_calib_valu(float* out, int64_t n, int iters, a3d_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif
