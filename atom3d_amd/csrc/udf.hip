@@ -46,10 +46,6 @@ constexpr int kLaneLds = 16;  // per-lane LDS stack entries in the lane path
 constexpr int kLaneOvf = kStackTotal - kLaneLds;
 
 __device__ __forceinline__ int32_t uni(int32_t x) { return __builtin_amdgcn_readfirstlane(x); }
-// lane j's value of x (j uniform)
-__device__ __forceinline__ float lane_f(float x, int j) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), j));
-}
 
 __device__ __forceinline__ void write_result(int64_t q, v3 p, float best, int32_t best_id, int32_t best_tri,
                                              const Tri48* __restrict__ tris, float* __restrict__ dist,
@@ -274,22 +270,17 @@ __device__ __forceinline__ void wave_walk(const Node64* __restrict__ nodes, cons
       if (br) { cur = uni(nd.d.y); continue; }
     } else {
       const int32_t s0 = leaf_start(cur), n = leaf_count(cur);
-      // the leaf's (<= 8 < 64) triangle boxes in one load, lane k holding box s0 + k: one
-      // memory round trip per leaf instead of one per triangle
-      float4 blo = make_float4(0.0f, 0.0f, 0.0f, 0.0f), bhi = blo;
-      if (QUEUE) {
-        const int32_t k = s0 + (lane < n ? lane : n - 1);
-        blo = boxes[k].lo;
-        bhi = boxes[k].hi;
-      }
+      // the leaf's triangle boxes by scalar loads (the address is the same on every lane):
+      // the bound test takes them as SGPR operands, with no readlane per coordinate
+      typedef const __attribute__((address_space(4))) float CFloat;
+      CFloat* cb = (CFloat*)boxes;
       for (int32_t t = s0; t < s0 + n; ++t) {
         v3 ta, tb, tc;
         int32_t id = 0;
         bool need;
         if (QUEUE) {  // bound test from the stored box; vertices only in the flush
-          const int j = t - s0;
-          need = box_d2(p, lane_f(blo.x, j), lane_f(blo.y, j), lane_f(blo.z, j), lane_f(bhi.x, j), lane_f(bhi.y, j),
-                        lane_f(bhi.z, j)) <= thr;
+          CFloat* b = cb + 8 * (int64_t)t;
+          need = box_d2(p, b[0], b[1], b[2], b[4], b[5], b[6]) <= thr;
         } else {
           load_tri(tris, t, ta, tb, tc, id);
           need = tri_box_d2(ta, tb, tc, tri_pad_of(tris, t), p) <= thr;
