@@ -118,6 +118,7 @@ SIGNATURES: dict[str, list] = {
     "a3d_graph_end": [_vp, _pi64],
     "a3d_graph_launch": [_i64, _vp],
     "a3d_graph_destroy": [_i64],
+    "a3d_calib_valu": [_vp, _i64, _i32, _vp],
 }
 
 

@@ -32,6 +32,40 @@ def test_library_exports_every_declared_symbol():
     assert set(_lib.SIGNATURES) <= set(names)
 
 
+def _declarations():
+    """name -> parameter type strings of every int-returning entry point in
+    include/atom3d_amd.h."""
+    src = open(os.path.join(ROOT, "include", "atom3d_amd.h")).read()
+    src = re.sub(r"/\*.*?\*/", " ", src, flags=re.S)
+    out = {}
+    for m in re.finditer(r"^\s*int\s+(a3d_\w+)\s*\(([^;]*?)\)\s*;", src, re.M | re.S):
+        params = " ".join(m.group(2).split())
+        out[m.group(1)] = [] if params in ("", "void") else [p.strip() for p in params.split(",")]
+    return out
+
+
+def _ctype_ok(param: str, t) -> bool:
+    if "*" in param or param.startswith("a3d_stream_t"):
+        return t is ctypes.c_void_p or (isinstance(t, type) and issubclass(t, ctypes._Pointer))
+    base = param.rsplit(" ", 1)[0].replace("const ", "").strip()
+    want = {"int64_t": ctypes.c_int64, "uint64_t": ctypes.c_uint64, "int": ctypes.c_int, "int32_t": ctypes.c_int32,
+            "float": ctypes.c_float, "double": ctypes.c_double}.get(base)
+    return want is not None and t is want
+
+
+def test_every_entry_point_has_a_matching_binding():
+    # a call without argtypes would pass 64-bit pointers and sizes as 32-bit C ints
+    decl = _declarations()
+    assert len(decl) >= 15
+    missing = sorted(set(decl) - set(_lib.SIGNATURES))
+    assert not missing, f"entry points without a ctypes signature in kernels/_lib.py: {missing}"
+    for name, params in decl.items():
+        sig = _lib.SIGNATURES[name]
+        assert len(sig) == len(params), f"{name}: header has {len(params)} parameters, binding {len(sig)}"
+        for i, (p, t) in enumerate(zip(params, sig)):
+            assert _ctype_ok(p, t), f"{name} parameter {i} ({p!r}) bound as {t}"
+
+
 def test_version_and_argument_errors():
     assert "gfx950" in _lib.version()
     b, w = ctypes.c_int64(), ctypes.c_int64()
