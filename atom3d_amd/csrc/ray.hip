@@ -29,7 +29,7 @@ namespace a3d {
 namespace {
 
 constexpr int kBlk = 256;
-constexpr int kRayLds = 16;  // per-lane LDS stack entries
+constexpr int kRayLds = 8;  // per-lane LDS stack entries: 16 KB per block of the nearest-hit walk, so 8 wavefronts per SIMD fit (16 entries: 5)
 constexpr int kRayOvf = kStackTotal - kRayLds;
 constexpr float kNoHit = 1e10f;
 
