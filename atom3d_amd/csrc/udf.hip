@@ -890,6 +890,13 @@ __global__ __launch_bounds__(kBlk) void k_qgather(const uint32_t* __restrict__ p
   perm[i] = i < nfull ? pm[(int64_t)porder[i >> 6] * 64 + (i & 63)] : pm[i];
 }
 
+// The query order sorts the 30-bit Morton codes (10 bits per axis over the
+// queries' box) on their top 24 bits, three 8-bit radix passes instead of
+// four: ties, queries within one 4 x 4 x 4 block of the finest cells, keep
+// their input order (the sort is stable), which changes nothing for a
+// 64-query packet.
+constexpr int kMortonLowBits = 6;
+
 struct QWs {
   uint32_t* red;
   uint32_t* k0;
@@ -919,7 +926,7 @@ QWs carve_q(void* base, int64_t K, bool packets = false) {
   w.v0 = (uint32_t*)take(4 * n);
   size_t tb = 0;
   (void)rocprim::radix_sort_pairs(nullptr, tb, (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
-                                           (uint32_t*)nullptr, (int)n, 0, 30);
+                                           (uint32_t*)nullptr, (int)n, kMortonLowBits, 30);
   if (packets) {
     const int64_t np = n / 64 > 1 ? n / 64 : 2;
     w.pm = (uint32_t*)take(4 * n);
@@ -948,7 +955,7 @@ int morton_sort(const QWs& w, const float* P, int64_t K, uint32_t* out, hipStrea
   hipLaunchKernelGGL(k_qmorton, dim3(g), dim3(kBlk), 0, s, P, K, w.red, w.k0, w.v0);
   A3D_LAUNCH_CHECK();
   size_t tb = w.cub_bytes;
-  A3D_HIP_CHECK(rocprim::radix_sort_pairs(w.cub, tb, w.k0, w.k1, w.v0, out, (int)K, 0, 30, s));
+  A3D_HIP_CHECK(rocprim::radix_sort_pairs(w.cub, tb, w.k0, w.k1, w.v0, out, (int)K, kMortonLowBits, 30, s));
   return A3D_OK;
 }
 
@@ -1326,10 +1333,22 @@ __global__ void k_udf_batch(const void* const* __restrict__ blobs, const float* 
 }
 
 size_t sort_bytes(int64_t K) {
-  size_t t = 0;
+  size_t t = 0, t32 = 0;
   (void)rocprim::radix_sort_pairs(nullptr, t, (uint64_t*)nullptr, (uint64_t*)nullptr, (uint32_t*)nullptr,
                                            (uint32_t*)nullptr, (int)(K > 1 ? K : 2), 0, 40);
-  return t;
+  (void)rocprim::radix_sort_pairs(nullptr, t32, (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                  (uint32_t*)nullptr, (int)(K > 1 ? K : 2), 0, 32);
+  return t > t32 ? t : t32;
+}
+
+// Up to 256 meshes the batch key fits 32 bits: mesh << 24 | the top 24 of the
+// 30 Morton bits (8 per axis over the mesh's box), which halves the sort's
+// key traffic and saves a radix pass against the 64-bit key.
+__global__ void k_batch_keys32(const uint64_t* __restrict__ k64, int64_t K, uint32_t* __restrict__ k32) {
+  const int64_t i = blockIdx.x * (int64_t)kBlkB + threadIdx.x;
+  if (i >= K) return;
+  const uint64_t k = k64[i];
+  k32[i] = (uint32_t)((k >> 30) << 24) | (uint32_t)((k & 0x3fffffffull) >> 6);
 }
 
 }  // namespace
@@ -1363,9 +1382,17 @@ extern "C" int a3d_batch_order(const void* const* blobs, int n_meshes, const flo
   hipLaunchKernelGGL(k_batch_keys, dim3((unsigned)div_up(K, kBlkB)), dim3(kBlkB), 0, s, blobs, n_meshes, P, K, q_off,
                      k0, v0);
   A3D_LAUNCH_CHECK();
-  int bits = 30;
-  while ((1ll << (bits - 30)) < n_meshes) ++bits;
-  A3D_HIP_CHECK(rocprim::radix_sort_pairs(tmp, tb, k0, k1, v0, perm, (int)K, 0, bits, s));
+  int mbits = 0;
+  while ((1ll << mbits) < n_meshes) ++mbits;
+  if (mbits <= 8) {
+    uint32_t* c0 = (uint32_t*)k1;  // the 64-bit key slots hold the 32-bit keys
+    uint32_t* c1 = c0 + K;
+    hipLaunchKernelGGL(k_batch_keys32, dim3((unsigned)div_up(K, kBlkB)), dim3(kBlkB), 0, s, k0, K, c0);
+    A3D_LAUNCH_CHECK();
+    A3D_HIP_CHECK(rocprim::radix_sort_pairs(tmp, tb, c0, c1, v0, perm, (int)K, 0, 24 + mbits, s));
+    return A3D_OK;
+  }
+  A3D_HIP_CHECK(rocprim::radix_sort_pairs(tmp, tb, k0, k1, v0, perm, (int)K, 0, 30 + mbits, s));
   return A3D_OK;
 }
 

@@ -92,8 +92,9 @@ int a3d_bvh_info(const void* blob, int64_t* out4, a3d_stream_t stream);
  * leaf triangles seen, exact triangle tests run (all summed over wavefronts)
  * and triangles whose bound admitted them per lane (summed over lanes).
  *
- * a3d_query_order: perm = the queries sorted along a 30-bit Morton curve of
- * their bounding box (stable), so each wavefront covers a compact region.
+ * a3d_query_order: perm = the queries sorted along a Morton curve of their
+ * bounding box (the top 24 bits of a 30-bit code; stable: ties keep their
+ * input order), so each wavefront covers a compact region.
  * Workspace of a3d_query_order_ws_bytes(K) bytes, 256-byte aligned; K < 2^31.
  */
 #define A3D_UDF_LANE 1
@@ -562,7 +563,8 @@ int a3d_sg_udf(const void* params, const void* blob, const float* P, int64_t K, 
  * blobs: device array of n_meshes BVH blob pointers; P: all queries
  * concatenated, mesh m owning [q_off[m], q_off[m+1]) (q_off device int64).
  * a3d_batch_order: perm (uint32 [K]) = queries sorted by mesh, then by Morton
- *   code in the mesh's box.  Workspace: a3d_batch_order_ws_bytes.
+ *   code in the mesh's box (8 bits per axis up to 256 meshes, 10 beyond;
+ *   ties keep their input order).  Workspace: a3d_batch_order_ws_bytes.
  * a3d_batch_udf: tiles = int64 pairs (start << 8 | count, mesh), count <= 64,
  *   covering perm[start, start + count); a persistent grid of `blocks` blocks
  *   of `waves_per_block` wavefronts takes tiles from *counter (device uint32,
