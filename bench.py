@@ -708,11 +708,35 @@ def measure_surrogate(mesh, V, P, a, fp64_check: bool, sweep: bool = False):
                   "losses": hist["losses"]},
         "errors_vs_bvh": ev, "errors_vs_fp64": fp64, "bbox_diag": diag,
         "meets_max_err_1e-5_diag": ev["max_err"] <= 1e-5 * diag,
-        "roofline": {"bound": "mfma", "achieved": tflops, "peak": 2500.0, "unit": "TFLOP/s",
-                     "frac": tflops / 2500.0, "traffic": None, "kernel": "k_sg_infer", "kernel_ms": infer_ms,
-                     "gather_bytes_per_query": 12 + 512 + 4,
-                     "gather_GBps": (12 + 512 + 4) * K / (infer_ms * 1e-3) / 1e9},
+        "roofline": sg_roofline(infer_ms, tflops, K),
     }
+
+
+def sg_roofline(infer_ms, tflops, K):
+    """Roofline of the surrogate's inference kernel: bound by the hash
+    table's gathers beyond L2 (measured FETCH_SIZE + WRITE_SIZE of
+    k_sg_infer in its committed counter summary, over the HIP-event-timed
+    launch), with the MLP's MFMA rate beside it (measured busy share and
+    BF16 FLOPs from the same summary; the FLOP count of the network per
+    query when the summary is absent)."""
+    f = profile_figures("k_sg_infer_sphere_union_counters")
+    traffic = f.get("infer_hbm_bytes_per_launch")
+    r = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None, "traffic": traffic,
+         "kernel": "k_sg_infer", "kernel_ms": infer_ms,
+         "profile": "profiles/k_sg_infer_sphere_union_counters.md" if f else None,
+         "gather_bytes_per_query": 12 + 512 + 4}
+    if traffic:
+        r["achieved"] = traffic / (infer_ms * 1e-3) / 1e9
+        r["frac"] = r["achieved"] / HBM_PEAK_GBS
+    mf = f.get("infer_mfma_flops_per_launch")
+    achieved = mf / (infer_ms * 1e-3) / 1e12 if mf else tflops
+    r["mfma"] = {"achieved": achieved, "peak": 2500.0, "unit": "TFLOP/s", "frac": achieved / 2500.0,
+                 "busy_percent": f.get("infer_mfma_busy_percent"),
+                 "source": "SQ_INSTS_VALU_MFMA_MOPS_BF16 x 512" if mf else "network FLOPs per query x queries"}
+    for k in ("infer_l2_hit_rate", "infer_kernel_ms_profiled"):
+        if k in f:
+            r[k[6:]] = f[k]
+    return r
 
 
 def bench_surrogate(a):
