@@ -16,7 +16,9 @@
 //     brick after brick so a brick loads as 256 coalesced 16-byte reads;
 //   - one workgroup per active brick loads its words plus a one-word halo
 //     into LDS and iterates R <- R | (dilate(R) & free) to a local fixpoint
-//     (32 voxels per bitwise op, two barriers per step);
+//     (32 voxels per bitwise op, two barriers per step), each step also
+//     filling every free k-run that holds a reached voxel (carry
+//     propagation, krun_fill) so water crosses a brick along k in one step;
 //   - a brick whose words changed re-activates its 26 neighbours for the next
 //     global pass (flag = pass number, so nothing is cleared between passes);
 //     the host checks for convergence once every 16 passes.
@@ -154,6 +156,30 @@ __device__ __forceinline__ uint32_t dilate(const uint32_t* s, int ii, int jj, in
          dk(s, hi + 1, hj + 1, m);
 }
 
+// Every free voxel of a 128-voxel k-row (4 words, bit b of word m = voxel
+// 32 m + b) joined along k to a voxel of x: a run of free voxels holding a
+// reached one is reached whole, whatever the connectivity (k steps are in all
+// of them).  Up: adding the seeds to the free mask carries from the lowest
+// seed of each run through the run's top, so ((F + S) ^ F) & F | S is each
+// run from its lowest seed up; down: the same on the bit-reversed row.
+__device__ __forceinline__ void krun_fill(uint32_t x[4], const uint32_t f[4]) {
+  const uint64_t F0 = f[0] | ((uint64_t)f[1] << 32), F1 = f[2] | ((uint64_t)f[3] << 32);
+  const uint64_t S0 = x[0] | ((uint64_t)x[1] << 32), S1 = x[2] | ((uint64_t)x[3] << 32);
+  uint64_t a0 = F0 + S0;
+  uint64_t a1 = F1 + S1 + (a0 < F0 ? 1ull : 0ull);
+  const uint64_t U0 = ((a0 ^ F0) & F0) | S0, U1 = ((a1 ^ F1) & F1) | S1;
+  const uint64_t rF0 = __builtin_bitreverse64(F1), rF1 = __builtin_bitreverse64(F0);
+  const uint64_t rS0 = __builtin_bitreverse64(U1), rS1 = __builtin_bitreverse64(U0);
+  a0 = rF0 + rS0;
+  a1 = rF1 + rS1 + (a0 < rF0 ? 1ull : 0ull);
+  const uint64_t D0 = ((a0 ^ rF0) & rF0) | rS0, D1 = ((a1 ^ rF1) & rF1) | rS1;
+  const uint64_t X0 = __builtin_bitreverse64(D1), X1 = __builtin_bitreverse64(D0);
+  x[0] = (uint32_t)X0;
+  x[1] = (uint32_t)(X0 >> 32);
+  x[2] = (uint32_t)X1;
+  x[3] = (uint32_t)(X1 >> 32);
+}
+
 __global__ __launch_bounds__(kBlk) void k_ff_pass(Geo g, int conn, int t, const uint32_t* __restrict__ occ,
                                                    uint32_t* reach, int32_t* active, int32_t* changed) {
   const int64_t b = blockIdx.x;
@@ -178,10 +204,10 @@ __global__ __launch_bounds__(kBlk) void k_ff_pass(Geo g, int conn, int t, const 
     uint32_t nx[4];
     bool ch = false;
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      nx[m] = cur[m] | (dilate(s, ii, jj, m, conn) & fr[m]);
-      ch |= nx[m] != cur[m];
-    }
+    for (int m = 0; m < 4; ++m) nx[m] = cur[m] | (dilate(s, ii, jj, m, conn) & fr[m]);
+    krun_fill(nx, fr);  // along k a step reaches the whole free run, not one voxel
+#pragma unroll
+    for (int m = 0; m < 4; ++m) ch |= nx[m] != cur[m];
     __syncthreads();
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
