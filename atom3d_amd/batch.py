@@ -75,7 +75,7 @@ assert _BOUND.itemsize == 64
 
 
 class BatchedUDF:
-    BOUND_GRID_RES = 31  # per-mesh distance-bound grid (32^3 points), built with the trees
+    BOUND_GRID_RES = 63  # per-mesh distance-bound grid (64^3 points), built with the trees
 
     def __init__(self, meshes, device="cuda", leaf: int = 8, bounds: bool = True):
         self.device = torch.device(device)
