@@ -282,10 +282,10 @@ class MeshBVH:
         gen.manual_seed(seed)
         dirs = torch.randn(n_dirs, 3, device=self.device, generator=gen)
         dirs = dirs / dirs.norm(dim=1, keepdim=True)
-        votes = torch.zeros(N, dtype=torch.int32, device=self.device)
-        for i in range(n_dirs):
-            sc = self._signed_crossing_count(points, dirs[i], max_t=max_t, coalesce_t=coalesce_t)
-            votes += ((sc.abs() % 2) == 1).int()
+        # all directions in one launch (a3d_ray_crossings_dirs): the counts
+        # _signed_crossing_count gives direction by direction
+        sc = self._bvh.signed_crossings_dirs(points, dirs, max_t=max_t, coalesce_t=coalesce_t)
+        votes = ((sc.abs() % 2) == 1).int().sum(0)
         inside = votes > (n_dirs // 2)
         sdf = udf.clone()
         sdf[inside] = -sdf[inside]

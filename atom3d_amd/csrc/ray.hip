@@ -327,6 +327,113 @@ __global__ __launch_bounds__(kBlk) void k_ray_crossings(float* __restrict__ t, i
   out[r] = total;
 }
 
+// Signed crossing counts of N points along n_dirs directions in one walk per
+// (point, direction): ray r = direction r / N from point r % N.  The ray's
+// hits (the all-hit walk's, same filters) are kept in registers sorted by
+// (t, sign) as they arrive, then grouped and counted exactly as
+// k_ray_crossings does over a sorted list.  A ray with more than
+// kCrossCap hits gets out[r] = kCrossOverflow and is left to the two-pass
+// path (count, write, k_ray_crossings).
+constexpr int kCrossCap = 16;
+constexpr int32_t kCrossOverflow = INT32_MIN;
+
+__global__ __launch_bounds__(kBlk) void k_ray_cross(const void* __restrict__ blob, const float* __restrict__ P,
+                                                     int64_t N, const float* __restrict__ dirs, int64_t K,
+                                                     float max_t, float coalesce_t, int32_t* __restrict__ out) {
+  __shared__ int32_t s_ref[kRayLds * kBlk];
+  const int tid = threadIdx.x;
+  const int64_t r = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlk + tid;
+  if (r >= K) return;
+  const int64_t pi = r % N, di = r / N;
+  const BvhHeader* H = header(blob);
+  const Node64* __restrict__ nodes = nodes_of(blob);
+  const Tri48* __restrict__ tris = tris_of(blob);
+  const v3 ro = mk(P[pi * 3], P[pi * 3 + 1], P[pi * 3 + 2]);
+  const v3 rd = mk(dirs[di * 3], dirs[di * 3 + 1], dirs[di * 3 + 2]);
+  const v3 inv = mk(safe_inv(rd.x), safe_inv(rd.y), safe_inv(rd.z));
+  const float wid = box_widen(ro, H->coord_max);
+  const float lim = prune_t(max_t);
+  // a hit is kept as sign * t (t > 0): |h| is its t exactly, and at equal t
+  // the negative sign orders first, as in k_ray_crossings' (t, sign) order
+  float hk[kCrossCap];
+#pragma unroll
+  for (int k = 0; k < kCrossCap; ++k) hk[k] = __int_as_float(0x7f800000);
+  int32_t n_hit = 0;
+  int32_t ovf_ref[kRayOvf];
+  int sp = 0;
+  int32_t cur = 0;
+  bool live = H->n_tris > 0;
+  while (live) {
+    if (cur >= 0) {
+      const Node64* nd = nodes + cur;
+      const float4 a = nd->a, b = nd->b, c = nd->c;
+      const int4 d = nd->d;
+      const bool vl = ray_box(ro, inv, a.x, a.y, a.z, a.w, b.x, b.y, wid) <= lim;
+      const bool vr = ray_box(ro, inv, b.z, b.w, c.x, c.y, c.z, c.w, wid) <= lim;
+      if (vl && vr) {
+        if (sp < kRayLds) s_ref[sp * kBlk + tid] = d.y;
+        else if (sp < kStackTotal) ovf_ref[sp - kRayLds] = d.y;
+        sp = sp < kStackTotal ? sp + 1 : sp;
+        cur = d.x;
+        continue;
+      }
+      if (vl) { cur = d.x; continue; }
+      if (vr) { cur = d.y; continue; }
+    } else {
+      const int32_t s0 = leaf_start(cur), n = leaf_count(cur);
+      for (int32_t t = s0; t < s0 + n; ++t) {
+        v3 ta, tb, tc;
+        int32_t id;
+        load_tri(tris, t, ta, tb, tc, id);
+        float th;
+        const int sg = all_hit_sign(ta, tb, tc, ro, rd, max_t, th);
+        if (!sg) continue;
+        ++n_hit;
+        // sorted insertion: the new hit moves down past every kept hit it precedes
+        float x = sg > 0 ? th : -th;
+#pragma unroll
+        for (int k = 0; k < kCrossCap; ++k) {
+          const float ax = fabsf(x), ah = fabsf(hk[k]);
+          const bool before = ax < ah || (ax == ah && x < hk[k]);
+          const float o = hk[k];
+          hk[k] = before ? x : o;
+          x = before ? o : x;
+        }
+      }
+    }
+    if (sp == 0) break;
+    --sp;
+    cur = sp < kRayLds ? s_ref[sp * kBlk + tid] : ovf_ref[sp - kRayLds];
+  }
+  if (n_hit > kCrossCap) {
+    out[r] = kCrossOverflow;
+    return;
+  }
+  // groups of consecutive hits closer than coalesce_t (k_ray_crossings' rule)
+  int32_t total = 0, cnt = 0, sum = 0;
+  float prev = 0.0f;
+#pragma unroll
+  for (int k = 0; k < kCrossCap; ++k) {
+    if (k < n_hit) {
+      const float tk = fabsf(hk[k]);
+      if (cnt > 0 && !(fabsf(tk - prev) < coalesce_t)) {
+        const int32_t g = (sum > 0) - (sum < 0);
+        total += (abs(sum) == cnt) ? g : (abs(sum) % 2) * g;
+        cnt = 0;
+        sum = 0;
+      }
+      ++cnt;
+      sum += hk[k] > 0.0f ? 1 : -1;
+      prev = tk;
+    }
+  }
+  if (cnt > 0) {
+    const int32_t g = (sum > 0) - (sum < 0);
+    total += (abs(sum) == cnt) ? g : (abs(sum) % 2) * g;
+  }
+  out[r] = total;
+}
+
 inline unsigned grid_for(int64_t n) { return (unsigned)div_up(n > 0 ? n : 1, kBlk); }
 
 int check_rays(const char* fn, const void* blob, const float* O, const float* D, int64_t K) {
@@ -388,6 +495,19 @@ extern "C" int a3d_ray_eval(const float* V, int64_t nV, const int32_t* F, int64_
     return fail(A3D_ERR_ARG, "a3d_ray_eval: null or empty input");
   hipLaunchKernelGGL(k_ray_eval, dim3(grid_for(n)), dim3(kBlk), 0, (hipStream_t)stream, V, F, O, D, n, ray_idx, face,
                      max_t, t, sign, points, uvw);
+  A3D_LAUNCH_CHECK();
+  return A3D_OK;
+}
+
+extern "C" int a3d_ray_crossings_dirs(const void* blob, const float* P, int64_t N, const float* dirs, int n_dirs,
+                                      float max_t, float coalesce_t, int32_t* out, a3d_stream_t stream) {
+  if (N < 0 || n_dirs < 0 || n_dirs > 4096 || N * n_dirs > (int64_t(1) << 31) - 1)
+    return fail(A3D_ERR_ARG, "a3d_ray_crossings_dirs: N=%lld x n_dirs=%d out of range", (long long)N, n_dirs);
+  const int64_t K = N * n_dirs;
+  if (K == 0) return A3D_OK;
+  if (!blob || !P || !dirs || !out) return fail(A3D_ERR_ARG, "a3d_ray_crossings_dirs: null argument");
+  hipLaunchKernelGGL(k_ray_cross, dim3(grid_for(K)), dim3(kBlk), 0, (hipStream_t)stream, blob, P, N, dirs, K, max_t,
+                     coalesce_t, out);
   A3D_LAUNCH_CHECK();
   return A3D_OK;
 }

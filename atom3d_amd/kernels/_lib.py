@@ -42,6 +42,7 @@ SIGNATURES: dict[str, list] = {
     "a3d_ray_all_hits": [_vp, _vp, _vp, _i64, _f32, _vp, _vp, _vp, _vp, _vp, _vp],
     "a3d_ray_eval": [_vp, _i64, _vp, _i64, _vp, _vp, _i64, _vp, _vp, _f32, _vp, _vp, _vp, _vp, _vp],
     "a3d_ray_crossings": [_vp, _vp, _vp, _i64, _f32, _vp, _vp],
+    "a3d_ray_crossings_dirs": [_vp, _vp, _i64, _vp, _i32, _f32, _f32, _vp, _vp],
     "a3d_bvh_build_ex": [_vp, _i64, _vp, _i64, _i32, _i32, _vp, _i64, _vp, _i64, _vp],
     "a3d_bvh_info": [_vp, _pi64, _vp],
     "a3d_sah_sizes": [_i64, _i32, _pi64, _pi64],

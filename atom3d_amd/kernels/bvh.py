@@ -474,6 +474,31 @@ class BVHAccelerator:
                       _lib.ptr(out), _lib.stream(self.device))
         return out
 
+    CROSS_OVERFLOW = -(2 ** 31)
+
+    def signed_crossings_dirs(self, points: torch.Tensor, dirs: torch.Tensor, max_t: float = 1e6,
+                              coalesce_t: float = 1e-4) -> torch.Tensor:
+        """Signed crossing counts (int32 [n_dirs, N]) of every point along
+        every direction, as signed_crossings gives them ray by ray: one walk
+        per (point, direction) with its hits sorted in registers; the rays
+        with more hits than that holds (a3d_ray_crossings_dirs marks them)
+        are counted again by the two-pass path."""
+        p = self._points(points)
+        dv = self._points(dirs)
+        N, n_dirs = p.shape[0], dv.shape[0]
+        out = torch.zeros((n_dirs, N), dtype=torch.int32, device=self.device)
+        if N == 0 or n_dirs == 0:
+            return out
+        with torch.cuda.device(self.device):
+            _lib.call("a3d_ray_crossings_dirs", _lib.ptr(self.blob), _lib.ptr(p), N, _lib.ptr(dv), n_dirs,
+                      float(max_t), float(coalesce_t), _lib.ptr(out), _lib.stream(self.device))
+        ovf = torch.nonzero(out.reshape(-1) == self.CROSS_OVERFLOW).reshape(-1)
+        if ovf.numel():
+            o = p[ovf % N].contiguous()
+            d = dv[ovf // N].contiguous()
+            out.view(-1)[ovf] = self.signed_crossings(o, d, max_t=max_t, coalesce_t=coalesce_t)
+        return out
+
     def aabb_hit(self, query_min: torch.Tensor, query_max: torch.Tensor) -> torch.Tensor:
         """Hit mask only (stops at the first intersecting face per box)."""
         lo, hi = self._points(query_min), self._points(query_max)

@@ -227,6 +227,12 @@ int a3d_sdf_sign(const float* P, int64_t K, const float* closest, const int32_t*
  *   from the mesh arrays V float32[nV*3], F int32[nF*3].
  * a3d_ray_crossings: signed crossing count out int32[K] from ray-major hit
  *   lists (offsets int64[K+1]); sorts each ray's (t, sign) in place by t.
+ * a3d_ray_crossings_dirs: the same counts for N points P float32[N*3] along
+ *   n_dirs directions dirs float32[n_dirs*3] (ray r = direction r / N from
+ *   point r % N, N * n_dirs < 2^31) in one walk per ray, its hits kept
+ *   sorted in registers: out int32[n_dirs*N]; a ray with more than 16 hits
+ *   gets INT32_MIN and is counted by the two-pass path instead.  Replaces
+ *   the per-direction loop of MeshBVH._sdf_parity (mesh_bvh.py:851-885).
  */
 int a3d_ray_nearest(const void* blob, const float* O, const float* D, int64_t K, float max_t,
                     uint8_t* hit, float* t, int32_t* face, float* points, float* uvw, a3d_stream_t stream);
@@ -240,6 +246,8 @@ int a3d_ray_eval(const float* V, int64_t nV, const int32_t* F, int64_t nF, const
                  int32_t* sign, float* points, float* uvw, a3d_stream_t stream);
 int a3d_ray_crossings(float* t, int32_t* sign, const int64_t* offsets, int64_t K, float coalesce_t,
                       int32_t* out, a3d_stream_t stream);
+int a3d_ray_crossings_dirs(const void* blob, const float* P, int64_t N, const float* dirs, int n_dirs, float max_t,
+                           float coalesce_t, int32_t* out, a3d_stream_t stream);
 
 /* ------------------------------------------------------------------ box-query tree
  * Replaces build_bvh (bvh_kernels.cu:860-925, bound at :1245) for the box

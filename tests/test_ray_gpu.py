@@ -98,6 +98,54 @@ def test_all_hits_bitexact_and_crossings(which):
     np.testing.assert_array_equal(c, oracle.crossings(ref["t"], ref["sign"], ref["offsets"], 1e-4))
 
 
+def _sphere_row(n=13, r=0.04):
+    """n small icospheres in a row along x: a ray along the row crosses 2n
+    surfaces, more than the fused walk keeps in registers."""
+    v, f = datasets.icosphere(2, r)
+    Vs, Fs = [], []
+    for k in range(n):
+        Vs.append(v + np.array([-0.9 + 0.15 * k, 0.0, 0.0]))
+        Fs.append(f + k * len(v))
+    return np.concatenate(Vs).astype(np.float32), np.concatenate(Fs).astype(np.int32)
+
+
+@pytest.mark.parametrize("which", [0, 1, 2])
+def test_crossings_dirs_equal_oracle_and_two_pass(which):
+    """The one-walk crossing counts of points x directions equal the
+    oracle's count over all hits and the two-pass path; on the row of
+    spheres the rays along the row have more hits than the registers hold
+    and take the fallback."""
+    if which < 2:
+        label, V, F = list(_meshes())[which]
+        P, _ = _rays(1500 if which == 0 else 500, 4)
+    else:
+        label = "sphere row"
+        V, F = _sphere_row()
+        rng = np.random.default_rng(6)
+        P = np.concatenate([rng.uniform(-1, 1, (400, 3)),
+                            np.stack([np.full(200, -1.2), rng.uniform(-0.01, 0.01, 200),
+                                      rng.uniform(-0.01, 0.01, 200)], 1)]).astype(np.float32)
+    rng = np.random.default_rng(7)
+    dirs = rng.normal(size=(6, 3))
+    dirs[0] = (1.0, 1e-3, 2e-3)  # along the row
+    dirs = (dirs / np.linalg.norm(dirs, axis=1, keepdims=True)).astype(np.float32)
+    print(f"[ray parity] crossings x dirs {label}: F={len(F)} N={len(P)} dirs={len(dirs)}", flush=True)
+    b = BVHAccelerator(torch.from_numpy(V).to(DEV), torch.from_numpy(F).to(DEV))
+    got = b.signed_crossings_dirs(torch.from_numpy(P).to(DEV), torch.from_numpy(dirs).to(DEV), max_t=1e6,
+                                  coalesce_t=1e-4).cpu().numpy()
+    O = np.tile(P, (len(dirs), 1))
+    D = np.repeat(dirs, len(P), axis=0)
+    ref = oracle.brute_ray_all(V, F, O, D, max_t=1e6)
+    want = oracle.crossings(ref["t"], ref["sign"], ref["offsets"], 1e-4).reshape(len(dirs), len(P))
+    np.testing.assert_array_equal(got, want)
+    two = b.signed_crossings(torch.from_numpy(O).to(DEV), torch.from_numpy(D).to(DEV), max_t=1e6,
+                             coalesce_t=1e-4).cpu().numpy().reshape(len(dirs), len(P))
+    np.testing.assert_array_equal(got, two)
+    if which == 2:
+        hits = np.diff(ref["offsets"])
+        assert hits.max() > 16  # the fallback ran
+
+
 def test_parity_sdf_matches_pseudo_normal_sign():
     V, F = datasets.icosphere(4, 0.5)
     mesh = MeshBVH(torch.from_numpy(V).float(), torch.from_numpy(F).int(), device=DEV)
