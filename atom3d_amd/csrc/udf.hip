@@ -831,6 +831,15 @@ __device__ __forceinline__ uint32_t spread10(uint32_t x) {
   return x;
 }
 
+__device__ __forceinline__ uint32_t compact3(uint32_t v) {  // every third bit -> low bits
+  v &= 0x09249249u;
+  v = (v ^ (v >> 2)) & 0x030c30c3u;
+  v = (v ^ (v >> 4)) & 0x0300f00fu;
+  v = (v ^ (v >> 8)) & 0xff0000ffu;
+  v = (v ^ (v >> 16)) & 0x000003ffu;
+  return v;
+}
+
 __global__ __launch_bounds__(kBlk) void k_qmorton(const float* __restrict__ P, int64_t K,
                                                    const uint32_t* __restrict__ red, uint32_t* __restrict__ keys,
                                                    uint32_t* __restrict__ vals) {
@@ -848,24 +857,30 @@ __global__ __launch_bounds__(kBlk) void k_qmorton(const float* __restrict__ P, i
   vals[i] = (uint32_t)i;
 }
 
-// Cost class of each full 64-query packet of the Morton order pm (one
+// Cost class of each full 64-query packet of the Morton order (one
 // wavefront per packet): the largest work of the grid points around the
-// packet's queries, each query reading the 8 corners of its grid cell (the
-// cell grid_bound reads, clamped the same way).  key = 32 - floor(log2(max +
-// 1)), so a stable ascending sort puts the heaviest class first and keeps the
-// Morton order inside each class.
-__global__ __launch_bounds__(kBlk) void k_qpacket(const float* __restrict__ P, const uint32_t* __restrict__ pm,
+// packet's queries, each query placed at the centre of its Morton cell
+// (decoded from its sorted key: 1/1024 of the queries' box per axis, read
+// coalesced instead of gathering the query) and reading the 8 corners of
+// its grid cell (clamped as grid_bound clamps).  key = 32 - floor(log2(max
+// + 1)), so a stable ascending sort puts the heaviest class first and keeps
+// the Morton order inside each class.  The class only orders the walk's
+// packets; no result depends on it.
+__global__ __launch_bounds__(kBlk) void k_qpacket(const uint32_t* __restrict__ mkey, const uint32_t* __restrict__ red,
                                                   int64_t nfull, const uint32_t* __restrict__ work, int R, float lo0,
                                                   float lo1, float lo2, float h, uint32_t* __restrict__ pkey,
                                                   uint32_t* __restrict__ pval) {
   const int64_t i = blockIdx.x * (int64_t)kBlk + threadIdx.x;
   if (i >= nfull) return;  // nfull is a multiple of 64: whole wavefronts leave together
-  const int64_t q = pm[i];
+  const uint32_t m = mkey[i];
+  const uint32_t cm[3] = {compact3(m >> 2), compact3(m >> 1), compact3(m)};
   const float lo[3] = {lo0, lo1, lo2};
   int c0[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    const float f = floorf((P[q * 3 + d] - lo[d]) / h);
+    const float qlo = ord2f(red[d]), ext = ord2f(red[3 + d]) - qlo;
+    const float x = qlo + ((float)cm[d] + 0.5f) * (ext * (1.0f / 1023.0f));
+    const float f = floorf((x - lo[d]) / h);
     c0[d] = f >= 0.0f ? (f < (float)(R - 1) ? (int)f : R - 1) : 0;  // NaN -> 0
   }
   uint32_t w = 0;
@@ -950,7 +965,7 @@ int morton_sort(const QWs& w, const float* P, int64_t K, uint32_t* out, hipStrea
   hipLaunchKernelGGL(k_qinit, dim3(1), dim3(64), 0, s, w.red);
   A3D_LAUNCH_CHECK();
   const unsigned g = (unsigned)div_up(K, kBlk);
-  hipLaunchKernelGGL(k_qbounds, dim3(g < 256 ? g : 256), dim3(kBlk), 0, s, P, K, w.red);
+  hipLaunchKernelGGL(k_qbounds, dim3(g < 1024 ? g : 1024), dim3(kBlk), 0, s, P, K, w.red);
   A3D_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_qmorton, dim3(g), dim3(kBlk), 0, s, P, K, w.red, w.k0, w.v0);
   A3D_LAUNCH_CHECK();
@@ -1013,8 +1028,8 @@ extern "C" int a3d_query_order_work(const float* P, int64_t K, const uint32_t* w
   const int64_t np = K / 64, nfull = np * 64;
   if (np < 2) return morton_sort(w, P, K, perm, s);
   if (int rc = morton_sort(w, P, K, w.pm, s)) return rc;
-  hipLaunchKernelGGL(k_qpacket, dim3(grid_for(nfull)), dim3(kBlk), 0, s, P, w.pm, nfull, work, R, lo3[0], lo3[1],
-                     lo3[2], h, w.pk0, w.pv0);
+  hipLaunchKernelGGL(k_qpacket, dim3(grid_for(nfull)), dim3(kBlk), 0, s, w.k1, w.red, nfull, work, R, lo3[0],
+                     lo3[1], lo3[2], h, w.pk0, w.pv0);
   A3D_LAUNCH_CHECK();
   size_t tb = w.cub_bytes;
   A3D_HIP_CHECK(rocprim::radix_sort_pairs(w.cub, tb, w.pk0, w.pk1, w.pv0, w.pv1, (int)np, 0, 6, s));
@@ -1136,14 +1151,6 @@ __global__ __launch_bounds__(kBlk) void k_grid_seed(GridBound gb, const float* _
 // The points of a side^3 grid, lo + (float)i * h per axis, at their own
 // index (i * side + j) * side + k, and a walk order covering them in 4x4x4
 // bricks (one wavefront each), the bricks along a Morton curve.
-__device__ __forceinline__ uint32_t compact3(uint32_t v) {  // every third bit -> low bits
-  v &= 0x09249249u;
-  v = (v ^ (v >> 2)) & 0x030c30c3u;
-  v = (v ^ (v >> 4)) & 0x0300f00fu;
-  v = (v ^ (v >> 8)) & 0xff0000ffu;
-  v = (v ^ (v >> 16)) & 0x000003ffu;
-  return v;
-}
 __global__ __launch_bounds__(kBlk) void k_grid_points(float lo0, float lo1, float lo2, float h, int side,
                                                       float* __restrict__ P, uint32_t* __restrict__ perm) {
   const int64_t n = (int64_t)side * side * side;

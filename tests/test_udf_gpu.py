@@ -151,23 +151,42 @@ def test_grid_seeded_walk_equals_plain_walk():
         assert torch.equal(x, y)
 
 
+def _spread10(x):
+    x = x.astype(np.uint64) & 0x3FF
+    out = np.zeros_like(x)
+    for bit in range(10):
+        out |= ((x >> bit) & 1) << (3 * bit)
+    return out
+
+
 def _cost_order_expected(b, P, morton):
-    """a3d_query_order_work restated on the CPU: each full 64-query packet of
-    the Morton order gets class floor(log2(m + 1)), m the largest grid work
-    over the 8 corners of its queries' cells (float32 (p - lo) / h, floored,
+    """a3d_query_order_work restated on the CPU: each query's 30-bit Morton
+    code over the queries' box (float32 (p - min) * (1023 / extent),
+    clamped, floored) is decoded back to its cell centre min + (c + 0.5) *
+    (extent * float32(1/1023)); each full 64-query packet of the Morton
+    order gets class floor(log2(m + 1)), m the largest grid work over the 8
+    corners of its queries' grid cells (float32 (x - lo) / h, floored,
     clamped to [0, R - 1]); packets sorted stably by class, heaviest first;
     the partial packet last."""
     R = b.SEED_GRID_RES
     _, _, lo, h = b.seed_grid()
     work = b._seed_work[R].cpu().numpy().astype(np.int64)
-    p = P.cpu().numpy()
-    f = np.floor((p - np.asarray(lo, np.float32)) / np.float32(h))
+    p = P.cpu().numpy().astype(np.float32)
+    qlo, qhi = p.min(0), p.max(0)
+    ext = (qhi - qlo).astype(np.float32)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        t = np.where(ext > 0, (p - qlo) * (np.float32(1023.0) / ext), np.float32(0.0)).astype(np.float32)
+    c = np.floor(np.clip(t, 0, 1023)).astype(np.uint64)
+    mo = morton.cpu().numpy().astype(np.int64)
+    key = (_spread10(c[:, 0]) << 2) | (_spread10(c[:, 1]) << 1) | _spread10(c[:, 2])
+    assert np.all(np.diff(key[mo] >> 6) >= 0)  # the Morton order sorts these keys' top 24 bits
+    x = (qlo + (c.astype(np.float32) + np.float32(0.5)) * (ext * np.float32(1.0 / 1023.0))).astype(np.float32)
+    f = np.floor((x - np.asarray(lo, np.float32)) / np.float32(h))
     c0 = np.where(f >= 0, np.where(f < R - 1, f, R - 1), 0).astype(np.int64)
     w = np.zeros(len(p), np.int64)
     for k in range(8):
-        c = c0 + np.array([k & 1, (k >> 1) & 1, k >> 2])
-        w = np.maximum(w, work[(c[:, 0] * (R + 1) + c[:, 1]) * (R + 1) + c[:, 2]])
-    mo = morton.cpu().numpy().astype(np.int64)
+        cc = c0 + np.array([k & 1, (k >> 1) & 1, k >> 2])
+        w = np.maximum(w, work[(cc[:, 0] * (R + 1) + cc[:, 1]) * (R + 1) + cc[:, 2]])
     n_full = len(p) // 64 * 64
     m = w[mo[:n_full]].reshape(-1, 64).max(1)
     cls = sum((m + 1 >= (1 << t)).astype(np.int64) for t in range(1, 34))  # floor(log2(m + 1))
