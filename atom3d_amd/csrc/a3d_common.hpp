@@ -367,4 +367,63 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
 
 inline int64_t div_up(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// ------------------------------------------------------------------ lane refill
+// Independent per-lane walks (one ray or box per lane) vary in length by
+// orders of magnitude; a wavefront that walked 64 of them side by side would
+// run as long as its longest one with most lanes idle.  Instead each
+// wavefront of a grid sized to the resident wavefronts works through a list
+// of items, and its idle lanes take the next items while the others walk on.
+// Wavefront wv of W takes the blocks of 64 consecutive items b = wv, wv + W,
+// ...; its job j is item ((j >> 6) * W + wv) * 64 + (j & 63).
+struct LaneJobs {
+  int64_t W, wv, njobs, next;  // wave-uniform
+};
+
+__device__ __forceinline__ LaneJobs lane_jobs(int64_t K, int block_threads) {
+  LaneJobs J;
+  const int wpb = block_threads >> 6;
+  J.W = (int64_t)gridDim.x * wpb;
+  J.wv = (int64_t)blockIdx.x * wpb + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int64_t nb = (K + 63) >> 6;
+  J.njobs = J.wv < nb ? ((nb - 1 - J.wv) / J.W + 1) * 64 : 0;
+  J.next = 0;
+  return J;
+}
+
+// Hands the next jobs to the lanes of `idle` in lane order (call with the
+// whole wavefront; the lanes outside `idle` get -1 and keep their item).
+// Returns this lane's new item, or -1 (no job left, or a job past K in the
+// last block).
+__device__ __forceinline__ int64_t lane_take(LaneJobs& J, uint64_t idle, int64_t K) {
+  const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
+                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+  const int64_t j = J.next + rank;
+  const bool mine = (idle >> (threadIdx.x & 63)) & 1ull;
+  J.next += __popcll(idle);
+  if (!mine || j >= J.njobs) return -1;
+  const int64_t item = ((j >> 6) * J.W + J.wv) * 64 + (j & 63);
+  return item < K ? item : -1;
+}
+
+// Lanes refill once at least this many are idle (or all of them are).
+constexpr int kRefillIdle = 16;
+
+// Grid for a refill kernel: at most the blocks resident at once on the
+// device (more would start late and end the launch late), at least enough
+// for one block of 64 items per wavefront.  cache: the call site's
+// per-device record of resident blocks (64 ints, zero-initialised).
+template <typename Kern>
+inline unsigned refill_grid(Kern kern, int block_threads, int64_t K, int* cache) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+  if (!cache[dev]) {
+    int n = 0, c = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern, block_threads, 0) != hipSuccess || n < 1) n = 1;
+    if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c < 1) c = 1;
+    cache[dev] = n * c;
+  }
+  const int64_t want = div_up(K > 0 ? K : 1, block_threads);
+  return (unsigned)(want < cache[dev] ? want : cache[dev]);
+}
+
 }  // namespace a3d

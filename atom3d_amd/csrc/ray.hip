@@ -98,7 +98,68 @@ __device__ __forceinline__ float box_widen(v3 ro, float coord_max) {
 
 __device__ __forceinline__ float prune_t(float best) { return best + best * 0.0078125f; }  // x (1 + 2^-7)
 
+// ------------------------------------------------------------ leaf tests dealt over the wavefront
+// In a per-lane walk a lane that reaches a leaf tests its (up to 8)
+// triangles one after another while the other lanes wait; with 64 lanes
+// some lane is at a leaf in most steps, so the wavefront would spend most
+// of its time in leaf loops with a few lanes active.  Instead the (lane,
+// triangle) pairs of every lane at a leaf are dealt out one per lane, in
+// rounds of 64, and tested side by side; each result goes back to the lane
+// that owns the pair.
+struct LeafLds {  // per wavefront
+  unsigned long long key[64];  // owner's best (t bits << 32 | face id) during a round
+  int32_t tri[64];             // triangle of the owner's winning pair
+  uint8_t own[64];             // owner lane of each pair of the round
+};
+
+__device__ __forceinline__ void wave_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// This lane's leaf (n triangles from start; n = 0: none) and the exclusive
+// prefix of n over the lanes; total pairs (wave-uniform).
+struct LeafDeal {
+  int32_t start, n, excl, total;
+};
+
+__device__ __forceinline__ LeafDeal leaf_deal(bool at_leaf, int32_t ref, int lane) {
+  LeafDeal d;
+  d.n = at_leaf ? leaf_count(ref) : 0;
+  d.start = at_leaf ? leaf_start(ref) : 0;
+  int incl = d.n;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int v = __shfl_up(incl, o);
+    if (lane >= o) incl += v;
+  }
+  d.excl = incl - d.n;
+  d.total = __shfl(incl, 63);
+  return d;
+}
+
+// Pair base + lane of a round: owner lane L and triangle (valid = the pair
+// exists).  Ends with the owner table read; the caller's next round starts
+// with a fence.
+__device__ __forceinline__ void leaf_pair(const LeafDeal& d, int base, LeafLds* S, int lane, bool& valid, int& L,
+                                          int32_t& tri) {
+  const int lo = d.excl > base ? d.excl : base;
+  const int hi = d.excl + d.n < base + 64 ? d.excl + d.n : base + 64;
+  for (int k = lo; k < hi; ++k) S->own[k - base] = (uint8_t)lane;
+  wave_fence();
+  const int j = base + lane;
+  valid = j < d.total;
+  L = valid ? (int)S->own[lane] : lane;
+  tri = __shfl(d.start, L) + (j - __shfl(d.excl, L));
+}
+
 // ------------------------------------------------------------ nearest hit
+// One walk per ray, the rays of a wavefront taken from its LaneJobs list as
+// lanes fall idle (see lane_take), leaf triangles dealt over the wavefront
+// (see leaf_pair); each ray's walk takes the same steps and gives the same
+// result as a walk on its own: a leaf's tests all complete before its lane
+// pops, and the lowest (t, face id) wins as in a one-by-one scan.
 __global__ __launch_bounds__(kBlk) void k_ray_nearest(const void* __restrict__ blob, const float* __restrict__ O,
                                                        const float* __restrict__ D, int64_t K, float max_t,
                                                        uint8_t* __restrict__ hit, float* __restrict__ t_out,
@@ -106,66 +167,116 @@ __global__ __launch_bounds__(kBlk) void k_ray_nearest(const void* __restrict__ b
                                                        float* __restrict__ uvw) {
   __shared__ int32_t s_ref[kRayLds * kBlk];
   __shared__ float s_t[kRayLds * kBlk];
-  const int tid = threadIdx.x;
-  const int64_t r = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlk + tid;
-  if (r >= K) return;
+  __shared__ LeafLds s_leaf[kBlk / 64];
+  const int tid = threadIdx.x, lane = tid & 63;
+  LeafLds* S = &s_leaf[tid >> 6];
   const BvhHeader* H = header(blob);
   const Node64* __restrict__ nodes = nodes_of(blob);
   const Tri48* __restrict__ tris = tris_of(blob);
-  const v3 ro = mk(O[r * 3], O[r * 3 + 1], O[r * 3 + 2]);
-  const v3 rd = mk(D[r * 3], D[r * 3 + 1], D[r * 3 + 2]);
-  const v3 inv = mk(safe_inv(rd.x), safe_inv(rd.y), safe_inv(rd.z));
-  const float wid = box_widen(ro, H->coord_max);
+  const bool empty = H->n_tris == 0;
+  LaneJobs J = lane_jobs(K, kBlk);
 
-  float best = max_t;
+  int64_t r = -1;  // this lane's ray
+  v3 ro = mk(0.0f, 0.0f, 0.0f), rd = ro, inv = ro;
+  float wid = 0.0f, best = max_t, thr = 0.0f;
   int32_t best_id = 0x7fffffff, best_tri = -1;
-  float thr = prune_t(best);
   int32_t ovf_ref[kRayOvf];
   float ovf_t[kRayOvf];
   int sp = 0;
-  int32_t cur = H->n_tris > 0 ? 0 : ~0;
-  if (H->n_tris == 0) sp = -1;
-  while (sp >= 0) {
-    if (cur >= 0) {
-      const Node64* nd = nodes + cur;
-      const float4 a = nd->a, b = nd->b, c = nd->c;
-      const int4 d = nd->d;
-      const float tl = ray_box(ro, inv, a.x, a.y, a.z, a.w, b.x, b.y, wid);
-      const float tr = ray_box(ro, inv, b.z, b.w, c.x, c.y, c.z, c.w, wid);
-      const bool vl = tl <= thr, vr = tr <= thr;
-      if (vl && vr) {
-        int32_t near_ref = d.x, far_ref = d.y;
-        float far_t = tr;
-        if (tr < tl) { near_ref = d.y; far_ref = d.x; far_t = tl; }
-        if (sp < kRayLds) {
-          s_ref[sp * kBlk + tid] = far_ref;
-          s_t[sp * kBlk + tid] = far_t;
-        } else if (sp < kStackTotal) {
-          ovf_ref[sp - kRayLds] = far_ref;
-          ovf_t[sp - kRayLds] = far_t;
-        }
-        sp = sp < kStackTotal ? sp + 1 : sp;
-        cur = near_ref;
-        continue;
+  int32_t cur = 0;
+  while (true) {
+    const uint64_t idle = __ballot(r < 0);
+    const int n_idle = __popcll(idle);
+    if (J.next < J.njobs && (n_idle >= kRefillIdle || n_idle == 64)) {
+      const int64_t nr = lane_take(J, idle, K);
+      if (nr >= 0) {
+        r = nr;
+        ro = mk(O[r * 3], O[r * 3 + 1], O[r * 3 + 2]);
+        rd = mk(D[r * 3], D[r * 3 + 1], D[r * 3 + 2]);
+        inv = mk(safe_inv(rd.x), safe_inv(rd.y), safe_inv(rd.z));
+        wid = box_widen(ro, H->coord_max);
+        best = max_t;
+        best_id = 0x7fffffff;
+        best_tri = -1;
+        thr = prune_t(best);
+        sp = 0;
+        cur = 0;
       }
-      if (vl) { cur = d.x; continue; }
-      if (vr) { cur = d.y; continue; }
-    } else {
-      const int32_t s0 = leaf_start(cur), n = leaf_count(cur);
-      for (int32_t t = s0; t < s0 + n; ++t) {
-        v3 ta, tb, tc;
-        int32_t id;
-        load_tri(tris, t, ta, tb, tc, id);
-        const float th = ray_tri(ta, tb, tc, ro, rd);
-        if (th == kNoHit || !(th < max_t)) continue;
-        if (th < best || (th == best && id < best_id)) {
-          best = th;
-          best_id = id;
-          best_tri = t;
+      continue;  // the new lanes' loads, and a refill that found no job, start the next round
+    }
+    if (n_idle == 64) break;  // no job left and every lane idle
+    bool pop = false, at_leaf = false;
+    if (r >= 0) {
+      if (empty) {
+        pop = true;
+      } else if (cur >= 0) {
+        const Node64* nd = nodes + cur;
+        const float4 a = nd->a, b = nd->b, c = nd->c;
+        const int4 d = nd->d;
+        const float tl = ray_box(ro, inv, a.x, a.y, a.z, a.w, b.x, b.y, wid);
+        const float tr = ray_box(ro, inv, b.z, b.w, c.x, c.y, c.z, c.w, wid);
+        const bool vl = tl <= thr, vr = tr <= thr;
+        if (vl && vr) {
+          int32_t near_ref = d.x, far_ref = d.y;
+          float far_t = tr;
+          if (tr < tl) { near_ref = d.y; far_ref = d.x; far_t = tl; }
+          if (sp < kRayLds) {
+            s_ref[sp * kBlk + tid] = far_ref;
+            s_t[sp * kBlk + tid] = far_t;
+          } else if (sp < kStackTotal) {
+            ovf_ref[sp - kRayLds] = far_ref;
+            ovf_t[sp - kRayLds] = far_t;
+          }
+          sp = sp < kStackTotal ? sp + 1 : sp;
+          cur = near_ref;
+        } else if (vl) {
+          cur = d.x;
+        } else if (vr) {
+          cur = d.y;
+        } else {
+          pop = true;
+        }
+      } else {
+        at_leaf = true;
+        pop = true;
+      }
+    }
+    if (__ballot(at_leaf)) {
+      const LeafDeal dl = leaf_deal(at_leaf, cur, lane);
+      for (int base = 0; base < dl.total; base += 64) {
+        wave_fence();
+        const unsigned long long mine = ((unsigned long long)__float_as_uint(best) << 32) | (uint32_t)best_id;
+        S->key[lane] = mine;
+        bool valid;
+        int L;
+        int32_t t;
+        leaf_pair(dl, base, S, lane, valid, L, t);
+        const v3 pro = mk(__shfl(ro.x, L), __shfl(ro.y, L), __shfl(ro.z, L));
+        const v3 prd = mk(__shfl(rd.x, L), __shfl(rd.y, L), __shfl(rd.z, L));
+        unsigned long long k = ~0ull;
+        if (valid) {
+          v3 ta, tb, tc;
+          int32_t id;
+          load_tri(tris, t, ta, tb, tc, id);
+          const float th = ray_tri(ta, tb, tc, pro, prd);
+          if (th != kNoHit && th < max_t) {
+            k = ((unsigned long long)__float_as_uint(th) << 32) | (uint32_t)id;
+            atomicMin(&S->key[L], k);
+          }
+        }
+        wave_fence();
+        if (valid && S->key[L] == k) S->tri[L] = t;
+        wave_fence();
+        const unsigned long long nk = S->key[lane];
+        if (nk != mine) {
+          best = __uint_as_float((uint32_t)(nk >> 32));
+          best_id = (int32_t)(uint32_t)nk;
+          best_tri = S->tri[lane];
           thr = prune_t(best);
         }
       }
     }
+    if (!pop) continue;
     bool found = false;
     while (sp > 0) {
       --sp;
@@ -180,21 +291,23 @@ __global__ __launch_bounds__(kBlk) void k_ray_nearest(const void* __restrict__ b
       }
       if (tt <= thr) { cur = rr; found = true; break; }
     }
-    if (!found) break;
+    if (found) continue;
+    // the walk of ray r is over
+    hit[r] = best_tri >= 0 ? 1 : 0;
+    t_out[r] = best;
+    face[r] = best_tri >= 0 ? best_id : -1;
+    v3 p = ro, w = mk(0.0f, 0.0f, 0.0f);
+    if (best_tri >= 0) {
+      p = add3(ro, mul3(rd, best));
+      v3 ta, tb, tc;
+      int32_t id;
+      load_tri(tris, best_tri, ta, tb, tc, id);
+      w = tri_bary(ta, tb, tc, p);
+    }
+    if (pts) { pts[r * 3] = p.x; pts[r * 3 + 1] = p.y; pts[r * 3 + 2] = p.z; }
+    if (uvw) { uvw[r * 3] = w.x; uvw[r * 3 + 1] = w.y; uvw[r * 3 + 2] = w.z; }
+    r = -1;
   }
-  hit[r] = best_tri >= 0 ? 1 : 0;
-  t_out[r] = best;
-  face[r] = best_tri >= 0 ? best_id : -1;
-  v3 p = ro, w = mk(0.0f, 0.0f, 0.0f);
-  if (best_tri >= 0) {
-    p = add3(ro, mul3(rd, best));
-    v3 ta, tb, tc;
-    int32_t id;
-    load_tri(tris, best_tri, ta, tb, tc, id);
-    w = tri_bary(ta, tb, tc, p);
-  }
-  if (pts) { pts[r * 3] = p.x; pts[r * 3 + 1] = p.y; pts[r * 3 + 2] = p.z; }
-  if (uvw) { uvw[r * 3] = w.x; uvw[r * 3 + 1] = w.y; uvw[r * 3 + 2] = w.z; }
 }
 
 // ------------------------------------------------------------ all hits
@@ -454,8 +567,9 @@ extern "C" int a3d_ray_nearest(const void* blob, const float* O, const float* D,
   if (rc) return rc;
   if (K == 0) return A3D_OK;
   if (!hit || !t || !face) return fail(A3D_ERR_ARG, "a3d_ray_nearest: null output");
-  hipLaunchKernelGGL(k_ray_nearest, dim3(grid_for(K)), dim3(kBlk), 0, (hipStream_t)stream, blob, O, D, K, max_t, hit,
-                     t, face, points, uvw);
+  static int resident[64];
+  hipLaunchKernelGGL(k_ray_nearest, dim3(refill_grid(k_ray_nearest, kBlk, K, resident)), dim3(kBlk), 0,
+                     (hipStream_t)stream, blob, O, D, K, max_t, hit, t, face, points, uvw);
   A3D_LAUNCH_CHECK();
   return A3D_OK;
 }
