@@ -106,12 +106,6 @@ __device__ __forceinline__ float prune_t(float best) { return best + best * 0.00
 // triangle) pairs of every lane at a leaf are dealt out one per lane, in
 // rounds of 64, and tested side by side; each result goes back to the lane
 // that owns the pair.
-struct LeafLds {  // per wavefront
-  unsigned long long key[64];  // owner's best (t bits << 32 | face id) during a round
-  int32_t tri[64];             // triangle of the owner's winning pair
-  uint8_t own[64];             // owner lane of each pair of the round
-};
-
 __device__ __forceinline__ void wave_fence() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -140,26 +134,76 @@ __device__ __forceinline__ LeafDeal leaf_deal(bool at_leaf, int32_t ref, int lan
 }
 
 // Pair base + lane of a round: owner lane L and triangle (valid = the pair
-// exists).  Ends with the owner table read; the caller's next round starts
-// with a fence.
-__device__ __forceinline__ void leaf_pair(const LeafDeal& d, int base, LeafLds* S, int lane, bool& valid, int& L,
+// exists).  own: the wavefront's 64-entry owner table.  Ends with the table
+// read; the caller's next round starts with a fence.
+__device__ __forceinline__ void leaf_pair(const LeafDeal& d, int base, uint8_t* own, int lane, bool& valid, int& L,
                                           int32_t& tri) {
   const int lo = d.excl > base ? d.excl : base;
   const int hi = d.excl + d.n < base + 64 ? d.excl + d.n : base + 64;
-  for (int k = lo; k < hi; ++k) S->own[k - base] = (uint8_t)lane;
+  for (int k = lo; k < hi; ++k) own[k - base] = (uint8_t)lane;
   wave_fence();
   const int j = base + lane;
   valid = j < d.total;
-  L = valid ? (int)S->own[lane] : lane;
+  L = valid ? (int)own[lane] : lane;
   tri = __shfl(d.start, L) + (j - __shfl(d.excl, L));
 }
 
+// Lanes [0, k) as a mask, k in [0, 64].
+__device__ __forceinline__ uint64_t lanes_below(int k) { return k >= 64 ? ~0ull : ((1ull << k) - 1ull); }
+
+// This lane's pairs among the lanes of round `base`, as a lane mask.
+__device__ __forceinline__ uint64_t own_pairs(const LeafDeal& d, int base) {
+  const int lo = d.excl - base > 0 ? d.excl - base : 0;
+  const int hi = d.excl + d.n - base < 64 ? d.excl + d.n - base : 64;
+  return (d.n > 0 && lo < hi) ? (lanes_below(hi) & ~lanes_below(lo)) : 0ull;
+}
+
+// One step of a walk that keeps every hit (no pruning by t): visit the inner
+// node cur (push the right child when both are entered), or report the leaf
+// cur; pop = the walk continues from the stack.
+__device__ __forceinline__ void all_step(const Node64* __restrict__ nodes, v3 ro, v3 inv, float wid, float lim,
+                                         int32_t& cur, int& sp, int32_t* s_ref, int tid, int32_t* ovf_ref,
+                                         bool& at_leaf, bool& pop) {
+  if (cur >= 0) {
+    const Node64* nd = nodes + cur;
+    const float4 a = nd->a, b = nd->b, c = nd->c;
+    const int4 d = nd->d;
+    const bool vl = ray_box(ro, inv, a.x, a.y, a.z, a.w, b.x, b.y, wid) <= lim;
+    const bool vr = ray_box(ro, inv, b.z, b.w, c.x, c.y, c.z, c.w, wid) <= lim;
+    if (vl && vr) {
+      if (sp < kRayLds) s_ref[sp * kBlk + tid] = d.y;
+      else if (sp < kStackTotal) ovf_ref[sp - kRayLds] = d.y;
+      sp = sp < kStackTotal ? sp + 1 : sp;
+      cur = d.x;
+    } else if (vl) {
+      cur = d.x;
+    } else if (vr) {
+      cur = d.y;
+    } else {
+      pop = true;
+    }
+  } else {
+    at_leaf = true;
+    pop = true;
+  }
+}
+
+// Pops the next node of a walk that keeps every hit; false when the stack is empty.
+__device__ __forceinline__ bool all_pop(int32_t& cur, int& sp, const int32_t* s_ref, int tid, const int32_t* ovf_ref) {
+  if (sp == 0) return false;
+  --sp;
+  cur = sp < kRayLds ? s_ref[sp * kBlk + tid] : ovf_ref[sp - kRayLds];
+  return true;
+}
+
 // ------------------------------------------------------------ nearest hit
-// One walk per ray, the rays of a wavefront taken from its LaneJobs list as
-// lanes fall idle (see lane_take), leaf triangles dealt over the wavefront
-// (see leaf_pair); each ray's walk takes the same steps and gives the same
-// result as a walk on its own: a leaf's tests all complete before its lane
-// pops, and the lowest (t, face id) wins as in a one-by-one scan.
+// One walk per ray.  The rays of a wavefront come from its LaneJobs list: a
+// lane whose walk is over holds its result until enough lanes are free,
+// then the free lanes write their results together and take the next rays
+// (see lane_take).  Leaf triangles are dealt over the wavefront (see
+// leaf_pair).  Each ray's walk takes the same steps and gives the same result
+// as a walk on its own: a leaf's tests all complete before its lane pops,
+// and the lowest (t, face id) wins as in a one-by-one scan.
 __global__ __launch_bounds__(kBlk) void k_ray_nearest(const void* __restrict__ blob, const float* __restrict__ O,
                                                        const float* __restrict__ D, int64_t K, float max_t,
                                                        uint8_t* __restrict__ hit, float* __restrict__ t_out,
@@ -167,16 +211,21 @@ __global__ __launch_bounds__(kBlk) void k_ray_nearest(const void* __restrict__ b
                                                        float* __restrict__ uvw) {
   __shared__ int32_t s_ref[kRayLds * kBlk];
   __shared__ float s_t[kRayLds * kBlk];
-  __shared__ LeafLds s_leaf[kBlk / 64];
+  __shared__ unsigned long long s_key[kBlk];  // owner's best (t bits << 32 | face id) during a leaf round
+  __shared__ int32_t s_tri[kBlk];             // triangle of the owner's winning pair
+  __shared__ uint8_t s_own[kBlk];
   const int tid = threadIdx.x, lane = tid & 63;
-  LeafLds* S = &s_leaf[tid >> 6];
+  unsigned long long* key = s_key + (tid & ~63);
+  int32_t* wtri = s_tri + (tid & ~63);
+  uint8_t* own = s_own + (tid & ~63);
   const BvhHeader* H = header(blob);
   const Node64* __restrict__ nodes = nodes_of(blob);
   const Tri48* __restrict__ tris = tris_of(blob);
   const bool empty = H->n_tris == 0;
   LaneJobs J = lane_jobs(K, kBlk);
 
-  int64_t r = -1;  // this lane's ray
+  int64_t r = -1;     // this lane's ray
+  bool done = false;  // its walk is over, its result not yet written
   v3 ro = mk(0.0f, 0.0f, 0.0f), rd = ro, inv = ro;
   float wid = 0.0f, best = max_t, thr = 0.0f;
   int32_t best_id = 0x7fffffff, best_tri = -1;
@@ -185,10 +234,29 @@ __global__ __launch_bounds__(kBlk) void k_ray_nearest(const void* __restrict__ b
   int sp = 0;
   int32_t cur = 0;
   while (true) {
-    const uint64_t idle = __ballot(r < 0);
-    const int n_idle = __popcll(idle);
-    if (J.next < J.njobs && (n_idle >= kRefillIdle || n_idle == 64)) {
-      const int64_t nr = lane_take(J, idle, K);
+    const uint64_t fm = __ballot(r < 0 || done);
+    const int n_free = __popcll(fm);
+    const bool more = J.next < J.njobs;
+    if (n_free == 64 || (more && n_free >= kRefillIdle)) {
+      if (done) {
+        hit[r] = best_tri >= 0 ? 1 : 0;
+        t_out[r] = best;
+        face[r] = best_tri >= 0 ? best_id : -1;
+        v3 p = ro, w = mk(0.0f, 0.0f, 0.0f);
+        if (best_tri >= 0) {
+          p = add3(ro, mul3(rd, best));
+          v3 ta, tb, tc;
+          int32_t id;
+          load_tri(tris, best_tri, ta, tb, tc, id);
+          w = tri_bary(ta, tb, tc, p);
+        }
+        if (pts) { pts[r * 3] = p.x; pts[r * 3 + 1] = p.y; pts[r * 3 + 2] = p.z; }
+        if (uvw) { uvw[r * 3] = w.x; uvw[r * 3 + 1] = w.y; uvw[r * 3 + 2] = w.z; }
+        done = false;
+        r = -1;
+      }
+      if (!more) break;  // every lane free and no ray left
+      const int64_t nr = lane_take(J, fm, K);
       if (nr >= 0) {
         r = nr;
         ro = mk(O[r * 3], O[r * 3 + 1], O[r * 3 + 2]);
@@ -202,11 +270,10 @@ __global__ __launch_bounds__(kBlk) void k_ray_nearest(const void* __restrict__ b
         sp = 0;
         cur = 0;
       }
-      continue;  // the new lanes' loads, and a refill that found no job, start the next round
+      continue;
     }
-    if (n_idle == 64) break;  // no job left and every lane idle
     bool pop = false, at_leaf = false;
-    if (r >= 0) {
+    if (r >= 0 && !done) {
       if (empty) {
         pop = true;
       } else if (cur >= 0) {
@@ -246,11 +313,11 @@ __global__ __launch_bounds__(kBlk) void k_ray_nearest(const void* __restrict__ b
       for (int base = 0; base < dl.total; base += 64) {
         wave_fence();
         const unsigned long long mine = ((unsigned long long)__float_as_uint(best) << 32) | (uint32_t)best_id;
-        S->key[lane] = mine;
+        key[lane] = mine;
         bool valid;
         int L;
         int32_t t;
-        leaf_pair(dl, base, S, lane, valid, L, t);
+        leaf_pair(dl, base, own, lane, valid, L, t);
         const v3 pro = mk(__shfl(ro.x, L), __shfl(ro.y, L), __shfl(ro.z, L));
         const v3 prd = mk(__shfl(rd.x, L), __shfl(rd.y, L), __shfl(rd.z, L));
         unsigned long long k = ~0ull;
@@ -261,17 +328,17 @@ __global__ __launch_bounds__(kBlk) void k_ray_nearest(const void* __restrict__ b
           const float th = ray_tri(ta, tb, tc, pro, prd);
           if (th != kNoHit && th < max_t) {
             k = ((unsigned long long)__float_as_uint(th) << 32) | (uint32_t)id;
-            atomicMin(&S->key[L], k);
+            atomicMin(&key[L], k);
           }
         }
         wave_fence();
-        if (valid && S->key[L] == k) S->tri[L] = t;
+        if (valid && key[L] == k) wtri[L] = t;
         wave_fence();
-        const unsigned long long nk = S->key[lane];
+        const unsigned long long nk = key[lane];
         if (nk != mine) {
           best = __uint_as_float((uint32_t)(nk >> 32));
           best_id = (int32_t)(uint32_t)nk;
-          best_tri = S->tri[lane];
+          best_tri = wtri[lane];
           thr = prune_t(best);
         }
       }
@@ -291,91 +358,108 @@ __global__ __launch_bounds__(kBlk) void k_ray_nearest(const void* __restrict__ b
       }
       if (tt <= thr) { cur = rr; found = true; break; }
     }
-    if (found) continue;
-    // the walk of ray r is over
-    hit[r] = best_tri >= 0 ? 1 : 0;
-    t_out[r] = best;
-    face[r] = best_tri >= 0 ? best_id : -1;
-    v3 p = ro, w = mk(0.0f, 0.0f, 0.0f);
-    if (best_tri >= 0) {
-      p = add3(ro, mul3(rd, best));
-      v3 ta, tb, tc;
-      int32_t id;
-      load_tri(tris, best_tri, ta, tb, tc, id);
-      w = tri_bary(ta, tb, tc, p);
-    }
-    if (pts) { pts[r * 3] = p.x; pts[r * 3 + 1] = p.y; pts[r * 3 + 2] = p.z; }
-    if (uvw) { uvw[r * 3] = w.x; uvw[r * 3 + 1] = w.y; uvw[r * 3 + 2] = w.z; }
-    r = -1;
+    if (!found) done = true;
   }
 }
 
 // ------------------------------------------------------------ all hits
 // offsets == nullptr: counts[r] = number of hits; else the hits of ray r are
-// written from offsets[r] on (walk order): ray index, face, t, sign.
+// written from offsets[r] on, in walk order (leaves in the order the walk
+// reaches them, a leaf's triangles in order): ray index, face, t, sign.
+// Rays from the wavefront's LaneJobs list, leaf triangles dealt over the
+// wavefront, as in k_ray_nearest; a hit's slot is its owner's position plus
+// the owner's hits before it in the round.
 __global__ __launch_bounds__(kBlk) void k_ray_all(const void* __restrict__ blob, const float* __restrict__ O,
                                                    const float* __restrict__ D, int64_t K, float max_t,
                                                    int32_t* __restrict__ counts, const int64_t* __restrict__ offsets,
                                                    int32_t* __restrict__ ray_out, int32_t* __restrict__ face_out,
                                                    float* __restrict__ t_out, int32_t* __restrict__ sign_out) {
   __shared__ int32_t s_ref[kRayLds * kBlk];
-  const int tid = threadIdx.x;
-  const int64_t r = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlk + tid;
-  if (r >= K) return;
+  __shared__ uint8_t s_own[kBlk];
+  const int tid = threadIdx.x, lane = tid & 63;
+  uint8_t* own = s_own + (tid & ~63);
   const BvhHeader* H = header(blob);
   const Node64* __restrict__ nodes = nodes_of(blob);
   const Tri48* __restrict__ tris = tris_of(blob);
-  const v3 ro = mk(O[r * 3], O[r * 3 + 1], O[r * 3 + 2]);
-  const v3 rd = mk(D[r * 3], D[r * 3 + 1], D[r * 3 + 2]);
-  const v3 inv = mk(safe_inv(rd.x), safe_inv(rd.y), safe_inv(rd.z));
-  const float wid = box_widen(ro, H->coord_max);
+  const bool empty = H->n_tris == 0;
   const float lim = prune_t(max_t);
-  int64_t pos = offsets ? offsets[r] : 0;
+  LaneJobs J = lane_jobs(K, kBlk);
+
+  int64_t r = -1;
+  bool done = false;
+  v3 ro = mk(0.0f, 0.0f, 0.0f), rd = ro, inv = ro;
+  float wid = 0.0f;
+  int64_t pos = 0;
   int32_t n_hit = 0;
   int32_t ovf_ref[kRayOvf];
   int sp = 0;
   int32_t cur = 0;
-  bool live = H->n_tris > 0;
-  while (live) {
-    if (cur >= 0) {
-      const Node64* nd = nodes + cur;
-      const float4 a = nd->a, b = nd->b, c = nd->c;
-      const int4 d = nd->d;
-      const bool vl = ray_box(ro, inv, a.x, a.y, a.z, a.w, b.x, b.y, wid) <= lim;
-      const bool vr = ray_box(ro, inv, b.z, b.w, c.x, c.y, c.z, c.w, wid) <= lim;
-      if (vl && vr) {
-        if (sp < kRayLds) s_ref[sp * kBlk + tid] = d.y;
-        else if (sp < kStackTotal) ovf_ref[sp - kRayLds] = d.y;
-        sp = sp < kStackTotal ? sp + 1 : sp;
-        cur = d.x;
-        continue;
+  while (true) {
+    const uint64_t fm = __ballot(r < 0 || done);
+    const int n_free = __popcll(fm);
+    const bool more = J.next < J.njobs;
+    if (n_free == 64 || (more && n_free >= kRefillIdle)) {
+      if (done) {
+        if (!offsets) counts[r] = n_hit;
+        done = false;
+        r = -1;
       }
-      if (vl) { cur = d.x; continue; }
-      if (vr) { cur = d.y; continue; }
-    } else {
-      const int32_t s0 = leaf_start(cur), n = leaf_count(cur);
-      for (int32_t t = s0; t < s0 + n; ++t) {
-        v3 ta, tb, tc;
-        int32_t id;
-        load_tri(tris, t, ta, tb, tc, id);
-        float th;
-        const int sg = all_hit_sign(ta, tb, tc, ro, rd, max_t, th);
-        if (!sg) continue;
-        if (offsets) {
-          ray_out[pos] = (int32_t)r;
-          face_out[pos] = id;
-          if (t_out) t_out[pos] = th;
-          if (sign_out) sign_out[pos] = sg;
-          ++pos;
+      if (!more) break;
+      const int64_t nr = lane_take(J, fm, K);
+      if (nr >= 0) {
+        r = nr;
+        ro = mk(O[r * 3], O[r * 3 + 1], O[r * 3 + 2]);
+        rd = mk(D[r * 3], D[r * 3 + 1], D[r * 3 + 2]);
+        inv = mk(safe_inv(rd.x), safe_inv(rd.y), safe_inv(rd.z));
+        wid = box_widen(ro, H->coord_max);
+        pos = offsets ? offsets[r] : 0;
+        n_hit = 0;
+        sp = 0;
+        cur = 0;
+      }
+      continue;
+    }
+    bool pop = false, at_leaf = false;
+    if (r >= 0 && !done) {
+      if (empty) pop = true;
+      else all_step(nodes, ro, inv, wid, lim, cur, sp, s_ref, tid, ovf_ref, at_leaf, pop);
+    }
+    if (__ballot(at_leaf)) {
+      const LeafDeal dl = leaf_deal(at_leaf, cur, lane);
+      for (int base = 0; base < dl.total; base += 64) {
+        wave_fence();
+        bool valid;
+        int L;
+        int32_t t;
+        leaf_pair(dl, base, own, lane, valid, L, t);
+        const v3 pro = mk(__shfl(ro.x, L), __shfl(ro.y, L), __shfl(ro.z, L));
+        const v3 prd = mk(__shfl(rd.x, L), __shfl(rd.y, L), __shfl(rd.z, L));
+        const long long oray = __shfl((long long)r, L);
+        const long long opos = __shfl((long long)pos, L);
+        const int first = __shfl(dl.excl, L) - base;  // the owner's first pair of the round, as a lane
+        float th = 0.0f;
+        int sg = 0;
+        int32_t id = 0;
+        if (valid) {
+          v3 ta, tb, tc;
+          load_tri(tris, t, ta, tb, tc, id);
+          sg = all_hit_sign(ta, tb, tc, pro, prd, max_t, th);
         }
-        ++n_hit;
+        const uint64_t hm = __ballot(sg != 0);
+        if (offsets && sg) {
+          const int64_t q = opos + __popcll(hm & lanes_below(lane) & ~lanes_below(first > 0 ? first : 0));
+          ray_out[q] = (int32_t)oray;
+          face_out[q] = id;
+          if (t_out) t_out[q] = th;
+          if (sign_out) sign_out[q] = sg;
+        }
+        const int c = __popcll(hm & own_pairs(dl, base));
+        n_hit += c;
+        pos += c;
       }
     }
-    if (sp == 0) break;
-    --sp;
-    cur = sp < kRayLds ? s_ref[sp * kBlk + tid] : ovf_ref[sp - kRayLds];
+    if (pop && !all_pop(cur, sp, s_ref, tid, ovf_ref)) done = true;
   }
-  if (!offsets) counts[r] = n_hit;
 }
 
 // t, sign, point and barycentrics of (ray, face) pairs from the original mesh
@@ -450,78 +534,8 @@ __global__ __launch_bounds__(kBlk) void k_ray_crossings(float* __restrict__ t, i
 constexpr int kCrossCap = 16;
 constexpr int32_t kCrossOverflow = INT32_MIN;
 
-__global__ __launch_bounds__(kBlk) void k_ray_cross(const void* __restrict__ blob, const float* __restrict__ P,
-                                                     int64_t N, const float* __restrict__ dirs, int64_t K,
-                                                     float max_t, float coalesce_t, int32_t* __restrict__ out) {
-  __shared__ int32_t s_ref[kRayLds * kBlk];
-  const int tid = threadIdx.x;
-  const int64_t r = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kBlk + tid;
-  if (r >= K) return;
-  const int64_t pi = r % N, di = r / N;
-  const BvhHeader* H = header(blob);
-  const Node64* __restrict__ nodes = nodes_of(blob);
-  const Tri48* __restrict__ tris = tris_of(blob);
-  const v3 ro = mk(P[pi * 3], P[pi * 3 + 1], P[pi * 3 + 2]);
-  const v3 rd = mk(dirs[di * 3], dirs[di * 3 + 1], dirs[di * 3 + 2]);
-  const v3 inv = mk(safe_inv(rd.x), safe_inv(rd.y), safe_inv(rd.z));
-  const float wid = box_widen(ro, H->coord_max);
-  const float lim = prune_t(max_t);
-  // a hit is kept as sign * t (t > 0): |h| is its t exactly, and at equal t
-  // the negative sign orders first, as in k_ray_crossings' (t, sign) order
-  float hk[kCrossCap];
-#pragma unroll
-  for (int k = 0; k < kCrossCap; ++k) hk[k] = __int_as_float(0x7f800000);
-  int32_t n_hit = 0;
-  int32_t ovf_ref[kRayOvf];
-  int sp = 0;
-  int32_t cur = 0;
-  bool live = H->n_tris > 0;
-  while (live) {
-    if (cur >= 0) {
-      const Node64* nd = nodes + cur;
-      const float4 a = nd->a, b = nd->b, c = nd->c;
-      const int4 d = nd->d;
-      const bool vl = ray_box(ro, inv, a.x, a.y, a.z, a.w, b.x, b.y, wid) <= lim;
-      const bool vr = ray_box(ro, inv, b.z, b.w, c.x, c.y, c.z, c.w, wid) <= lim;
-      if (vl && vr) {
-        if (sp < kRayLds) s_ref[sp * kBlk + tid] = d.y;
-        else if (sp < kStackTotal) ovf_ref[sp - kRayLds] = d.y;
-        sp = sp < kStackTotal ? sp + 1 : sp;
-        cur = d.x;
-        continue;
-      }
-      if (vl) { cur = d.x; continue; }
-      if (vr) { cur = d.y; continue; }
-    } else {
-      const int32_t s0 = leaf_start(cur), n = leaf_count(cur);
-      for (int32_t t = s0; t < s0 + n; ++t) {
-        v3 ta, tb, tc;
-        int32_t id;
-        load_tri(tris, t, ta, tb, tc, id);
-        float th;
-        const int sg = all_hit_sign(ta, tb, tc, ro, rd, max_t, th);
-        if (!sg) continue;
-        ++n_hit;
-        // sorted insertion: the new hit moves down past every kept hit it precedes
-        float x = sg > 0 ? th : -th;
-#pragma unroll
-        for (int k = 0; k < kCrossCap; ++k) {
-          const float ax = fabsf(x), ah = fabsf(hk[k]);
-          const bool before = ax < ah || (ax == ah && x < hk[k]);
-          const float o = hk[k];
-          hk[k] = before ? x : o;
-          x = before ? o : x;
-        }
-      }
-    }
-    if (sp == 0) break;
-    --sp;
-    cur = sp < kRayLds ? s_ref[sp * kBlk + tid] : ovf_ref[sp - kRayLds];
-  }
-  if (n_hit > kCrossCap) {
-    out[r] = kCrossOverflow;
-    return;
-  }
+__device__ __forceinline__ int32_t cross_total(const float (&hk)[kCrossCap], int32_t n_hit, float coalesce_t) {
+  if (n_hit > kCrossCap) return kCrossOverflow;
   // groups of consecutive hits closer than coalesce_t (k_ray_crossings' rule)
   int32_t total = 0, cnt = 0, sum = 0;
   float prev = 0.0f;
@@ -544,7 +558,115 @@ __global__ __launch_bounds__(kBlk) void k_ray_cross(const void* __restrict__ blo
     const int32_t g = (sum > 0) - (sum < 0);
     total += (abs(sum) == cnt) ? g : (abs(sum) % 2) * g;
   }
-  out[r] = total;
+  return total;
+}
+
+// Rays from the wavefront's LaneJobs list, leaf triangles dealt over the
+// wavefront, as in k_ray_nearest; each owner inserts its pairs' hits.
+__global__ __launch_bounds__(kBlk) void k_ray_cross(const void* __restrict__ blob, const float* __restrict__ P,
+                                                     int64_t N, const float* __restrict__ dirs, int64_t K,
+                                                     float max_t, float coalesce_t, int32_t* __restrict__ out) {
+  __shared__ int32_t s_ref[kRayLds * kBlk];
+  __shared__ uint8_t s_own[kBlk];
+  __shared__ float s_hv[kBlk];  // the signed t of each pair of a round (0: no hit)
+  const int tid = threadIdx.x, lane = tid & 63;
+  uint8_t* own = s_own + (tid & ~63);
+  float* hv = s_hv + (tid & ~63);
+  const BvhHeader* H = header(blob);
+  const Node64* __restrict__ nodes = nodes_of(blob);
+  const Tri48* __restrict__ tris = tris_of(blob);
+  const bool empty = H->n_tris == 0;
+  const float lim = prune_t(max_t);
+  LaneJobs J = lane_jobs(K, kBlk);
+
+  int64_t r = -1;
+  bool done = false;
+  v3 ro = mk(0.0f, 0.0f, 0.0f), rd = ro, inv = ro;
+  float wid = 0.0f;
+  // a hit is kept as sign * t (t > 0): |h| is its t exactly, and at equal t
+  // the negative sign orders first, as in k_ray_crossings' (t, sign) order
+  float hk[kCrossCap];
+#pragma unroll
+  for (int k = 0; k < kCrossCap; ++k) hk[k] = __int_as_float(0x7f800000);
+  int32_t n_hit = 0;
+  int32_t ovf_ref[kRayOvf];
+  int sp = 0;
+  int32_t cur = 0;
+  while (true) {
+    const uint64_t fm = __ballot(r < 0 || done);
+    const int n_free = __popcll(fm);
+    const bool more = J.next < J.njobs;
+    if (n_free == 64 || (more && n_free >= kRefillIdle)) {
+      if (done) {
+        out[r] = cross_total(hk, n_hit, coalesce_t);
+        done = false;
+        r = -1;
+      }
+      if (!more) break;
+      const int64_t nr = lane_take(J, fm, K);
+      if (nr >= 0) {
+        r = nr;
+        const int64_t pi = r % N, di = r / N;
+        ro = mk(P[pi * 3], P[pi * 3 + 1], P[pi * 3 + 2]);
+        rd = mk(dirs[di * 3], dirs[di * 3 + 1], dirs[di * 3 + 2]);
+        inv = mk(safe_inv(rd.x), safe_inv(rd.y), safe_inv(rd.z));
+        wid = box_widen(ro, H->coord_max);
+#pragma unroll
+        for (int k = 0; k < kCrossCap; ++k) hk[k] = __int_as_float(0x7f800000);
+        n_hit = 0;
+        sp = 0;
+        cur = 0;
+      }
+      continue;
+    }
+    bool pop = false, at_leaf = false;
+    if (r >= 0 && !done) {
+      if (empty) pop = true;
+      else all_step(nodes, ro, inv, wid, lim, cur, sp, s_ref, tid, ovf_ref, at_leaf, pop);
+    }
+    if (__ballot(at_leaf)) {
+      const LeafDeal dl = leaf_deal(at_leaf, cur, lane);
+      for (int base = 0; base < dl.total; base += 64) {
+        wave_fence();
+        bool valid;
+        int L;
+        int32_t t;
+        leaf_pair(dl, base, own, lane, valid, L, t);
+        const v3 pro = mk(__shfl(ro.x, L), __shfl(ro.y, L), __shfl(ro.z, L));
+        const v3 prd = mk(__shfl(rd.x, L), __shfl(rd.y, L), __shfl(rd.z, L));
+        float x = 0.0f;
+        int sg = 0;
+        if (valid) {
+          v3 ta, tb, tc;
+          int32_t id;
+          load_tri(tris, t, ta, tb, tc, id);
+          float th;
+          sg = all_hit_sign(ta, tb, tc, pro, prd, max_t, th);
+          x = sg > 0 ? th : -th;
+        }
+        hv[lane] = x;
+        const uint64_t hm = __ballot(sg != 0);
+        wave_fence();
+        uint64_t mine = hm & own_pairs(dl, base);
+        n_hit += __popcll(mine);
+        while (mine) {
+          const int k = __builtin_ctzll(mine);
+          mine &= mine - 1ull;
+          // sorted insertion: the new hit moves down past every kept hit it precedes
+          float y = hv[k];
+#pragma unroll
+          for (int m = 0; m < kCrossCap; ++m) {
+            const float ay = fabsf(y), ah = fabsf(hk[m]);
+            const bool before = ay < ah || (ay == ah && y < hk[m]);
+            const float o = hk[m];
+            hk[m] = before ? y : o;
+            y = before ? o : y;
+          }
+        }
+      }
+    }
+    if (pop && !all_pop(cur, sp, s_ref, tid, ovf_ref)) done = true;
+  }
 }
 
 inline unsigned grid_for(int64_t n) { return (unsigned)div_up(n > 0 ? n : 1, kBlk); }
@@ -580,9 +702,10 @@ extern "C" int a3d_ray_all_count(const void* blob, const float* O, const float* 
   if (rc) return rc;
   if (K == 0) return A3D_OK;
   if (!counts) return fail(A3D_ERR_ARG, "a3d_ray_all_count: null counts");
-  hipLaunchKernelGGL(k_ray_all, dim3(grid_for(K)), dim3(kBlk), 0, (hipStream_t)stream, blob, O, D, K, max_t, counts,
-                     (const int64_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr, (float*)nullptr,
-                     (int32_t*)nullptr);
+  static int resident[64];
+  hipLaunchKernelGGL(k_ray_all, dim3(refill_grid(k_ray_all, kBlk, K, resident)), dim3(kBlk), 0, (hipStream_t)stream,
+                     blob, O, D, K, max_t, counts, (const int64_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr,
+                     (float*)nullptr, (int32_t*)nullptr);
   A3D_LAUNCH_CHECK();
   return A3D_OK;
 }
@@ -594,8 +717,9 @@ extern "C" int a3d_ray_all_hits(const void* blob, const float* O, const float* D
   if (rc) return rc;
   if (K == 0) return A3D_OK;
   if (!offsets || !ray_idx || !face) return fail(A3D_ERR_ARG, "a3d_ray_all_hits: null offsets or outputs");
-  hipLaunchKernelGGL(k_ray_all, dim3(grid_for(K)), dim3(kBlk), 0, (hipStream_t)stream, blob, O, D, K, max_t,
-                     (int32_t*)nullptr, offsets, ray_idx, face, t, sign);
+  static int resident[64];
+  hipLaunchKernelGGL(k_ray_all, dim3(refill_grid(k_ray_all, kBlk, K, resident)), dim3(kBlk), 0, (hipStream_t)stream,
+                     blob, O, D, K, max_t, (int32_t*)nullptr, offsets, ray_idx, face, t, sign);
   A3D_LAUNCH_CHECK();
   return A3D_OK;
 }
@@ -620,8 +744,9 @@ extern "C" int a3d_ray_crossings_dirs(const void* blob, const float* P, int64_t 
   const int64_t K = N * n_dirs;
   if (K == 0) return A3D_OK;
   if (!blob || !P || !dirs || !out) return fail(A3D_ERR_ARG, "a3d_ray_crossings_dirs: null argument");
-  hipLaunchKernelGGL(k_ray_cross, dim3(grid_for(K)), dim3(kBlk), 0, (hipStream_t)stream, blob, P, N, dirs, K, max_t,
-                     coalesce_t, out);
+  static int resident[64];
+  hipLaunchKernelGGL(k_ray_cross, dim3(refill_grid(k_ray_cross, kBlk, K, resident)), dim3(kBlk), 0,
+                     (hipStream_t)stream, blob, P, N, dirs, K, max_t, coalesce_t, out);
   A3D_LAUNCH_CHECK();
   return A3D_OK;
 }
