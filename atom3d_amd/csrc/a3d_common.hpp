@@ -368,13 +368,14 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
 inline int64_t div_up(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // ------------------------------------------------------------------ lane refill
-// Independent per-lane walks (one ray or box per lane) vary in length by
+// Independent per-lane walks (one ray per lane) vary in length by
 // orders of magnitude; a wavefront that walked 64 of them side by side would
 // run as long as its longest one with most lanes idle.  Instead each
 // wavefront of a grid sized to the resident wavefronts works through a list
 // of items, and its idle lanes take the next items while the others walk on.
 // Wavefront wv of W takes the blocks of 64 consecutive items b = wv, wv + W,
-// ...; its job j is item ((j >> 6) * W + wv) * 64 + (j & 63).
+// ...; its job j is item ((j >> 6) * W + wv) * 64 + (j & 63).  (Blocks in
+// runs per XCD, as xcd_block maps them, made random rays 7 % slower.)
 struct LaneJobs {
   int64_t W, wv, njobs, next;  // wave-uniform
 };
@@ -424,6 +425,66 @@ inline unsigned refill_grid(Kern kern, int block_threads, int64_t K, int* cache)
   }
   const int64_t want = div_up(K > 0 ? K : 1, block_threads);
   return (unsigned)(want < cache[dev] ? want : cache[dev]);
+}
+
+// ------------------------------------------------------------------ leaf tests dealt over the wavefront
+// In a per-lane ray walk a lane that reaches a leaf tests its (up to 8)
+// triangles one after another while the other lanes wait; with 64 lanes
+// some lane is at a leaf in most steps, so the wavefront would spend most
+// of its time in leaf loops with a few lanes active.  Instead the (lane,
+// triangle) pairs of every lane at a leaf are dealt out one per lane, in
+// rounds of 64, and tested side by side; each result goes back to the lane
+// that owns the pair.
+__device__ __forceinline__ void wave_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// This lane's leaf (n triangles from start; n = 0: none) and the exclusive
+// prefix of n over the lanes; total pairs (wave-uniform).
+struct LeafDeal {
+  int32_t start, n, excl, total;
+};
+
+__device__ __forceinline__ LeafDeal leaf_deal(bool at_leaf, int32_t ref, int lane) {
+  LeafDeal d;
+  d.n = at_leaf ? leaf_count(ref) : 0;
+  d.start = at_leaf ? leaf_start(ref) : 0;
+  int incl = d.n;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int v = __shfl_up(incl, o);
+    if (lane >= o) incl += v;
+  }
+  d.excl = incl - d.n;
+  d.total = __shfl(incl, 63);
+  return d;
+}
+
+// Pair base + lane of a round: owner lane L and triangle (valid = the pair
+// exists).  own: the wavefront's 64-entry owner table.  Ends with the table
+// read; the caller's next round starts with a fence.
+__device__ __forceinline__ void leaf_pair(const LeafDeal& d, int base, uint8_t* own, int lane, bool& valid, int& L,
+                                          int32_t& tri) {
+  const int lo = d.excl > base ? d.excl : base;
+  const int hi = d.excl + d.n < base + 64 ? d.excl + d.n : base + 64;
+  for (int k = lo; k < hi; ++k) own[k - base] = (uint8_t)lane;
+  wave_fence();
+  const int j = base + lane;
+  valid = j < d.total;
+  L = valid ? (int)own[lane] : lane;
+  tri = __shfl(d.start, L) + (j - __shfl(d.excl, L));
+}
+
+// Lanes [0, k) as a mask, k in [0, 64].
+__device__ __forceinline__ uint64_t lanes_below(int k) { return k >= 64 ? ~0ull : ((1ull << k) - 1ull); }
+
+// This lane's pairs among the lanes of round `base`, as a lane mask.
+__device__ __forceinline__ uint64_t own_pairs(const LeafDeal& d, int base) {
+  const int lo = d.excl - base > 0 ? d.excl - base : 0;
+  const int hi = d.excl + d.n - base < 64 ? d.excl + d.n - base : 64;
+  return (d.n > 0 && lo < hi) ? (lanes_below(hi) & ~lanes_below(lo)) : 0ull;
 }
 
 }  // namespace a3d

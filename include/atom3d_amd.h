@@ -230,7 +230,7 @@ int a3d_sdf_sign(const float* P, int64_t K, const float* closest, const int32_t*
  * a3d_ray_crossings_dirs: the same counts for N points P float32[N*3] along
  *   n_dirs directions dirs float32[n_dirs*3] (ray r = direction r / N from
  *   point r % N, N * n_dirs < 2^31) in one walk per ray, its hits kept
- *   sorted in registers: out int32[n_dirs*N]; a ray with more than 16 hits
+ *   sorted in registers: out int32[n_dirs*N]; a ray with more than 8 hits
  *   gets INT32_MIN and is counted by the two-pass path instead.  Replaces
  *   the per-direction loop of MeshBVH._sdf_parity (mesh_bvh.py:851-885).
  */
