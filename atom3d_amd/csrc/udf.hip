@@ -196,9 +196,11 @@ __device__ __forceinline__ void q_flush(WaveQ* __restrict__ Q, int n, int qn, co
 // Per-lane bounds of the RING most recent stack entries (RING a power of
 // two, or 0 for none), indexed by the wave-uniform stack position modulo
 // RING.  4 entries: the closest-point walk 12.62 -> 12.12 ms on the 4.1M-face
-// union; the batched walk keeps none, since the registers would cost it a
-// wavefront per SIMD (79 -> 86 VGPRs, 43.1 -> 45.6 ms).
+// union.  The batched walk keeps 2, its kernel held to 80 VGPRs (6
+// wavefronts per SIMD, as without the ring): 43.2 -> 41.6 ms on 256 meshes;
+// unheld, 4 entries took it to 86 VGPRs, 5 wavefronts and 45.6 ms.
 constexpr int kRingUdf = 4;
+constexpr int kRingBatch = 2;
 template <int RING>
 struct WalkRing {
   static_assert(RING >= 0 && (RING & (RING - 1)) == 0, "ring size: 0 or a power of two");
@@ -531,7 +533,8 @@ __device__ __forceinline__ void batch_walk_tile(const void* __restrict__ blob, c
   }
   int32_t best_id = 0x7fffffff, best_tri = -1;
   uint32_t n0 = 0, n1 = 0, n2 = 0, n3 = 0;
-  wave_walk<false, true, 0>(nodes, tris, boxes_of(blob), obbs_of(blob), tobbs_of(blob), p, valid, slack, stk, Q, lane,
+  wave_walk<false, true, kRingBatch>(nodes, tris, boxes_of(blob), obbs_of(blob), tobbs_of(blob), p, valid, slack,
+                                  stk, Q, lane,
                          best, best_id,
                          best_tri, n0, n1, n2, n3);
   if (valid) write_result(q, p, best, best_id, best_tri, tris, dist, face, closest, uvw);
@@ -1340,7 +1343,8 @@ __global__ void k_batch_reset(unsigned int* counter) {
 // tiles: int64 pairs (start << 8 | count, mesh), start = index into perm.
 // bounds (nullable): one GridBound per mesh; redo: walk only the lanes whose
 // face is -1 (the second pass after a bounded one), without bounds.
-__global__ void k_udf_batch(const void* const* __restrict__ blobs, const float* __restrict__ P,
+__global__ __attribute__((amdgpu_waves_per_eu(6, 6))) void k_udf_batch(const void* const* __restrict__ blobs,
+                                                                          const float* __restrict__ P,
                             const uint32_t* __restrict__ perm, const int64_t* __restrict__ tiles, int64_t n_tiles,
                             unsigned int* __restrict__ counter, float* __restrict__ dist, int32_t* __restrict__ face,
                             float* __restrict__ closest, float* __restrict__ uvw,
