@@ -29,7 +29,10 @@ namespace a3d {
 namespace {
 
 constexpr int kBlk = 256;
-constexpr int kRayLds = 8;  // per-lane LDS stack entries: 16 KB per block of the nearest-hit walk, so 8 wavefronts per SIMD fit (16 entries: 5)
+// Per-lane LDS stack entries (the rest in private memory): 24 KB per block of the
+// nearest-hit walk; its registers allow 5 wavefronts per SIMD, and 5 blocks' LDS fits a CU
+// (8 entries: 6.75 ms, 12: 6.66 ms on 2^24 rays)
+constexpr int kRayLds = 12;
 constexpr int kRayOvf = kStackTotal - kRayLds;
 constexpr float kNoHit = 1e10f;
 
